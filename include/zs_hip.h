@@ -1,0 +1,174 @@
+/* zs_hip.h -- C ABI of the MI355X (gfx950) M31 trace-commitment path.
+ *
+ * Replaces zig-stark's src/cuda layer (src/cuda/circlefft_gpu.zig,
+ * src/cuda/cuda.zig) and follows the conventions of zig-capi.h:
+ *   - return codes: 0 ok, -1 generic, -2 malformed input, -3 out of memory,
+ *     -4 protocol error (zig-capi.h:20-21), plus -5 for "no usable HIP device
+ *     or a HIP runtime failure" (the reference's error.CudaFailure /
+ *     error.NoCudaDevice, src/cuda/cuda.zig:54-106).  There is no CPU
+ *     fallback: without a device every compute entry point returns -5;
+ *   - buffers passed in are borrowed for the duration of the call;
+ *   - buffers the library returns are allocated with the caller's
+ *     zs_host_allocator and released with zs_free (zig-capi.h:33-39, 73-74).
+ *
+ * Layouts: M31 values are uint32_t in [0, 2^31 - 1).  QM31 values are four
+ * uint32_t (a.c0, a.c1, b.c0, b.c1; src/m31/field/qm31.zig:91-101).  Digests
+ * are 32 bytes.  Columns are column-major: column c starts at c * col_stride.
+ * A Merkle node array holds every level, leaves first (level 0 = 2^k
+ * digests, ..., the root last), as MerkleTree.nodes does
+ * (src/core/merkle/merkle.zig:17-45).
+ *
+ * Functions whose name ends in _dev take device pointers and run
+ * asynchronously on the context's stream; the others take host pointers and
+ * return when the result is in host memory (as circlefft_gpu.zig does).
+ * Calls on one context must not overlap from several threads; use one
+ * context per thread (contexts on one device share nothing).
+ */
+#ifndef ZS_HIP_H
+#define ZS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ZS_OK 0
+#define ZS_ERR_GENERIC -1
+#define ZS_ERR_INPUT -2
+#define ZS_ERR_OOM -3
+#define ZS_ERR_PROTOCOL -4
+#define ZS_ERR_DEVICE -5
+
+/* flags for zs_m31_lde_commit_dev / zs_fri_commit_dev */
+#define ZS_FLAG_PLAIN 1u /* reference-shaped per-column, per-layer launches (comparison path) */
+
+typedef struct zs_hip_ctx zs_hip_ctx;
+typedef struct zs_tree zs_tree;
+typedef struct zs_channel zs_channel;
+typedef struct zs_fri_result zs_fri_result;
+
+/* zig-capi.h:35-39 */
+typedef struct {
+    void *ctx;
+    void *(*alloc)(void *ctx, size_t size);
+    void (*free)(void *ctx, void *ptr, size_t size);
+} zs_host_allocator;
+
+/* CircleCoset (src/m31/circle/coset.zig:10-14): offset + <step>, 2^log_size points */
+typedef struct {
+    uint32_t offset_x, offset_y, step_x, step_y, log_size;
+} zs_circle_coset;
+
+/* ---- library and context --------------------------------------------------- */
+/* zig-capi.h:76-77 (zs_version) */
+const char *zs_hip_version(void);
+/* Lazily-initialised context on `device` (src/cuda/circlefft_gpu.zig:20-39,
+ * src/cuda/cuda.zig:81-106).  flags: reserved, pass 0. */
+int32_t zs_hip_init(int32_t device, uint32_t flags, zs_hip_ctx **out_ctx);
+void zs_hip_destroy(zs_hip_ctx *ctx);
+const char *zs_hip_last_error(const zs_hip_ctx *ctx);
+/* Run _dev calls on this hipStream_t (NULL = the null stream). */
+int32_t zs_hip_set_stream(zs_hip_ctx *ctx, void *hip_stream);
+int32_t zs_hip_synchronize(zs_hip_ctx *ctx);
+/* zig-capi.h:73-74 */
+void zs_free(zs_host_allocator host, uint8_t *ptr, size_t len);
+
+/* CircleCoset.canonicHalf(log_size), 1 <= log_size <= 28 (coset.zig:55-65). */
+int32_t zs_circle_coset_canonic_half(uint32_t log_size, zs_circle_coset *out);
+
+/* ---- circle FFT ---------------------------------------------------------------- */
+/* circleFFT / circleIFFT (src/cuda/circlefft_gpu.zig:46 and :111, the GPU
+ * replacements of src/m31/ntt/circle.zig:195 and :272).  n = 2 * 2^half_coset.log_size,
+ * n >= 2.  Host buffers; every size runs on the device (no CPU fallback). */
+int32_t zs_m31_circle_fft(zs_hip_ctx *ctx, const uint32_t *coeffs, size_t n, const zs_circle_coset *half_coset,
+                          uint32_t *evals);
+int32_t zs_m31_circle_ifft(zs_hip_ctx *ctx, const uint32_t *evals, size_t n, const zs_circle_coset *half_coset,
+                           uint32_t *coeffs);
+/* Batched device variants on canonicHalf(log_n): n_cols columns, column c at c * col_stride. */
+int32_t zs_m31_circle_fft_batch_dev(zs_hip_ctx *ctx, const uint32_t *d_coeffs, uint32_t *d_evals, uint32_t log_n,
+                                    uint32_t n_cols, size_t col_stride);
+int32_t zs_m31_circle_ifft_batch_dev(zs_hip_ctx *ctx, const uint32_t *d_evals, uint32_t *d_coeffs, uint32_t log_n,
+                                     uint32_t n_cols, size_t col_stride);
+
+/* ---- low-degree extension and row-leaf commitment -------------------------- */
+/* Blowup LDE of each column (circleIFFT on canonicHalf(log_n), zero-pad the A and
+ * B halves, circleFFT on canonicHalf(log_n + log_blowup)); then leaf i =
+ * hashM31s(lde_0[i], ..., lde_{c-1}[i]) (src/m31/hash.zig:15-25) and
+ * MerkleTree.init over the 2^(log_n + log_blowup) leaves (merkle.zig:17-45).
+ *   d_cols:  n_cols * 2^log_n words, column-major, contiguous
+ *   d_lde:   n_cols * 2^(log_n+log_blowup) words, column-major, contiguous
+ *   d_nodes: (2^(log_n+log_blowup+1) - 1) * 32 bytes
+ * 1 <= log_n, log_n + log_blowup <= 28, 1 <= n_cols. */
+int32_t zs_m31_lde_commit_dev(zs_hip_ctx *ctx, const uint32_t *d_cols, uint32_t n_cols, uint32_t log_n,
+                              uint32_t log_blowup, uint32_t *d_lde, uint8_t *d_nodes, uint32_t flags);
+/* The LDE alone (no commitment); same layouts as zs_m31_lde_commit_dev. */
+int32_t zs_m31_lde_dev(zs_hip_ctx *ctx, const uint32_t *d_cols, uint32_t n_cols, uint32_t log_n, uint32_t log_blowup,
+                       uint32_t *d_lde);
+/* Row leaves alone: d_leaves[i] = hashM31s(row i), 2^log_rows digests (level 0 only). */
+int32_t zs_m31_row_leaves_dev(zs_hip_ctx *ctx, const uint32_t *d_cols, uint32_t n_cols, uint32_t log_rows,
+                              size_t col_stride, uint8_t *d_leaves);
+/* Host-buffer form: returns a device-resident tree; lde_out may be NULL. */
+int32_t zs_m31_lde_commit(zs_hip_ctx *ctx, const uint32_t *cols, uint32_t n_cols, uint32_t log_n,
+                          uint32_t log_blowup, zs_tree **out_tree, uint8_t root[32], uint32_t *lde_out);
+/* Row-leaf tree over existing device columns of 2^log_rows values. */
+int32_t zs_m31_row_commit_dev(zs_hip_ctx *ctx, const uint32_t *d_cols, uint32_t n_cols, uint32_t log_rows,
+                              size_t col_stride, uint8_t *d_nodes, uint32_t flags);
+
+/* ---- Merkle trees (src/core/merkle/merkle.zig) ---------------------------- */
+uint32_t zs_tree_log_leaves(const zs_tree *tree);
+/* MerkleTree.root (merkle.zig:47-49) */
+int32_t zs_tree_root(const zs_tree *tree, uint8_t out[32]);
+/* MerkleTree.open (merkle.zig:57-68): log_leaves sibling digests, bottom-up. */
+int32_t zs_merkle_open(const zs_tree *tree, size_t index, uint8_t *path_out);
+/* Copy one whole level (2^(log_leaves - level) digests) to host memory. */
+int32_t zs_tree_read_level(const zs_tree *tree, uint32_t level, uint8_t *out);
+void zs_tree_free(zs_tree *tree);
+
+/* ---- Fiat-Shamir channel (src/core/channel/channel.zig), host side ------ */
+zs_channel *zs_channel_new(const uint8_t *domain_separator, size_t len);
+zs_channel *zs_channel_clone(const zs_channel *ch);
+void zs_channel_free(zs_channel *ch);
+void zs_channel_absorb(zs_channel *ch, const uint8_t *data, size_t len);      /* channel.zig:23-29 */
+void zs_channel_sample_bytes(zs_channel *ch, uint8_t *out, size_t len);       /* channel.zig:49-69 */
+void zs_channel_sample_qm31(zs_channel *ch, uint32_t out[4]);                 /* channel.zig:72-76 */
+size_t zs_channel_sample_index(zs_channel *ch, size_t n);                     /* channel.zig:79-101 */
+
+/* ---- FRI commit phase (src/m31/fri.zig:157-233) ----------------------------- */
+/* Commit a QM31 codeword of 2^dlog values on D_0 = FRI_OFFSET * <w_0>:
+ * L = dlog - remainder_log layers are Merkle-committed (hashQM31 leaves) and
+ * folded; the remainder is the 2^(remainder_log - log_blowup) interpolation
+ * coefficients of the last layer.
+ *   alphas: L fold challenges (QM31), or NULL to draw them from `chan`
+ *           (absorb root_i, sample QM31) exactly as proveCodeword does; with
+ *           NULL the remainder is absorbed into `chan` afterwards.
+ *   roots_out: L * 32 bytes; remainder_out: 4 * 2^(remainder_log - log_blowup) words.
+ *   out_result: optional; keeps every layer and tree on the device for queries.
+ * d_codeword is a device pointer; it is not modified. */
+int32_t zs_fri_commit_dev(zs_hip_ctx *ctx, const uint32_t *d_codeword, uint32_t dlog, uint32_t log_blowup,
+                          uint32_t remainder_log, const uint32_t *alphas, zs_channel *chan, uint32_t flags,
+                          uint8_t *roots_out, uint32_t *remainder_out, zs_fri_result **out_result);
+/* Host-buffer form of zs_fri_commit_dev. */
+int32_t zs_fri_commit(zs_hip_ctx *ctx, const uint32_t *codeword, uint32_t dlog, uint32_t log_blowup,
+                      uint32_t remainder_log, const uint32_t *alphas, zs_channel *chan, uint8_t *roots_out,
+                      uint32_t *remainder_out, zs_fri_result **out_result);
+/* Query phase (fri.zig:236-256): num_queries indices from `chan`, then per
+ * layer the pair (p mod n/2, p mod n/2 + n/2) values and both Merkle paths.
+ *   idx_out: num_queries; vals_out: num_queries * L * 2 * 4 words;
+ *   paths_out: per query, per layer i, path0 then path1 of (dlog - i) digests. */
+int32_t zs_fri_query(zs_fri_result *res, zs_channel *chan, size_t num_queries, size_t *idx_out, uint32_t *vals_out,
+                     uint8_t *paths_out);
+/* Copy FRI layer `layer` (2^(dlog - layer) QM31 values) to host memory. */
+int32_t zs_fri_read_layer(const zs_fri_result *res, uint32_t layer, uint32_t *out);
+void zs_fri_result_free(zs_fri_result *res);
+
+/* ---- synthetic input ---------------------------------------------------------- */
+/* Seeded uniform M31 values: element i is the top 31 bits of the (i+1)-th
+ * SplitMix64 output from `seed` (2^31 - 1 rejected). */
+int32_t zs_fill_m31_dev(zs_hip_ctx *ctx, uint64_t seed, uint32_t *d_out, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -12,3 +12,39 @@ for p in (ROOT, TESTS):
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) device and the built HIP library")
+
+
+def _load_pkg():
+    import __graft_entry__
+    return __graft_entry__.load_package()
+
+
+_load_pkg()
+
+
+@pytest.fixture(scope="session")
+def zctx():
+    """One zs_hip context on device 0 for the whole GPU session."""
+    import zig_stark_amd as Z
+    ctx = Z.Context(0)
+    yield ctx
+    ctx.close()
+
+
+@pytest.fixture(scope="session")
+def torch_dev():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a ROCm device"
+    return torch.device("cuda:0")
+
+
+def to_dev(arr, device):
+    """numpy uint32 -> torch int32 tensor on the device (same bits)."""
+    import numpy as np
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(arr, dtype=np.uint32).view(np.int32)).to(device)
+
+
+def from_dev(t):
+    import numpy as np
+    return t.cpu().numpy().view(np.uint32)

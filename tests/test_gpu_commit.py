@@ -1,0 +1,109 @@
+"""GPU parity: blowup LDE + row-leaf BLAKE3 Merkle commitment against the
+oracle (circleIFFT/circleFFT composition, hashM31s leaves, MerkleTree),
+bit-exact; plus the full 64 x 2^20 -> 2^22 workload through
+size-independent properties."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import zig_stark_amd as Z
+from conftest import from_dev, to_dev
+
+pytestmark = pytest.mark.gpu
+P = O.P
+
+
+def oracle_commit(cols, beta):
+    lde = np.stack([O.lde(c, beta) for c in cols])
+    return lde, O.row_commit(lde)
+
+
+@pytest.mark.parametrize("n_cols,lg,beta", [
+    (1, 1, 1), (2, 1, 2), (3, 2, 2), (5, 3, 1), (64, 6, 2), (16, 10, 2), (7, 11, 2),
+    (4, 12, 3), (17, 8, 1), (300, 4, 2), (16, 9, 4),
+])
+def test_lde_commit_matches_oracle(zctx, n_cols, lg, beta):
+    rng = np.random.default_rng(n_cols * 1000 + lg * 10 + beta)
+    cols = rng.integers(0, P, (n_cols, 1 << lg), dtype=np.uint32)
+    tree, root, lde = zctx.lde_commit(cols, beta)
+    ref_lde, ref_nodes = oracle_commit(cols, beta)
+    assert np.array_equal(lde, ref_lde)
+    assert root == O.tree_root(ref_nodes)
+    N = 1 << (lg + beta)
+    # every level equals the oracle's
+    off = 0
+    for lvl in range(lg + beta + 1):
+        cnt = N >> lvl
+        assert tree.level(lvl) == ref_nodes[32 * off: 32 * (off + cnt)], lvl
+        off += cnt
+    # openings equal MerkleTree.open and verify
+    for i in rng.integers(0, N, 6).tolist():
+        path = tree.open(i)
+        assert path == O.merkle_open(ref_nodes, N, i)
+        assert O.merkle_verify(root, i, ref_nodes[32 * i: 32 * i + 32], path)
+
+
+@pytest.mark.parametrize("n_cols,lg,beta", [(8, 3, 2), (64, 8, 2), (9, 12, 2)])
+def test_plain_path_matches_fused(zctx, torch_dev, n_cols, lg, beta):
+    import torch
+    rng = np.random.default_rng(lg)
+    cols = rng.integers(0, P, (n_cols, 1 << lg), dtype=np.uint32)
+    N = 1 << (lg + beta)
+    d_cols = to_dev(cols, torch_dev)
+    outs = []
+    for flags in (0, Z.ZS_FLAG_PLAIN):
+        d_lde = torch.zeros((n_cols, N), dtype=torch.int32, device=torch_dev)
+        d_nodes = torch.zeros((2 * N - 1) * 8, dtype=torch.int32, device=torch_dev)
+        zctx.lde_commit_dev(d_cols.data_ptr(), n_cols, lg, beta, d_lde.data_ptr(), d_nodes.data_ptr(), flags)
+        zctx.synchronize()
+        outs.append((from_dev(d_lde), from_dev(d_nodes)))
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1], outs[1][1])
+    ref_lde, ref_nodes = oracle_commit(cols, beta)
+    assert np.array_equal(outs[0][0], ref_lde)
+    assert outs[0][1].tobytes() == ref_nodes
+
+
+def test_full_size_64x2p20_properties(zctx, torch_dev):
+    """64 columns x 2^20 -> blowup 4 -> 2^22 rows, as the benchmark runs it."""
+    import torch
+    n_cols, lg, beta = 64, 20, 2
+    n, N = 1 << lg, 1 << (lg + beta)
+    d_cols = torch.empty((n_cols, n), dtype=torch.int32, device=torch_dev)
+    zctx.fill_m31_dev(2, d_cols.data_ptr(), n_cols * n)
+    d_lde = torch.empty((n_cols, N), dtype=torch.int32, device=torch_dev)
+    d_nodes = torch.empty((2 * N - 1) * 8, dtype=torch.int32, device=torch_dev)
+    zctx.lde_commit_dev(d_cols.data_ptr(), n_cols, lg, beta, d_lde.data_ptr(), d_nodes.data_ptr())
+    zctx.synchronize()
+    cols_h = from_dev(d_cols)
+    # the device generator is the oracle's seeded stream
+    assert np.array_equal(cols_h[0, :4096], O.fill_m31(2, 4096))
+    # (1) round trip: the IFFT of each LDE column is the zero-padded interpolant
+    d_back = torch.empty_like(d_lde)
+    zctx.circle_ifft_batch_dev(d_lde.data_ptr(), d_back.data_ptr(), lg + beta, n_cols, N)
+    d_coef = torch.empty_like(d_cols)
+    zctx.circle_ifft_batch_dev(d_cols.data_ptr(), d_coef.data_ptr(), lg, n_cols, n)
+    zctx.synchronize()
+    back, coef = from_dev(d_back), from_dev(d_coef)
+    h, H = n // 2, N // 2
+    assert np.array_equal(back[:, :h], coef[:, :h])
+    assert np.array_equal(back[:, H:H + h], coef[:, h:])
+    assert not back[:, h:H].any() and not back[:, H + h:].any()
+    # (2) two whole columns against the oracle LDE
+    lde_h = from_dev(d_lde)
+    for c in (0, 63):
+        assert np.array_equal(lde_h[c], O.lde(cols_h[c], beta))
+    # (3) sampled leaves and nodes against the oracle hashes, and the tree is consistent
+    nodes = from_dev(d_nodes).tobytes()
+    rng = np.random.default_rng(5)
+    for i in rng.integers(0, N, 64).tolist():
+        assert nodes[32 * i: 32 * i + 32] == O.hash_m31s(lde_h[:, i])
+    off = 0
+    for lvl in range(lg + beta):
+        cnt = N >> lvl
+        for j in rng.integers(0, cnt // 2, 8).tolist():
+            l = nodes[32 * (off + 2 * j): 32 * (off + 2 * j + 1)]
+            r = nodes[32 * (off + 2 * j + 1): 32 * (off + 2 * j + 2)]
+            assert nodes[32 * (off + cnt + j): 32 * (off + cnt + j + 1)] == O.hash2(l, r)
+        off += cnt
+    assert off == 2 * N - 2

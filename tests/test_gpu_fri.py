@@ -1,0 +1,100 @@
+"""GPU parity: FRI commit phase (src/m31/fri.zig:157-256) against the oracle:
+every layer, every root, the remainder, the channel-drawn challenges, query
+indices, revealed values and Merkle paths; and a full-size 2^22 commit whose
+proof the oracle's verifier accepts."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import zig_stark_amd as Z
+
+pytestmark = pytest.mark.gpu
+P = O.P
+
+
+@pytest.mark.parametrize("dlog,beta,rem_log", [(3, 0, 1), (6, 2, 3), (10, 2, 5), (14, 2, 5), (16, 3, 4)])
+def test_fri_fixed_alphas_match_oracle(zctx, dlog, beta, rem_log):
+    rng = np.random.default_rng(dlog)
+    cw = rng.integers(0, P, (1 << dlog, 4), dtype=np.uint32)
+    L = dlog - rem_log
+    alphas = rng.integers(0, P, (L, 4), dtype=np.uint32)
+    roots, rem, res = zctx.fri_commit(cw, beta, rem_log, alphas=alphas)
+    oroots, olayers, orem, _ = O.fri_commit(cw, beta, rem_log, alphas=alphas)
+    assert roots == oroots
+    assert np.array_equal(rem, orem)
+    for i in range(L + 1):
+        assert np.array_equal(res.layer(i), olayers[i]), i
+
+
+@pytest.mark.parametrize("log_size,beta,rem_log,queries", [(4, 3, 3, 8), (8, 2, 5, 20), (12, 2, 5, 20)])
+def test_fri_channel_prove_matches_oracle(zctx, log_size, beta, rem_log, queries):
+    rng = np.random.default_rng(log_size)
+    dlog = log_size + beta
+    cw = rng.integers(0, P, (1 << dlog, 4), dtype=np.uint32)
+    ch = Z.Channel(b"zig-stark:fri:test")
+    roots, rem, res = zctx.fri_commit(cw, beta, rem_log, channel=ch)
+    idx, vals, paths = res.query(ch, queries)
+    proof = O.fri_prove(cw, beta, rem_log, queries, O.Channel(b"zig-stark:fri:test"))
+    assert b"".join(roots) == proof["roots"]
+    assert np.array_equal(rem.reshape(-1), proof["remainder"])
+    assert idx == proof["qidx"]
+    assert np.array_equal(vals, proof["qvals"])
+    assert paths == proof["qpaths"]
+
+
+def low_degree_codeword_on_gpu(zctx, rng, log_size, beta):
+    """A QM31 codeword of degree < 2^log_size on D_0: evaluate random
+    coefficients by Horner over the FRI domain (oracle domain points)."""
+    coeffs = rng.integers(0, P, (1 << log_size, 4), dtype=np.uint32)
+    dlog = log_size + beta
+    return coeffs, np.array([O.poly_eval(coeffs, O.fri_domain_point(dlog, 0, j)) for j in range(1 << dlog)])
+
+
+def test_fri_low_degree_proof_verifies(zctx):
+    rng = np.random.default_rng(3)
+    _, cw = low_degree_codeword_on_gpu(zctx, rng, 8, 2)
+    ch = Z.Channel(b"zig-stark:fri:test")
+    roots, rem, res = zctx.fri_commit(cw, 2, 3, channel=ch)
+    idx, vals, paths = res.query(ch, 16)
+    proof = dict(dlog=10, L=7, roots=b"".join(roots), remainder=rem.reshape(-1), qidx=idx, qvals=vals,
+                 qpaths=paths, num_queries=16)
+    assert O.fri_verify(proof, 2, 3, O.Channel(b"zig-stark:fri:test"))
+    bad = dict(proof, qvals=vals.copy())
+    bad["qvals"][5] = (int(bad["qvals"][5]) + 1) % P
+    assert not O.fri_verify(bad, 2, 3, O.Channel(b"zig-stark:fri:test"))
+
+
+def test_fri_full_size_2p22(zctx):
+    """One QM31 column of 2^22, folded to 2^5 (17 committed layers), seeded alphas."""
+    rng = np.random.default_rng(3)
+    dlog, beta, rem_log = 22, 2, 5
+    cw = rng.integers(0, P, (1 << dlog, 4), dtype=np.uint32)
+    L = dlog - rem_log
+    alphas = rng.integers(0, P, (L, 4), dtype=np.uint32)
+    roots, rem, res = zctx.fri_commit(cw, beta, rem_log, alphas=alphas)
+    assert len(roots) == 17
+    # each layer is the fold of the previous one (oracle fold on sampled slices)
+    prev = res.layer(0)
+    assert np.array_equal(prev, cw)
+    for i in range(L):
+        nxt = res.layer(i + 1)
+        n = prev.shape[0]
+        for j in rng.integers(0, n // 2, 4).tolist():
+            # fold a 2-element layer made of the pair (j, j + n/2): fold index 0 uses D_i[j]
+            y0, y1 = prev[j], prev[j + n // 2]
+            x = O.fri_domain_point(22, i, j)
+            xi = O.qm31_inv(x)
+            half = (P + 1) // 2
+            hs = (O.qm31_mul((y0.astype(np.uint64) + y1) % P, [half, 0, 0, 0]))
+            hd = O.qm31_mul(O.qm31_mul(((y0.astype(np.int64) - y1) % P).astype(np.uint32), xi), [half, 0, 0, 0])
+            v = (hs.astype(np.uint64) + O.qm31_mul(alphas[i], hd)) % P
+            assert np.array_equal(nxt[j], v.astype(np.uint32)), (i, j)
+        prev = nxt
+    # the last layer's remainder interpolant reproduces its first points
+    last = res.layer(L)
+    for j in range(1 << (rem_log - beta)):
+        assert np.array_equal(O.poly_eval(rem, O.fri_domain_point(22, L, j)), last[j])
+    # the last committed layers' roots equal the oracle's tree over the same values
+    for i in range(L - 4, L):
+        leaves = b"".join(O.hash_qm31(v) for v in res.layer(i))
+        assert roots[i] == O.tree_root(O.merkle_build(leaves))

@@ -1,0 +1,348 @@
+"""Python binding of libzs_hip.so (include/zs_hip.h), the MI355X M31
+trace-commitment path of zig-stark.
+
+The names mirror the reference interface: circle_fft / circle_ifft follow
+src/cuda/circlefft_gpu.zig:46,111 (and src/m31/ntt/circle.zig:195,272),
+lde_commit follows the per-column commitment of src/m31/stark.zig:396-414
+with row leaves (src/m31/hash.zig:15-25), fri_commit follows
+src/m31/fri.zig:157-233 and fri_query fri.zig:236-256, Channel follows
+src/core/channel/channel.zig.
+
+There is no CPU fallback: if the library or a HIP device is missing, the
+calls raise ZsError.  Device buffers are torch tensors on a ROCm device
+(torch is only used for device memory and streams).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libzs_hip.so")
+HEADER = os.path.join(os.path.dirname(PKG), "include", "zs_hip.h")
+P = 2147483647
+
+ZS_OK, ZS_ERR_GENERIC, ZS_ERR_INPUT, ZS_ERR_OOM, ZS_ERR_PROTOCOL, ZS_ERR_DEVICE = 0, -1, -2, -3, -4, -5
+ZS_FLAG_PLAIN = 1
+
+_lib = None
+
+
+class ZsError(RuntimeError):
+    def __init__(self, code, msg=""):
+        super().__init__(f"zs_hip error {code}: {msg}")
+        self.code = code
+
+
+class Coset(ctypes.Structure):
+    """CircleCoset (src/m31/circle/coset.zig:10-14)."""
+    _fields_ = [("offset_x", ctypes.c_uint32), ("offset_y", ctypes.c_uint32), ("step_x", ctypes.c_uint32),
+                ("step_y", ctypes.c_uint32), ("log_size", ctypes.c_uint32)]
+
+    def as_list(self):
+        return [self.offset_x, self.offset_y, self.step_x, self.step_y, self.log_size]
+
+
+u32p = ctypes.POINTER(ctypes.c_uint32)
+vp = ctypes.c_void_p
+sz = ctypes.c_size_t
+i32 = ctypes.c_int32
+u32 = ctypes.c_uint32
+
+
+def lib():
+    """Load libzs_hip.so; raise if it is not built (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ZsError(ZS_ERR_GENERIC, f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def _declare(L):
+    L.zs_hip_version.restype = ctypes.c_char_p
+    L.zs_hip_init.argtypes = [i32, u32, ctypes.POINTER(vp)]
+    L.zs_hip_init.restype = i32
+    L.zs_hip_destroy.argtypes = [vp]
+    L.zs_hip_last_error.argtypes = [vp]
+    L.zs_hip_last_error.restype = ctypes.c_char_p
+    L.zs_hip_set_stream.argtypes = [vp, vp]
+    L.zs_hip_synchronize.argtypes = [vp]
+    L.zs_circle_coset_canonic_half.argtypes = [u32, ctypes.POINTER(Coset)]
+    L.zs_m31_circle_fft.argtypes = [vp, u32p, sz, ctypes.POINTER(Coset), u32p]
+    L.zs_m31_circle_ifft.argtypes = [vp, u32p, sz, ctypes.POINTER(Coset), u32p]
+    L.zs_m31_circle_fft_batch_dev.argtypes = [vp, vp, vp, u32, u32, sz]
+    L.zs_m31_circle_ifft_batch_dev.argtypes = [vp, vp, vp, u32, u32, sz]
+    L.zs_m31_lde_commit_dev.argtypes = [vp, vp, u32, u32, u32, vp, vp, u32]
+    L.zs_m31_lde_commit.argtypes = [vp, u32p, u32, u32, u32, ctypes.POINTER(vp), ctypes.c_char_p, u32p]
+    L.zs_m31_row_commit_dev.argtypes = [vp, vp, u32, u32, sz, vp, u32]
+    L.zs_m31_lde_dev.argtypes = [vp, vp, u32, u32, u32, vp]
+    L.zs_m31_row_leaves_dev.argtypes = [vp, vp, u32, u32, sz, vp]
+    L.zs_tree_log_leaves.argtypes = [vp]
+    L.zs_tree_log_leaves.restype = u32
+    L.zs_tree_root.argtypes = [vp, ctypes.c_char_p]
+    L.zs_merkle_open.argtypes = [vp, sz, ctypes.c_char_p]
+    L.zs_tree_read_level.argtypes = [vp, u32, ctypes.c_char_p]
+    L.zs_tree_free.argtypes = [vp]
+    L.zs_channel_new.argtypes = [ctypes.c_char_p, sz]
+    L.zs_channel_new.restype = vp
+    L.zs_channel_clone.argtypes = [vp]
+    L.zs_channel_clone.restype = vp
+    L.zs_channel_free.argtypes = [vp]
+    L.zs_channel_absorb.argtypes = [vp, ctypes.c_char_p, sz]
+    L.zs_channel_sample_bytes.argtypes = [vp, ctypes.c_char_p, sz]
+    L.zs_channel_sample_qm31.argtypes = [vp, u32p]
+    L.zs_channel_sample_index.argtypes = [vp, sz]
+    L.zs_channel_sample_index.restype = sz
+    L.zs_fri_commit_dev.argtypes = [vp, vp, u32, u32, u32, u32p, vp, u32, ctypes.c_char_p, u32p, ctypes.POINTER(vp)]
+    L.zs_fri_commit.argtypes = [vp, u32p, u32, u32, u32, u32p, vp, ctypes.c_char_p, u32p, ctypes.POINTER(vp)]
+    L.zs_fri_query.argtypes = [vp, vp, sz, ctypes.POINTER(sz), u32p, ctypes.c_char_p]
+    L.zs_fri_read_layer.argtypes = [vp, u32, u32p]
+    L.zs_fri_result_free.argtypes = [vp]
+    L.zs_fill_m31_dev.argtypes = [vp, ctypes.c_uint64, vp, sz]
+    for name in ["zs_hip_set_stream", "zs_hip_synchronize", "zs_circle_coset_canonic_half", "zs_m31_circle_fft",
+                 "zs_m31_circle_ifft", "zs_m31_circle_fft_batch_dev", "zs_m31_circle_ifft_batch_dev",
+                 "zs_m31_lde_commit_dev", "zs_m31_lde_commit", "zs_m31_row_commit_dev", "zs_m31_lde_dev",
+                 "zs_m31_row_leaves_dev", "zs_tree_root",
+                 "zs_merkle_open", "zs_tree_read_level", "zs_fri_commit_dev", "zs_fri_commit", "zs_fri_query",
+                 "zs_fri_read_layer", "zs_fill_m31_dev"]:
+        getattr(L, name).restype = i32
+
+
+def header_functions():
+    """Names of the functions include/zs_hip.h declares."""
+    import re
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(zs_[a-z0-9_]+)\s*\(", text)))
+
+
+def _p(a):
+    return a.ctypes.data_as(u32p)
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def canonic_half(log_size):
+    """CircleCoset.canonicHalf (coset.zig:55-65)."""
+    c = Coset()
+    rc = lib().zs_circle_coset_canonic_half(log_size, ctypes.byref(c))
+    if rc:
+        raise ZsError(rc, "canonic_half")
+    return c
+
+
+def version():
+    return lib().zs_hip_version().decode()
+
+
+class Channel:
+    """Fiat-Shamir transcript (src/core/channel/channel.zig), host side."""
+
+    def __init__(self, sep=b"", _handle=None):
+        self.h = _handle if _handle is not None else lib().zs_channel_new(bytes(sep), len(sep))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.zs_channel_free(self.h)
+            self.h = None
+
+    def clone(self):
+        return Channel(_handle=lib().zs_channel_clone(self.h))
+
+    def absorb(self, data):
+        data = bytes(data)
+        lib().zs_channel_absorb(self.h, data, len(data))
+
+    def sample_bytes(self, n):
+        out = ctypes.create_string_buffer(n)
+        lib().zs_channel_sample_bytes(self.h, out, n)
+        return out.raw
+
+    def sample_qm31(self):
+        o = np.zeros(4, np.uint32)
+        lib().zs_channel_sample_qm31(self.h, _p(o))
+        return o
+
+    def sample_index(self, n):
+        return lib().zs_channel_sample_index(self.h, n)
+
+
+class Tree:
+    """A device-resident MerkleTree (src/core/merkle/merkle.zig)."""
+
+    def __init__(self, handle):
+        self.h = handle
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.zs_tree_free(self.h)
+            self.h = None
+
+    @property
+    def log_leaves(self):
+        return lib().zs_tree_log_leaves(self.h)
+
+    def root(self):
+        out = ctypes.create_string_buffer(32)
+        _check(lib().zs_tree_root(self.h, out))
+        return out.raw
+
+    def open(self, index):
+        n = self.log_leaves
+        out = ctypes.create_string_buffer(max(1, 32 * n))
+        _check(lib().zs_merkle_open(self.h, index, out))
+        return out.raw[: 32 * n]
+
+    def level(self, level):
+        cnt = (1 << self.log_leaves) >> level
+        out = ctypes.create_string_buffer(32 * cnt)
+        _check(lib().zs_tree_read_level(self.h, level, out))
+        return out.raw
+
+
+class FriResult:
+    def __init__(self, handle, dlog, L, ctx):
+        self.h, self.dlog, self.L, self.ctx = handle, dlog, L, ctx
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.zs_fri_result_free(self.h)
+            self.h = None
+
+    def layer(self, i):
+        o = np.zeros(4 << (self.dlog - i), np.uint32)
+        _check(lib().zs_fri_read_layer(self.h, i, _p(o)), self.ctx)
+        return o.reshape(-1, 4)
+
+    def query(self, channel, num_queries):
+        """fri.zig:236-256: indices, revealed pair values and both paths per layer."""
+        idx = (sz * max(1, num_queries))()
+        vals = np.zeros(max(1, num_queries * self.L * 8), np.uint32)
+        per_q = sum(2 * (self.dlog - i) for i in range(self.L)) * 32
+        paths = ctypes.create_string_buffer(max(1, per_q * num_queries))
+        _check(lib().zs_fri_query(self.h, channel.h, num_queries, idx, _p(vals), paths), self.ctx)
+        return list(idx)[:num_queries], vals[: num_queries * self.L * 8], paths.raw[: per_q * num_queries]
+
+
+def _check(rc, ctx=None):
+    if rc != ZS_OK:
+        msg = ""
+        if ctx is not None:
+            msg = lib().zs_hip_last_error(ctx.h).decode(errors="replace")
+        raise ZsError(rc, msg)
+
+
+class Context:
+    """A zs_hip_ctx on one device (circlefft_gpu.zig:20-39's lazily-initialised context)."""
+
+    def __init__(self, device=0):
+        h = vp()
+        rc = lib().zs_hip_init(device, 0, ctypes.byref(h))
+        if rc != ZS_OK:
+            raise ZsError(rc, "zs_hip_init: no usable HIP device")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().zs_hip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        if _lib is not None:
+            self.close()
+
+    def check(self, rc):
+        _check(rc, self)
+
+    def set_stream(self, stream_ptr):
+        self.check(lib().zs_hip_set_stream(self.h, stream_ptr))
+
+    def synchronize(self):
+        self.check(lib().zs_hip_synchronize(self.h))
+
+    # ---- circle FFT (circlefft_gpu.zig:46, :111) ----
+    def circle_fft(self, coeffs, half_coset=None):
+        coeffs = _u32(coeffs)
+        n = coeffs.size
+        hc = half_coset if half_coset is not None else canonic_half(n.bit_length() - 1)
+        out = np.zeros(n, np.uint32)
+        self.check(lib().zs_m31_circle_fft(self.h, _p(coeffs), n, ctypes.byref(hc), _p(out)))
+        return out
+
+    def circle_ifft(self, evals, half_coset=None):
+        evals = _u32(evals)
+        n = evals.size
+        hc = half_coset if half_coset is not None else canonic_half(n.bit_length() - 1)
+        out = np.zeros(n, np.uint32)
+        self.check(lib().zs_m31_circle_ifft(self.h, _p(evals), n, ctypes.byref(hc), _p(out)))
+        return out
+
+    def circle_fft_batch_dev(self, d_in, d_out, log_n, n_cols, col_stride):
+        self.check(lib().zs_m31_circle_fft_batch_dev(self.h, d_in, d_out, log_n, n_cols, col_stride))
+
+    def circle_ifft_batch_dev(self, d_in, d_out, log_n, n_cols, col_stride):
+        self.check(lib().zs_m31_circle_ifft_batch_dev(self.h, d_in, d_out, log_n, n_cols, col_stride))
+
+    # ---- LDE + row commitment ----
+    def lde_commit(self, cols, log_blowup, want_lde=True):
+        """cols: (n_cols, 2^log_n) array. Returns (Tree, root, lde or None)."""
+        cols = _u32(cols)
+        n_cols, n = cols.shape
+        lg = n.bit_length() - 1
+        lde = np.zeros((n_cols, n << log_blowup), np.uint32) if want_lde else None
+        t = vp()
+        root = ctypes.create_string_buffer(32)
+        self.check(lib().zs_m31_lde_commit(self.h, _p(cols), n_cols, lg, log_blowup, ctypes.byref(t), root,
+                                           _p(lde) if lde is not None else None))
+        return Tree(t), root.raw, lde
+
+    def lde_commit_dev(self, d_cols, n_cols, log_n, log_blowup, d_lde, d_nodes, flags=0):
+        self.check(lib().zs_m31_lde_commit_dev(self.h, d_cols, n_cols, log_n, log_blowup, d_lde, d_nodes, flags))
+
+    def lde_dev(self, d_cols, n_cols, log_n, log_blowup, d_lde):
+        self.check(lib().zs_m31_lde_dev(self.h, d_cols, n_cols, log_n, log_blowup, d_lde))
+
+    def row_leaves_dev(self, d_cols, n_cols, log_rows, col_stride, d_leaves):
+        self.check(lib().zs_m31_row_leaves_dev(self.h, d_cols, n_cols, log_rows, col_stride, d_leaves))
+
+    def row_commit_dev(self, d_cols, n_cols, log_rows, col_stride, d_nodes, flags=0):
+        self.check(lib().zs_m31_row_commit_dev(self.h, d_cols, n_cols, log_rows, col_stride, d_nodes, flags))
+
+    def fill_m31_dev(self, seed, d_out, n):
+        self.check(lib().zs_fill_m31_dev(self.h, seed, d_out, n))
+
+    # ---- FRI (fri.zig:157-233) ----
+    def fri_commit(self, codeword, log_blowup, remainder_log, alphas=None, channel=None, keep=True):
+        cw = _u32(codeword).reshape(-1)
+        n0 = cw.size // 4
+        dlog = n0.bit_length() - 1
+        L = dlog - remainder_log
+        roots = ctypes.create_string_buffer(32 * L)
+        rem = np.zeros(4 << (remainder_log - log_blowup), np.uint32)
+        al = _u32(alphas).reshape(-1) if alphas is not None else None
+        res = vp()
+        self.check(lib().zs_fri_commit(self.h, _p(cw), dlog, log_blowup, remainder_log,
+                                       _p(al) if al is not None else None,
+                                       channel.h if channel is not None else None, roots, _p(rem),
+                                       ctypes.byref(res) if keep else None))
+        fr = FriResult(res, dlog, L, self) if keep else None
+        return [roots.raw[32 * i:32 * (i + 1)] for i in range(L)], rem.reshape(-1, 4), fr
+
+    def fri_commit_dev(self, d_codeword, dlog, log_blowup, remainder_log, alphas=None, channel=None, keep=False):
+        L = dlog - remainder_log
+        roots = ctypes.create_string_buffer(32 * L)
+        rem = np.zeros(4 << (remainder_log - log_blowup), np.uint32)
+        al = _u32(alphas).reshape(-1) if alphas is not None else None
+        res = vp()
+        self.check(lib().zs_fri_commit_dev(self.h, d_codeword, dlog, log_blowup, remainder_log,
+                                           _p(al) if al is not None else None,
+                                           channel.h if channel is not None else None, 0, roots, _p(rem),
+                                           ctypes.byref(res) if keep else None))
+        fr = FriResult(res, dlog, L, self) if keep else None
+        return [roots.raw[32 * i:32 * (i + 1)] for i in range(L)], rem.reshape(-1, 4), fr

@@ -1,0 +1,474 @@
+// capi.cpp -- the extern "C" entry points of include/zs_hip.h.
+//
+// Mirrors the reference's device boundary (src/cuda/circlefft_gpu.zig,
+// src/cuda/cuda.zig): a lazily initialised per-device context, host-buffer
+// transforms with the CPU transform's signature, and error codes in the
+// zig-capi.h convention.  Unlike the reference there is no CPU fallback: a
+// missing device or a HIP failure is returned as ZS_ERR_DEVICE.
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "blake3.h"
+#include "internal.h"
+
+using namespace zs;
+
+struct zs_hip_ctx {
+    Ctx c;
+};
+
+struct zs_tree {
+    Ctx *ctx = nullptr;
+    DevBuf nodes;
+    uint32_t log_leaves = 0;
+    uint8_t root[32];
+};
+
+struct zs_fri_result {
+    Ctx *ctx = nullptr;
+    uint32_t dlog = 0, L = 0, log_blowup = 0, remainder_log = 0;
+    DevBuf layers;  // layer i at layer_off[i] words, 2^(dlog-i) QM31 values
+    DevBuf trees;   // tree i at tree_off[i] words, every level
+    std::vector<size_t> layer_off, tree_off;
+};
+
+namespace zs {
+int32_t Ctx::ensure_scratch(int slot, size_t bytes) {
+    DevBuf &b = scratch[slot];
+    if (b.bytes >= bytes) return ZS_OK;
+    if (b.ptr) {
+        ZS_HIP(this, hipStreamSynchronize(stream));
+        ZS_HIP(this, hipFree(b.ptr));
+        b.ptr = nullptr;
+        b.bytes = 0;
+    }
+    hipError_t e = hipMalloc(&b.ptr, bytes);
+    if (e != hipSuccess) {
+        b.ptr = nullptr;
+        set_error(std::string("hipMalloc scratch: ") + hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? ZS_ERR_OOM : ZS_ERR_DEVICE;
+    }
+    b.bytes = bytes;
+    return ZS_OK;
+}
+}  // namespace zs
+
+static int32_t dev_alloc(Ctx *ctx, DevBuf &b, size_t bytes) {
+    hipError_t e = hipMalloc(&b.ptr, bytes ? bytes : 4);
+    if (e != hipSuccess) {
+        b.ptr = nullptr;
+        ctx->set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? ZS_ERR_OOM : ZS_ERR_DEVICE;
+    }
+    b.bytes = bytes;
+    return ZS_OK;
+}
+
+static bool is_pow2(size_t n) { return n && !(n & (n - 1)); }
+static uint32_t log2u(size_t n) {
+    uint32_t l = 0;
+    while ((size_t(1) << l) < n) l++;
+    return l;
+}
+
+extern "C" {
+
+const char *zs_hip_version(void) { return "zig-stark-hip 0.1.0 (gfx950; M31 circle LDE, BLAKE3 Merkle, QM31 FRI)"; }
+
+int32_t zs_hip_init(int32_t device, uint32_t flags, zs_hip_ctx **out_ctx) {
+    (void)flags;
+    if (!out_ctx) return ZS_ERR_INPUT;
+    *out_ctx = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0 || device < 0 || device >= count) return ZS_ERR_DEVICE;
+    if (hipSetDevice(device) != hipSuccess) return ZS_ERR_DEVICE;
+    auto *c = new (std::nothrow) zs_hip_ctx();
+    if (!c) return ZS_ERR_OOM;
+    c->c.device = device;
+    *out_ctx = c;
+    return ZS_OK;
+}
+
+void zs_hip_destroy(zs_hip_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->c.device);
+    (void)hipStreamSynchronize(ctx->c.stream);
+    delete ctx;
+}
+
+const char *zs_hip_last_error(const zs_hip_ctx *ctx) { return ctx ? ctx->c.last_error.c_str() : "no context"; }
+
+int32_t zs_hip_set_stream(zs_hip_ctx *ctx, void *hip_stream) {
+    if (!ctx) return ZS_ERR_INPUT;
+    ctx->c.stream = static_cast<hipStream_t>(hip_stream);
+    return ZS_OK;
+}
+
+int32_t zs_hip_synchronize(zs_hip_ctx *ctx) {
+    if (!ctx) return ZS_ERR_INPUT;
+    ZS_HIP(&ctx->c, hipStreamSynchronize(ctx->c.stream));
+    return ZS_OK;
+}
+
+void zs_free(zs_host_allocator host, uint8_t *ptr, size_t len) {
+    if (ptr && host.free) host.free(host.ctx, ptr, len);
+}
+
+// coset.zig:55-65
+int32_t zs_circle_coset_canonic_half(uint32_t log_size, zs_circle_coset *out) {
+    if (!out || log_size < 1 || log_size > 28) return ZS_ERR_INPUT;
+    cpoint g = cp_neg(cpoint{1268011823u, 2u});  // point.zig:49-51, negated
+    cpoint off = cp_mul(g, uint64_t(1) << (30 - log_size));
+    cpoint st = cp_mul(g, uint64_t(1) << (32 - log_size));
+    *out = {off.x, off.y, st.x, st.y, log_size - 1};
+    return ZS_OK;
+}
+
+// ---- circle FFT, host buffers ------------------------------------------------
+static int32_t host_transform(zs_hip_ctx *h, bool inverse, const uint32_t *in, size_t n, const zs_circle_coset *hc,
+                              uint32_t *out) {
+    if (!h || !in || !out || !hc) return ZS_ERR_INPUT;
+    if (n < 2 || !is_pow2(n) || (size_t(1) << hc->log_size) != n / 2 || hc->log_size > 28) return ZS_ERR_INPUT;
+    Ctx *ctx = &h->c;
+    ZS_HIP(ctx, hipSetDevice(ctx->device));
+    DevBuf a, b;
+    ZS_TRY(dev_alloc(ctx, a, n * 4));
+    ZS_TRY(dev_alloc(ctx, b, n * 4));
+    ZS_HIP(ctx, hipMemcpy(a.ptr, in, n * 4, hipMemcpyHostToDevice));
+    if (inverse) ZS_TRY(launch_ifft(ctx, *hc, a.as<uint32_t>(), n, b.as<uint32_t>(), n, 1));
+    else ZS_TRY(launch_fft(ctx, *hc, a.as<uint32_t>(), n, b.as<uint32_t>(), n, 1));
+    ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ZS_HIP(ctx, hipMemcpy(out, b.ptr, n * 4, hipMemcpyDeviceToHost));
+    return ZS_OK;
+}
+
+int32_t zs_m31_circle_fft(zs_hip_ctx *ctx, const uint32_t *coeffs, size_t n, const zs_circle_coset *half_coset,
+                          uint32_t *evals) {
+    return host_transform(ctx, false, coeffs, n, half_coset, evals);
+}
+
+int32_t zs_m31_circle_ifft(zs_hip_ctx *ctx, const uint32_t *evals, size_t n, const zs_circle_coset *half_coset,
+                           uint32_t *coeffs) {
+    return host_transform(ctx, true, evals, n, half_coset, coeffs);
+}
+
+int32_t zs_m31_circle_fft_batch_dev(zs_hip_ctx *h, const uint32_t *d_coeffs, uint32_t *d_evals, uint32_t log_n,
+                                    uint32_t n_cols, size_t col_stride) {
+    if (!h || log_n < 1 || log_n > 28 || col_stride < (size_t(1) << log_n)) return ZS_ERR_INPUT;
+    zs_circle_coset hc;
+    zs_circle_coset_canonic_half(log_n, &hc);
+    return launch_fft(&h->c, hc, d_coeffs, col_stride, d_evals, col_stride, n_cols);
+}
+
+int32_t zs_m31_circle_ifft_batch_dev(zs_hip_ctx *h, const uint32_t *d_evals, uint32_t *d_coeffs, uint32_t log_n,
+                                     uint32_t n_cols, size_t col_stride) {
+    if (!h || log_n < 1 || log_n > 28 || col_stride < (size_t(1) << log_n)) return ZS_ERR_INPUT;
+    zs_circle_coset hc;
+    zs_circle_coset_canonic_half(log_n, &hc);
+    return launch_ifft(&h->c, hc, d_evals, col_stride, d_coeffs, col_stride, n_cols);
+}
+
+// ---- LDE + row commitment ------------------------------------------------------
+int32_t zs_m31_lde_commit_dev(zs_hip_ctx *h, const uint32_t *d_cols, uint32_t n_cols, uint32_t log_n,
+                              uint32_t log_blowup, uint32_t *d_lde, uint8_t *d_nodes, uint32_t flags) {
+    if (!h || !d_cols || !d_lde || !d_nodes || n_cols == 0 || log_n < 1 || log_n + log_blowup > 28)
+        return ZS_ERR_INPUT;
+    Ctx *ctx = &h->c;
+    uint32_t LT = log_n + log_blowup;
+    size_t n = size_t(1) << log_n, N = size_t(1) << LT;
+    if (flags & ZS_FLAG_PLAIN) {
+        if (log_n < 3) return ZS_ERR_INPUT;
+        ZS_TRY(ctx->ensure_scratch(3, 2 * N * 4));
+        ZS_TRY(launch_lde_plain(ctx, d_cols, n, n_cols, log_n, log_blowup, d_lde, N, ctx->scratch[3].as<uint32_t>()));
+        return launch_row_commit_plain(ctx, d_lde, N, n_cols, LT, reinterpret_cast<uint32_t *>(d_nodes));
+    }
+    ZS_TRY(zs_m31_lde_dev(h, d_cols, n_cols, log_n, log_blowup, d_lde));
+    return launch_row_commit(ctx, d_lde, N, n_cols, LT, reinterpret_cast<uint32_t *>(d_nodes));
+}
+
+int32_t zs_m31_lde_dev(zs_hip_ctx *h, const uint32_t *d_cols, uint32_t n_cols, uint32_t log_n, uint32_t log_blowup,
+                       uint32_t *d_lde) {
+    if (!h || !d_cols || !d_lde || n_cols == 0 || log_n < 1 || log_n + log_blowup > 28) return ZS_ERR_INPUT;
+    Ctx *ctx = &h->c;
+    size_t n = size_t(1) << log_n, N = size_t(1) << (log_n + log_blowup);
+    size_t coef_words;
+    size_t big_words = lde_scratch_words(n_cols, log_n, log_blowup, &coef_words);
+    ZS_TRY(ctx->ensure_scratch(1, coef_words * 4));
+    if (big_words) ZS_TRY(ctx->ensure_scratch(2, big_words * 4));
+    return launch_lde(ctx, d_cols, n, n_cols, log_n, log_blowup, d_lde, N, ctx->scratch[1].as<uint32_t>(),
+                      big_words ? ctx->scratch[2].as<uint32_t>() : nullptr);
+}
+
+int32_t zs_m31_row_leaves_dev(zs_hip_ctx *h, const uint32_t *d_cols, uint32_t n_cols, uint32_t log_rows,
+                              size_t col_stride, uint8_t *d_leaves) {
+    if (!h || !d_cols || !d_leaves || n_cols == 0 || log_rows > 28 || col_stride < (size_t(1) << log_rows))
+        return ZS_ERR_INPUT;
+    return launch_row_leaves(&h->c, d_cols, col_stride, n_cols, uint64_t(1) << log_rows,
+                             reinterpret_cast<uint32_t *>(d_leaves));
+}
+
+int32_t zs_m31_row_commit_dev(zs_hip_ctx *h, const uint32_t *d_cols, uint32_t n_cols, uint32_t log_rows,
+                              size_t col_stride, uint8_t *d_nodes, uint32_t flags) {
+    if (!h || !d_cols || !d_nodes || n_cols == 0 || log_rows > 28 || col_stride < (size_t(1) << log_rows))
+        return ZS_ERR_INPUT;
+    if (flags & ZS_FLAG_PLAIN)
+        return launch_row_commit_plain(&h->c, d_cols, col_stride, n_cols, log_rows,
+                                       reinterpret_cast<uint32_t *>(d_nodes));
+    return launch_row_commit(&h->c, d_cols, col_stride, n_cols, log_rows, reinterpret_cast<uint32_t *>(d_nodes));
+}
+
+int32_t zs_m31_lde_commit(zs_hip_ctx *h, const uint32_t *cols, uint32_t n_cols, uint32_t log_n, uint32_t log_blowup,
+                          zs_tree **out_tree, uint8_t root[32], uint32_t *lde_out) {
+    if (!h || !cols || n_cols == 0 || log_n < 1 || log_n + log_blowup > 28) return ZS_ERR_INPUT;
+    Ctx *ctx = &h->c;
+    ZS_HIP(ctx, hipSetDevice(ctx->device));
+    uint32_t LT = log_n + log_blowup;
+    size_t n = size_t(1) << log_n, N = size_t(1) << LT;
+    DevBuf d_cols, d_lde;
+    auto *t = new (std::nothrow) zs_tree();
+    if (!t) return ZS_ERR_OOM;
+    t->ctx = ctx;
+    t->log_leaves = LT;
+    int32_t rc = dev_alloc(ctx, d_cols, n * n_cols * 4);
+    if (!rc) rc = dev_alloc(ctx, d_lde, N * n_cols * 4);
+    if (!rc) rc = dev_alloc(ctx, t->nodes, tree_words(LT) * 4);
+    if (!rc && hipMemcpy(d_cols.ptr, cols, n * n_cols * 4, hipMemcpyHostToDevice) != hipSuccess) rc = ZS_ERR_DEVICE;
+    if (!rc)
+        rc = zs_m31_lde_commit_dev(h, d_cols.as<uint32_t>(), n_cols, log_n, log_blowup, d_lde.as<uint32_t>(),
+                                   t->nodes.as<uint8_t>(), 0);
+    if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = ZS_ERR_DEVICE;
+    if (!rc && hipMemcpy(t->root, t->nodes.as<uint8_t>() + (tree_words(LT) - 8) * 4, 32, hipMemcpyDeviceToHost) !=
+                   hipSuccess)
+        rc = ZS_ERR_DEVICE;
+    if (!rc && lde_out && hipMemcpy(lde_out, d_lde.ptr, N * n_cols * 4, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = ZS_ERR_DEVICE;
+    if (rc) {
+        delete t;
+        return rc;
+    }
+    if (root) std::memcpy(root, t->root, 32);
+    if (out_tree) *out_tree = t;
+    else delete t;
+    return ZS_OK;
+}
+
+// ---- trees -----------------------------------------------------------------------
+uint32_t zs_tree_log_leaves(const zs_tree *t) { return t ? t->log_leaves : 0; }
+
+int32_t zs_tree_root(const zs_tree *t, uint8_t out[32]) {
+    if (!t || !out) return ZS_ERR_INPUT;
+    std::memcpy(out, t->root, 32);
+    return ZS_OK;
+}
+
+int32_t zs_merkle_open(const zs_tree *t, size_t index, uint8_t *path_out) {
+    if (!t || !path_out || index >= (size_t(1) << t->log_leaves)) return ZS_ERR_INPUT;
+    if (t->log_leaves == 0) return ZS_OK;
+    Ctx *ctx = t->ctx;
+    DevBuf out;
+    ZS_TRY(dev_alloc(ctx, out, size_t(t->log_leaves) * 32));
+    std::vector<PathReq> reqs{{0, 0, t->log_leaves, (uint32_t)index}};
+    ZS_TRY(launch_gather_paths(ctx, t->nodes.as<uint32_t>(), reqs, out.as<uint32_t>()));
+    ZS_HIP(ctx, hipMemcpy(path_out, out.ptr, size_t(t->log_leaves) * 32, hipMemcpyDeviceToHost));
+    return ZS_OK;
+}
+
+int32_t zs_tree_read_level(const zs_tree *t, uint32_t level, uint8_t *out) {
+    if (!t || !out || level > t->log_leaves) return ZS_ERR_INPUT;
+    Ctx *ctx = t->ctx;
+    size_t cnt = (size_t(1) << t->log_leaves) >> level;
+    ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ZS_HIP(ctx, hipMemcpy(out, t->nodes.as<uint32_t>() + level_offset_words(t->log_leaves, level), cnt * 32,
+                          hipMemcpyDeviceToHost));
+    return ZS_OK;
+}
+
+void zs_tree_free(zs_tree *t) { delete t; }
+
+// ---- FRI ---------------------------------------------------------------------------
+// univariate.zig:79-139 (O(n^2) Lagrange; the interpolant is unique)
+static void interpolate_host(const std::vector<qm31> &xs, const std::vector<qm31> &ys, std::vector<qm31> &coeffs) {
+    size_t n = xs.size();
+    const qm31 zero{{0, 0}, {0, 0}}, one{{1, 0}, {0, 0}};
+    std::vector<qm31> p(n + 1, zero), acc(n, zero), tmp(n, zero);
+    p[0] = one;
+    size_t plen = 1;
+    for (size_t i = 0; i < n; i++) {
+        for (size_t j = plen; j > 0; j--) p[j] = qm_sub(p[j - 1], qm_mul(p[j], xs[i]));
+        p[0] = qm_mul(qm_sub(zero, p[0]), xs[i]);
+        plen++;
+    }
+    for (size_t i = 0; i < n; i++) {
+        qm31 w = one;
+        for (size_t j = 0; j < n; j++)
+            if (i != j) w = qm_mul(w, qm_sub(xs[i], xs[j]));
+        qm31 sc = qm_mul(ys[i], qm_inv(w));
+        tmp[n - 1] = p[n];
+        for (size_t j = n - 1; j > 0; j--) tmp[j - 1] = qm_add(p[j], qm_mul(xs[i], tmp[j]));
+        for (size_t k = 0; k < n; k++) acc[k] = qm_add(acc[k], qm_mul(tmp[k], sc));
+    }
+    coeffs = acc;
+}
+
+int32_t zs_fri_commit_dev(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t dlog, uint32_t log_blowup,
+                          uint32_t remainder_log, const uint32_t *alphas, zs_channel *chan, uint32_t flags,
+                          uint8_t *roots_out, uint32_t *remainder_out, zs_fri_result **out_result) {
+    (void)flags;
+    if (!h || !d_codeword || !roots_out || !remainder_out) return ZS_ERR_INPUT;
+    if (dlog > 28 || remainder_log > dlog || remainder_log < log_blowup || dlog - remainder_log < 1)
+        return ZS_ERR_INPUT;
+    if (!alphas && !chan) return ZS_ERR_INPUT;
+    Ctx *ctx = &h->c;
+    ZS_HIP(ctx, hipSetDevice(ctx->device));
+    auto *r = new (std::nothrow) zs_fri_result();
+    if (!r) return ZS_ERR_OOM;
+    std::unique_ptr<zs_fri_result> guard(r);
+    r->ctx = ctx;
+    r->dlog = dlog;
+    r->L = dlog - remainder_log;
+    r->log_blowup = log_blowup;
+    r->remainder_log = remainder_log;
+    const uint32_t L = r->L;
+    size_t n0 = size_t(1) << dlog;
+    size_t lw = 0, tw = 0;
+    for (uint32_t i = 0; i <= L; i++) {
+        r->layer_off.push_back(lw);
+        lw += (n0 >> i) * 4;
+    }
+    for (uint32_t i = 0; i < L; i++) {
+        r->tree_off.push_back(tw);
+        tw += tree_words(dlog - i);
+    }
+    ZS_TRY(dev_alloc(ctx, r->layers, lw * 4));
+    ZS_TRY(dev_alloc(ctx, r->trees, tw * 4));
+    DevBuf d_alphas;
+    ZS_TRY(dev_alloc(ctx, d_alphas, size_t(L) * 16));
+    uint32_t *layers = r->layers.as<uint32_t>(), *trees = r->trees.as<uint32_t>();
+    ZS_HIP(ctx, hipMemcpyAsync(layers, d_codeword, n0 * 16, hipMemcpyDeviceToDevice, ctx->stream));
+    if (alphas)
+        ZS_HIP(ctx, hipMemcpyAsync(d_alphas.ptr, alphas, size_t(L) * 16, hipMemcpyHostToDevice, ctx->stream));
+    ZS_TRY(launch_qm31_leaves(ctx, layers, n0, trees));
+    std::vector<uint32_t> host_alpha(size_t(L) * 4);
+    for (uint32_t i = 0; i < L; i++) {
+        uint32_t li = dlog - i;
+        ZS_TRY(launch_tree_levels(ctx, trees + r->tree_off[i], li, 0));
+        uint32_t *root = trees + r->tree_off[i] + tree_words(li) - 8;
+        if (!alphas) {
+            uint8_t rb[32];
+            ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            ZS_HIP(ctx, hipMemcpy(rb, root, 32, hipMemcpyDeviceToHost));
+            zs_channel_absorb(chan, rb, 32);
+            zs_channel_sample_qm31(chan, &host_alpha[4 * i]);
+            ZS_HIP(ctx, hipMemcpy(d_alphas.as<uint32_t>() + 4 * i, &host_alpha[4 * i], 16, hipMemcpyHostToDevice));
+        }
+        uint32_t *next_leaves = (i + 1 < L) ? trees + r->tree_off[i + 1] : nullptr;
+        ZS_TRY(launch_fri_fold(ctx, layers + r->layer_off[i], size_t(1) << li, dlog, i,
+                               d_alphas.as<uint32_t>() + 4 * i, layers + r->layer_off[i + 1], next_leaves));
+    }
+    ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (uint32_t i = 0; i < L; i++) {
+        uint32_t li = dlog - i;
+        ZS_HIP(ctx, hipMemcpy(roots_out + 32 * i, trees + r->tree_off[i] + tree_words(li) - 8, 32,
+                              hipMemcpyDeviceToHost));
+    }
+    // remainder: interpolate the first 2^(remainder_log - log_blowup) points of layer L (fri.zig:228-233)
+    size_t rc = size_t(1) << (remainder_log - log_blowup);
+    std::vector<uint32_t> last(rc * 4);
+    ZS_HIP(ctx, hipMemcpy(last.data(), layers + r->layer_off[L], rc * 16, hipMemcpyDeviceToHost));
+    std::vector<qm31> xs(rc), ys(rc), co;
+    qm31 w = qm31_root_of_unity(dlog - L);
+    qm31 acc = fri_offset_pow(L);
+    for (size_t j = 0; j < rc; j++) {
+        xs[j] = acc;
+        acc = qm_mul(acc, w);
+        ys[j] = qm_from_words(&last[4 * j]);
+    }
+    interpolate_host(xs, ys, co);
+    for (size_t k = 0; k < rc; k++) qm_to_words(co[k], remainder_out + 4 * k);
+    if (!alphas) {
+        uint8_t b[16];
+        for (size_t k = 0; k < rc; k++) {
+            for (int q = 0; q < 4; q++) {
+                uint32_t v = remainder_out[4 * k + q];
+                b[4 * q] = (uint8_t)v; b[4 * q + 1] = (uint8_t)(v >> 8);
+                b[4 * q + 2] = (uint8_t)(v >> 16); b[4 * q + 3] = (uint8_t)(v >> 24);
+            }
+            zs_channel_absorb(chan, b, 16);
+        }
+    }
+    if (out_result) *out_result = guard.release();
+    return ZS_OK;
+}
+
+int32_t zs_fri_commit(zs_hip_ctx *h, const uint32_t *codeword, uint32_t dlog, uint32_t log_blowup,
+                      uint32_t remainder_log, const uint32_t *alphas, zs_channel *chan, uint8_t *roots_out,
+                      uint32_t *remainder_out, zs_fri_result **out_result) {
+    if (!h || !codeword || dlog > 28) return ZS_ERR_INPUT;
+    Ctx *ctx = &h->c;
+    ZS_HIP(ctx, hipSetDevice(ctx->device));
+    DevBuf d;
+    size_t bytes = (size_t(1) << dlog) * 16;
+    ZS_TRY(dev_alloc(ctx, d, bytes));
+    ZS_HIP(ctx, hipMemcpy(d.ptr, codeword, bytes, hipMemcpyHostToDevice));
+    return zs_fri_commit_dev(h, d.as<uint32_t>(), dlog, log_blowup, remainder_log, alphas, chan, 0, roots_out,
+                             remainder_out, out_result);
+}
+
+int32_t zs_fri_query(zs_fri_result *r, zs_channel *chan, size_t num_queries, size_t *idx_out, uint32_t *vals_out,
+                     uint8_t *paths_out) {
+    if (!r || !chan || (num_queries && (!idx_out || !vals_out || !paths_out))) return ZS_ERR_INPUT;
+    Ctx *ctx = r->ctx;
+    const uint32_t L = r->L, dlog = r->dlog;
+    size_t n0 = size_t(1) << dlog;
+    size_t per_q_digests = 0;
+    for (uint32_t i = 0; i < L; i++) per_q_digests += 2 * size_t(dlog - i);
+    std::vector<PathReq> preqs;
+    std::vector<uint64_t> vreqs;
+    for (size_t q = 0; q < num_queries; q++) {
+        size_t p = zs_channel_sample_index(chan, n0);  // fri.zig:241
+        idx_out[q] = p;
+        uint64_t pout = q * per_q_digests * 8;
+        for (uint32_t i = 0; i < L; i++) {
+            size_t ni = n0 >> i, nn = ni / 2, pn = p % nn;
+            uint64_t vout = ((q * L + i) * 2) * 4;
+            vreqs.push_back(r->layer_off[i] + pn * 4);
+            vreqs.push_back(vout);
+            vreqs.push_back(r->layer_off[i] + (pn + nn) * 4);
+            vreqs.push_back(vout + 4);
+            uint32_t li = dlog - i;
+            preqs.push_back({r->tree_off[i], pout, li, (uint32_t)pn});
+            pout += uint64_t(li) * 8;
+            preqs.push_back({r->tree_off[i], pout, li, (uint32_t)(pn + nn)});
+            pout += uint64_t(li) * 8;
+            p = pn;
+        }
+    }
+    if (num_queries == 0) return ZS_OK;
+    DevBuf dv, dp;
+    ZS_TRY(dev_alloc(ctx, dv, num_queries * L * 2 * 16));
+    ZS_TRY(dev_alloc(ctx, dp, num_queries * per_q_digests * 32));
+    ZS_TRY(launch_gather_words(ctx, r->layers.as<uint32_t>(), vreqs, 4, dv.as<uint32_t>()));
+    ZS_TRY(launch_gather_paths(ctx, r->trees.as<uint32_t>(), preqs, dp.as<uint32_t>()));
+    ZS_HIP(ctx, hipMemcpy(vals_out, dv.ptr, num_queries * L * 2 * 16, hipMemcpyDeviceToHost));
+    ZS_HIP(ctx, hipMemcpy(paths_out, dp.ptr, num_queries * per_q_digests * 32, hipMemcpyDeviceToHost));
+    return ZS_OK;
+}
+
+int32_t zs_fri_read_layer(const zs_fri_result *r, uint32_t layer, uint32_t *out) {
+    if (!r || !out || layer > r->L) return ZS_ERR_INPUT;
+    Ctx *ctx = r->ctx;
+    ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ZS_HIP(ctx, hipMemcpy(out, r->layers.as<uint32_t>() + r->layer_off[layer], (size_t(1) << (r->dlog - layer)) * 16,
+                          hipMemcpyDeviceToHost));
+    return ZS_OK;
+}
+
+void zs_fri_result_free(zs_fri_result *r) { delete r; }
+
+int32_t zs_fill_m31_dev(zs_hip_ctx *h, uint64_t seed, uint32_t *d_out, size_t n) {
+    if (!h || (!d_out && n)) return ZS_ERR_INPUT;
+    return launch_fill_m31(&h->c, seed, d_out, n);
+}
+
+}  // extern "C"

@@ -1,0 +1,616 @@
+// fft.hip -- circle FFT / IFFT / LDE kernels for gfx950.
+//
+// The transform is the one of src/m31/ntt/circle.zig:195-346: the A-then-B
+// coefficients are interleaved into k-space ([A0, B0, A1, B1, ...]), the
+// forward transform runs butterfly stages s = L-1 .. 0 (stage s pairs k and
+// k + 2^s with twiddle tw_s[k >> (s+1)]), and the result is read out in
+// bit-reversed order; the inverse runs the stages upward with inverted
+// twiddles and scales by 1/n.
+//
+// Twiddles live on the device in one table per half-coset ("heap" layout:
+// stage s of a log-L transform at [2^(L-1-s), 2^(L-s))).  Entry values are
+// those of slowPrecomputeTwiddles / lineTwiddleSlice / circleTwiddles
+// (circle.zig:65-106): stage s >= 1 holds x(2^(s-1) * at(bitrev(h))) of the
+// half-coset, stage 0 is the [y, -y, -x, x] expansion of stage 1.
+//
+// The fast path groups C consecutive stages into one pass: a workgroup loads
+// G*Q tiles of 2^C elements into LDS (coalesced: tiles with adjacent addresses
+// are loaded together), runs the C stages in LDS, and writes back.  Natural-
+// order input (inverse) and output (forward) are handled in the first / last
+// pass by choosing tiles whose natural addresses are adjacent, so the
+// bit-reversal costs no extra pass.  For the blowup LDE the first log_blowup
+// forward stages act on zeros and reduce to copies, so the LDE runs 2^beta
+// size-2^log_n transforms per column (q = 0..2^beta-1) on twiddle sub-tables.
+#include <vector>
+
+#include "internal.h"
+
+namespace zs {
+
+// ---------------------------------------------------------------------------
+// twiddle generation
+// ---------------------------------------------------------------------------
+struct CosetPowers {
+    cpoint off[32];   // offset doubled j times
+    cpoint step[32];  // step doubled k times
+};
+
+__device__ __forceinline__ uint32_t coset_point_x(const CosetPowers &cp, uint32_t j, uint32_t idx) {
+    cpoint r = cp.off[j];
+    uint32_t b = 0;
+    while (idx) {
+        if (idx & 1u) r = cp_add(r, cp.step[j + b]);
+        idx >>= 1;
+        b++;
+    }
+    return r.x;
+}
+
+__global__ void twiddle_gen_kernel(uint32_t *fwd, uint32_t *inv, uint32_t L, CosetPowers cp) {
+    uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t n = 1u << L;
+    if (e == 0) {
+        fwd[0] = 0;
+        inv[0] = 0;
+        return;
+    }
+    if (e >= n) return;
+    uint32_t lvl = 31 - __builtin_clz(e);
+    uint32_t s = L - 1 - lvl;
+    uint32_t h = e - (1u << lvl);
+    uint32_t v;
+    if (s >= 1) {
+        v = coset_point_x(cp, s - 1, bitrev32(h, lvl));
+    } else {
+        uint32_t i = h >> 2, r = h & 3u;
+        uint32_t x = coset_point_x(cp, 0, bitrev32(2 * i, L - 2));
+        uint32_t y = coset_point_x(cp, 0, bitrev32(2 * i + 1, L - 2));
+        v = r == 0 ? y : r == 1 ? m31_neg(y) : r == 2 ? m31_neg(x) : x;
+    }
+    fwd[e] = v;
+    inv[e] = m31_inv(v);
+}
+
+cpoint coset_at_host(const zs_circle_coset &c, uint64_t i) {
+    return cp_add({c.offset_x, c.offset_y}, cp_mul({c.step_x, c.step_y}, i));
+}
+
+int32_t get_twiddles(Ctx *ctx, const zs_circle_coset &hc, const TwiddleTable **out) {
+    auto key = std::make_tuple(hc.offset_x, hc.offset_y, hc.step_x, hc.step_y, hc.log_size);
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    auto it = ctx->twiddles.find(key);
+    if (it != ctx->twiddles.end()) {
+        *out = it->second.get();
+        return ZS_OK;
+    }
+    uint32_t L = hc.log_size + 1;
+    if (L < 3 || L > 29) return ZS_ERR_INPUT;
+    auto t = std::make_unique<TwiddleTable>();
+    t->log_n = L;
+    size_t bytes = (size_t(1) << L) * 4;
+    ZS_HIP(ctx, hipMalloc(&t->fwd.ptr, bytes));
+    t->fwd.bytes = bytes;
+    ZS_HIP(ctx, hipMalloc(&t->inv.ptr, bytes));
+    t->inv.bytes = bytes;
+    CosetPowers cp;
+    cpoint o{hc.offset_x, hc.offset_y}, st{hc.step_x, hc.step_y};
+    for (int j = 0; j < 32; j++) {
+        cp.off[j] = o;
+        cp.step[j] = st;
+        o = cp_double(o);
+        st = cp_double(st);
+    }
+    uint32_t n = 1u << L;
+    hipLaunchKernelGGL(twiddle_gen_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, t->fwd.as<uint32_t>(),
+                       t->inv.as<uint32_t>(), L, cp);
+    ZS_HIP(ctx, hipGetLastError());
+    *out = t.get();
+    ctx->twiddles[key] = std::move(t);
+    return ZS_OK;
+}
+
+int32_t get_canonic_twiddles(Ctx *ctx, uint32_t log_n, const TwiddleTable **out) {
+    zs_circle_coset hc;
+    if (zs_circle_coset_canonic_half(log_n, &hc) != ZS_OK) return ZS_ERR_INPUT;
+    return get_twiddles(ctx, hc, out);
+}
+
+// ---------------------------------------------------------------------------
+// multi-stage LDS pass
+// ---------------------------------------------------------------------------
+enum : uint32_t { IN_KSPACE = 0, IN_SHARED = 1, IN_NATURAL = 2, IN_AB = 3 };
+enum : uint32_t { OUT_KSPACE = 0, OUT_NATURAL = 1, OUT_AB = 2 };
+
+struct FftPass {
+    const uint32_t *in;
+    uint32_t *out;
+    uint64_t in_cs, out_cs;  // column strides (elements)
+    const uint32_t *tw;      // heap table of log size LT
+    uint32_t L, LT, beta;    // transform log size, table log size, blowup log
+    uint32_t s_lo, C, G_log, Q_log;
+    uint32_t in_mode, out_mode;
+    uint32_t scale;          // 0 = none
+    uint32_t natural_tiles;  // tiles chosen so bitrev addresses are adjacent
+};
+
+constexpr int PASS_THREADS = 256;
+
+template <bool INV>
+__global__ __launch_bounds__(PASS_THREADS) void fft_pass_kernel(FftPass p) {
+    extern __shared__ uint32_t lds[];
+    const uint32_t C = p.C, T = 1u << C, TP = T + 1;
+    const uint32_t G = 1u << p.G_log, Q = 1u << p.Q_log;
+    const uint32_t col = blockIdx.y;
+    const uint32_t qblk = blockIdx.z << p.Q_log;
+    const uint32_t tid = threadIdx.x;
+
+    uint32_t hi = 0, lo0 = 0, tau0 = 0;
+    if (p.s_lo > 0) {
+        uint32_t lo_groups = (1u << p.s_lo) >> p.G_log;
+        hi = blockIdx.x / lo_groups;
+        lo0 = (blockIdx.x % lo_groups) << p.G_log;
+    } else {
+        tau0 = blockIdx.x << p.G_log;
+    }
+    auto fixed = [&](uint32_t g) -> uint32_t {
+        if (p.s_lo > 0) return (hi << (p.s_lo + C)) | (lo0 + g);
+        uint32_t h = tau0 + g;
+        if (p.natural_tiles) h = bitrev32(h, p.L - C);
+        return h << C;
+    };
+    const bool g_fast = (p.s_lo > 0) || p.natural_tiles;
+    const uint32_t half_n = 1u << (p.L - 1);
+
+    // ---- load ----
+    const uint32_t *in_col = p.in + (uint64_t)col * p.in_cs;
+    const uint32_t qload_log = (p.in_mode == IN_SHARED) ? 0 : p.Q_log;
+    const uint32_t nload = (G << qload_log) << C;
+    for (uint32_t e = tid; e < nload; e += PASS_THREADS) {
+        uint32_t g, t, ql;
+        if (g_fast) {
+            g = e & (G - 1);
+            uint32_t r = e >> p.G_log;
+            ql = r & ((1u << qload_log) - 1);
+            t = r >> qload_log;
+        } else {
+            t = e & (T - 1);
+            uint32_t r = e >> C;
+            g = r & (G - 1);
+            ql = r >> p.G_log;
+        }
+        uint32_t k = fixed(g) | (t << p.s_lo);
+        uint64_t addr;
+        switch (p.in_mode) {
+            case IN_KSPACE: addr = ((uint64_t)(qblk + ql) << p.L) | k; break;
+            case IN_SHARED: addr = k; break;
+            case IN_NATURAL: addr = bitrev32(k, p.L); break;
+            default: addr = (k & 1u) ? (half_n + (k >> 1)) : (k >> 1); break;
+        }
+        uint32_t v = in_col[addr];
+        if (p.in_mode == IN_SHARED) {
+            for (uint32_t qq = 0; qq < Q; qq++) lds[(qq * G + g) * TP + t] = v;
+        } else {
+            lds[(ql * G + g) * TP + t] = v;
+        }
+    }
+    __syncthreads();
+
+    // ---- stages ----
+    const uint32_t half_pairs = T >> 1;
+    const uint32_t npairs = (Q * G) * half_pairs;
+    for (uint32_t si = 0; si < C; si++) {
+        const uint32_t ls = INV ? si : (C - 1 - si);
+        const uint32_t s = p.s_lo + ls;
+        const uint32_t *tws = p.tw + (1u << (p.LT - 1 - s));
+        for (uint32_t b = tid; b < npairs; b += PASS_THREADS) {
+            uint32_t j = b & (half_pairs - 1);
+            uint32_t tile = b >> (C - 1);
+            uint32_t g = tile & (G - 1), ql = tile >> p.G_log;
+            uint32_t t0 = ((j >> ls) << (ls + 1)) | (j & ((1u << ls) - 1));
+            uint32_t t1 = t0 | (1u << ls);
+            uint32_t K = ((qblk + ql) << p.L) | fixed(g) | (t0 << p.s_lo);
+            uint32_t w = tws[K >> (s + 1)];
+            uint32_t *base = lds + tile * TP;
+            uint32_t v0 = base[t0], v1 = base[t1];
+            if (INV) {
+                base[t0] = m31_add(v0, v1);
+                base[t1] = m31_mul(m31_sub(v0, v1), w);
+            } else {
+                uint32_t m = m31_mul(v1, w);
+                base[t0] = m31_add(v0, m);
+                base[t1] = m31_sub(v0, m);
+            }
+        }
+        __syncthreads();
+    }
+
+    // ---- store ----
+    uint32_t *out_col = p.out + (uint64_t)col * p.out_cs;
+    const uint32_t nstore = (G * Q) << C;
+    for (uint32_t e = tid; e < nstore; e += PASS_THREADS) {
+        uint32_t g, t, ql;
+        if (p.out_mode == OUT_NATURAL) {
+            uint32_t rq = e & (Q - 1);
+            g = (e >> p.Q_log) & (G - 1);
+            t = e >> (p.Q_log + p.G_log);
+            ql = bitrev32(rq, p.Q_log);
+        } else if (g_fast) {
+            g = e & (G - 1);
+            uint32_t r = e >> p.G_log;
+            ql = r & (Q - 1);
+            t = r >> p.Q_log;
+        } else {
+            t = e & (T - 1);
+            uint32_t r = e >> C;
+            g = r & (G - 1);
+            ql = r >> p.G_log;
+        }
+        uint32_t v = lds[(ql * G + g) * TP + t];
+        if (p.scale) v = m31_mul(v, p.scale);
+        uint32_t k = fixed(g) | (t << p.s_lo);
+        uint32_t q = qblk + ql;
+        uint64_t addr;
+        switch (p.out_mode) {
+            case OUT_KSPACE: addr = ((uint64_t)q << p.L) | k; break;
+            case OUT_NATURAL: addr = ((uint64_t)bitrev32(k, p.L) << p.beta) | bitrev32(q, p.beta); break;
+            default: addr = (k & 1u) ? (half_n + (k >> 1)) : (k >> 1); break;
+        }
+        out_col[addr] = v;
+    }
+}
+
+struct PassRange {
+    uint32_t s_lo, C;
+};
+
+static std::vector<PassRange> plan_passes(uint32_t L, bool inverse, uint32_t cmax) {
+    uint32_t np = (L + cmax - 1) / cmax;
+    std::vector<uint32_t> sizes(np, L / np);
+    for (uint32_t i = 0; i < L % np; i++) sizes[i]++;
+    std::vector<PassRange> out;
+    if (inverse) {
+        uint32_t s = 0;
+        for (uint32_t c : sizes) { out.push_back({s, c}); s += c; }
+    } else {
+        uint32_t s = L;
+        for (uint32_t c : sizes) { s -= c; out.push_back({s, c}); }
+    }
+    return out;
+}
+
+constexpr uint32_t PASS_CMAX = 10;
+constexpr uint32_t PASS_ELEMS_LOG = 13;
+
+static int32_t run_pass(Ctx *ctx, FftPass p, bool inverse, uint32_t n_cols) {
+    uint32_t avail = p.s_lo > 0 ? p.s_lo : (p.L - p.C);
+    int32_t want = (int32_t)PASS_ELEMS_LOG - (int32_t)p.C - (int32_t)p.Q_log;
+    p.G_log = (uint32_t)std::max<int32_t>(0, std::min<int32_t>((int32_t)avail, want));
+    uint32_t tiles = 1u << (p.L - p.C);
+    dim3 grid(tiles >> p.G_log, n_cols, (1u << p.beta) >> p.Q_log);
+    size_t lds = (size_t(1) << (p.G_log + p.Q_log)) * ((size_t(1) << p.C) + 1) * 4;
+    if (inverse)
+        hipLaunchKernelGGL(fft_pass_kernel<true>, grid, dim3(PASS_THREADS), lds, ctx->stream, p);
+    else
+        hipLaunchKernelGGL(fft_pass_kernel<false>, grid, dim3(PASS_THREADS), lds, ctx->stream, p);
+    ZS_HIP(ctx, hipGetLastError());
+    return ZS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// n = 2, 4: the reference's special cases (circle.zig:210-220, 290-319)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void bfly(uint32_t &v0, uint32_t &v1, uint32_t t) {
+    uint32_t m = m31_mul(v1, t);
+    v1 = m31_sub(v0, m);
+    v0 = m31_add(v0, m);
+}
+__device__ __forceinline__ void ibfly(uint32_t &v0, uint32_t &v1, uint32_t t) {
+    uint32_t a = v0;
+    v0 = m31_add(a, v1);
+    v1 = m31_mul(m31_sub(a, v1), t);
+}
+
+// out_kspace: write the inverse's interleaved coefficients instead of A|B
+template <bool INV>
+__global__ void fft_small_kernel(const uint32_t *in, uint64_t in_cs, uint32_t *out, uint64_t out_cs, uint32_t n_cols,
+                                 uint32_t lg, uint32_t px, uint32_t py, uint32_t out_kspace) {
+    uint32_t col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= n_cols) return;
+    const uint32_t *a = in + (uint64_t)col * in_cs;
+    uint32_t *o = out + (uint64_t)col * out_cs;
+    if (!INV) {
+        if (lg == 1) {
+            uint32_t e0 = a[0], e1 = a[1];
+            bfly(e0, e1, py);
+            o[0] = e0; o[1] = e1;
+        } else {
+            uint32_t e0 = a[0], e1 = a[2], e2 = a[1], e3 = a[3];
+            bfly(e0, e2, px);
+            bfly(e1, e3, px);
+            bfly(e0, e1, py);
+            bfly(e2, e3, m31_neg(py));
+            o[0] = e0; o[1] = e2; o[2] = e1; o[3] = e3;
+        }
+    } else {
+        if (lg == 1) {
+            uint32_t yn_inv = m31_inv(m31_add(py, py));
+            uint32_t y_inv = m31_add(yn_inv, yn_inv);
+            uint32_t n_inv = m31_mul(yn_inv, py);
+            uint32_t v0 = a[0], v1 = a[1];
+            ibfly(v0, v1, y_inv);
+            o[0] = m31_mul(v0, n_inv);
+            o[1] = m31_mul(v1, n_inv);
+        } else {
+            uint32_t xy4 = m31_mul(m31_mul(px, py), 4);
+            uint32_t xyn_inv = m31_inv(xy4);
+            uint32_t x_inv = m31_mul(m31_mul(xyn_inv, py), 4);
+            uint32_t y_inv = m31_mul(m31_mul(xyn_inv, px), 4);
+            uint32_t n_inv = m31_mul(m31_mul(xyn_inv, px), py);
+            uint32_t v0 = a[0], v1 = a[2], v2 = a[1], v3 = a[3];  // bit-reversed
+            ibfly(v0, v1, y_inv);
+            ibfly(v2, v3, m31_neg(y_inv));
+            ibfly(v0, v2, x_inv);
+            ibfly(v1, v3, x_inv);
+            v0 = m31_mul(v0, n_inv); v1 = m31_mul(v1, n_inv);
+            v2 = m31_mul(v2, n_inv); v3 = m31_mul(v3, n_inv);
+            if (out_kspace) {
+                o[0] = v0; o[1] = v1; o[2] = v2; o[3] = v3;
+            } else {
+                o[0] = v0; o[1] = v2; o[2] = v1; o[3] = v3;
+            }
+        }
+    }
+}
+
+static int32_t run_small(Ctx *ctx, bool inverse, const zs_circle_coset &hc, const uint32_t *in, uint64_t in_cs,
+                         uint32_t *out, uint64_t out_cs, uint32_t n_cols, bool out_kspace) {
+    cpoint p0 = coset_at_host(hc, 0);
+    uint32_t lg = hc.log_size + 1;
+    dim3 grid((n_cols + 127) / 128);
+    if (inverse)
+        hipLaunchKernelGGL(fft_small_kernel<true>, grid, dim3(128), 0, ctx->stream, in, in_cs, out, out_cs, n_cols, lg,
+                           p0.x, p0.y, (uint32_t)out_kspace);
+    else
+        hipLaunchKernelGGL(fft_small_kernel<false>, grid, dim3(128), 0, ctx->stream, in, in_cs, out, out_cs, n_cols,
+                           lg, p0.x, p0.y, 0u);
+    ZS_HIP(ctx, hipGetLastError());
+    return ZS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// public launchers
+// ---------------------------------------------------------------------------
+static int32_t transform(Ctx *ctx, bool inverse, const zs_circle_coset &hc, const uint32_t *d_in, uint64_t in_stride,
+                         uint32_t *d_out, uint64_t out_stride, uint32_t n_cols) {
+    uint32_t L = hc.log_size + 1;
+    if (n_cols == 0) return ZS_OK;
+    if (L < 3) return run_small(ctx, inverse, hc, d_in, in_stride, d_out, out_stride, n_cols, false);
+    const TwiddleTable *tt;
+    ZS_TRY(get_twiddles(ctx, hc, &tt));
+    auto plan = plan_passes(L, inverse, PASS_CMAX);
+    size_t n = size_t(1) << L;
+    uint32_t *tmp = nullptr;
+    if (plan.size() > 1) {
+        ZS_TRY(ctx->ensure_scratch(0, n * n_cols * 4));
+        tmp = ctx->scratch[0].as<uint32_t>();
+    }
+    for (size_t i = 0; i < plan.size(); i++) {
+        FftPass p{};
+        bool first = i == 0, last = i + 1 == plan.size();
+        p.in = first ? d_in : tmp;
+        p.in_cs = first ? in_stride : n;
+        p.out = last ? d_out : tmp;
+        p.out_cs = last ? out_stride : n;
+        p.tw = inverse ? tt->inv.as<uint32_t>() : tt->fwd.as<uint32_t>();
+        p.L = L; p.LT = L; p.beta = 0;
+        p.s_lo = plan[i].s_lo; p.C = plan[i].C; p.Q_log = 0;
+        if (inverse) {
+            p.in_mode = first ? IN_NATURAL : IN_KSPACE;
+            p.out_mode = last ? OUT_AB : OUT_KSPACE;
+            p.natural_tiles = first ? 1 : 0;
+            p.scale = last ? m31_inv((uint32_t)(n % P)) : 0;
+        } else {
+            p.in_mode = first ? IN_AB : IN_KSPACE;
+            p.out_mode = last ? OUT_NATURAL : OUT_KSPACE;
+            p.natural_tiles = last ? 1 : 0;
+            p.scale = 0;
+        }
+        ZS_TRY(run_pass(ctx, p, inverse, n_cols));
+    }
+    return ZS_OK;
+}
+
+int32_t launch_fft(Ctx *ctx, const zs_circle_coset &hc, const uint32_t *d_in, uint64_t in_stride, uint32_t *d_out,
+                   uint64_t out_stride, uint32_t n_cols) {
+    return transform(ctx, false, hc, d_in, in_stride, d_out, out_stride, n_cols);
+}
+int32_t launch_ifft(Ctx *ctx, const zs_circle_coset &hc, const uint32_t *d_in, uint64_t in_stride, uint32_t *d_out,
+                    uint64_t out_stride, uint32_t n_cols) {
+    return transform(ctx, true, hc, d_in, in_stride, d_out, out_stride, n_cols);
+}
+
+size_t lde_scratch_words(uint32_t n_cols, uint32_t log_n, uint32_t log_blowup, size_t *coef_words) {
+    *coef_words = (size_t(n_cols) << log_n);
+    uint32_t LT = log_n + log_blowup;
+    bool multi = log_n > PASS_CMAX || LT < 3;
+    return multi ? (size_t(n_cols) << LT) : 0;
+}
+
+// zero-pad A|B coefficients of size 2^log_n into A|B of size 2^LT (tiny-size path)
+__global__ void pad_ab_kernel(const uint32_t *in, uint32_t *out, uint32_t n_cols, uint32_t log_n, uint32_t LT) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t N = uint64_t(1) << LT;
+    if (i >= N * n_cols) return;
+    uint64_t col = i >> LT, r = i & (N - 1);
+    uint64_t h = uint64_t(1) << (log_n - 1), H = N >> 1;
+    uint32_t v = 0;
+    if (r < h) v = in[(col << log_n) + r];
+    else if (r >= H && r < H + h) v = in[(col << log_n) + h + (r - H)];
+    out[i] = v;
+}
+
+int32_t launch_lde(Ctx *ctx, const uint32_t *d_in, uint64_t in_stride, uint32_t n_cols, uint32_t log_n,
+                   uint32_t log_blowup, uint32_t *d_out, uint64_t out_stride, uint32_t *d_coef, uint32_t *d_big) {
+    uint32_t LT = log_n + log_blowup;
+    if (log_n < 1 || LT > 28) return ZS_ERR_INPUT;
+    if (n_cols == 0) return ZS_OK;
+    size_t n = size_t(1) << log_n;
+    zs_circle_coset hc_small, hc_big;
+    zs_circle_coset_canonic_half(log_n, &hc_small);
+    zs_circle_coset_canonic_half(LT, &hc_big);
+    if (LT < 3) {
+        // log_n = 1 (and log_blowup <= 1): A|B interpolant, zero-pad, forward transform
+        ZS_TRY(run_small(ctx, true, hc_small, d_in, in_stride, d_coef, n, n_cols, false));
+        uint64_t tot = uint64_t(n_cols) << LT;
+        hipLaunchKernelGGL(pad_ab_kernel, dim3((tot + 255) / 256), dim3(256), 0, ctx->stream, d_coef, d_big, n_cols,
+                           log_n, LT);
+        ZS_HIP(ctx, hipGetLastError());
+        return run_small(ctx, false, hc_big, d_big, size_t(1) << LT, d_out, out_stride, n_cols, false);
+    }
+    // 1) interpolate: natural evaluations -> interleaved coefficients scaled by 1/n (k-space)
+    if (log_n < 3) {
+        ZS_TRY(run_small(ctx, true, hc_small, d_in, in_stride, d_coef, n, n_cols, true));
+    } else {
+        const TwiddleTable *ti;
+        ZS_TRY(get_twiddles(ctx, hc_small, &ti));
+        auto plan = plan_passes(log_n, true, PASS_CMAX);
+        for (size_t i = 0; i < plan.size(); i++) {
+            FftPass p{};
+            bool first = i == 0, last = i + 1 == plan.size();
+            p.in = first ? d_in : d_coef;
+            p.in_cs = first ? in_stride : n;
+            p.out = d_coef;
+            p.out_cs = n;
+            p.tw = ti->inv.as<uint32_t>();
+            p.L = log_n; p.LT = log_n; p.beta = 0;
+            p.s_lo = plan[i].s_lo; p.C = plan[i].C; p.Q_log = 0;
+            p.in_mode = first ? IN_NATURAL : IN_KSPACE;
+            p.out_mode = OUT_KSPACE;
+            p.natural_tiles = first ? 1 : 0;
+            p.scale = last ? m31_inv((uint32_t)n) : 0;
+            ZS_TRY(run_pass(ctx, p, true, n_cols));
+        }
+    }
+    // 2) evaluate on the 2^beta quarter transforms of the big canonic domain
+    const TwiddleTable *tf;
+    ZS_TRY(get_twiddles(ctx, hc_big, &tf));
+    auto plan = plan_passes(log_n, false, PASS_CMAX);
+    size_t N = size_t(1) << LT;
+    for (size_t i = 0; i < plan.size(); i++) {
+        FftPass p{};
+        bool first = i == 0, last = i + 1 == plan.size();
+        p.in = first ? d_coef : d_big;
+        p.in_cs = first ? n : N;
+        p.out = last ? d_out : d_big;
+        p.out_cs = last ? out_stride : N;
+        p.tw = tf->fwd.as<uint32_t>();
+        p.L = log_n; p.LT = LT; p.beta = log_blowup;
+        p.s_lo = plan[i].s_lo; p.C = plan[i].C;
+        // the last pass keeps 2^Q_log of the 2^beta replicas in one workgroup so its natural-order
+        // stores are contiguous; capped so the LDS tile stays at 2^13 elements
+        p.Q_log = last ? std::min<uint32_t>(log_blowup, PASS_ELEMS_LOG > p.C ? PASS_ELEMS_LOG - p.C : 0) : 0;
+        p.in_mode = first ? IN_SHARED : IN_KSPACE;
+        p.out_mode = last ? OUT_NATURAL : OUT_KSPACE;
+        p.natural_tiles = last ? 1 : 0;
+        p.scale = 0;
+        ZS_TRY(run_pass(ctx, p, false, n_cols));
+    }
+    return ZS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// reference-shaped plain path: one launch per butterfly layer, per column
+// (src/cuda/circlefft_gpu.zig:46-175, src/cuda/kernels/circlefft.cu)
+// ---------------------------------------------------------------------------
+__global__ void plain_interleave_kernel(const uint32_t *ab, uint32_t *out, uint32_t half) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= half) return;
+    out[2 * i] = ab[i];
+    out[2 * i + 1] = ab[half + i];
+}
+__global__ void plain_layer_kernel(uint32_t *v, const uint32_t *tw, uint32_t n, uint32_t s, uint32_t inverse) {
+    uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n / 2) return;
+    uint32_t b = x >> s, l = x & ((1u << s) - 1);
+    uint32_t i0 = (b << (s + 1)) + l, i1 = i0 + (1u << s);
+    uint32_t t = tw[b];
+    uint32_t v0 = v[i0], v1 = v[i1];
+    if (inverse) ibfly(v0, v1, t);
+    else bfly(v0, v1, t);
+    v[i0] = v0;
+    v[i1] = v1;
+}
+__global__ void plain_bitrev_kernel(const uint32_t *in, uint32_t *out, uint32_t L) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (1u << L)) return;
+    out[i] = in[bitrev32(i, L)];
+}
+__global__ void plain_deinterleave_kernel(const uint32_t *c, uint32_t *ab, uint32_t half, uint32_t inv_n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= half) return;
+    ab[i] = m31_mul(c[2 * i], inv_n);
+    ab[half + i] = m31_mul(c[2 * i + 1], inv_n);
+}
+
+int32_t launch_lde_plain(Ctx *ctx, const uint32_t *d_in, uint64_t in_stride, uint32_t n_cols, uint32_t log_n,
+                         uint32_t log_blowup, uint32_t *d_out, uint64_t out_stride, uint32_t *d_tmp) {
+    uint32_t LT = log_n + log_blowup;
+    if (log_n < 3) return ZS_ERR_INPUT;
+    size_t n = size_t(1) << log_n, N = size_t(1) << LT;
+    const TwiddleTable *ti, *tf;
+    ZS_TRY(get_canonic_twiddles(ctx, log_n, &ti));
+    ZS_TRY(get_canonic_twiddles(ctx, LT, &tf));
+    uint32_t *a = d_tmp, *b = d_tmp + N;  // two work vectors of N words
+    uint32_t inv_n = m31_inv((uint32_t)n);
+    auto grid = [](size_t cnt) { return dim3((uint32_t)((cnt + 255) / 256)); };
+    for (uint32_t c = 0; c < n_cols; c++) {
+        const uint32_t *src = d_in + (uint64_t)c * in_stride;
+        // circleIFFT: bit-reverse, circle layer, line layers 1..L-1, scale + deinterleave
+        hipLaunchKernelGGL(plain_bitrev_kernel, grid(n), dim3(256), 0, ctx->stream, src, a, log_n);
+        for (uint32_t s = 0; s < log_n; s++)
+            hipLaunchKernelGGL(plain_layer_kernel, grid(n / 2), dim3(256), 0, ctx->stream, a,
+                               ti->inv.as<uint32_t>() + (size_t(1) << (log_n - 1 - s)), (uint32_t)n, s, 1u);
+        hipLaunchKernelGGL(plain_deinterleave_kernel, grid(n / 2), dim3(256), 0, ctx->stream, a, b, (uint32_t)(n / 2),
+                           inv_n);
+        // zero-pad into A|B of size N
+        hipLaunchKernelGGL(pad_ab_kernel, grid(N), dim3(256), 0, ctx->stream, b, a, 1u, log_n, LT);
+        // circleFFT: interleave, line layers L-1..1, circle layer, bit-reverse
+        hipLaunchKernelGGL(plain_interleave_kernel, grid(N / 2), dim3(256), 0, ctx->stream, a, b, (uint32_t)(N / 2));
+        for (int s = (int)LT - 1; s >= 0; s--)
+            hipLaunchKernelGGL(plain_layer_kernel, grid(N / 2), dim3(256), 0, ctx->stream, b,
+                               tf->fwd.as<uint32_t>() + (size_t(1) << (LT - 1 - s)), (uint32_t)N, (uint32_t)s, 0u);
+        hipLaunchKernelGGL(plain_bitrev_kernel, grid(N), dim3(256), 0, ctx->stream, b,
+                           d_out + (uint64_t)c * out_stride, LT);
+    }
+    ZS_HIP(ctx, hipGetLastError());
+    return ZS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// seeded synthetic input
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t splitmix_mix(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+__global__ void fill_m31_kernel(uint64_t seed, uint32_t *out, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t k = i + 1;
+    uint32_t v;
+    for (;;) {
+        v = (uint32_t)(splitmix_mix(seed + k * 0x9E3779B97F4A7C15ull) >> 33);
+        if (v != P) break;
+        k += uint64_t(1) << 40;
+    }
+    out[i] = v;
+}
+int32_t launch_fill_m31(Ctx *ctx, uint64_t seed, uint32_t *d_out, uint64_t n) {
+    if (n == 0) return ZS_OK;
+    hipLaunchKernelGGL(fill_m31_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, ctx->stream, seed, d_out, n);
+    ZS_HIP(ctx, hipGetLastError());
+    return ZS_OK;
+}
+
+}  // namespace zs

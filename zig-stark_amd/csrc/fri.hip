@@ -1,0 +1,108 @@
+// fri.hip -- FRI commit-phase folding over QM31 for gfx950 (src/m31/fri.zig).
+//
+// Layer i lives on D_i = FRI_OFFSET^(2^i) * <w_i>, w_i = w_0^(2^i), and is
+// folded with challenge alpha_i (fri.zig:211-218):
+//   next[j] = (y0 + y1)/2 + alpha * (y0 - y1) * D_i[j]^{-1} / 2,
+//   y0 = cur[j], y1 = cur[j + n/2].
+// The reference inverts D_i[j] per element.  Here D_i[j]^{-1} =
+// FRI_OFFSET^(-2^i) * w_0^{-(j * 2^i)} is read from two small tables of
+// powers of w_0^{-1} (exponent split 10 / rest bits), so a fold costs three
+// QM31 products and two halvings (x/2 is a shift and a conditional add of p),
+// with no inversion.  Field results are canonical, so the output equals the
+// reference's value for value.  The kernel also writes the hashQM31 leaf of
+// each new value, fusing the next layer's leaf pass into the fold.
+#include "blake3.h"
+#include "internal.h"
+
+namespace zs {
+
+static const uint32_t FRI_OFFSET_W[4] = {1, 0, 1, 0};  // fri.zig:36, (1 + 0i) + (1 + 0i)u
+static const uint8_t TWO_ADIC_ROOT_BYTES[16] = {0x60, 0x1f, 0x3d, 0x34, 0x32, 0x39, 0xa0, 0x6a,
+                                                0xa0, 0xd6, 0x51, 0x61, 0x91, 0xe7, 0x86, 0x3e};
+
+qm31 qm31_root_of_unity(uint32_t log_size) {  // qm31.zig:80-89
+    uint32_t w[4];
+    for (int i = 0; i < 4; i++) {
+        const uint8_t *b = TWO_ADIC_ROOT_BYTES + 4 * i;
+        w[i] = m31_from_u32((uint32_t)b[0] | ((uint32_t)b[1] << 8) | ((uint32_t)b[2] << 16) | ((uint32_t)b[3] << 24));
+    }
+    return qm_pow(qm_from_words(w), uint64_t(1) << (32 - log_size));
+}
+
+qm31 fri_offset_pow(uint32_t layer) { return qm_pow(qm_from_words(FRI_OFFSET_W), uint64_t(1) << layer); }
+
+int32_t get_fri_tables(Ctx *ctx, uint32_t dlog, const uint32_t **out) {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    auto it = ctx->fri_tables.find(dlog);
+    if (it != ctx->fri_tables.end()) {
+        *out = it->second->as<uint32_t>();
+        return ZS_OK;
+    }
+    size_t n_hi = dlog > 11 ? (size_t(1) << (dlog - 11)) : 1;
+    std::vector<uint32_t> host((1024 + n_hi) * 4);
+    qm31 winv = qm_inv(qm31_root_of_unity(dlog));
+    qm31 acc = {{1, 0}, {0, 0}};
+    for (size_t a = 0; a < 1024; a++) {
+        qm_to_words(acc, &host[4 * a]);
+        acc = qm_mul(acc, winv);
+    }
+    qm31 step = acc;  // w^{-1024}
+    acc = {{1, 0}, {0, 0}};
+    for (size_t b = 0; b < n_hi; b++) {
+        qm_to_words(acc, &host[4 * (1024 + b)]);
+        acc = qm_mul(acc, step);
+    }
+    auto buf = std::make_unique<DevBuf>();
+    ZS_HIP(ctx, hipMalloc(&buf->ptr, host.size() * 4));
+    buf->bytes = host.size() * 4;
+    ZS_HIP(ctx, hipMemcpy(buf->ptr, host.data(), host.size() * 4, hipMemcpyHostToDevice));
+    *out = buf->as<uint32_t>();
+    ctx->fri_tables[dlog] = std::move(buf);
+    return ZS_OK;
+}
+
+__device__ __forceinline__ qm31 ld_qm(const uint32_t *p) {
+    uint4 v = *reinterpret_cast<const uint4 *>(p);
+    return {{v.x, v.y}, {v.z, v.w}};
+}
+__device__ __forceinline__ void st_qm(uint32_t *p, qm31 x) {
+    *reinterpret_cast<uint4 *>(p) = make_uint4(x.a.c0, x.a.c1, x.b.c0, x.b.c1);
+}
+
+__global__ __launch_bounds__(256) void fri_fold_kernel(const uint32_t *cur, uint64_t half, uint32_t layer,
+                                                       const uint32_t *tables, qm31 inv_off, const uint32_t *alpha_p,
+                                                       uint32_t *next, uint32_t *next_leaves) {
+    uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= half) return;
+    qm31 y0 = ld_qm(cur + 4 * j);
+    qm31 y1 = ld_qm(cur + 4 * (j + half));
+    uint64_t e = j << layer;
+    qm31 xinv = qm_mul(inv_off, qm_mul(ld_qm(tables + 4 * (e & 1023)), ld_qm(tables + 4 * (1024 + (e >> 10)))));
+    qm31 alpha = ld_qm(alpha_p);
+    qm31 hs = qm_half(qm_add(y0, y1));
+    qm31 hd = qm_half(qm_mul(qm_sub(y0, y1), xinv));
+    qm31 v = qm_add(hs, qm_mul(alpha, hd));
+    st_qm(next + 4 * j, v);
+    if (next_leaves) {
+        uint32_t w[4] = {v.a.c0, v.a.c1, v.b.c0, v.b.c1};
+        uint32_t d[8];
+        b3_hash_qm31(w, d);
+        uint4 *o = reinterpret_cast<uint4 *>(next_leaves + j * 8);
+        o[0] = make_uint4(d[0], d[1], d[2], d[3]);
+        o[1] = make_uint4(d[4], d[5], d[6], d[7]);
+    }
+}
+
+int32_t launch_fri_fold(Ctx *ctx, const uint32_t *d_cur, uint64_t n, uint32_t dlog, uint32_t layer,
+                        const uint32_t *d_alpha, uint32_t *d_next, uint32_t *d_next_leaves) {
+    const uint32_t *tables;
+    ZS_TRY(get_fri_tables(ctx, dlog, &tables));
+    qm31 inv_off = qm_inv(fri_offset_pow(layer));
+    uint64_t half = n / 2;
+    hipLaunchKernelGGL(fri_fold_kernel, dim3((uint32_t)((half + 255) / 256)), dim3(256), 0, ctx->stream, d_cur, half,
+                       layer, tables, inv_off, d_alpha, d_next, d_next_leaves);
+    ZS_HIP(ctx, hipGetLastError());
+    return ZS_OK;
+}
+
+}  // namespace zs

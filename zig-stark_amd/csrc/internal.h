@@ -1,0 +1,138 @@
+// internal.h -- context, device buffers and the launch entry points shared by
+// the kernel files and the C ABI (capi.cpp).  Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/zs_hip.h"
+#include "field.h"
+
+namespace zs {
+
+// HIP call checking: record the failure on the context and return ZS_ERR_DEVICE.
+#define ZS_HIP(ctx, call)                                                              \
+    do {                                                                               \
+        hipError_t _e = (call);                                                        \
+        if (_e != hipSuccess) {                                                        \
+            (ctx)->set_error(std::string(#call) + ": " + hipGetErrorString(_e));       \
+            return ZS_ERR_DEVICE;                                                      \
+        }                                                                              \
+    } while (0)
+
+#define ZS_TRY(expr)                 \
+    do {                             \
+        int32_t _rc = (expr);        \
+        if (_rc != ZS_OK) return _rc; \
+    } while (0)
+
+// A device allocation that frees itself.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() {
+        if (ptr) (void)hipFree(ptr);
+    }
+    template <class T>
+    T *as() const { return static_cast<T *>(ptr); }
+};
+
+// Twiddle tables for one half-coset: heap layout, stage s of a transform of
+// log size L at [2^(L-1-s), 2^(L-s)); forward values and their inverses.
+struct TwiddleTable {
+    uint32_t log_n = 0;
+    DevBuf fwd, inv;
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;  // nullptr = the null stream
+    std::mutex mu;                  // guards the caches below
+    std::string last_error;
+    std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t, uint32_t>, std::unique_ptr<TwiddleTable>> twiddles;
+    // FRI inverse-domain tables keyed by dlog: [lo table 2^10 | hi table] QM31 words
+    std::map<uint32_t, std::unique_ptr<DevBuf>> fri_tables;
+    // grow-only scratch arenas
+    DevBuf scratch[4];
+
+    void set_error(const std::string &s) { last_error = s; }
+    int32_t ensure_scratch(int slot, size_t bytes);
+};
+
+// ---- twiddles (fft.hip) ----
+int32_t get_twiddles(Ctx *ctx, const zs_circle_coset &hc, const TwiddleTable **out);
+int32_t get_canonic_twiddles(Ctx *ctx, uint32_t log_n, const TwiddleTable **out);
+cpoint coset_at_host(const zs_circle_coset &c, uint64_t i);
+
+// ---- FFT launches (fft.hip); all device pointers, column-major, async on ctx->stream ----
+// Forward circle FFT of n_cols columns: A-then-B coefficients -> natural-order evaluations.
+int32_t launch_fft(Ctx *ctx, const zs_circle_coset &hc, const uint32_t *d_in, uint64_t in_stride, uint32_t *d_out,
+                   uint64_t out_stride, uint32_t n_cols);
+// Inverse circle FFT: natural-order evaluations -> A-then-B coefficients.
+int32_t launch_ifft(Ctx *ctx, const zs_circle_coset &hc, const uint32_t *d_in, uint64_t in_stride, uint32_t *d_out,
+                    uint64_t out_stride, uint32_t n_cols);
+// Blowup LDE on canonic cosets: columns of 2^log_n values -> 2^(log_n+log_blowup) values.
+// d_coef: scratch of n_cols * 2^log_n words; d_big: scratch of n_cols * 2^(log_n+log_blowup) words
+// (may be nullptr when the transform fits one pass).
+int32_t launch_lde(Ctx *ctx, const uint32_t *d_in, uint64_t in_stride, uint32_t n_cols, uint32_t log_n,
+                   uint32_t log_blowup, uint32_t *d_out, uint64_t out_stride, uint32_t *d_coef, uint32_t *d_big);
+size_t lde_scratch_words(uint32_t n_cols, uint32_t log_n, uint32_t log_blowup, size_t *coef_words);
+// Reference-shaped per-column, per-layer path (one launch per butterfly layer).
+int32_t launch_lde_plain(Ctx *ctx, const uint32_t *d_in, uint64_t in_stride, uint32_t n_cols, uint32_t log_n,
+                         uint32_t log_blowup, uint32_t *d_out, uint64_t out_stride, uint32_t *d_tmp);
+int32_t launch_fill_m31(Ctx *ctx, uint64_t seed, uint32_t *d_out, uint64_t n);
+
+// ---- Merkle (merkle.hip) ----
+// leaves = hashM31s(row) over n_cols columns at stride col_stride, then every
+// node level; d_nodes holds (2 * 2^log_rows - 1) digests, leaves first.
+int32_t launch_row_commit(Ctx *ctx, const uint32_t *d_cols, uint64_t col_stride, uint32_t n_cols, uint32_t log_rows,
+                          uint32_t *d_nodes);
+int32_t launch_row_commit_plain(Ctx *ctx, const uint32_t *d_cols, uint64_t col_stride, uint32_t n_cols,
+                                uint32_t log_rows, uint32_t *d_nodes);
+int32_t launch_row_leaves(Ctx *ctx, const uint32_t *d_cols, uint64_t col_stride, uint32_t n_cols, uint64_t n,
+                          uint32_t *d_nodes);
+// node levels above an already written level 0 of 2^log_leaves digests
+int32_t launch_tree_levels(Ctx *ctx, uint32_t *d_nodes, uint32_t log_leaves, uint32_t from_level);
+int32_t launch_qm31_leaves(Ctx *ctx, const uint32_t *d_vals, uint64_t n, uint32_t *d_nodes);
+// Merkle path gather: for each request, log_leaves sibling digests written at out_off_words.
+struct PathReq {
+    uint64_t tree_off_words;
+    uint64_t out_off_words;
+    uint32_t log_leaves;
+    uint32_t index;
+};
+int32_t launch_gather_paths(Ctx *ctx, const uint32_t *d_trees, const std::vector<PathReq> &reqs, uint32_t *d_out);
+// copies `words` words from src + pairs[2r] to out + pairs[2r+1] for each r; synchronous
+int32_t launch_gather_words(Ctx *ctx, const uint32_t *d_src, const std::vector<uint64_t> &pairs, uint32_t words,
+                            uint32_t *d_out);
+
+// ---- FRI (fri.hip) ----
+int32_t get_fri_tables(Ctx *ctx, uint32_t dlog, const uint32_t **out);
+// next[j] = fold(D_layer[j], cur[j], cur[j + n/2], alpha); also writes the
+// hashQM31 leaves of `next` into d_next_leaves when non-null.
+int32_t launch_fri_fold(Ctx *ctx, const uint32_t *d_cur, uint64_t n, uint32_t dlog, uint32_t layer,
+                        const uint32_t *d_alpha, uint32_t *d_next, uint32_t *d_next_leaves);
+
+// host-side constants
+qm31 fri_offset_pow(uint32_t layer);  // FRI_OFFSET^(2^layer)
+qm31 qm31_root_of_unity(uint32_t log_size);
+
+inline size_t tree_words(uint32_t log_leaves) { return ((size_t(2) << log_leaves) - 1) * 8; }
+inline size_t level_offset_words(uint32_t log_leaves, uint32_t level) {
+    // digests before `level`: sum_{l<level} 2^(log_leaves - l)
+    size_t n = size_t(1) << log_leaves;
+    size_t off = 0;
+    for (uint32_t l = 0; l < level; l++) off += n >> l;
+    return off * 8;
+}
+
+}  // namespace zs
