@@ -34,8 +34,9 @@ def zctx():
 @pytest.fixture(scope="session")
 def torch_dev():
     import torch
-    assert torch.cuda.is_available(), "GPU tests need a ROCm device"
-    return torch.device("cuda:0")
+    dev = torch.device("cuda:0")
+    torch.zeros(1, device=dev)  # raises with the runtime's reason if the device is unusable
+    return dev
 
 
 def to_dev(arr, device):

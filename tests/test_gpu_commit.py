@@ -107,3 +107,21 @@ def test_full_size_64x2p20_properties(zctx, torch_dev):
             assert nodes[32 * (off + cnt + j): 32 * (off + cnt + j + 1)] == O.hash2(l, r)
         off += cnt
     assert off == 2 * N - 2
+
+
+@pytest.mark.parametrize("n_cols,lg,beta", [(3, 16, 2), (2, 17, 3), (2, 18, 1), (1, 21, 1), (5, 10, 3), (3, 11, 3)])
+def test_lde_commit_register_group_path(zctx, n_cols, lg, beta):
+    """Shapes that take the register-group passes and the fused interpolation/evaluation kernel
+    (single-workgroup form for lg 8..11, two passes for lg 16..22)."""
+    rng = np.random.default_rng(n_cols * 7 + lg)
+    cols = rng.integers(0, P, (n_cols, 1 << lg), dtype=np.uint32)
+    tree, root, lde = zctx.lde_commit(cols, beta)
+    ref_lde = np.stack([O.lde(c, beta) for c in cols])
+    assert np.array_equal(lde, ref_lde)
+    N = 1 << (lg + beta)
+    leaves = b"".join(O.hash_m31s(ref_lde[:, i]) for i in range(0, N, max(1, N // 512)))
+    got = tree.level(0)
+    step = max(1, N // 512)
+    assert b"".join(got[32 * i: 32 * i + 32] for i in range(0, N, step)) == leaves
+    if N <= 1 << 17:
+        assert root == O.tree_root(O.row_commit(ref_lde))

@@ -56,6 +56,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ZsError(ZS_ERR_GENERIC, f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        # torch ships its own libamdhip64.so.7 under the same soname as the system
+        # ROCm runtime; whichever is loaded first serves the whole process.  Load
+        # torch first (when installed) so torch and this library share its runtime.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         _declare(L)
         _lib = L

@@ -340,14 +340,24 @@ int32_t zs_fri_commit_dev(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t dl
         r->tree_off.push_back(tw);
         tw += tree_words(dlog - i);
     }
-    ZS_TRY(dev_alloc(ctx, r->layers, lw * 4));
-    ZS_TRY(dev_alloc(ctx, r->trees, tw * 4));
-    DevBuf d_alphas;
-    ZS_TRY(dev_alloc(ctx, d_alphas, size_t(L) * 16));
-    uint32_t *layers = r->layers.as<uint32_t>(), *trees = r->trees.as<uint32_t>();
+    // a kept result owns its layers and trees; otherwise they live in the context's scratch
+    uint32_t *layers, *trees, *alpha_dev;
+    if (out_result) {
+        ZS_TRY(dev_alloc(ctx, r->layers, lw * 4));
+        ZS_TRY(dev_alloc(ctx, r->trees, tw * 4));
+        layers = r->layers.as<uint32_t>();
+        trees = r->trees.as<uint32_t>();
+    } else {
+        ZS_TRY(ctx->ensure_scratch(4, lw * 4));
+        ZS_TRY(ctx->ensure_scratch(5, tw * 4));
+        layers = ctx->scratch[4].as<uint32_t>();
+        trees = ctx->scratch[5].as<uint32_t>();
+    }
+    ZS_TRY(ctx->ensure_scratch(6, size_t(L) * 16));
+    alpha_dev = ctx->scratch[6].as<uint32_t>();
     ZS_HIP(ctx, hipMemcpyAsync(layers, d_codeword, n0 * 16, hipMemcpyDeviceToDevice, ctx->stream));
     if (alphas)
-        ZS_HIP(ctx, hipMemcpyAsync(d_alphas.ptr, alphas, size_t(L) * 16, hipMemcpyHostToDevice, ctx->stream));
+        ZS_HIP(ctx, hipMemcpyAsync(alpha_dev, alphas, size_t(L) * 16, hipMemcpyHostToDevice, ctx->stream));
     ZS_TRY(launch_qm31_leaves(ctx, layers, n0, trees));
     std::vector<uint32_t> host_alpha(size_t(L) * 4);
     for (uint32_t i = 0; i < L; i++) {
@@ -360,11 +370,11 @@ int32_t zs_fri_commit_dev(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t dl
             ZS_HIP(ctx, hipMemcpy(rb, root, 32, hipMemcpyDeviceToHost));
             zs_channel_absorb(chan, rb, 32);
             zs_channel_sample_qm31(chan, &host_alpha[4 * i]);
-            ZS_HIP(ctx, hipMemcpy(d_alphas.as<uint32_t>() + 4 * i, &host_alpha[4 * i], 16, hipMemcpyHostToDevice));
+            ZS_HIP(ctx, hipMemcpy(alpha_dev + 4 * i, &host_alpha[4 * i], 16, hipMemcpyHostToDevice));
         }
         uint32_t *next_leaves = (i + 1 < L) ? trees + r->tree_off[i + 1] : nullptr;
         ZS_TRY(launch_fri_fold(ctx, layers + r->layer_off[i], size_t(1) << li, dlog, i,
-                               d_alphas.as<uint32_t>() + 4 * i, layers + r->layer_off[i + 1], next_leaves));
+                               alpha_dev + 4 * i, layers + r->layer_off[i + 1], next_leaves));
     }
     ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     for (uint32_t i = 0; i < L; i++) {

@@ -23,7 +23,7 @@
 // size-2^log_n transforms per column (q = 0..2^beta-1) on twiddle sub-tables.
 #include <vector>
 
-#include "internal.h"
+#include "fft_common.h"
 
 namespace zs {
 
@@ -118,21 +118,6 @@ int32_t get_canonic_twiddles(Ctx *ctx, uint32_t log_n, const TwiddleTable **out)
 // ---------------------------------------------------------------------------
 // multi-stage LDS pass
 // ---------------------------------------------------------------------------
-enum : uint32_t { IN_KSPACE = 0, IN_SHARED = 1, IN_NATURAL = 2, IN_AB = 3 };
-enum : uint32_t { OUT_KSPACE = 0, OUT_NATURAL = 1, OUT_AB = 2 };
-
-struct FftPass {
-    const uint32_t *in;
-    uint32_t *out;
-    uint64_t in_cs, out_cs;  // column strides (elements)
-    const uint32_t *tw;      // heap table of log size LT
-    uint32_t L, LT, beta;    // transform log size, table log size, blowup log
-    uint32_t s_lo, C, G_log, Q_log;
-    uint32_t in_mode, out_mode;
-    uint32_t scale;          // 0 = none
-    uint32_t natural_tiles;  // tiles chosen so bitrev addresses are adjacent
-};
-
 constexpr int PASS_THREADS = 256;
 
 template <bool INV>
@@ -259,29 +244,7 @@ __global__ __launch_bounds__(PASS_THREADS) void fft_pass_kernel(FftPass p) {
     }
 }
 
-struct PassRange {
-    uint32_t s_lo, C;
-};
-
-static std::vector<PassRange> plan_passes(uint32_t L, bool inverse, uint32_t cmax) {
-    uint32_t np = (L + cmax - 1) / cmax;
-    std::vector<uint32_t> sizes(np, L / np);
-    for (uint32_t i = 0; i < L % np; i++) sizes[i]++;
-    std::vector<PassRange> out;
-    if (inverse) {
-        uint32_t s = 0;
-        for (uint32_t c : sizes) { out.push_back({s, c}); s += c; }
-    } else {
-        uint32_t s = L;
-        for (uint32_t c : sizes) { s -= c; out.push_back({s, c}); }
-    }
-    return out;
-}
-
-constexpr uint32_t PASS_CMAX = 10;
-constexpr uint32_t PASS_ELEMS_LOG = 13;
-
-static int32_t run_pass(Ctx *ctx, FftPass p, bool inverse, uint32_t n_cols) {
+int32_t run_pass(Ctx *ctx, FftPass p, bool inverse, uint32_t n_cols) {
     uint32_t avail = p.s_lo > 0 ? p.s_lo : (p.L - p.C);
     int32_t want = (int32_t)PASS_ELEMS_LOG - (int32_t)p.C - (int32_t)p.Q_log;
     p.G_log = (uint32_t)std::max<int32_t>(0, std::min<int32_t>((int32_t)avail, want));
@@ -415,7 +378,7 @@ static int32_t transform(Ctx *ctx, bool inverse, const zs_circle_coset &hc, cons
             p.natural_tiles = last ? 1 : 0;
             p.scale = 0;
         }
-        ZS_TRY(run_pass(ctx, p, inverse, n_cols));
+        ZS_TRY(run_tile_pass(ctx, p, inverse, n_cols));
     }
     return ZS_OK;
 }
@@ -432,7 +395,7 @@ int32_t launch_ifft(Ctx *ctx, const zs_circle_coset &hc, const uint32_t *d_in, u
 size_t lde_scratch_words(uint32_t n_cols, uint32_t log_n, uint32_t log_blowup, size_t *coef_words) {
     *coef_words = (size_t(n_cols) << log_n);
     uint32_t LT = log_n + log_blowup;
-    bool multi = log_n > PASS_CMAX || LT < 3;
+    bool multi = log_n > PASS_CMAX || LT < 3;  // multi-pass forward, or the tiny pad path
     return multi ? (size_t(n_cols) << LT) : 0;
 }
 
@@ -447,6 +410,60 @@ __global__ void pad_ab_kernel(const uint32_t *in, uint32_t *out, uint32_t n_cols
     if (r < h) v = in[(col << log_n) + r];
     else if (r >= H && r < H + h) v = in[(col << log_n) + h + (r - H)];
     out[i] = v;
+}
+
+// LDE with the register-group kernels: interpolation passes below the top one,
+// the fused top pass (interpolation top + 1/n + every q's evaluation top), then
+// the remaining evaluation passes, the last writing natural order.
+static int32_t launch_lde_tiled(Ctx *ctx, const uint32_t *d_in, uint64_t in_stride, uint32_t n_cols, uint32_t log_n,
+                                uint32_t log_blowup, uint32_t *d_out, uint64_t out_stride, uint32_t *d_coef,
+                                uint32_t *d_big, const zs_circle_coset &hc_small, const zs_circle_coset &hc_big,
+                                const std::vector<PassRange> &pi, const std::vector<PassRange> &pf) {
+    const TwiddleTable *ti, *tf;
+    ZS_TRY(get_twiddles(ctx, hc_small, &ti));
+    ZS_TRY(get_twiddles(ctx, hc_big, &tf));
+    const uint32_t LT = log_n + log_blowup;
+    const size_t n = size_t(1) << log_n, N = size_t(1) << LT;
+    const size_t np = pf.size();
+    if (np == 1)
+        return run_lde_fused(ctx, d_in, in_stride, d_out, out_stride, ti->inv.as<uint32_t>(), tf->fwd.as<uint32_t>(),
+                             log_n, log_blowup, 0, log_n, n_cols);
+    for (size_t i = 0; i + 1 < np; i++) {
+        FftPass p{};
+        bool first = i == 0;
+        p.in = first ? d_in : d_coef;
+        p.in_cs = first ? in_stride : n;
+        p.out = d_coef;
+        p.out_cs = n;
+        p.tw = ti->inv.as<uint32_t>();
+        p.L = log_n; p.LT = log_n; p.beta = 0;
+        p.s_lo = pi[i].s_lo; p.C = pi[i].C; p.Q_log = 0;
+        p.in_mode = first ? IN_NATURAL : IN_KSPACE;
+        p.out_mode = OUT_KSPACE;
+        p.natural_tiles = first ? 1 : 0;
+        p.scale = 0;
+        ZS_TRY(run_tile_pass(ctx, p, true, n_cols));
+    }
+    ZS_TRY(run_lde_fused(ctx, d_coef, n, d_big, N, ti->inv.as<uint32_t>(), tf->fwd.as<uint32_t>(), log_n,
+                         log_blowup, pf[0].s_lo, pf[0].C, n_cols));
+    for (size_t i = 1; i < np; i++) {
+        FftPass p{};
+        bool last = i + 1 == np;
+        p.in = d_big;
+        p.in_cs = N;
+        p.out = last ? d_out : d_big;
+        p.out_cs = last ? out_stride : N;
+        p.tw = tf->fwd.as<uint32_t>();
+        p.L = log_n; p.LT = LT; p.beta = log_blowup;
+        p.s_lo = pf[i].s_lo; p.C = pf[i].C;
+        p.Q_log = last ? std::min<uint32_t>(log_blowup, PASS_ELEMS_LOG > p.C ? PASS_ELEMS_LOG - p.C : 0) : 0;
+        p.in_mode = IN_KSPACE;
+        p.out_mode = last ? OUT_NATURAL : OUT_KSPACE;
+        p.natural_tiles = last ? 1 : 0;
+        p.scale = 0;
+        ZS_TRY(run_tile_pass(ctx, p, false, n_cols));
+    }
+    return ZS_OK;
 }
 
 int32_t launch_lde(Ctx *ctx, const uint32_t *d_in, uint64_t in_stride, uint32_t n_cols, uint32_t log_n,
@@ -466,6 +483,14 @@ int32_t launch_lde(Ctx *ctx, const uint32_t *d_in, uint64_t in_stride, uint32_t 
                            log_n, LT);
         ZS_HIP(ctx, hipGetLastError());
         return run_small(ctx, false, hc_big, d_big, size_t(1) << LT, d_out, out_stride, n_cols, false);
+    }
+    {
+        auto pi = plan_passes(log_n, true, PASS_CMAX);
+        auto pf = plan_passes(log_n, false, PASS_CMAX);
+        bool tiled = log_n >= 8 && lde_fused_ok(pf[0].C, log_blowup, pf.size() == 1);
+        for (auto &r : pf) tiled = tiled && tile_pass_ok(r.C);
+        if (tiled) return launch_lde_tiled(ctx, d_in, in_stride, n_cols, log_n, log_blowup, d_out, out_stride,
+                                           d_coef, d_big, hc_small, hc_big, pi, pf);
     }
     // 1) interpolate: natural evaluations -> interleaved coefficients scaled by 1/n (k-space)
     if (log_n < 3) {
@@ -488,7 +513,7 @@ int32_t launch_lde(Ctx *ctx, const uint32_t *d_in, uint64_t in_stride, uint32_t 
             p.out_mode = OUT_KSPACE;
             p.natural_tiles = first ? 1 : 0;
             p.scale = last ? m31_inv((uint32_t)n) : 0;
-            ZS_TRY(run_pass(ctx, p, true, n_cols));
+            ZS_TRY(run_tile_pass(ctx, p, true, n_cols));
         }
     }
     // 2) evaluate on the 2^beta quarter transforms of the big canonic domain
@@ -513,7 +538,7 @@ int32_t launch_lde(Ctx *ctx, const uint32_t *d_in, uint64_t in_stride, uint32_t 
         p.out_mode = last ? OUT_NATURAL : OUT_KSPACE;
         p.natural_tiles = last ? 1 : 0;
         p.scale = 0;
-        ZS_TRY(run_pass(ctx, p, false, n_cols));
+        ZS_TRY(run_tile_pass(ctx, p, false, n_cols));
     }
     return ZS_OK;
 }

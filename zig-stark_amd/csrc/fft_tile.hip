@@ -1,0 +1,540 @@
+// fft_tile.hip -- register-group circle-FFT passes for gfx950.
+//
+// A pass covers C consecutive butterfly stages (8 <= C <= 11) of a tile of
+// 2^C elements.  The workgroup stages its tiles through LDS with coalesced
+// global accesses; then each wave transforms one tile with its 64 lanes
+// holding R = 2^(C-6) elements each.  The C stages are split into groups of
+// log2 R stages: a group's stages run entirely in registers (the group's bits
+// of the element index come from the lane's register index j, the other bits
+// from the lane id), and the tile goes through LDS only between groups, so a
+// 10-stage pass makes three LDS round trips instead of ten.  Twiddles for a
+// stage are loaded once per distinct index (2^r - 1 loads per r-stage group
+// per lane, shared by r * 2^(r-1) butterflies).
+//
+// The fused LDE kernels run the interpolation's top pass, scale by 1/n, and
+// then the 2^beta forward transforms' top pass straight from the registers
+// (the interpolation's last group and the evaluation's first group cover the
+// same index bits), so the interleaved coefficients of those stages never go
+// to HBM.
+#include "fft_common.h"
+
+namespace zs {
+
+template <int C>
+struct TC {
+    static constexpr int RB = C - 6;  // log2 elements per lane
+    static constexpr int R = 1 << RB;
+    static constexpr int T = 1 << C;
+    static constexpr int TS = T + (T >> 4) + 1;  // padded tile stride in LDS (words)
+    static constexpr int NG = (C + RB - 1) / RB;  // register groups per pass
+};
+
+// LDS index of tile element t: one pad word per 16 keeps the group access
+// patterns (stride 1, 4 and 16 across lanes) free of bank conflicts
+__device__ __forceinline__ uint32_t lidx(uint32_t t) { return t + (t >> 4); }
+
+// element index of register j of `lane` in a group covering bits [LO, LO + r):
+// the group bits are j's low r bits, the other C - r bits are (lane, j >> r)
+template <int RB, int LO, int r>
+__device__ __forceinline__ uint32_t t_of(uint32_t lane, uint32_t j) {
+    const uint32_t g = j & ((1u << r) - 1);
+    const uint32_t other = (lane << (RB - r)) | (j >> r);
+    return ((other >> LO) << (LO + r)) | (g << LO) | (other & ((1u << LO) - 1));
+}
+
+// LDS writes of a wave must land before its lanes read each other's values
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+template <int C, int LO, int r>
+__device__ __forceinline__ void g_load(uint32_t (&v)[TC<C>::R], const uint32_t *tl, uint32_t lane) {
+#pragma unroll
+    for (int j = 0; j < TC<C>::R; j++) v[j] = tl[lidx(t_of<TC<C>::RB, LO, r>(lane, j))];
+}
+template <int C, int LO, int r>
+__device__ __forceinline__ void g_store(const uint32_t (&v)[TC<C>::R], uint32_t *tl, uint32_t lane) {
+#pragma unroll
+    for (int j = 0; j < TC<C>::R; j++) tl[lidx(t_of<TC<C>::RB, LO, r>(lane, j))] = v[j];
+}
+
+// one butterfly stage on local bit LO + b.  twl is the tile set's twiddles in
+// LDS, heap layout: local stage ls at [2^(C-1-ls), 2^(C-ls)), entry i being the
+// twiddle of the pairs whose bits above ls are i.
+template <bool INV, int C, int LO, int r, int b>
+__device__ __forceinline__ void g_stage(uint32_t (&v)[TC<C>::R], uint32_t lane, const uint32_t *twl) {
+    constexpr int RB = TC<C>::RB;
+    constexpr int ls = LO + b;
+    const uint32_t *tws = twl + (1u << (C - 1 - ls));
+#pragma unroll
+    for (int o = 0; o < (1 << (RB - r)); o++) {
+#pragma unroll
+        for (int u = 0; u < (1 << (r - 1 - b)); u++) {
+            const uint32_t jb = ((uint32_t)o << r) | ((uint32_t)u << (b + 1));
+            const uint32_t t0 = t_of<RB, LO, r>(lane, jb);
+            const uint32_t w = tws[t0 >> (ls + 1)];
+#pragma unroll
+            for (int l = 0; l < (1 << b); l++) {
+                const int j0 = (int)jb | l, j1 = j0 | (1 << b);
+                const uint32_t a = v[j0];
+                if (INV) {
+                    v[j0] = m31_add(a, v[j1]);
+                    v[j1] = m31_mul(m31_sub(a, v[j1]), w);
+                } else {
+                    const uint32_t m = m31_mul(v[j1], w);
+                    v[j0] = m31_add(a, m);
+                    v[j1] = m31_sub(a, m);
+                }
+            }
+        }
+    }
+}
+
+template <bool INV, int C, int LO, int r, int i>
+__device__ __forceinline__ void g_stages(uint32_t (&v)[TC<C>::R], uint32_t lane, const uint32_t *twl) {
+    constexpr int b = INV ? i : (r - 1 - i);
+    g_stage<INV, C, LO, r, b>(v, lane, twl);
+    if constexpr (i + 1 < r) g_stages<INV, C, LO, r, i + 1>(v, lane, twl);
+}
+
+// group GI in execution order (forward: top bits first; inverse: bottom first)
+template <bool INV, int C, int GI>
+struct Grp {
+    static constexpr int RB = TC<C>::RB, NG = TC<C>::NG;
+    static constexpr int GF = INV ? (NG - 1 - GI) : GI;
+    static constexpr int HI = C - 1 - GF * RB;
+    static constexpr int LO = (C - (GF + 1) * RB) > 0 ? (C - (GF + 1) * RB) : 0;
+    static constexpr int r = HI - LO + 1;
+};
+
+// groups GI.. of a tile held in LDS
+template <bool INV, int C, int GI>
+__device__ __forceinline__ void tile_run(uint32_t *tl, uint32_t lane, const uint32_t *twl) {
+    using Gp = Grp<INV, C, GI>;
+    uint32_t v[TC<C>::R];
+    g_load<C, Gp::LO, Gp::r>(v, tl, lane);
+    g_stages<INV, C, Gp::LO, Gp::r, 0>(v, lane, twl);
+    g_store<C, Gp::LO, Gp::r>(v, tl, lane);
+    wave_lds_sync();
+    if constexpr (GI + 1 < TC<C>::NG) tile_run<INV, C, GI + 1>(tl, lane, twl);
+}
+
+// Cooperatively copy the 2^C - 1 twiddles a tile set needs into LDS (heap
+// layout).  Stage s = s_lo + ls of the global table (log size LT) is read at
+// (Kbase >> (s+1)) + i: the tile's fixed bits select one contiguous segment
+// per stage, so the copy is coalesced.
+template <int C>
+__device__ __forceinline__ void stage_twiddles(uint32_t *twl, const uint32_t *tw, uint32_t LT, uint32_t s_lo,
+                                               uint32_t Kbase) {
+    for (uint32_t e = threadIdx.x; e + 1 < (1u << C); e += blockDim.x) {
+        uint32_t idx = e + 1;
+        uint32_t lvl = 31 - __builtin_clz(idx);
+        uint32_t ls = C - 1 - lvl;
+        uint32_t s = s_lo + ls;
+        twl[idx] = tw[(1u << (LT - 1 - s)) + (Kbase >> (s + 1)) + (idx - (1u << lvl))];
+    }
+}
+
+// workgroup -> tile group, remapped so consecutive tile groups share an XCD
+// (workgroups are dealt round-robin over the 8 XCDs, each with its own L2)
+__device__ __forceinline__ uint32_t xcd_block(uint32_t bid, uint32_t nblk) {
+    if (nblk % 8) return bid;
+    return (bid & 7u) * (nblk >> 3) + (bid >> 3);
+}
+
+struct TileGeom {
+    uint32_t hi, lo0, tau0;
+};
+
+__device__ __forceinline__ TileGeom tile_geom(const FftPass &p, uint32_t blk) {
+    TileGeom g{0, 0, 0};
+    if (p.s_lo > 0) {
+        uint32_t lo_groups = (1u << p.s_lo) >> p.G_log;
+        g.hi = blk / lo_groups;
+        g.lo0 = (blk % lo_groups) << p.G_log;
+    } else {
+        g.tau0 = blk << p.G_log;
+    }
+    return g;
+}
+__device__ __forceinline__ uint32_t tile_fixed(const FftPass &p, const TileGeom &tg, uint32_t g) {
+    if (p.s_lo > 0) return (tg.hi << (p.s_lo + p.C)) | (tg.lo0 + g);
+    uint32_t h = tg.tau0 + g;
+    if (p.natural_tiles) h = bitrev32(h, p.L - p.C);
+    return h << p.C;
+}
+
+// cooperative global -> LDS load of the workgroup's G * 2^qload tiles
+template <int C>
+__device__ __forceinline__ void coop_load(const FftPass &p, const TileGeom &tg, uint32_t *lds, uint32_t col,
+                                          uint32_t qblk, uint32_t nq_log, bool replicate_q) {
+    constexpr uint32_t T = 1u << C, TS = TC<C>::TS;
+    const uint32_t G = 1u << p.G_log, Q = 1u << p.Q_log;
+    const bool g_fast = (p.s_lo > 0) || p.natural_tiles;
+    const uint32_t half_n = 1u << (p.L - 1);
+    const uint32_t *in_col = p.in + (uint64_t)col * p.in_cs;
+    const uint32_t nload = (G << nq_log) << C;
+    for (uint32_t e = threadIdx.x; e < nload; e += blockDim.x) {
+        uint32_t g, t, ql;
+        if (g_fast) {
+            g = e & (G - 1);
+            uint32_t rr = e >> p.G_log;
+            ql = rr & ((1u << nq_log) - 1);
+            t = rr >> nq_log;
+        } else {
+            t = e & (T - 1);
+            uint32_t rr = e >> C;
+            g = rr & (G - 1);
+            ql = rr >> p.G_log;
+        }
+        uint32_t k = tile_fixed(p, tg, g) | (t << p.s_lo);
+        uint64_t addr;
+        switch (p.in_mode) {
+            case IN_KSPACE: addr = ((uint64_t)(qblk + ql) << p.L) | k; break;
+            case IN_SHARED: addr = k; break;
+            case IN_NATURAL: addr = bitrev32(k, p.L); break;
+            default: addr = (k & 1u) ? (half_n + (k >> 1)) : (k >> 1); break;
+        }
+        uint32_t v = in_col[addr];
+        if (replicate_q) {
+            for (uint32_t qq = 0; qq < Q; qq++) lds[(qq * G + g) * TS + lidx(t)] = v;
+        } else {
+            lds[(ql * G + g) * TS + lidx(t)] = v;
+        }
+    }
+}
+
+// cooperative LDS -> global store of nq tiles-per-g (q runs over [qblk, qblk + 2^nq_log))
+template <int C>
+__device__ __forceinline__ void coop_store(const FftPass &p, const TileGeom &tg, const uint32_t *lds, uint32_t col,
+                                           uint32_t qblk, uint32_t nq_log) {
+    constexpr uint32_t T = 1u << C, TS = TC<C>::TS;
+    const uint32_t G = 1u << p.G_log, NQ = 1u << nq_log;
+    const bool g_fast = (p.s_lo > 0) || p.natural_tiles;
+    const uint32_t half_n = 1u << (p.L - 1);
+    uint32_t *out_col = p.out + (uint64_t)col * p.out_cs;
+    const uint32_t nstore = (G * NQ) << C;
+    for (uint32_t e = threadIdx.x; e < nstore; e += blockDim.x) {
+        uint32_t g, t, ql;
+        if (p.out_mode == OUT_NATURAL) {
+            uint32_t rq = e & (NQ - 1);
+            g = (e >> nq_log) & (G - 1);
+            t = e >> (nq_log + p.G_log);
+            ql = bitrev32(rq, nq_log);
+        } else if (g_fast) {
+            g = e & (G - 1);
+            uint32_t rr = e >> p.G_log;
+            ql = rr & (NQ - 1);
+            t = rr >> nq_log;
+        } else {
+            t = e & (T - 1);
+            uint32_t rr = e >> C;
+            g = rr & (G - 1);
+            ql = rr >> p.G_log;
+        }
+        uint32_t v = lds[(ql * G + g) * TS + lidx(t)];
+        if (p.scale) v = m31_mul(v, p.scale);
+        uint32_t k = tile_fixed(p, tg, g) | (t << p.s_lo);
+        uint32_t q = qblk + ql;
+        uint64_t addr;
+        switch (p.out_mode) {
+            case OUT_KSPACE: addr = ((uint64_t)q << p.L) | k; break;
+            case OUT_NATURAL: addr = ((uint64_t)bitrev32(k, p.L) << p.beta) | bitrev32(q, p.beta); break;
+            default: addr = (k & 1u) ? (half_n + (k >> 1)) : (k >> 1); break;
+        }
+        out_col[addr] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// one pass (same contract as fft_pass_kernel)
+// ---------------------------------------------------------------------------
+template <int C>
+constexpr int tile_block_max() { return C >= 11 ? 512 : 1024; }
+
+template <bool INV, int C>
+__global__ __launch_bounds__(tile_block_max<C>()) void fft_tile_kernel(FftPass p) {
+    extern __shared__ uint32_t lds[];
+    constexpr uint32_t TS = TC<C>::TS;
+    const uint32_t G = 1u << p.G_log, Q = 1u << p.Q_log;
+    const uint32_t qblk = blockIdx.z << p.Q_log;
+    const TileGeom tg = tile_geom(p, xcd_block(blockIdx.x, gridDim.x));
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nwaves = blockDim.x >> 6;
+    // tiles of a strided pass share their twiddles per q; contiguous-pass tiles each have their own.
+    // The sets are staged once and reused for the workgroup's ncb columns.
+    const bool per_tile_tw = p.s_lo == 0;
+    const uint32_t nsets = per_tile_tw ? Q * G : Q;
+    uint32_t *twl = lds + Q * G * TS;
+    for (uint32_t set = 0; set < nsets; set++) {
+        uint32_t ql = per_tile_tw ? (set >> p.G_log) : set;
+        uint32_t g = per_tile_tw ? (set & (G - 1)) : 0;
+        uint32_t Kbase = ((qblk + ql) << p.L) | tile_fixed(p, tg, g);
+        stage_twiddles<C>(twl + (set << C), p.tw, p.LT, p.s_lo, Kbase);
+    }
+    const bool shared_in = p.in_mode == IN_SHARED;
+    const uint32_t c0 = blockIdx.y * p.ncb;
+    const uint32_t c1 = min(c0 + p.ncb, p.n_cols);
+    for (uint32_t col = c0; col < c1; col++) {
+        coop_load<C>(p, tg, lds, col, qblk, shared_in ? 0 : p.Q_log, shared_in);
+        __syncthreads();
+        for (uint32_t tile = wave; tile < Q * G; tile += nwaves) {
+            uint32_t set = per_tile_tw ? tile : (tile >> p.G_log);
+            tile_run<INV, C, 0>(lds + tile * TS, lane, twl + (set << C));
+        }
+        __syncthreads();
+        coop_store<C>(p, tg, lds, col, qblk, p.Q_log);
+        __syncthreads();
+    }
+}
+
+template <int C>
+static size_t tile_lds_bytes(uint32_t ntiles, uint32_t nsets) {
+    return (size_t(ntiles) * TC<C>::TS + (size_t(nsets) << C)) * 4;
+}
+
+// Launch-shape knobs (read once from the environment; defaults from measurement):
+//   ZS_TILE_ELEMS_LOG: elements of the tiles one workgroup transforms (default 2^12)
+//   ZS_COLS_PER_WG:    columns a workgroup runs with one set of staged twiddles (default 4)
+static int env_int(const char *name, int dflt, int lo, int hi) {
+    const char *e = getenv(name);
+    int x = e ? atoi(e) : dflt;
+    return std::min(hi, std::max(lo, x));
+}
+static uint32_t tile_elems_log() {
+    static uint32_t v = (uint32_t)env_int("ZS_TILE_ELEMS_LOG", 12, 11, 15);
+    return v;
+}
+static uint32_t cols_per_wg() {
+    static uint32_t v = (uint32_t)env_int("ZS_COLS_PER_WG", 4, 1, 64);
+    return v;
+}
+// columns per workgroup, reduced so the grid still has >= 2048 workgroups
+static uint32_t pick_ncb(uint32_t n_cols, uint32_t wgs_per_col) {
+    uint32_t ncb = std::min(cols_per_wg(), std::max<uint32_t>(n_cols, 1));
+    while (ncb > 1 && uint64_t(wgs_per_col) * ((n_cols + ncb - 1) / ncb) < 2048) ncb >>= 1;
+    return ncb;
+}
+
+int32_t run_tile_pass(Ctx *ctx, FftPass p, bool inverse, uint32_t n_cols) {
+    uint32_t avail = p.s_lo > 0 ? p.s_lo : (p.L - p.C);
+    int32_t want = (int32_t)tile_elems_log() - (int32_t)p.C - (int32_t)p.Q_log;
+    p.G_log = (uint32_t)std::max<int32_t>(0, std::min<int32_t>((int32_t)avail, want));
+    uint32_t tiles = 1u << (p.L - p.C);
+    uint32_t ntile = 1u << (p.G_log + p.Q_log);
+    uint32_t nsets = p.s_lo == 0 ? ntile : (1u << p.Q_log);
+    uint32_t gx = tiles >> p.G_log, gz = (1u << p.beta) >> p.Q_log;
+    p.n_cols = n_cols;
+    p.ncb = pick_ncb(n_cols, gx * gz);
+    dim3 grid(gx, (n_cols + p.ncb - 1) / p.ncb, gz);
+    uint32_t maxw = p.C >= 11 ? 8 : 16;
+    dim3 block(64 * std::min<uint32_t>(ntile, maxw));
+#define ZS_TILE_CASE(CC)                                                                                         \
+    case CC: {                                                                                                   \
+        size_t lds = tile_lds_bytes<CC>(ntile, nsets);                                                           \
+        if (lds > 160 * 1024) return run_pass(ctx, p, inverse, n_cols);                                          \
+        if (inverse) hipLaunchKernelGGL((fft_tile_kernel<true, CC>), grid, block, lds, ctx->stream, p);          \
+        else hipLaunchKernelGGL((fft_tile_kernel<false, CC>), grid, block, lds, ctx->stream, p);                 \
+        break;                                                                                                   \
+    }
+    switch (p.C) {
+        ZS_TILE_CASE(8)
+        ZS_TILE_CASE(9)
+        ZS_TILE_CASE(10)
+        ZS_TILE_CASE(11)
+        default: return run_pass(ctx, p, inverse, n_cols);
+    }
+#undef ZS_TILE_CASE
+    ZS_HIP(ctx, hipGetLastError());
+    return ZS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// fused LDE: interpolation top pass + scale + forward top pass for every q
+// ---------------------------------------------------------------------------
+// Forward groups 0.. of a tile whose top-group values are in registers.
+template <int C>
+__device__ __forceinline__ void fwd_from_regs(uint32_t (&v)[TC<C>::R], uint32_t *tl, uint32_t lane,
+                                              const uint32_t *twl) {
+    using G0 = Grp<false, C, 0>;
+    g_stages<false, C, G0::LO, G0::r, 0>(v, lane, twl);
+    g_store<C, G0::LO, G0::r>(v, tl, lane);
+    wave_lds_sync();
+    if constexpr (TC<C>::NG > 1) tile_run<false, C, 1>(tl, lane, twl);
+}
+
+struct LdeFused {
+    FftPass p;               // geometry: L, C, s_lo, G_log, beta; in/out/strides; in_mode/out_mode
+    const uint32_t *tw_inv;  // table of log L (inverse)
+    const uint32_t *tw_fwd;  // table of log L + beta (forward)
+    uint32_t inv_n;
+};
+
+// s_lo > 0: G tiles per workgroup (one wave each).  Twiddle sets in LDS: the
+// interpolation's (shared by the G tiles) and one per q for the evaluation.
+// Per q the workgroup's G output tiles are stored together (k-space).
+template <int C>
+__global__ __launch_bounds__(tile_block_max<C>()) void lde_mid_kernel(LdeFused f) {
+    extern __shared__ uint32_t lds[];
+    constexpr uint32_t TS = TC<C>::TS;
+    const FftPass &p = f.p;
+    const TileGeom tg = tile_geom(p, xcd_block(blockIdx.x, gridDim.x));
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t G = 1u << p.G_log;  // == number of waves
+    const uint32_t NQ = 1u << p.beta;
+    uint32_t *twl = lds + G * TS;  // set 0: interpolation, set 1 + q: evaluation of q
+    const uint32_t hi_bits = tile_fixed(p, tg, 0) & ~((1u << p.s_lo) - 1);
+    stage_twiddles<C>(twl, f.tw_inv, p.L, p.s_lo, hi_bits);
+    for (uint32_t q = 0; q < NQ; q++)
+        stage_twiddles<C>(twl + ((1 + q) << C), f.tw_fwd, p.L + p.beta, p.s_lo, (q << p.L) | hi_bits);
+    using Top = Grp<false, C, 0>;
+    uint32_t *tl = lds + wave * TS;
+    const uint32_t c0 = blockIdx.y * p.ncb;
+    const uint32_t c1 = min(c0 + p.ncb, p.n_cols);
+    for (uint32_t col = c0; col < c1; col++) {
+        coop_load<C>(p, tg, lds, col, 0, 0, false);
+        __syncthreads();
+        uint32_t c[TC<C>::R];
+        if (wave < G) {
+            tile_run<true, C, 0>(tl, lane, twl);
+            g_load<C, Top::LO, Top::r>(c, tl, lane);
+#pragma unroll
+            for (int j = 0; j < TC<C>::R; j++) c[j] = m31_mul(c[j], f.inv_n);
+        }
+        for (uint32_t q = 0; q < NQ; q++) {
+            __syncthreads();  // the previous store (or the interpolation's reads) are done with the tiles
+            if (wave < G) {
+                uint32_t v[TC<C>::R];
+#pragma unroll
+                for (int j = 0; j < TC<C>::R; j++) v[j] = c[j];
+                fwd_from_regs<C>(v, tl, lane, twl + ((1 + q) << C));
+            }
+            __syncthreads();
+            coop_store<C>(p, tg, lds, col, q, 0);
+        }
+        __syncthreads();
+    }
+}
+
+// s_lo == 0 and L == C: the whole LDE of a column in one workgroup; wave 0
+// interpolates, every wave evaluates some q, all 2^beta outputs leave together
+template <int C>
+__global__ __launch_bounds__(256) void lde_single_kernel(LdeFused f) {
+    extern __shared__ uint32_t lds[];
+    constexpr uint32_t TS = TC<C>::TS;
+    const FftPass &p = f.p;
+    const TileGeom tg{0, 0, 0};
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nwaves = blockDim.x >> 6;
+    const uint32_t NQ = 1u << p.beta;
+    uint32_t *src = lds + NQ * TS;       // the interpolation's tile, after the NQ output tiles
+    uint32_t *twl = lds + (NQ + 1) * TS;  // set 0: interpolation, set 1 + q: evaluation of q
+    stage_twiddles<C>(twl, f.tw_inv, p.L, 0, 0);
+    for (uint32_t q = 0; q < NQ; q++) stage_twiddles<C>(twl + ((1 + q) << C), f.tw_fwd, p.L + p.beta, 0, q << p.L);
+    using Top = Grp<false, C, 0>;
+    const uint32_t c0 = blockIdx.y * p.ncb;
+    const uint32_t c1 = min(c0 + p.ncb, p.n_cols);
+    for (uint32_t col = c0; col < c1; col++) {
+        coop_load<C>(p, tg, src, col, 0, 0, false);
+        __syncthreads();
+        if (wave == 0) tile_run<true, C, 0>(src, lane, twl);
+        __syncthreads();
+        uint32_t c[TC<C>::R];
+        g_load<C, Top::LO, Top::r>(c, src, lane);
+#pragma unroll
+        for (int j = 0; j < TC<C>::R; j++) c[j] = m31_mul(c[j], f.inv_n);
+        for (uint32_t q = wave; q < NQ; q += nwaves) {
+            uint32_t v[TC<C>::R];
+#pragma unroll
+            for (int j = 0; j < TC<C>::R; j++) v[j] = c[j];
+            fwd_from_regs<C>(v, lds + q * TS, lane, twl + ((1 + q) << C));
+        }
+        __syncthreads();
+        coop_store<C>(p, tg, lds, col, 0, p.beta);
+        __syncthreads();
+    }
+}
+
+static size_t TC_TS(uint32_t C) { return (size_t(1) << C) + (size_t(1) << (C - 4)) + 1; }
+
+bool lde_fused_ok(uint32_t C, uint32_t beta, bool single) {
+    if (!tile_pass_ok(C) || beta > 8) return false;
+    if (!single) return true;
+    // the single-workgroup form keeps 2^beta output tiles, the interpolation tile and
+    // 1 + 2^beta twiddle sets in LDS
+    size_t ts = (size_t(1) << C) + (size_t(1) << (C - 4)) + 1;
+    size_t nq = size_t(1) << beta;
+    return ((nq + 1) * ts + ((nq + 1) << C)) * 4 <= 160 * 1024;
+}
+
+int32_t run_lde_fused(Ctx *ctx, const uint32_t *d_in, uint64_t in_cs, uint32_t *d_out, uint64_t out_cs,
+                      const uint32_t *tw_inv, const uint32_t *tw_fwd, uint32_t L, uint32_t beta, uint32_t s_lo,
+                      uint32_t C, uint32_t n_cols) {
+    if (!tile_pass_ok(C) || s_lo + C != L) return ZS_ERR_INPUT;
+    LdeFused f{};
+    FftPass &p = f.p;
+    p.in = d_in;
+    p.in_cs = in_cs;
+    p.out = d_out;
+    p.out_cs = out_cs;
+    p.L = L;
+    p.LT = L;
+    p.beta = beta;
+    p.s_lo = s_lo;
+    p.C = C;
+    p.scale = 0;
+    f.tw_inv = tw_inv;
+    f.tw_fwd = tw_fwd;
+    f.inv_n = m31_inv(uint32_t(1) << L);
+    dim3 grid;
+    dim3 block;
+    size_t lds;
+    bool single = s_lo == 0;
+    if (single) {
+        if (!lde_fused_ok(L, beta, true)) return ZS_ERR_INPUT;
+        p.G_log = 0;
+        p.Q_log = beta;
+        p.in_mode = IN_NATURAL;
+        p.out_mode = OUT_NATURAL;
+        p.natural_tiles = 1;
+        p.n_cols = n_cols;
+        p.ncb = pick_ncb(n_cols, 1);
+        grid = dim3(1, (n_cols + p.ncb - 1) / p.ncb, 1);
+        block = dim3(64 * std::min<uint32_t>(1u << beta, 4));
+        lds = ((size_t(1) << beta) + 1) * TC_TS(C) + (((size_t(1) << beta) + 1) << C);
+    } else {
+        // one wave per tile, at most 8 (C = 11) or 16 tiles per workgroup
+        int32_t want = std::min<int32_t>((int32_t)tile_elems_log() - (int32_t)C, C >= 11 ? 3 : 4);
+        p.G_log = (uint32_t)std::max<int32_t>(0, std::min<int32_t>((int32_t)s_lo, want));
+        p.Q_log = 0;
+        p.in_mode = IN_KSPACE;
+        p.out_mode = OUT_KSPACE;
+        p.natural_tiles = 0;
+        uint32_t gx = (1u << (L - C)) >> p.G_log;
+        p.n_cols = n_cols;
+        p.ncb = pick_ncb(n_cols, gx);
+        grid = dim3(gx, (n_cols + p.ncb - 1) / p.ncb, 1);
+        block = dim3(64u << p.G_log);
+        lds = (size_t(1) << p.G_log) * TC_TS(C) + (((size_t(1) << beta) + 1) << C);
+        if (lds * 4 > 160 * 1024) return ZS_ERR_INPUT;
+    }
+#define ZS_FUSED_CASE(CC)                                                                                          \
+    case CC:                                                                                                       \
+        if (single)                                                                                                \
+            hipLaunchKernelGGL((lde_single_kernel<CC>), grid, block, lds * 4, ctx->stream, f);                     \
+        else                                                                                                       \
+            hipLaunchKernelGGL((lde_mid_kernel<CC>), grid, block, lds * 4, ctx->stream, f);                        \
+        break;
+    switch (C) {
+        ZS_FUSED_CASE(8)
+        ZS_FUSED_CASE(9)
+        ZS_FUSED_CASE(10)
+        ZS_FUSED_CASE(11)
+        default: return ZS_ERR_INPUT;
+    }
+#undef ZS_FUSED_CASE
+    ZS_HIP(ctx, hipGetLastError());
+    return ZS_OK;
+}
+
+}  // namespace zs

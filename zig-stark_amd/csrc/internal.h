@@ -62,7 +62,7 @@ struct Ctx {
     // FRI inverse-domain tables keyed by dlog: [lo table 2^10 | hi table] QM31 words
     std::map<uint32_t, std::unique_ptr<DevBuf>> fri_tables;
     // grow-only scratch arenas
-    DevBuf scratch[4];
+    DevBuf scratch[8];  // 0 transforms, 1-2 LDE, 3 plain path, 4-6 FRI
 
     void set_error(const std::string &s) { last_error = s; }
     int32_t ensure_scratch(int slot, size_t bytes);

@@ -10,9 +10,9 @@
 // rows, so every column load is a coalesced 256-byte wave access) and
 // compresses 16 columns per BLAKE3 block; rows of more than 256 columns span
 // several 1024-byte chunks and are merged with BLAKE3's chunk tree.  Node
-// levels are built bottom-up; the lowest levels are hashed inside a wave
-// (a lane keeps its node, the partner's digest arrives by lane shuffle) so
-// that a launch writes several levels.
+// levels are built bottom-up, three per launch: a lane owns an 8-child
+// subtree and hashes all 7 of its nodes, so no lane idles while the level
+// narrows; the last 12 levels run in one workgroup.
 #include "blake3.h"
 #include "internal.h"
 
@@ -108,47 +108,69 @@ __global__ __launch_bounds__(256) void qm31_leaf_kernel(const uint32_t *vals, ui
     o[1] = make_uint4(d[4], d[5], d[6], d[7]);
 }
 
-// Up to `nlev` levels per launch: lane i hashes node i of the first new level
-// from two children in memory, then pairs of lanes combine by shuffle.
-__global__ __launch_bounds__(256) void tree_levels_kernel(uint32_t *child_level, uint32_t *parent_level,
-                                                          uint64_t n_parents, uint32_t nlev, uint64_t n_parents_total) {
+__device__ __forceinline__ void ld_digest(const uint32_t *p, uint32_t d[8]) {
+    const uint4 *q = reinterpret_cast<const uint4 *>(p);
+    uint4 a = q[0], b = q[1];
+    d[0] = a.x; d[1] = a.y; d[2] = a.z; d[3] = a.w; d[4] = b.x; d[5] = b.y; d[6] = b.z; d[7] = b.w;
+}
+__device__ __forceinline__ void st_digest(uint32_t *p, const uint32_t d[8]) {
+    uint4 *q = reinterpret_cast<uint4 *>(p);
+    q[0] = make_uint4(d[0], d[1], d[2], d[3]);
+    q[1] = make_uint4(d[4], d[5], d[6], d[7]);
+}
+
+// NLEV (1..3) levels per launch with no idle lanes: lane i owns the subtree
+// over children 2^NLEV * i .. 2^NLEV * (i + 1) - 1 and writes all its nodes.
+template <int NLEV>
+__global__ __launch_bounds__(256) void tree_subtree_kernel(const uint32_t *child, uint32_t *lvl1, uint64_t n_lanes) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool active = i < n_parents;
-    uint32_t d[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (active) {
-        const uint4 *c = reinterpret_cast<const uint4 *>(child_level + i * 16);
-        uint4 a0 = c[0], a1 = c[1], b0 = c[2], b1 = c[3];
-        uint32_t l[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
-        uint32_t r[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
-        b3_hash2(l, r, d);
-        uint4 *o = reinterpret_cast<uint4 *>(parent_level + i * 8);
-        o[0] = make_uint4(d[0], d[1], d[2], d[3]);
-        o[1] = make_uint4(d[4], d[5], d[6], d[7]);
-    }
-    // further levels inside the wave: node i at this level pairs with i ^ 1
-    uint32_t *lvl = parent_level;
-    uint64_t cnt = n_parents;
-    uint64_t idx = i;
-    for (uint32_t k = 1; k < nlev; k++) {
-        uint32_t partner[8];
-        uint32_t lane_off = 1u << (k - 1);  // partner's lane distance at level k-1
+    if (i >= n_lanes) return;
+    constexpr int W = 1 << NLEV;
+    uint32_t *lvl2 = lvl1 + n_lanes * (W / 2) * 8;
+    uint32_t *lvl3 = lvl2 + n_lanes * (W / 4) * 8;
+    uint32_t a[8], b[8], n1[W / 2][8];
 #pragma unroll
-        for (int w = 0; w < 8; w++) partner[w] = __shfl_xor(d[w], lane_off);
-        uint64_t next_cnt = cnt >> 1;
-        uint32_t *next_lvl = lvl + cnt * 8;
-        bool left = ((idx >> (k - 1)) & 1u) == 0 && ((i & ((1u << (k - 1)) - 1)) == 0);
-        if (active && left && (idx >> k) < next_cnt) {
-            uint32_t t[8];
-            b3_hash2(d, partner, t);
-            for (int w = 0; w < 8; w++) d[w] = t[w];
-            uint4 *o = reinterpret_cast<uint4 *>(next_lvl + (idx >> k) * 8);
-            o[0] = make_uint4(d[0], d[1], d[2], d[3]);
-            o[1] = make_uint4(d[4], d[5], d[6], d[7]);
-        }
-        lvl = next_lvl;
-        cnt = next_cnt;
+    for (int k = 0; k < W / 2; k++) {
+        ld_digest(child + (i * W + 2 * k) * 8, a);
+        ld_digest(child + (i * W + 2 * k + 1) * 8, b);
+        b3_hash2(a, b, n1[k]);
+        st_digest(lvl1 + (i * (W / 2) + k) * 8, n1[k]);
     }
-    (void)n_parents_total;
+    if constexpr (NLEV >= 2) {
+        uint32_t n2[W / 4][8];
+#pragma unroll
+        for (int k = 0; k < W / 4; k++) {
+            b3_hash2(n1[2 * k], n1[2 * k + 1], n2[k]);
+            st_digest(lvl2 + (i * (W / 4) + k) * 8, n2[k]);
+        }
+        if constexpr (NLEV >= 3) {
+            uint32_t n3[8];
+            b3_hash2(n2[0], n2[1], n3);
+            st_digest(lvl3 + i * 8, n3);
+        }
+    }
+}
+
+// the top levels in one workgroup: one node per thread per level, levels
+// separated by a device-scope fence and a barrier
+__global__ __launch_bounds__(1024) void tree_tail_kernel(uint32_t *child, uint32_t log_children) {
+    uint32_t *lvl = child;
+    uint32_t cnt = 1u << log_children;
+    for (uint32_t l = 0; l < log_children; l++) {
+        uint32_t *next = lvl + (uint64_t)cnt * 8;
+        uint32_t half = cnt >> 1;
+        for (uint32_t i = threadIdx.x; i < half; i += blockDim.x) {
+            uint32_t a[8], b[8], d[8];
+            ld_digest(lvl + (uint64_t)(2 * i) * 8, a);
+            ld_digest(lvl + (uint64_t)(2 * i + 1) * 8, b);
+            b3_hash2(a, b, d);
+            st_digest(next + (uint64_t)i * 8, d);
+        }
+        __threadfence();
+        __syncthreads();
+        lvl = next;
+        cnt = half;
+    }
 }
 
 // Build levels from_level+1 .. top over level from_level (already written).
@@ -156,15 +178,20 @@ int32_t launch_tree_levels(Ctx *ctx, uint32_t *d_nodes, uint32_t log_leaves, uin
     uint32_t level = from_level;
     while (level < log_leaves) {
         uint32_t remaining = log_leaves - level;
-        uint32_t nlev = remaining < 6 ? remaining : 6;  // 6 levels fit a 64-lane wave
-        uint64_t n_parents = (uint64_t(1) << log_leaves) >> (level + 1);
+        uint32_t log_children = remaining;  // children at `level`: 2^remaining
         uint32_t *child = d_nodes + level_offset_words(log_leaves, level);
-        uint32_t *parent = d_nodes + level_offset_words(log_leaves, level + 1);
-        // shuffle levels only combine lanes of one wave
-        if ((uint64_t(1) << (nlev - 1)) > n_parents) nlev = 1;
-        uint32_t blocks = (uint32_t)((n_parents + 255) / 256);
-        hipLaunchKernelGGL(tree_levels_kernel, dim3(blocks), dim3(256), 0, ctx->stream, child, parent, n_parents, nlev,
-                           n_parents);
+        if (log_children <= 12) {
+            hipLaunchKernelGGL(tree_tail_kernel, dim3(1), dim3(1024), 0, ctx->stream, child, log_children);
+            ZS_HIP(ctx, hipGetLastError());
+            break;
+        }
+        uint32_t nlev = remaining < 3 ? remaining : 3;
+        uint64_t n_lanes = (uint64_t(1) << log_children) >> nlev;
+        uint32_t *lvl1 = d_nodes + level_offset_words(log_leaves, level + 1);
+        dim3 grid((uint32_t)((n_lanes + 255) / 256));
+        if (nlev == 3) hipLaunchKernelGGL(tree_subtree_kernel<3>, grid, dim3(256), 0, ctx->stream, child, lvl1, n_lanes);
+        else if (nlev == 2) hipLaunchKernelGGL(tree_subtree_kernel<2>, grid, dim3(256), 0, ctx->stream, child, lvl1, n_lanes);
+        else hipLaunchKernelGGL(tree_subtree_kernel<1>, grid, dim3(256), 0, ctx->stream, child, lvl1, n_lanes);
         ZS_HIP(ctx, hipGetLastError());
         level += nlev;
     }
