@@ -168,7 +168,7 @@ __device__ __forceinline__ uint32_t tile_fixed(const FftPass &p, const TileGeom 
 
 // cooperative global -> LDS load of the workgroup's G * 2^qload tiles
 template <int C>
-__device__ __forceinline__ void coop_load(const FftPass &p, const TileGeom &tg, uint32_t *lds, uint32_t col,
+__device__ __forceinline__ void coop_load_generic(const FftPass &p, const TileGeom &tg, uint32_t *lds, uint32_t col,
                                           uint32_t qblk, uint32_t nq_log, bool replicate_q) {
     constexpr uint32_t T = 1u << C, TS = TC<C>::TS;
     const uint32_t G = 1u << p.G_log, Q = 1u << p.Q_log;
@@ -208,7 +208,7 @@ __device__ __forceinline__ void coop_load(const FftPass &p, const TileGeom &tg, 
 
 // cooperative LDS -> global store of nq tiles-per-g (q runs over [qblk, qblk + 2^nq_log))
 template <int C>
-__device__ __forceinline__ void coop_store(const FftPass &p, const TileGeom &tg, const uint32_t *lds, uint32_t col,
+__device__ __forceinline__ void coop_store_generic(const FftPass &p, const TileGeom &tg, const uint32_t *lds, uint32_t col,
                                            uint32_t qblk, uint32_t nq_log) {
     constexpr uint32_t T = 1u << C, TS = TC<C>::TS;
     const uint32_t G = 1u << p.G_log, NQ = 1u << nq_log;
@@ -246,6 +246,98 @@ __device__ __forceinline__ void coop_store(const FftPass &p, const TileGeom &tg,
         }
         out_col[addr] = v;
     }
+}
+
+
+// Cooperative copies with the hot layouts specialised: the per-thread part of
+// each address (its run position) is computed once, and k-space tiles are
+// streamed with t fastest so every wave access is one contiguous run.
+template <int C>
+__device__ __forceinline__ void coop_load(const FftPass &p, const TileGeom &tg, uint32_t *lds, uint32_t col,
+                                          uint32_t qblk, uint32_t nq_log, bool replicate_q) {
+    constexpr uint32_t T = 1u << C, TS = TC<C>::TS;
+    const uint32_t G = 1u << p.G_log, nt = blockDim.x, tid = threadIdx.x;
+    const uint32_t *in_col = p.in + (uint64_t)col * p.in_cs;
+    if (p.in_mode == IN_NATURAL && p.s_lo == 0 && p.natural_tiles && nq_log == 0 && G <= nt) {
+        // element (g, t) sits at natural (bitrev_C(t) << (L - C)) | (tau0 + g): runs of G along g
+        const uint32_t g = tid & (G - 1), tstep = nt >> p.G_log;
+        const uint32_t *src = in_col + tg.tau0 + g;
+        uint32_t *dst = lds + g * TS;
+        for (uint32_t t = tid >> p.G_log; t < T; t += tstep) dst[lidx(t)] = src[(uint64_t)bitrev32(t, C) << (p.L - C)];
+        return;
+    }
+    if ((p.in_mode == IN_KSPACE || p.in_mode == IN_SHARED) && p.s_lo > 0 && G <= nt) {
+        // strided tiles: (q, hi, t, lo0 + g) -> runs of G along g
+        const uint32_t g = tid & (G - 1), tstep = nt >> p.G_log;
+        const uint32_t nq = (p.in_mode == IN_SHARED) ? 1u : (1u << nq_log);
+        for (uint32_t ql = 0; ql < nq; ql++) {
+            uint64_t base = (p.in_mode == IN_SHARED ? 0ull : ((uint64_t)(qblk + ql) << p.L)) |
+                            ((uint64_t)tg.hi << (p.s_lo + C)) | (tg.lo0 + g);
+            const uint32_t *src = in_col + base;
+            for (uint32_t t = tid >> p.G_log; t < T; t += tstep) {
+                uint32_t v = src[(uint64_t)t << p.s_lo];
+                if (replicate_q) {
+                    for (uint32_t qq = 0; qq < (1u << p.Q_log); qq++) lds[(qq * G + g) * TS + lidx(t)] = v;
+                } else {
+                    lds[(ql * G + g) * TS + lidx(t)] = v;
+                }
+            }
+        }
+        return;
+    }
+    if (p.in_mode == IN_KSPACE && p.s_lo == 0 && !replicate_q) {
+        // contiguous tiles: stream each tile with t fastest
+        for (uint32_t tile = 0; tile < (G << nq_log); tile++) {
+            const uint32_t g = tile & (G - 1), ql = tile >> p.G_log;
+            const uint32_t *src = in_col + (((uint64_t)(qblk + ql) << p.L) | tile_fixed(p, tg, g));
+            uint32_t *dst = lds + tile * TS;
+            for (uint32_t t = tid; t < T; t += nt) dst[lidx(t)] = src[t];
+        }
+        return;
+    }
+    coop_load_generic<C>(p, tg, lds, col, qblk, nq_log, replicate_q);
+}
+
+template <int C>
+__device__ __forceinline__ void coop_store(const FftPass &p, const TileGeom &tg, const uint32_t *lds, uint32_t col,
+                                           uint32_t qblk, uint32_t nq_log) {
+    constexpr uint32_t T = 1u << C, TS = TC<C>::TS;
+    const uint32_t G = 1u << p.G_log, nt = blockDim.x, tid = threadIdx.x;
+    uint32_t *out_col = p.out + (uint64_t)col * p.out_cs;
+    const uint32_t run_log = p.G_log + nq_log;
+    if (p.out_mode == OUT_NATURAL && p.s_lo == 0 && p.natural_tiles && (1u << run_log) <= nt && !p.scale) {
+        // element (q, g, t) goes to (bitrev_C(t) << (L-C+beta)) | ((tau0 + g) << beta) | bitrev_beta(q):
+        // a thread keeps one run position rp = (g, bitrev(q)) and walks t
+        const uint32_t rp = tid & ((1u << run_log) - 1), tstep = nt >> run_log;
+        const uint32_t rq = rp & ((1u << nq_log) - 1), g = rp >> nq_log;
+        const uint32_t ql = bitrev32(rq, nq_log), q = qblk + ql;
+        const uint64_t low = ((uint64_t)(tg.tau0 + g) << p.beta) | bitrev32(q, p.beta);
+        const uint32_t sh = p.L - C + p.beta;
+        const uint32_t *src = lds + (ql * G + g) * TS;
+        for (uint32_t t = tid >> run_log; t < T; t += tstep)
+            out_col[((uint64_t)bitrev32(t, C) << sh) | low] = src[lidx(t)];
+        return;
+    }
+    if (p.out_mode == OUT_KSPACE && p.s_lo > 0 && G <= nt && !p.scale) {
+        const uint32_t g = tid & (G - 1), tstep = nt >> p.G_log;
+        for (uint32_t ql = 0; ql < (1u << nq_log); ql++) {
+            uint64_t base = ((uint64_t)(qblk + ql) << p.L) | ((uint64_t)tg.hi << (p.s_lo + C)) | (tg.lo0 + g);
+            uint32_t *dst = out_col + base;
+            const uint32_t *src = lds + (ql * G + g) * TS;
+            for (uint32_t t = tid >> p.G_log; t < T; t += tstep) dst[(uint64_t)t << p.s_lo] = src[lidx(t)];
+        }
+        return;
+    }
+    if (p.out_mode == OUT_KSPACE && p.s_lo == 0 && !p.scale) {
+        for (uint32_t tile = 0; tile < (G << nq_log); tile++) {
+            const uint32_t g = tile & (G - 1), ql = tile >> p.G_log;
+            uint32_t *dst = out_col + (((uint64_t)(qblk + ql) << p.L) | tile_fixed(p, tg, g));
+            const uint32_t *src = lds + tile * TS;
+            for (uint32_t t = tid; t < T; t += nt) dst[t] = src[lidx(t)];
+        }
+        return;
+    }
+    coop_store_generic<C>(p, tg, lds, col, qblk, nq_log);
 }
 
 // ---------------------------------------------------------------------------
@@ -295,7 +387,7 @@ static size_t tile_lds_bytes(uint32_t ntiles, uint32_t nsets) {
 }
 
 // Launch-shape knobs (read once from the environment; defaults from measurement):
-//   ZS_TILE_ELEMS_LOG: elements of the tiles one workgroup transforms (default 2^12)
+//   ZS_TILE_ELEMS_LOG: elements of the tiles one workgroup transforms (default 2^13)
 //   ZS_COLS_PER_WG:    columns a workgroup runs with one set of staged twiddles (default 4)
 static int env_int(const char *name, int dflt, int lo, int hi) {
     const char *e = getenv(name);
@@ -303,7 +395,7 @@ static int env_int(const char *name, int dflt, int lo, int hi) {
     return std::min(hi, std::max(lo, x));
 }
 static uint32_t tile_elems_log() {
-    static uint32_t v = (uint32_t)env_int("ZS_TILE_ELEMS_LOG", 12, 11, 15);
+    static uint32_t v = (uint32_t)env_int("ZS_TILE_ELEMS_LOG", 13, 11, 15);
     return v;
 }
 static uint32_t cols_per_wg() {
