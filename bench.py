@@ -28,10 +28,32 @@ def algorithmic_bytes(n_cols, log_n, log_blowup):
     return 4 * n_cols * n + 4 * n_cols * N + 32 * (2 * N - 1)
 
 
-def leaf_kernel_bytes(n_cols, log_rows):
-    """row_leaf_kernel per launch: read every LDE value once, write one digest per row."""
-    N = 1 << log_rows
-    return 4 * n_cols * N + 32 * N
+def kernel_bytes(n_cols, log_n, log_blowup):
+    """Algorithmic HBM bytes per launch of the kernels of one step (each value read
+    once and each result written once):
+      interpolation pass over the natural-order columns: read n, write n words per column;
+      fused interpolation-top / evaluation-top pass: read n, write N words per column;
+      last evaluation pass to natural order: read N, write N words per column;
+      row leaves: read every LDE value, write one 32-byte digest per row."""
+    n, N = 1 << log_n, 1 << (log_n + log_blowup)
+    return {
+        "fft_tile_kernel<inv,10,natural-in>": 8 * n_cols * n,
+        "lde_mid_kernel<10>": 4 * n_cols * n + 4 * n_cols * N,
+        "fft_tile_kernel<fwd,10,natural-out>": 8 * n_cols * N,
+        "row_leaf_kernel": 4 * n_cols * N + 32 * N,
+    }
+
+
+def load_traffic(path):
+    """Measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated),
+    from the summary committed under profiles/: lines 'traffic <kernel> <bytes>'."""
+    out = {}
+    if os.path.exists(path):
+        for line in open(path):
+            f = line.split()
+            if len(f) == 3 and f[0] == "traffic":
+                out[f[1]] = int(f[2])
+    return out
 
 
 def cpu_baseline(n_cols, log_n, log_blowup):
@@ -73,6 +95,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fri", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one 256 MiB device copy (for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build measures one GPU "
@@ -116,10 +140,25 @@ def main():
     root_fused = d_nodes[-8:].cpu().numpy().view(np.uint32).tobytes()
 
     # stages, each timed alone on the same stream
-    d_leaves = torch.empty(N * 8, dtype=torch.int32, device=dev)
     lde_ms = timed(lambda: ctx.lde_dev(d_cols.data_ptr(), n_cols, lg, beta, d_lde.data_ptr()), 5)
     commit_ms = timed(lambda: ctx.row_commit_dev(d_lde.data_ptr(), n_cols, lg + beta, N, d_nodes.data_ptr()), 5)
-    leaf_ms = timed(lambda: ctx.row_leaves_dev(d_lde.data_ptr(), n_cols, lg + beta, N, d_leaves.data_ptr()), 5)
+
+    # per-kernel durations: every hot launch bracketed by HIP events on this stream
+    ctx.profile(True)
+    for _ in range(3):
+        step()
+    kern = ctx.profile_read()
+    ctx.profile(False)
+    kbytes = kernel_bytes(n_cols, lg, beta)
+    dom = max((k for k in kern if k in kbytes), key=lambda k: kern[k][0])
+    dom_ms = kern[dom][0] / kern[dom][1]
+
+    if args.copy_calibration:
+        a = torch.empty(1 << 26, dtype=torch.int32, device=dev)
+        bb = torch.empty_like(a)
+        a.fill_(1)
+        bb.copy_(a)
+        torch.cuda.synchronize()
 
     # reference-shaped plain path (per column, one launch per butterfly layer)
     plain_ms = None
@@ -148,8 +187,8 @@ def main():
     elems = n_cols * n
     value = elems / (ms * 1e-3) / 1e6
     alg = algorithmic_bytes(n_cols, lg, beta)
-    lk_bytes = leaf_kernel_bytes(n_cols, lg + beta)
-    achieved = lk_bytes / (leaf_ms * 1e-3) / 1e9
+    achieved = kbytes[dom] / (dom_ms * 1e-3) / 1e9
+    traffic = load_traffic(os.path.join(ROOT, "profiles", "hbm_traffic_lde_commit_64x2p20.txt")).get(dom)
     out = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -166,12 +205,13 @@ def main():
         "config": {"workload": f"{n_cols} cols x 2^{lg} uniform M31 -> circle LDE blowup {1 << beta} "
                                f"(2^{lg + beta} rows, canonic cosets) -> BLAKE3 row-leaf Merkle tree",
                    "n_cols": n_cols, "log_n": lg, "log_blowup": beta, "seed": SEED},
-        "roofline": {"bound": "hbm", "kernel": "row_leaf_kernel", "achieved": round(achieved, 1),
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None},
+                     "traffic": traffic, "algorithmic_bytes": kbytes[dom], "avg_launch_ms": round(dom_ms, 4)},
         "step_hbm": {"algorithmic_bytes": alg, "GBps": round(alg / (ms * 1e-3) / 1e9, 1),
                      "frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
-        "stages_ms": {"lde": round(lde_ms, 4), "commit": round(commit_ms, 4), "row_leaf_kernel": round(leaf_ms, 4)},
+        "stages_ms": {"lde": round(lde_ms, 4), "commit": round(commit_ms, 4)},
+        "kernels_ms_per_step": {k: round(v[0] / 3, 4) for k, v in sorted(kern.items(), key=lambda kv: -kv[1][0])},
         "plain": {"ms_per_step": round(plain_ms, 3) if plain_ms else None,
                   "speedup_fused_vs_plain": round(plain_ms / ms, 2) if plain_ms else None,
                   "bit_exact_root": bit_exact},

@@ -72,6 +72,16 @@ const char *zs_hip_last_error(const zs_hip_ctx *ctx);
 /* Run _dev calls on this hipStream_t (NULL = the null stream). */
 int32_t zs_hip_set_stream(zs_hip_ctx *ctx, void *hip_stream);
 int32_t zs_hip_synchronize(zs_hip_ctx *ctx);
+/* Per-kernel timing (no reference counterpart; the reference times whole calls
+ * with clock_gettime, benchmarks/ntt/src/main.zig:15-47).  enable = 1 clears the
+ * records and starts bracketing every hot kernel with HIP events on the
+ * context's stream; 0 stops. */
+int32_t zs_hip_profile(zs_hip_ctx *ctx, int32_t enable);
+/* Synchronise and aggregate the recorded launches by kernel name: names are
+ * written NUL-separated into `names` (names_cap bytes), ms[i] is the total
+ * event time and counts[i] the launches of entry i; *n_out entries (<= max). */
+int32_t zs_hip_profile_read(zs_hip_ctx *ctx, char *names, size_t names_cap, double *ms, uint32_t *counts,
+                            uint32_t max, uint32_t *n_out);
 /* zig-capi.h:73-74 */
 void zs_free(zs_host_allocator host, uint8_t *ptr, size_t len);
 

@@ -110,12 +110,15 @@ def _declare(L):
     L.zs_fri_read_layer.argtypes = [vp, u32, u32p]
     L.zs_fri_result_free.argtypes = [vp]
     L.zs_fill_m31_dev.argtypes = [vp, ctypes.c_uint64, vp, sz]
+    L.zs_hip_profile.argtypes = [vp, i32]
+    L.zs_hip_profile_read.argtypes = [vp, ctypes.c_char_p, sz, ctypes.POINTER(ctypes.c_double),
+                                      ctypes.POINTER(u32), u32, ctypes.POINTER(u32)]
     for name in ["zs_hip_set_stream", "zs_hip_synchronize", "zs_circle_coset_canonic_half", "zs_m31_circle_fft",
                  "zs_m31_circle_ifft", "zs_m31_circle_fft_batch_dev", "zs_m31_circle_ifft_batch_dev",
                  "zs_m31_lde_commit_dev", "zs_m31_lde_commit", "zs_m31_row_commit_dev", "zs_m31_lde_dev",
                  "zs_m31_row_leaves_dev", "zs_tree_root",
                  "zs_merkle_open", "zs_tree_read_level", "zs_fri_commit_dev", "zs_fri_commit", "zs_fri_query",
-                 "zs_fri_read_layer", "zs_fill_m31_dev"]:
+                 "zs_fri_read_layer", "zs_fill_m31_dev", "zs_hip_profile", "zs_hip_profile_read"]:
         getattr(L, name).restype = i32
 
 
@@ -272,6 +275,21 @@ class Context:
 
     def synchronize(self):
         self.check(lib().zs_hip_synchronize(self.h))
+
+    def profile(self, enable=True):
+        """Bracket every hot kernel launch with HIP events (clears earlier records)."""
+        self.check(lib().zs_hip_profile(self.h, 1 if enable else 0))
+
+    def profile_read(self):
+        """{kernel name: (total ms, launches)} of the launches recorded since profile(True)."""
+        cap, mx = 1 << 16, 256
+        names = ctypes.create_string_buffer(cap)
+        ms = (ctypes.c_double * mx)()
+        cnt = (u32 * mx)()
+        n = u32()
+        self.check(lib().zs_hip_profile_read(self.h, names, cap, ms, cnt, mx, ctypes.byref(n)))
+        keys = names.raw.split(b"\0")[: n.value]
+        return {k.decode(): (ms[i], cnt[i]) for i, k in enumerate(keys)}
 
     # ---- circle FFT (circlefft_gpu.zig:46, :111) ----
     def circle_fft(self, coeffs, half_coset=None):

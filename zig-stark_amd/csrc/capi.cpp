@@ -6,6 +6,7 @@
 // zig-capi.h convention.  Unlike the reference there is no CPU fallback: a
 // missing device or a HIP failure is returned as ZS_ERR_DEVICE.
 #include <cstring>
+#include <map>
 #include <new>
 #include <vector>
 
@@ -90,10 +91,13 @@ int32_t zs_hip_init(int32_t device, uint32_t flags, zs_hip_ctx **out_ctx) {
     return ZS_OK;
 }
 
+static void prof_clear(Ctx *ctx);
+
 void zs_hip_destroy(zs_hip_ctx *ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->c.device);
     (void)hipStreamSynchronize(ctx->c.stream);
+    prof_clear(&ctx->c);
     delete ctx;
 }
 
@@ -108,6 +112,55 @@ int32_t zs_hip_set_stream(zs_hip_ctx *ctx, void *hip_stream) {
 int32_t zs_hip_synchronize(zs_hip_ctx *ctx) {
     if (!ctx) return ZS_ERR_INPUT;
     ZS_HIP(&ctx->c, hipStreamSynchronize(ctx->c.stream));
+    return ZS_OK;
+}
+
+static void prof_clear(Ctx *ctx) {
+    for (auto &r : ctx->prof) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    ctx->prof.clear();
+}
+
+int32_t zs_hip_profile(zs_hip_ctx *h, int32_t enable) {
+    if (!h) return ZS_ERR_INPUT;
+    ZS_HIP(&h->c, hipStreamSynchronize(h->c.stream));
+    prof_clear(&h->c);
+    h->c.profiling = enable != 0;
+    return ZS_OK;
+}
+
+int32_t zs_hip_profile_read(zs_hip_ctx *h, char *names, size_t names_cap, double *ms, uint32_t *counts, uint32_t max,
+                            uint32_t *n_out) {
+    if (!h || !n_out) return ZS_ERR_INPUT;
+    Ctx *ctx = &h->c;
+    ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::vector<std::string> order;
+    std::map<std::string, std::pair<double, uint32_t>> agg;
+    for (auto &r : ctx->prof) {
+        float t = 0;
+        ZS_HIP(ctx, hipEventElapsedTime(&t, r.a, r.b));
+        auto it = agg.find(r.name);
+        if (it == agg.end()) {
+            order.push_back(r.name);
+            agg[r.name] = {t, 1};
+        } else {
+            it->second.first += t;
+            it->second.second++;
+        }
+    }
+    uint32_t n = 0;
+    size_t off = 0;
+    for (auto &name : order) {
+        if (n >= max || off + name.size() + 1 > names_cap) break;
+        std::memcpy(names + off, name.c_str(), name.size() + 1);
+        off += name.size() + 1;
+        ms[n] = agg[name].first;
+        counts[n] = agg[name].second;
+        n++;
+    }
+    *n_out = n;
     return ZS_OK;
 }
 

@@ -426,8 +426,13 @@ int32_t run_tile_pass(Ctx *ctx, FftPass p, bool inverse, uint32_t n_cols) {
     case CC: {                                                                                                   \
         size_t lds = tile_lds_bytes<CC>(ntile, nsets);                                                           \
         if (lds > 160 * 1024) return run_pass(ctx, p, inverse, n_cols);                                          \
-        if (inverse) hipLaunchKernelGGL((fft_tile_kernel<true, CC>), grid, block, lds, ctx->stream, p);          \
-        else hipLaunchKernelGGL((fft_tile_kernel<false, CC>), grid, block, lds, ctx->stream, p);                 \
+        if (inverse)                                                                                             \
+            ZS_LAUNCH(ctx, p.s_lo == 0 ? "fft_tile_kernel<inv," #CC ",natural-in>" : "fft_tile_kernel<inv," #CC ">", \
+                      (fft_tile_kernel<true, CC>), grid, block, lds, ctx->stream, p);                            \
+        else                                                                                                     \
+            ZS_LAUNCH(ctx, p.out_mode == OUT_NATURAL ? "fft_tile_kernel<fwd," #CC ",natural-out>"                \
+                                                     : "fft_tile_kernel<fwd," #CC ">",                           \
+                      (fft_tile_kernel<false, CC>), grid, block, lds, ctx->stream, p);                           \
         break;                                                                                                   \
     }
     switch (p.C) {
@@ -613,9 +618,9 @@ int32_t run_lde_fused(Ctx *ctx, const uint32_t *d_in, uint64_t in_cs, uint32_t *
 #define ZS_FUSED_CASE(CC)                                                                                          \
     case CC:                                                                                                       \
         if (single)                                                                                                \
-            hipLaunchKernelGGL((lde_single_kernel<CC>), grid, block, lds * 4, ctx->stream, f);                     \
+            ZS_LAUNCH(ctx, "lde_single_kernel<" #CC ">", (lde_single_kernel<CC>), grid, block, lds * 4, ctx->stream, f); \
         else                                                                                                       \
-            hipLaunchKernelGGL((lde_mid_kernel<CC>), grid, block, lds * 4, ctx->stream, f);                        \
+            ZS_LAUNCH(ctx, "lde_mid_kernel<" #CC ">", (lde_mid_kernel<CC>), grid, block, lds * 4, ctx->stream, f);  \
         break;
     switch (C) {
         ZS_FUSED_CASE(8)

@@ -99,8 +99,8 @@ int32_t launch_fri_fold(Ctx *ctx, const uint32_t *d_cur, uint64_t n, uint32_t dl
     ZS_TRY(get_fri_tables(ctx, dlog, &tables));
     qm31 inv_off = qm_inv(fri_offset_pow(layer));
     uint64_t half = n / 2;
-    hipLaunchKernelGGL(fri_fold_kernel, dim3((uint32_t)((half + 255) / 256)), dim3(256), 0, ctx->stream, d_cur, half,
-                       layer, tables, inv_off, d_alpha, d_next, d_next_leaves);
+    ZS_LAUNCH(ctx, "fri_fold_kernel", fri_fold_kernel, dim3((uint32_t)((half + 255) / 256)), dim3(256), 0, ctx->stream,
+              d_cur, half, layer, tables, inv_off, d_alpha, d_next, d_next_leaves);
     ZS_HIP(ctx, hipGetLastError());
     return ZS_OK;
 }

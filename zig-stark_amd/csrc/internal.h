@@ -26,6 +26,20 @@ namespace zs {
         }                                                                              \
     } while (0)
 
+// Launch a hot kernel; when the context is profiling, bracket it with events
+// on the launch stream so its duration can be read back (zs_hip_profile_read).
+#define ZS_LAUNCH(ctx, label, ...)                                                 \
+    do {                                                                           \
+        hipEvent_t _za = nullptr, _zb = nullptr;                                   \
+        if ((ctx)->profiling && hipEventCreate(&_za) == hipSuccess)                \
+            (void)hipEventRecord(_za, (ctx)->stream);                              \
+        hipLaunchKernelGGL(__VA_ARGS__);                                           \
+        if (_za && hipEventCreate(&_zb) == hipSuccess) {                           \
+            (void)hipEventRecord(_zb, (ctx)->stream);                              \
+            (ctx)->prof.push_back({std::string(label), _za, _zb});                 \
+        }                                                                          \
+    } while (0)
+
 #define ZS_TRY(expr)                 \
     do {                             \
         int32_t _rc = (expr);        \
@@ -53,8 +67,15 @@ struct TwiddleTable {
     DevBuf fwd, inv;
 };
 
+struct ProfRec {
+    std::string name;
+    hipEvent_t a, b;
+};
+
 struct Ctx {
     int device = 0;
+    bool profiling = false;           // record HIP events around the hot kernels
+    std::vector<ProfRec> prof;
     hipStream_t stream = nullptr;  // nullptr = the null stream
     std::mutex mu;                  // guards the caches below
     std::string last_error;

@@ -88,7 +88,8 @@ int32_t launch_row_leaves(Ctx *ctx, const uint32_t *d_cols, uint64_t col_stride,
                                  uint64_t n, uint32_t *d_nodes) {
     dim3 grid((uint32_t)((n + 255) / 256));
     if (n_cols <= 256)
-        hipLaunchKernelGGL(row_leaf_kernel, grid, dim3(256), 0, ctx->stream, d_cols, col_stride, n_cols, n, d_nodes);
+        ZS_LAUNCH(ctx, "row_leaf_kernel", row_leaf_kernel, grid, dim3(256), 0, ctx->stream, d_cols, col_stride, n_cols, n,
+                  d_nodes);
     else
         hipLaunchKernelGGL(row_leaf_multichunk_kernel, grid, dim3(256), 0, ctx->stream, d_cols, col_stride, n_cols,
                            n, d_nodes);
@@ -181,7 +182,7 @@ int32_t launch_tree_levels(Ctx *ctx, uint32_t *d_nodes, uint32_t log_leaves, uin
         uint32_t log_children = remaining;  // children at `level`: 2^remaining
         uint32_t *child = d_nodes + level_offset_words(log_leaves, level);
         if (log_children <= 12) {
-            hipLaunchKernelGGL(tree_tail_kernel, dim3(1), dim3(1024), 0, ctx->stream, child, log_children);
+            ZS_LAUNCH(ctx, "tree_tail_kernel", tree_tail_kernel, dim3(1), dim3(1024), 0, ctx->stream, child, log_children);
             ZS_HIP(ctx, hipGetLastError());
             break;
         }
@@ -189,9 +190,15 @@ int32_t launch_tree_levels(Ctx *ctx, uint32_t *d_nodes, uint32_t log_leaves, uin
         uint64_t n_lanes = (uint64_t(1) << log_children) >> nlev;
         uint32_t *lvl1 = d_nodes + level_offset_words(log_leaves, level + 1);
         dim3 grid((uint32_t)((n_lanes + 255) / 256));
-        if (nlev == 3) hipLaunchKernelGGL(tree_subtree_kernel<3>, grid, dim3(256), 0, ctx->stream, child, lvl1, n_lanes);
-        else if (nlev == 2) hipLaunchKernelGGL(tree_subtree_kernel<2>, grid, dim3(256), 0, ctx->stream, child, lvl1, n_lanes);
-        else hipLaunchKernelGGL(tree_subtree_kernel<1>, grid, dim3(256), 0, ctx->stream, child, lvl1, n_lanes);
+        if (nlev == 3)
+            ZS_LAUNCH(ctx, "tree_subtree_kernel<3>", tree_subtree_kernel<3>, grid, dim3(256), 0, ctx->stream, child, lvl1,
+                      n_lanes);
+        else if (nlev == 2)
+            ZS_LAUNCH(ctx, "tree_subtree_kernel<2>", tree_subtree_kernel<2>, grid, dim3(256), 0, ctx->stream, child, lvl1,
+                      n_lanes);
+        else
+            ZS_LAUNCH(ctx, "tree_subtree_kernel<1>", tree_subtree_kernel<1>, grid, dim3(256), 0, ctx->stream, child, lvl1,
+                      n_lanes);
         ZS_HIP(ctx, hipGetLastError());
         level += nlev;
     }
@@ -229,7 +236,7 @@ int32_t launch_row_commit_plain(Ctx *ctx, const uint32_t *d_cols, uint64_t col_s
 }
 
 int32_t launch_qm31_leaves(Ctx *ctx, const uint32_t *d_vals, uint64_t n, uint32_t *d_nodes) {
-    hipLaunchKernelGGL(qm31_leaf_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, ctx->stream, d_vals, n,
+    ZS_LAUNCH(ctx, "qm31_leaf_kernel", qm31_leaf_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, ctx->stream, d_vals, n,
                        d_nodes);
     ZS_HIP(ctx, hipGetLastError());
     return ZS_OK;
