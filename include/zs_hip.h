@@ -173,6 +173,68 @@ int32_t zs_fri_query(zs_fri_result *res, zs_channel *chan, size_t num_queries, s
 int32_t zs_fri_read_layer(const zs_fri_result *res, uint32_t layer, uint32_t *out);
 void zs_fri_result_free(zs_fri_result *res);
 
+/* ---- one trace's full commitment, batches, and the serving loop --------------- */
+/* The commitment of one trace (the composition the reference leaves open; the
+ * channel, FRI and tree steps each follow the reference):
+ *   1. LDE of the columns (blowup 2^log_blowup) and the row-leaf Merkle tree -> trace root;
+ *   2. channel = Channel.init("zig-stark:hip:commit"), absorbDigest(trace root),
+ *      gamma = sample(QM31)                     (src/core/channel/channel.zig)
+ *   3. g[i] = sum_c gamma^c * lde_c[i] over the LDE rows (QM31);
+ *   4. the FRI commit phase of g with challenges drawn from the same channel
+ *      (src/m31/fri.zig:157-233), dlog = log_n + log_blowup.
+ * The tree stays in d_nodes; fri_roots gets (dlog - remainder_log) digests and
+ * remainder 4 * 2^(remainder_log - log_blowup) words.  Requires
+ * log_blowup <= remainder_log < log_n + log_blowup and n_cols <= 4096. */
+int32_t zs_commit_job_dev(zs_hip_ctx *ctx, const uint32_t *d_cols, uint32_t n_cols, uint32_t log_n,
+                          uint32_t log_blowup, uint32_t remainder_log, uint32_t *d_lde, uint8_t *d_nodes,
+                          uint8_t trace_root[32], uint8_t *fri_roots, uint32_t *remainder, uint32_t flags);
+
+/* One trace of a batch: device input, device outputs (LDE, tree), host results. */
+typedef struct {
+    const uint32_t *d_cols;  /* n_cols * 2^log_n words, column-major */
+    uint32_t n_cols, log_n;
+    uint32_t *d_lde;         /* n_cols * 2^(log_n+log_blowup) words */
+    uint8_t *d_nodes;        /* (2^(log_n+log_blowup+1) - 1) * 32 bytes */
+    uint8_t trace_root[32];  /* out */
+    uint8_t *fri_roots;      /* out, (log_n + log_blowup - remainder_log) * 32 bytes */
+    uint32_t *remainder;     /* out, 4 * 2^(remainder_log - log_blowup) words */
+    int32_t status;          /* out, ZS_OK or an error code */
+} zs_job;
+/* Commit many traces of uneven sizes.  n_workers > 1 runs them on that many
+ * worker contexts with their own streams, taking jobs from a shared queue so
+ * their kernels overlap on the device; ZS_FLAG_PLAIN runs the reference-shaped
+ * per-trace loop (one trace after another, per-column per-layer launches).
+ * Returns ZS_OK when every job's status is ZS_OK. */
+int32_t zs_commit_batch_dev(zs_hip_ctx *ctx, zs_job *jobs, size_t n_jobs, uint32_t log_blowup,
+                            uint32_t remainder_log, uint32_t n_workers, uint32_t flags);
+
+/* Serving loop: requests are admitted while their device memory fits the HBM
+ * budget, run on worker streams concurrently with the ones already on the
+ * device, and answered with their results and timestamps. */
+typedef struct zs_server zs_server;
+typedef struct {
+    uint32_t n_cols, log_n, log_blowup, remainder_log;
+    uint64_t seed;            /* synthetic input: zs_fill_m31_dev stream of `seed` */
+    const uint32_t *host_cols;/* or NULL; else n_cols * 2^log_n host words (copied at submit) */
+} zs_request;
+typedef struct {
+    int32_t status;
+    uint32_t fri_layers;
+    uint8_t trace_root[32];
+    uint8_t fri_roots[28 * 32];
+    uint32_t remainder[4 * 64];
+    double t_submit, t_admit, t_done;  /* seconds, steady clock */
+    uint64_t device_bytes;             /* the request's admitted footprint */
+} zs_response;
+zs_server *zs_server_new(int32_t device, uint32_t n_workers, uint64_t hbm_budget_bytes);
+/* Enqueue; returns immediately with a ticket. ZS_ERR_OOM if the request alone exceeds the budget. */
+int32_t zs_server_submit(zs_server *srv, const zs_request *req, uint64_t *ticket);
+/* Block until the request is answered (the response is then released). */
+int32_t zs_server_wait(zs_server *srv, uint64_t ticket, zs_response *out);
+/* Device bytes admitted right now, and the largest value seen. */
+int32_t zs_server_memory(zs_server *srv, uint64_t *in_use, uint64_t *peak);
+void zs_server_free(zs_server *srv);
+
 /* ---- synthetic input ---------------------------------------------------------- */
 /* Seeded uniform M31 values: element i is the top 31 bits of the (i+1)-th
  * SplitMix64 output from `seed` (2^31 - 1 rejected). */

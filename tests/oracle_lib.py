@@ -430,3 +430,32 @@ def fill_m31(seed, n):
     o = np.zeros(n, np.uint32)
     lib().zo_fill_m31(seed, _p(o), n)
     return o
+
+
+def qm31_combine(lde, gamma):
+    """g[i] = sum_c gamma^c * lde[c][i] as (N, 4) QM31 values (lde: (n_cols, N) M31)."""
+    n_cols, N = lde.shape
+    g = np.zeros((N, 4), np.uint64)
+    pw = np.array([1, 0, 0, 0], np.uint32)
+    col = lde.astype(np.uint64)
+    for c in range(n_cols):
+        for k in range(4):
+            g[:, k] = (g[:, k] + (int(pw[k]) * col[c]) % P) % P
+        pw = qm31_mul(pw, gamma)
+    return g.astype(np.uint32)
+
+
+def commit_job(cols, log_blowup, remainder_log):
+    """The trace commitment the library's zs_commit_job_dev computes, from oracle parts:
+    LDE -> row tree -> Channel("zig-stark:hip:commit") absorbs the root -> gamma ->
+    g = sum_c gamma^c lde_c -> FRI commit phase with the same channel.
+    Returns (trace_root, fri_roots, remainder, lde, nodes)."""
+    lde_ = np.stack([lde(c, log_blowup) for c in cols])
+    nodes = row_commit(lde_)
+    root = tree_root(nodes)
+    ch = Channel(b"zig-stark:hip:commit")
+    ch.absorb(root)
+    gamma = ch.sample_qm31()
+    g = qm31_combine(lde_, gamma)
+    roots, _, rem, _ = fri_commit(g, log_blowup, remainder_log, channel=ch, want_layers=False)
+    return root, roots, rem, lde_, nodes

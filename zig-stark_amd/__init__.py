@@ -110,6 +110,14 @@ def _declare(L):
     L.zs_fri_read_layer.argtypes = [vp, u32, u32p]
     L.zs_fri_result_free.argtypes = [vp]
     L.zs_fill_m31_dev.argtypes = [vp, ctypes.c_uint64, vp, sz]
+    L.zs_commit_job_dev.argtypes = [vp, vp, u32, u32, u32, u32, vp, vp, ctypes.c_char_p, ctypes.c_char_p, u32p, u32]
+    L.zs_commit_batch_dev.argtypes = [vp, ctypes.c_void_p, sz, u32, u32, u32, u32]
+    L.zs_server_new.argtypes = [i32, u32, ctypes.c_uint64]
+    L.zs_server_new.restype = vp
+    L.zs_server_submit.argtypes = [vp, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]
+    L.zs_server_wait.argtypes = [vp, ctypes.c_uint64, ctypes.c_void_p]
+    L.zs_server_memory.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]
+    L.zs_server_free.argtypes = [vp]
     L.zs_hip_profile.argtypes = [vp, i32]
     L.zs_hip_profile_read.argtypes = [vp, ctypes.c_char_p, sz, ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(u32), u32, ctypes.POINTER(u32)]
@@ -118,7 +126,9 @@ def _declare(L):
                  "zs_m31_lde_commit_dev", "zs_m31_lde_commit", "zs_m31_row_commit_dev", "zs_m31_lde_dev",
                  "zs_m31_row_leaves_dev", "zs_tree_root",
                  "zs_merkle_open", "zs_tree_read_level", "zs_fri_commit_dev", "zs_fri_commit", "zs_fri_query",
-                 "zs_fri_read_layer", "zs_fill_m31_dev", "zs_hip_profile", "zs_hip_profile_read"]:
+                 "zs_fri_read_layer", "zs_fill_m31_dev", "zs_hip_profile", "zs_hip_profile_read",
+                 "zs_commit_job_dev", "zs_commit_batch_dev", "zs_server_submit", "zs_server_wait",
+                 "zs_server_memory"]:
         getattr(L, name).restype = i32
 
 
@@ -248,6 +258,72 @@ def _check(rc, ctx=None):
         raise ZsError(rc, msg)
 
 
+class Job(ctypes.Structure):
+    """zs_job: one trace of a batch (device input and outputs, host results)."""
+    _fields_ = [("d_cols", vp), ("n_cols", u32), ("log_n", u32), ("d_lde", vp), ("d_nodes", vp),
+                ("trace_root", ctypes.c_uint8 * 32), ("fri_roots", vp), ("remainder", vp), ("status", i32)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("n_cols", u32), ("log_n", u32), ("log_blowup", u32), ("remainder_log", u32),
+                ("seed", ctypes.c_uint64), ("host_cols", u32p)]
+
+
+class Response(ctypes.Structure):
+    _fields_ = [("status", i32), ("fri_layers", u32), ("trace_root", ctypes.c_uint8 * 32),
+                ("fri_roots", ctypes.c_uint8 * (28 * 32)), ("remainder", u32 * (4 * 64)),
+                ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double),
+                ("device_bytes", ctypes.c_uint64)]
+
+    def result(self, log_blowup, remainder_log):
+        L = self.fri_layers
+        rc = 1 << (remainder_log - log_blowup)
+        return (bytes(self.trace_root), [bytes(self.fri_roots[32 * i:32 * (i + 1)]) for i in range(L)],
+                np.array(self.remainder[:4 * rc], np.uint32).reshape(rc, 4))
+
+
+class Server:
+    """Continuous-batching request loop with HBM-budget admission (zs_server_*)."""
+
+    def __init__(self, device=0, n_workers=4, hbm_budget=200 * 10**9):
+        self.h = lib().zs_server_new(device, n_workers, hbm_budget)
+        if not self.h:
+            raise ZsError(ZS_ERR_DEVICE, "zs_server_new")
+
+    def submit(self, n_cols, log_n, log_blowup=2, remainder_log=5, seed=0, host_cols=None):
+        r = Request(n_cols, log_n, log_blowup, remainder_log, seed, None)
+        keep = None
+        if host_cols is not None:
+            keep = _u32(host_cols).reshape(-1)
+            r.host_cols = _p(keep)
+        t = ctypes.c_uint64()
+        rc = lib().zs_server_submit(self.h, ctypes.byref(r), ctypes.byref(t))
+        if rc:
+            raise ZsError(rc, "zs_server_submit")
+        return t.value
+
+    def wait(self, ticket):
+        out = Response()
+        rc = lib().zs_server_wait(self.h, ticket, ctypes.byref(out))
+        if rc:
+            raise ZsError(rc, "zs_server_wait")
+        return out
+
+    def memory(self):
+        a, b = ctypes.c_uint64(), ctypes.c_uint64()
+        lib().zs_server_memory(self.h, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().zs_server_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        if _lib is not None:
+            self.close()
+
+
 class Context:
     """A zs_hip_ctx on one device (circlefft_gpu.zig:20-39's lazily-initialised context)."""
 
@@ -341,6 +417,36 @@ class Context:
 
     def fill_m31_dev(self, seed, d_out, n):
         self.check(lib().zs_fill_m31_dev(self.h, seed, d_out, n))
+
+    # ---- one trace's commitment, and batches of uneven traces ----
+    def commit_job_dev(self, d_cols, n_cols, log_n, log_blowup, remainder_log, d_lde, d_nodes, flags=0):
+        """LDE + row tree + gamma from the trace root + QM31 combination + FRI commit.
+        Returns (trace_root, fri_roots, remainder)."""
+        L = log_n + log_blowup - remainder_log
+        root = ctypes.create_string_buffer(32)
+        roots = ctypes.create_string_buffer(32 * L)
+        rem = np.zeros(4 << (remainder_log - log_blowup), np.uint32)
+        self.check(lib().zs_commit_job_dev(self.h, d_cols, n_cols, log_n, log_blowup, remainder_log, d_lde, d_nodes,
+                                           root, roots, _p(rem), flags))
+        return root.raw, [roots.raw[32 * i:32 * (i + 1)] for i in range(L)], rem.reshape(-1, 4)
+
+    def commit_batch_dev(self, specs, log_blowup, remainder_log, n_workers=4, flags=0):
+        """specs: list of (d_cols, n_cols, log_n, d_lde, d_nodes).  Returns per job
+        (trace_root, fri_roots, remainder)."""
+        jobs = (Job * len(specs))()
+        keep = []
+        for k, (d_cols, n_cols, log_n, d_lde, d_nodes) in enumerate(specs):
+            L = log_n + log_blowup - remainder_log
+            fr = ctypes.create_string_buffer(32 * L)
+            rem = np.zeros(4 << (remainder_log - log_blowup), np.uint32)
+            keep.append((fr, rem, L))
+            jobs[k].d_cols, jobs[k].n_cols, jobs[k].log_n = d_cols, n_cols, log_n
+            jobs[k].d_lde, jobs[k].d_nodes = d_lde, d_nodes
+            jobs[k].fri_roots = ctypes.cast(fr, vp)
+            jobs[k].remainder = rem.ctypes.data
+        self.check(lib().zs_commit_batch_dev(self.h, jobs, len(specs), log_blowup, remainder_log, n_workers, flags))
+        return [(bytes(jobs[k].trace_root), [keep[k][0].raw[32 * i:32 * (i + 1)] for i in range(keep[k][2])],
+                 keep[k][1].reshape(-1, 4)) for k in range(len(specs))]
 
     # ---- FRI (fri.zig:157-233) ----
     def fri_commit(self, codeword, log_blowup, remainder_log, alphas=None, channel=None, keep=True):

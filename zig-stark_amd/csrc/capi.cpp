@@ -15,43 +15,39 @@
 
 using namespace zs;
 
-struct zs_hip_ctx {
-    Ctx c;
-};
-
-struct zs_tree {
-    Ctx *ctx = nullptr;
-    DevBuf nodes;
-    uint32_t log_leaves = 0;
-    uint8_t root[32];
-};
-
-struct zs_fri_result {
-    Ctx *ctx = nullptr;
-    uint32_t dlog = 0, L = 0, log_blowup = 0, remainder_log = 0;
-    DevBuf layers;  // layer i at layer_off[i] words, 2^(dlog-i) QM31 values
-    DevBuf trees;   // tree i at tree_off[i] words, every level
-    std::vector<size_t> layer_off, tree_off;
-};
-
 namespace zs {
 int32_t Ctx::ensure_scratch(int slot, size_t bytes) {
     DevBuf &b = scratch[slot];
     if (b.bytes >= bytes) return ZS_OK;
     if (b.ptr) {
-        ZS_HIP(this, hipStreamSynchronize(stream));
-        ZS_HIP(this, hipFree(b.ptr));
+        if (stream_ordered) {
+            ZS_HIP(this, hipFreeAsync(b.ptr, stream));
+        } else {
+            ZS_HIP(this, hipStreamSynchronize(stream));
+            ZS_HIP(this, hipFree(b.ptr));
+        }
         b.ptr = nullptr;
         b.bytes = 0;
     }
-    hipError_t e = hipMalloc(&b.ptr, bytes);
+    hipError_t e = stream_ordered ? hipMallocAsync(&b.ptr, bytes, stream) : hipMalloc(&b.ptr, bytes);
     if (e != hipSuccess) {
         b.ptr = nullptr;
-        set_error(std::string("hipMalloc scratch: ") + hipGetErrorString(e));
+        set_error(std::string("scratch allocation: ") + hipGetErrorString(e));
         return e == hipErrorOutOfMemory ? ZS_ERR_OOM : ZS_ERR_DEVICE;
     }
     b.bytes = bytes;
     return ZS_OK;
+}
+
+// give the scratch arenas back (stream-ordered contexts free them on their stream)
+void Ctx::release_scratch() {
+    for (DevBuf &b : scratch) {
+        if (!b.ptr) continue;
+        if (stream_ordered) (void)hipFreeAsync(b.ptr, stream);
+        else (void)hipFree(b.ptr);
+        b.ptr = nullptr;
+        b.bytes = 0;
+    }
 }
 }  // namespace zs
 
@@ -96,8 +92,13 @@ static void prof_clear(Ctx *ctx);
 void zs_hip_destroy(zs_hip_ctx *ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->c.device);
+    for (void *w : ctx->c.workers) zs_hip_destroy(static_cast<zs_hip_ctx *>(w));
+    ctx->c.workers.clear();
     (void)hipStreamSynchronize(ctx->c.stream);
     prof_clear(&ctx->c);
+    ctx->c.release_scratch();
+    (void)hipStreamSynchronize(ctx->c.stream);
+    if (ctx->c.own_stream) (void)hipStreamDestroy(ctx->c.stream);
     delete ctx;
 }
 
@@ -188,11 +189,13 @@ static int32_t host_transform(zs_hip_ctx *h, bool inverse, const uint32_t *in, s
     DevBuf a, b;
     ZS_TRY(dev_alloc(ctx, a, n * 4));
     ZS_TRY(dev_alloc(ctx, b, n * 4));
-    ZS_HIP(ctx, hipMemcpy(a.ptr, in, n * 4, hipMemcpyHostToDevice));
+    ZS_HIP(ctx, hipMemcpyAsync(a.ptr, in, n * 4, hipMemcpyHostToDevice, ctx->stream));
+    ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (inverse) ZS_TRY(launch_ifft(ctx, *hc, a.as<uint32_t>(), n, b.as<uint32_t>(), n, 1));
     else ZS_TRY(launch_fft(ctx, *hc, a.as<uint32_t>(), n, b.as<uint32_t>(), n, 1));
     ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    ZS_HIP(ctx, hipMemcpy(out, b.ptr, n * 4, hipMemcpyDeviceToHost));
+    ZS_HIP(ctx, hipMemcpyAsync(out, b.ptr, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return ZS_OK;
 }
 
@@ -286,16 +289,19 @@ int32_t zs_m31_lde_commit(zs_hip_ctx *h, const uint32_t *cols, uint32_t n_cols, 
     int32_t rc = dev_alloc(ctx, d_cols, n * n_cols * 4);
     if (!rc) rc = dev_alloc(ctx, d_lde, N * n_cols * 4);
     if (!rc) rc = dev_alloc(ctx, t->nodes, tree_words(LT) * 4);
-    if (!rc && hipMemcpy(d_cols.ptr, cols, n * n_cols * 4, hipMemcpyHostToDevice) != hipSuccess) rc = ZS_ERR_DEVICE;
+    if (!rc && hipMemcpyAsync(d_cols.ptr, cols, n * n_cols * 4, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+        rc = ZS_ERR_DEVICE;
     if (!rc)
         rc = zs_m31_lde_commit_dev(h, d_cols.as<uint32_t>(), n_cols, log_n, log_blowup, d_lde.as<uint32_t>(),
                                    t->nodes.as<uint8_t>(), 0);
     if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = ZS_ERR_DEVICE;
-    if (!rc && hipMemcpy(t->root, t->nodes.as<uint8_t>() + (tree_words(LT) - 8) * 4, 32, hipMemcpyDeviceToHost) !=
-                   hipSuccess)
+    if (!rc && hipMemcpyAsync(t->root, t->nodes.as<uint8_t>() + (tree_words(LT) - 8) * 4, 32, hipMemcpyDeviceToHost,
+                              ctx->stream) != hipSuccess)
         rc = ZS_ERR_DEVICE;
-    if (!rc && lde_out && hipMemcpy(lde_out, d_lde.ptr, N * n_cols * 4, hipMemcpyDeviceToHost) != hipSuccess)
+    if (!rc && lde_out &&
+        hipMemcpyAsync(lde_out, d_lde.ptr, N * n_cols * 4, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
         rc = ZS_ERR_DEVICE;
+    if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = ZS_ERR_DEVICE;
     if (rc) {
         delete t;
         return rc;
@@ -323,7 +329,8 @@ int32_t zs_merkle_open(const zs_tree *t, size_t index, uint8_t *path_out) {
     ZS_TRY(dev_alloc(ctx, out, size_t(t->log_leaves) * 32));
     std::vector<PathReq> reqs{{0, 0, t->log_leaves, (uint32_t)index}};
     ZS_TRY(launch_gather_paths(ctx, t->nodes.as<uint32_t>(), reqs, out.as<uint32_t>()));
-    ZS_HIP(ctx, hipMemcpy(path_out, out.ptr, size_t(t->log_leaves) * 32, hipMemcpyDeviceToHost));
+    ZS_HIP(ctx, hipMemcpyAsync(path_out, out.ptr, size_t(t->log_leaves) * 32, hipMemcpyDeviceToHost, ctx->stream));
+    ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return ZS_OK;
 }
 
@@ -332,8 +339,9 @@ int32_t zs_tree_read_level(const zs_tree *t, uint32_t level, uint8_t *out) {
     Ctx *ctx = t->ctx;
     size_t cnt = (size_t(1) << t->log_leaves) >> level;
     ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    ZS_HIP(ctx, hipMemcpy(out, t->nodes.as<uint32_t>() + level_offset_words(t->log_leaves, level), cnt * 32,
-                          hipMemcpyDeviceToHost));
+    ZS_HIP(ctx, hipMemcpyAsync(out, t->nodes.as<uint32_t>() + level_offset_words(t->log_leaves, level), cnt * 32,
+                          hipMemcpyDeviceToHost, ctx->stream));
+    ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return ZS_OK;
 }
 
@@ -419,26 +427,26 @@ int32_t zs_fri_commit_dev(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t dl
         uint32_t *root = trees + r->tree_off[i] + tree_words(li) - 8;
         if (!alphas) {
             uint8_t rb[32];
+            ZS_HIP(ctx, hipMemcpyAsync(rb, root, 32, hipMemcpyDeviceToHost, ctx->stream));
             ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
-            ZS_HIP(ctx, hipMemcpy(rb, root, 32, hipMemcpyDeviceToHost));
             zs_channel_absorb(chan, rb, 32);
             zs_channel_sample_qm31(chan, &host_alpha[4 * i]);
-            ZS_HIP(ctx, hipMemcpy(alpha_dev + 4 * i, &host_alpha[4 * i], 16, hipMemcpyHostToDevice));
+            ZS_HIP(ctx, hipMemcpyAsync(alpha_dev + 4 * i, &host_alpha[4 * i], 16, hipMemcpyHostToDevice, ctx->stream));
         }
         uint32_t *next_leaves = (i + 1 < L) ? trees + r->tree_off[i + 1] : nullptr;
         ZS_TRY(launch_fri_fold(ctx, layers + r->layer_off[i], size_t(1) << li, dlog, i,
                                alpha_dev + 4 * i, layers + r->layer_off[i + 1], next_leaves));
     }
-    ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     for (uint32_t i = 0; i < L; i++) {
         uint32_t li = dlog - i;
-        ZS_HIP(ctx, hipMemcpy(roots_out + 32 * i, trees + r->tree_off[i] + tree_words(li) - 8, 32,
-                              hipMemcpyDeviceToHost));
+        ZS_HIP(ctx, hipMemcpyAsync(roots_out + 32 * i, trees + r->tree_off[i] + tree_words(li) - 8, 32,
+                                   hipMemcpyDeviceToHost, ctx->stream));
     }
     // remainder: interpolate the first 2^(remainder_log - log_blowup) points of layer L (fri.zig:228-233)
     size_t rc = size_t(1) << (remainder_log - log_blowup);
     std::vector<uint32_t> last(rc * 4);
-    ZS_HIP(ctx, hipMemcpy(last.data(), layers + r->layer_off[L], rc * 16, hipMemcpyDeviceToHost));
+    ZS_HIP(ctx, hipMemcpyAsync(last.data(), layers + r->layer_off[L], rc * 16, hipMemcpyDeviceToHost, ctx->stream));
+    ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     std::vector<qm31> xs(rc), ys(rc), co;
     qm31 w = qm31_root_of_unity(dlog - L);
     qm31 acc = fri_offset_pow(L);
@@ -473,7 +481,8 @@ int32_t zs_fri_commit(zs_hip_ctx *h, const uint32_t *codeword, uint32_t dlog, ui
     DevBuf d;
     size_t bytes = (size_t(1) << dlog) * 16;
     ZS_TRY(dev_alloc(ctx, d, bytes));
-    ZS_HIP(ctx, hipMemcpy(d.ptr, codeword, bytes, hipMemcpyHostToDevice));
+    ZS_HIP(ctx, hipMemcpyAsync(d.ptr, codeword, bytes, hipMemcpyHostToDevice, ctx->stream));
+    ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return zs_fri_commit_dev(h, d.as<uint32_t>(), dlog, log_blowup, remainder_log, alphas, chan, 0, roots_out,
                              remainder_out, out_result);
 }
@@ -513,8 +522,10 @@ int32_t zs_fri_query(zs_fri_result *r, zs_channel *chan, size_t num_queries, siz
     ZS_TRY(dev_alloc(ctx, dp, num_queries * per_q_digests * 32));
     ZS_TRY(launch_gather_words(ctx, r->layers.as<uint32_t>(), vreqs, 4, dv.as<uint32_t>()));
     ZS_TRY(launch_gather_paths(ctx, r->trees.as<uint32_t>(), preqs, dp.as<uint32_t>()));
-    ZS_HIP(ctx, hipMemcpy(vals_out, dv.ptr, num_queries * L * 2 * 16, hipMemcpyDeviceToHost));
-    ZS_HIP(ctx, hipMemcpy(paths_out, dp.ptr, num_queries * per_q_digests * 32, hipMemcpyDeviceToHost));
+    ZS_HIP(ctx, hipMemcpyAsync(vals_out, dv.ptr, num_queries * L * 2 * 16, hipMemcpyDeviceToHost, ctx->stream));
+    ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ZS_HIP(ctx, hipMemcpyAsync(paths_out, dp.ptr, num_queries * per_q_digests * 32, hipMemcpyDeviceToHost, ctx->stream));
+    ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return ZS_OK;
 }
 
@@ -522,8 +533,9 @@ int32_t zs_fri_read_layer(const zs_fri_result *r, uint32_t layer, uint32_t *out)
     if (!r || !out || layer > r->L) return ZS_ERR_INPUT;
     Ctx *ctx = r->ctx;
     ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    ZS_HIP(ctx, hipMemcpy(out, r->layers.as<uint32_t>() + r->layer_off[layer], (size_t(1) << (r->dlog - layer)) * 16,
-                          hipMemcpyDeviceToHost));
+    ZS_HIP(ctx, hipMemcpyAsync(out, r->layers.as<uint32_t>() + r->layer_off[layer], (size_t(1) << (r->dlog - layer)) * 16,
+                          hipMemcpyDeviceToHost, ctx->stream));
+    ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return ZS_OK;
 }
 

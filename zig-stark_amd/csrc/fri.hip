@@ -55,7 +55,8 @@ int32_t get_fri_tables(Ctx *ctx, uint32_t dlog, const uint32_t **out) {
     auto buf = std::make_unique<DevBuf>();
     ZS_HIP(ctx, hipMalloc(&buf->ptr, host.size() * 4));
     buf->bytes = host.size() * 4;
-    ZS_HIP(ctx, hipMemcpy(buf->ptr, host.data(), host.size() * 4, hipMemcpyHostToDevice));
+    ZS_HIP(ctx, hipMemcpyAsync(buf->ptr, host.data(), host.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     *out = buf->as<uint32_t>();
     ctx->fri_tables[dlog] = std::move(buf);
     return ZS_OK;
@@ -101,6 +102,39 @@ int32_t launch_fri_fold(Ctx *ctx, const uint32_t *d_cur, uint64_t n, uint32_t dl
     uint64_t half = n / 2;
     ZS_LAUNCH(ctx, "fri_fold_kernel", fri_fold_kernel, dim3((uint32_t)((half + 255) / 256)), dim3(256), 0, ctx->stream,
               d_cur, half, layer, tables, inv_off, d_alpha, d_next, d_next_leaves);
+    ZS_HIP(ctx, hipGetLastError());
+    return ZS_OK;
+}
+
+}  // namespace zs
+
+namespace zs {
+
+// g[i] = sum_c gamma^c * lde_c[i] (QM31 combination of M31 columns); powers: n_cols QM31 values
+__global__ __launch_bounds__(256) void combine_kernel(const uint32_t *lde, uint64_t cs, uint32_t n_cols, uint64_t n,
+                                                      const uint32_t *powers, uint32_t *out) {
+    extern __shared__ uint32_t pw[];
+    for (uint32_t k = threadIdx.x; k < 4 * n_cols; k += blockDim.x) pw[k] = powers[k];
+    __syncthreads();
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    for (uint32_t c = 0; c < n_cols; c++) {
+        uint32_t v = lde[(uint64_t)c * cs + i];
+        a0 = m31_add(a0, m31_mul(pw[4 * c], v));
+        a1 = m31_add(a1, m31_mul(pw[4 * c + 1], v));
+        a2 = m31_add(a2, m31_mul(pw[4 * c + 2], v));
+        a3 = m31_add(a3, m31_mul(pw[4 * c + 3], v));
+    }
+    *reinterpret_cast<uint4 *>(out + 4 * i) = make_uint4(a0, a1, a2, a3);
+}
+
+int32_t launch_combine(Ctx *ctx, const uint32_t *d_lde, uint64_t cs, uint32_t n_cols, uint64_t n,
+                       const uint32_t *d_powers, uint32_t *d_out) {
+    size_t lds = size_t(n_cols) * 16;
+    if (lds > 64 * 1024) return ZS_ERR_INPUT;
+    ZS_LAUNCH(ctx, "combine_kernel", combine_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), lds, ctx->stream,
+              d_lde, cs, n_cols, n, d_powers, d_out);
     ZS_HIP(ctx, hipGetLastError());
     return ZS_OK;
 }

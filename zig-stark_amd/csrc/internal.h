@@ -74,6 +74,9 @@ struct ProfRec {
 
 struct Ctx {
     int device = 0;
+    bool own_stream = false;          // the stream was created for this context
+    bool stream_ordered = false;      // scratch comes from the stream-ordered allocator
+    std::vector<void *> workers;      // worker contexts (zs_hip_ctx *) of batched commits
     bool profiling = false;           // record HIP events around the hot kernels
     std::vector<ProfRec> prof;
     hipStream_t stream = nullptr;  // nullptr = the null stream
@@ -83,10 +86,11 @@ struct Ctx {
     // FRI inverse-domain tables keyed by dlog: [lo table 2^10 | hi table] QM31 words
     std::map<uint32_t, std::unique_ptr<DevBuf>> fri_tables;
     // grow-only scratch arenas
-    DevBuf scratch[8];  // 0 transforms, 1-2 LDE, 3 plain path, 4-6 FRI
+    DevBuf scratch[10];  // 0 transforms, 1-2 LDE, 3 plain path, 4-6 FRI, 7-9 commit jobs
 
     void set_error(const std::string &s) { last_error = s; }
     int32_t ensure_scratch(int slot, size_t bytes);
+    void release_scratch();
 };
 
 // ---- twiddles (fft.hip) ----
@@ -143,6 +147,10 @@ int32_t get_fri_tables(Ctx *ctx, uint32_t dlog, const uint32_t **out);
 int32_t launch_fri_fold(Ctx *ctx, const uint32_t *d_cur, uint64_t n, uint32_t dlog, uint32_t layer,
                         const uint32_t *d_alpha, uint32_t *d_next, uint32_t *d_next_leaves);
 
+// g[i] = sum_c powers[c] * lde_c[i]: QM31 combination of n_cols M31 columns of n values
+int32_t launch_combine(Ctx *ctx, const uint32_t *d_lde, uint64_t cs, uint32_t n_cols, uint64_t n,
+                       const uint32_t *d_powers, uint32_t *d_out);
+
 // host-side constants
 qm31 fri_offset_pow(uint32_t layer);  // FRI_OFFSET^(2^layer)
 qm31 qm31_root_of_unity(uint32_t log_size);
@@ -157,3 +165,25 @@ inline size_t level_offset_words(uint32_t log_leaves, uint32_t level) {
 }
 
 }  // namespace zs
+
+// ---- handle types of the C ABI (defined here so every translation unit can use them) ----
+struct zs_hip_ctx {
+    zs::Ctx c;
+};
+
+struct zs_tree {
+    zs::Ctx *ctx = nullptr;
+    zs::DevBuf nodes;
+    uint32_t log_leaves = 0;
+    uint8_t root[32];
+};
+
+struct zs_fri_result {
+    zs::Ctx *ctx = nullptr;
+    uint32_t dlog = 0, L = 0, log_blowup = 0, remainder_log = 0;
+    zs::DevBuf layers;  // layer i at layer_off[i] words, 2^(dlog-i) QM31 values
+    zs::DevBuf trees;   // tree i at tree_off[i] words, every level
+    std::vector<size_t> layer_off, tree_off;
+};
+
+

@@ -1,0 +1,320 @@
+// job.cpp -- one trace's full commitment, uneven batches, and the serving loop.
+//
+// The reference commits traces one at a time on the CPU (src/m31/stark.zig:
+// 335-414 for the trace trees, src/m31/fri.zig:157-233 for FRI) and has no
+// batching or serving layer.  Here a trace's commitment is one call
+// (commit_job), a batch spreads uneven traces over worker contexts that each
+// own a HIP stream and take jobs from a shared queue, and the server admits
+// requests against an HBM budget and runs them on its workers while others
+// are in flight (continuous batching).  Results do not depend on the mix: every
+// job runs the same kernels on its own buffers and scratch.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "internal.h"
+
+using namespace zs;
+
+namespace {
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+constexpr char kCommitSep[] = "zig-stark:hip:commit";
+
+int32_t commit_job(zs_hip_ctx *h, const uint32_t *d_cols, uint32_t n_cols, uint32_t log_n, uint32_t log_blowup,
+                   uint32_t remainder_log, uint32_t *d_lde, uint8_t *d_nodes, uint8_t trace_root[32],
+                   uint8_t *fri_roots, uint32_t *remainder, uint32_t flags) {
+    if (!h || !d_cols || !d_lde || !d_nodes || !trace_root || !fri_roots || !remainder) return ZS_ERR_INPUT;
+    const uint32_t LT = log_n + log_blowup;
+    if (n_cols == 0 || n_cols > 4096 || log_n < 1 || LT > 28 || remainder_log < log_blowup || remainder_log >= LT)
+        return ZS_ERR_INPUT;
+    Ctx *ctx = &h->c;
+    const size_t N = size_t(1) << LT;
+    // 1. LDE and the row-leaf tree
+    ZS_TRY(zs_m31_lde_commit_dev(h, d_cols, n_cols, log_n, log_blowup, d_lde, d_nodes, flags & ZS_FLAG_PLAIN));
+    ZS_HIP(ctx, hipMemcpyAsync(trace_root, d_nodes + (tree_words(LT) - 8) * 4, 32, hipMemcpyDeviceToHost,
+                               ctx->stream));
+    ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    // 2. gamma from the trace root
+    std::unique_ptr<zs_channel, void (*)(zs_channel *)> chan(
+        zs_channel_new(reinterpret_cast<const uint8_t *>(kCommitSep), sizeof(kCommitSep) - 1), zs_channel_free);
+    zs_channel_absorb(chan.get(), trace_root, 32);
+    uint32_t gw[4];
+    zs_channel_sample_qm31(chan.get(), gw);
+    // 3. g = sum_c gamma^c lde_c
+    std::vector<uint32_t> pw(size_t(n_cols) * 4);
+    qm31 gamma = qm_from_words(gw), acc = {{1, 0}, {0, 0}};
+    for (uint32_t c = 0; c < n_cols; c++) {
+        qm_to_words(acc, &pw[4 * c]);
+        acc = qm_mul(acc, gamma);
+    }
+    ZS_TRY(ctx->ensure_scratch(9, pw.size() * 4));
+    ZS_TRY(ctx->ensure_scratch(8, N * 16));
+    ZS_HIP(ctx, hipMemcpyAsync(ctx->scratch[9].ptr, pw.data(), pw.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    ZS_TRY(launch_combine(ctx, d_lde, N, n_cols, N, ctx->scratch[9].as<uint32_t>(), ctx->scratch[8].as<uint32_t>()));
+    // 4. FRI commit phase on g, challenges from the same channel
+    return zs_fri_commit_dev(h, ctx->scratch[8].as<uint32_t>(), LT, log_blowup, remainder_log, nullptr, chan.get(), 0,
+                             fri_roots, remainder, nullptr);
+}
+
+// device bytes one request holds while it runs (its buffers and the scratch it grows)
+uint64_t job_footprint(uint32_t n_cols, uint32_t log_n, uint32_t log_blowup, uint32_t remainder_log) {
+    const uint64_t n = uint64_t(1) << log_n, N = uint64_t(1) << (log_n + log_blowup), c = n_cols;
+    uint64_t fri_trees = 0, fri_layers = 0;
+    for (uint32_t i = 0; i + remainder_log < log_n + log_blowup; i++) fri_trees += ((2 * N) >> i) * 32;
+    for (uint32_t i = 0; i + remainder_log <= log_n + log_blowup; i++) fri_layers += (N >> i) * 16;
+    return 4 * c * n              // input
+           + 4 * c * N            // LDE
+           + 32 * (2 * N - 1)     // row tree
+           + 4 * c * n + 4 * c * N  // LDE scratch: coefficients and the multi-pass array
+           + 16 * N + 16 * c      // combination codeword and powers
+           + fri_layers + fri_trees;
+}
+
+zs_hip_ctx *new_worker(int device, bool stream_ordered) {
+    zs_hip_ctx *w = nullptr;
+    if (zs_hip_init(device, 0, &w) != ZS_OK) return nullptr;
+    hipStream_t s = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+        zs_hip_destroy(w);
+        return nullptr;
+    }
+    w->c.stream = s;
+    w->c.own_stream = true;
+    w->c.stream_ordered = stream_ordered;
+    return w;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t zs_commit_job_dev(zs_hip_ctx *ctx, const uint32_t *d_cols, uint32_t n_cols, uint32_t log_n,
+                          uint32_t log_blowup, uint32_t remainder_log, uint32_t *d_lde, uint8_t *d_nodes,
+                          uint8_t trace_root[32], uint8_t *fri_roots, uint32_t *remainder, uint32_t flags) {
+    return commit_job(ctx, d_cols, n_cols, log_n, log_blowup, remainder_log, d_lde, d_nodes, trace_root, fri_roots,
+                      remainder, flags);
+}
+
+int32_t zs_commit_batch_dev(zs_hip_ctx *h, zs_job *jobs, size_t n_jobs, uint32_t log_blowup, uint32_t remainder_log,
+                            uint32_t n_workers, uint32_t flags) {
+    if (!h || (!jobs && n_jobs)) return ZS_ERR_INPUT;
+    Ctx *ctx = &h->c;
+    auto run_one = [&](zs_hip_ctx *w, zs_job &j) {
+        j.status = commit_job(w, j.d_cols, j.n_cols, j.log_n, log_blowup, remainder_log, j.d_lde, j.d_nodes,
+                              j.trace_root, j.fri_roots, j.remainder, flags);
+    };
+    if (n_workers <= 1 || (flags & ZS_FLAG_PLAIN)) {
+        for (size_t i = 0; i < n_jobs; i++) run_one(h, jobs[i]);
+    } else {
+        ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));  // inputs written on the caller's stream are complete
+        auto &ws = ctx->workers;
+        while (ws.size() < n_workers) {
+            zs_hip_ctx *w = new_worker(ctx->device, false);
+            if (!w) return ZS_ERR_DEVICE;
+            ws.push_back(w);
+        }
+        std::atomic<size_t> next{0};
+        std::vector<std::thread> th;
+        for (uint32_t t = 0; t < n_workers; t++) {
+            th.emplace_back([&, t] {
+                zs_hip_ctx *w = static_cast<zs_hip_ctx *>(ws[t]);
+                (void)hipSetDevice(ctx->device);
+                for (size_t i; (i = next.fetch_add(1)) < n_jobs;) run_one(w, jobs[i]);
+                (void)hipStreamSynchronize(w->c.stream);
+            });
+        }
+        for (auto &x : th) x.join();
+    }
+    for (size_t i = 0; i < n_jobs; i++)
+        if (jobs[i].status != ZS_OK) return jobs[i].status;
+    return ZS_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// serving loop
+// ---------------------------------------------------------------------------
+struct ServReq {
+    uint64_t id;
+    zs_request r;
+    std::vector<uint32_t> host;
+    zs_response resp;
+    uint64_t bytes;
+    bool done = false;
+};
+
+struct zs_server {
+    int device = 0;
+    uint64_t budget = 0;
+    uint64_t in_use = 0, peak = 0;
+    std::vector<zs_hip_ctx *> workers;
+    std::vector<std::thread> threads;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<ServReq *> queue;
+    std::map<uint64_t, std::unique_ptr<ServReq>> reqs;
+    uint64_t next_id = 1;
+    bool stop = false;
+};
+
+static void serve_one(zs_hip_ctx *w, ServReq *q) {
+    Ctx *ctx = &w->c;
+    const zs_request &r = q->r;
+    const uint32_t LT = r.log_n + r.log_blowup;
+    const size_t n = size_t(1) << r.log_n, N = size_t(1) << LT;
+    void *d_cols = nullptr, *d_lde = nullptr, *d_nodes = nullptr;
+    int32_t rc = ZS_OK;
+    if (hipMallocAsync(&d_cols, n * r.n_cols * 4, ctx->stream) != hipSuccess ||
+        hipMallocAsync(&d_lde, N * r.n_cols * 4, ctx->stream) != hipSuccess ||
+        hipMallocAsync(&d_nodes, tree_words(LT) * 4, ctx->stream) != hipSuccess)
+        rc = ZS_ERR_OOM;
+    if (!rc) {
+        if (r.host_cols)
+            rc = hipMemcpyAsync(d_cols, q->host.data(), n * r.n_cols * 4, hipMemcpyHostToDevice, ctx->stream) ==
+                         hipSuccess
+                     ? ZS_OK
+                     : ZS_ERR_DEVICE;
+        else
+            rc = zs_fill_m31_dev(w, r.seed, static_cast<uint32_t *>(d_cols), n * r.n_cols);
+    }
+    if (!rc)
+        rc = commit_job(w, static_cast<uint32_t *>(d_cols), r.n_cols, r.log_n, r.log_blowup, r.remainder_log,
+                        static_cast<uint32_t *>(d_lde), static_cast<uint8_t *>(d_nodes), q->resp.trace_root,
+                        q->resp.fri_roots, q->resp.remainder, 0);
+    if (d_cols) (void)hipFreeAsync(d_cols, ctx->stream);
+    if (d_lde) (void)hipFreeAsync(d_lde, ctx->stream);
+    if (d_nodes) (void)hipFreeAsync(d_nodes, ctx->stream);
+    ctx->release_scratch();
+    (void)hipStreamSynchronize(ctx->stream);
+    q->resp.status = rc;
+    q->resp.fri_layers = LT - r.remainder_log;
+}
+
+static void serve_loop(zs_server *s, zs_hip_ctx *w) {
+    (void)hipSetDevice(s->device);
+    for (;;) {
+        ServReq *q = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_work.wait(lk, [&] {
+                if (s->stop && s->queue.empty()) return true;
+                for (ServReq *x : s->queue)
+                    if (s->in_use + x->bytes <= s->budget) return true;
+                return false;
+            });
+            if (s->stop && s->queue.empty()) return;
+            // first request that fits what is free (earlier large ones wait for memory)
+            for (auto it = s->queue.begin(); it != s->queue.end(); ++it) {
+                if (s->in_use + (*it)->bytes <= s->budget) {
+                    q = *it;
+                    s->queue.erase(it);
+                    break;
+                }
+            }
+            if (!q) continue;
+            s->in_use += q->bytes;
+            s->peak = std::max(s->peak, s->in_use);
+            q->resp.t_admit = now_s();
+        }
+        serve_one(w, q);
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            q->resp.t_done = now_s();
+            q->done = true;
+            s->in_use -= q->bytes;
+        }
+        s->cv_done.notify_all();
+        s->cv_work.notify_all();
+    }
+}
+
+extern "C" {
+
+zs_server *zs_server_new(int32_t device, uint32_t n_workers, uint64_t hbm_budget_bytes) {
+    if (n_workers == 0 || n_workers > 64) return nullptr;
+    auto *s = new zs_server();
+    s->device = device;
+    s->budget = hbm_budget_bytes;
+    for (uint32_t i = 0; i < n_workers; i++) {
+        zs_hip_ctx *w = new_worker(device, true);
+        if (!w) {
+            zs_server_free(s);
+            return nullptr;
+        }
+        s->workers.push_back(w);
+    }
+    for (zs_hip_ctx *w : s->workers) s->threads.emplace_back(serve_loop, s, w);
+    return s;
+}
+
+int32_t zs_server_submit(zs_server *s, const zs_request *req, uint64_t *ticket) {
+    if (!s || !req || !ticket) return ZS_ERR_INPUT;
+    const uint32_t LT = req->log_n + req->log_blowup;
+    if (req->n_cols == 0 || req->n_cols > 4096 || req->log_n < 1 || LT > 28 || req->remainder_log < req->log_blowup ||
+        req->remainder_log >= LT || req->remainder_log - req->log_blowup > 6)
+        return ZS_ERR_INPUT;
+    auto q = std::make_unique<ServReq>();
+    q->r = *req;
+    std::memset(&q->resp, 0, sizeof q->resp);
+    q->resp.t_submit = now_s();
+    if (req->host_cols) q->host.assign(req->host_cols, req->host_cols + (size_t(req->n_cols) << req->log_n));
+    q->bytes = job_footprint(req->n_cols, req->log_n, req->log_blowup, req->remainder_log);
+    q->resp.device_bytes = q->bytes;
+    if (q->bytes > s->budget) return ZS_ERR_OOM;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        q->id = s->next_id++;
+        *ticket = q->id;
+        s->queue.push_back(q.get());
+        s->reqs[q->id] = std::move(q);
+    }
+    s->cv_work.notify_all();
+    return ZS_OK;
+}
+
+int32_t zs_server_wait(zs_server *s, uint64_t ticket, zs_response *out) {
+    if (!s || !out) return ZS_ERR_INPUT;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return ZS_ERR_INPUT;
+    ServReq *q = it->second.get();
+    s->cv_done.wait(lk, [&] { return q->done; });
+    *out = q->resp;
+    s->reqs.erase(it);
+    return ZS_OK;
+}
+
+int32_t zs_server_memory(zs_server *s, uint64_t *in_use, uint64_t *peak) {
+    if (!s) return ZS_ERR_INPUT;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (in_use) *in_use = s->in_use;
+    if (peak) *peak = s->peak;
+    return ZS_OK;
+}
+
+void zs_server_free(zs_server *s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    for (auto &t : s->threads) t.join();
+    for (zs_hip_ctx *w : s->workers) zs_hip_destroy(w);
+    delete s;
+}
+
+}  // extern "C"

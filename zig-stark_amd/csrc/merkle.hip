@@ -274,7 +274,8 @@ int32_t launch_gather_paths(Ctx *ctx, const uint32_t *d_trees, const std::vector
     if (reqs.empty()) return ZS_OK;
     DevBuf r;
     ZS_HIP(ctx, hipMalloc(&r.ptr, reqs.size() * sizeof(PathReq)));
-    ZS_HIP(ctx, hipMemcpy(r.ptr, reqs.data(), reqs.size() * sizeof(PathReq), hipMemcpyHostToDevice));
+    ZS_HIP(ctx, hipMemcpyAsync(r.ptr, reqs.data(), reqs.size() * sizeof(PathReq), hipMemcpyHostToDevice, ctx->stream));
+    ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     uint32_t n = (uint32_t)reqs.size();
     hipLaunchKernelGGL(gather_paths_kernel, dim3((n + 127) / 128), dim3(128), 0, ctx->stream, d_trees,
                        r.as<PathReq>(), n, d_out);
@@ -288,7 +289,8 @@ int32_t launch_gather_words(Ctx *ctx, const uint32_t *d_src, const std::vector<u
     if (pairs.empty()) return ZS_OK;
     DevBuf r;
     ZS_HIP(ctx, hipMalloc(&r.ptr, pairs.size() * 8));
-    ZS_HIP(ctx, hipMemcpy(r.ptr, pairs.data(), pairs.size() * 8, hipMemcpyHostToDevice));
+    ZS_HIP(ctx, hipMemcpyAsync(r.ptr, pairs.data(), pairs.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+    ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     uint32_t n = (uint32_t)(pairs.size() / 2);
     hipLaunchKernelGGL(gather_words_kernel, dim3((n + 127) / 128), dim3(128), 0, ctx->stream, d_src,
                        r.as<uint64_t>(), n, words, d_out);
