@@ -3,10 +3,23 @@
 per-column plain path, with the oracle timed on the host as the CPU baseline.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--plain-steps P] [--no-cpu]
+                [--no-fri] [--batch-traces T] [--serve-requests R]
 
 Prints one JSON line.  A "step" is one zs_m31_lde_commit_dev call on inputs
 already resident in HBM (IFFT + blowup-4 FFT of every column, 2^22 row
-leaves, every Merkle level).
+leaves, every Merkle level).  Beside the headline the line carries:
+  fri:    one QM31 column of 2^22 folded to 2^5 with seeded challenges (graph
+          replay and per-kernel launches);
+  full_commit: the whole trace commitment of the same 64 x 2^20 trace (LDE,
+          row tree, channel challenge, QM31 combination, FRI with channel
+          challenges);
+  batch:  256 traces of uneven size (log rows 12..20, 8..128 columns) through
+          the same commitment, spread over worker streams (worker count picked
+          by measurement), against one trace at a time and the reference-shaped
+          per-trace plain loop;
+  serve:  Poisson arrivals (mean 200/s) of commit requests (log rows 14..22,
+          16..256 columns) under a 200 GB device-memory budget: throughput and
+          p50/p99 latency.
 """
 import argparse
 import json
@@ -56,6 +69,151 @@ def load_traffic(path):
     return out
 
 
+def fri_algorithmic_bytes(dlog, rem_log):
+    """Per committed element of a layer: read 16 B, write a 32-byte leaf, ~32 B of
+    nodes, and 8 B of fold output (half as many 16-byte values)."""
+    return 88 * sum(1 << (dlog - i) for i in range(dlog - rem_log))
+
+
+def fri_leg(Z, ctx, torch, dev):
+    import numpy as np
+    dlog, beta, rem_log = 22, 2, 5
+    d_cw = torch.empty((1 << dlog) * 4, dtype=torch.int32, device=dev)
+    ctx.fill_m31_dev(3, d_cw.data_ptr(), (1 << dlog) * 4)
+    rng = np.random.default_rng(3)
+    alphas = rng.integers(0, Z.P, (dlog - rem_log, 4), dtype=np.uint32)
+
+    def call(flags=0):
+        return ctx.fri_commit_dev(d_cw.data_ptr(), dlog, beta, rem_log, alphas=alphas, flags=flags)
+
+    r_graph = call()
+    r_eager = call(Z.ZS_FLAG_NO_GRAPH)
+    same = r_graph[0] == r_eager[0] and bool(np.array_equal(r_graph[1], r_eager[1]))
+    reps = 10
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        call()
+    graph_ms = (time.perf_counter() - t0) * 1e3 / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        call(Z.ZS_FLAG_NO_GRAPH)
+    eager_ms = (time.perf_counter() - t0) * 1e3 / reps
+    ctx.profile(True)
+    call()
+    kern = ctx.profile_read()
+    ctx.profile(False)
+    alg = fri_algorithmic_bytes(dlog, rem_log)
+    best = min(graph_ms, eager_ms)
+    return {"workload": f"one QM31 column of 2^{dlog} on the FRI domain, {dlog - rem_log} committed layers "
+                        f"folded to 2^{rem_log}, seeded challenges (seed 3)",
+            "ms_graph": round(graph_ms, 3), "ms_launches": round(eager_ms, 3), "graph_equals_launches": same,
+            "algorithmic_bytes": alg, "GBps": round(alg / (best * 1e-3) / 1e9, 1),
+            "frac_hbm": round(alg / (best * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "kernels_ms": {k: round(v[0], 4) for k, v in sorted(kern.items(), key=lambda kv: -kv[1][0])[:6]},
+            "root0": r_graph[0][0].hex()}
+
+
+def full_commit_leg(ctx, d_cols, n_cols, lg, beta, d_lde, d_nodes):
+    ctx.commit_job_dev(d_cols.data_ptr(), n_cols, lg, beta, 5, d_lde.data_ptr(), d_nodes.data_ptr())
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        root, roots, rem = ctx.commit_job_dev(d_cols.data_ptr(), n_cols, lg, beta, 5, d_lde.data_ptr(),
+                                              d_nodes.data_ptr())
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    return {"workload": f"{n_cols} x 2^{lg}: LDE x{1 << beta}, row tree, gamma from the trace root, "
+                        f"QM31 combination, FRI to 2^5 with channel challenges",
+            "ms": round(ms, 3), "Melem_per_s": round(n_cols * (1 << lg) / (ms * 1e-3) / 1e6, 1),
+            "trace_root": root.hex(), "fri_root0": roots[0].hex()}
+
+
+def batch_leg(Z, torch, dev, stream, n_traces):
+    """Uneven batch: per-trace buffers resident in HBM, seeded inputs (seed 4, stream k)."""
+    import numpy as np
+    rng = np.random.default_rng(4)
+    lgs = rng.integers(12, 21, n_traces)
+    cols = rng.integers(8, 129, n_traces)
+    bctx = Z.Context(0)
+    bctx.set_stream(stream.cuda_stream)
+    bufs, specs = [], []
+    for k in range(n_traces):
+        c, lg = int(cols[k]), int(lgs[k])
+        n, N = 1 << lg, 1 << (lg + 2)
+        d_c = torch.empty((c, n), dtype=torch.int32, device=dev)
+        d_l = torch.empty((c, N), dtype=torch.int32, device=dev)
+        d_t = torch.empty((2 * N - 1) * 8, dtype=torch.int32, device=dev)
+        bctx.fill_m31_dev((4 << 32) + k, d_c.data_ptr(), c * n)
+        bufs.append((d_c, d_l, d_t))
+        specs.append((d_c.data_ptr(), c, lg, d_l.data_ptr(), d_t.data_ptr()))
+    torch.cuda.synchronize()
+    elems = int(sum(int(c) << int(lg) for c, lg in zip(cols, lgs)))
+
+    def run(workers, flags=0):
+        t0 = time.perf_counter()
+        out = bctx.commit_batch_dev(specs, 2, 5, n_workers=workers, flags=flags)
+        return (time.perf_counter() - t0) * 1e3, out
+
+    run(4)  # warm: twiddles, FRI tables, worker scratch
+    tune = {}
+    for w in (2, 4, 8):
+        run(w)
+        tune[w] = round(run(w)[0], 2)
+    best_w = min(tune, key=tune.get)
+    best_ms, best = run(best_w)
+    run(1)
+    seq_ms, seq = run(1)
+    plain_ms, plain = run(1, Z.ZS_FLAG_PLAIN)
+    same = all(a[0] == b[0] == c[0] and a[1] == b[1] == c[1] and np.array_equal(a[2], b[2])
+               and np.array_equal(a[2], c[2]) for a, b, c in zip(best, seq, plain))
+    bctx.close()
+    del bufs, specs
+    torch.cuda.empty_cache()
+    return {"workload": f"{n_traces} traces, log rows U{{12..20}}, columns U{{8..128}} (seed 4), blowup 4, "
+                        f"LDE + row tree + FRI to 2^5 each",
+            "elements": elems, "ms": round(best_ms, 2), "Melem_per_s": round(elems / (best_ms * 1e-3) / 1e6, 1),
+            "workers": best_w, "workers_ms": tune,
+            "one_at_a_time_ms": round(seq_ms, 2), "plain_loop_ms": round(plain_ms, 2),
+            "speedup_vs_plain_loop": round(plain_ms / best_ms, 2), "bit_exact_all_paths": same}
+
+
+def serve_leg(Z, n_requests, n_workers=4, budget=200 * 10**9):
+    import numpy as np
+    rng = np.random.default_rng(5)
+    gaps = rng.exponential(1 / 200.0, n_requests)
+    lgs = rng.integers(14, 23, n_requests)
+    cols = rng.integers(16, 257, n_requests)
+    srv = Z.Server(0, n_workers=n_workers, hbm_budget=budget)
+    # warm every size on the workers (twiddle tables and FRI tables stay resident)
+    warm = [srv.submit(16, lg, 2, 5, seed=lg) for lg in range(14, 23) for _ in range(n_workers)]
+    for t in warm:
+        srv.wait(t)
+    t0 = time.perf_counter()
+    at = np.cumsum(gaps)
+    tickets = []
+    for k in range(n_requests):
+        d = t0 + at[k] - time.perf_counter()
+        if d > 0:
+            time.sleep(d)
+        tickets.append(srv.submit(int(cols[k]), int(lgs[k]), 2, 5, seed=(5 << 32) + k))
+    resp = [srv.wait(t) for t in tickets]
+    _, peak = srv.memory()
+    srv.close()
+    ok = all(r.status == 0 for r in resp)
+    lat = np.array([r.t_done - r.t_submit for r in resp]) * 1e3
+    qwait = np.array([r.t_admit - r.t_submit for r in resp]) * 1e3
+    span = max(r.t_done for r in resp) - min(r.t_submit for r in resp)
+    elems = int(sum(int(c) << int(lg) for c, lg in zip(cols, lgs)))
+    return {"workload": f"{n_requests} requests, Poisson arrivals mean 200/s, log rows U{{14..22}}, "
+                        f"columns U{{16..256}} (seed 5), blowup 4, LDE + row tree + FRI to 2^5, "
+                        f"inputs generated on the device from the request seed",
+            "workers": n_workers, "budget_bytes": budget, "peak_admitted_bytes": peak, "all_ok": ok,
+            "offered_rate_per_s": round(n_requests / at[-1], 1),
+            "completed_per_s": round(n_requests / span, 1), "Melem_per_s": round(elems / span / 1e6, 1),
+            "latency_ms_p50": round(float(np.percentile(lat, 50)), 2),
+            "latency_ms_p99": round(float(np.percentile(lat, 99)), 2),
+            "queue_wait_ms_p50": round(float(np.percentile(qwait, 50)), 2)}
+
+
 def cpu_baseline(n_cols, log_n, log_blowup):
     """The oracle (C restatement of the reference algorithm, one thread) on a bounded
     sample: two full columns' LDE and the row leaves + tree of 2^13 rows x n_cols,
@@ -95,6 +253,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fri", action="store_true")
+    ap.add_argument("--batch-traces", type=int, default=256, help="uneven-batch leg size (0 = skip)")
+    ap.add_argument("--serve-requests", type=int, default=200, help="serving leg size (0 = skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device copy (for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -169,20 +329,10 @@ def main():
         root_plain = d_nodes[-8:].cpu().numpy().view(np.uint32).tobytes()
         bit_exact = root_plain == root_fused
 
-    # FRI commit phase: one QM31 column of 2^22 folded to 2^5, seeded challenges
-    fri_ms = None
-    if not args.no_fri:
-        dlog, rem_log = 22, 5
-        d_cw = torch.empty((1 << dlog) * 4, dtype=torch.int32, device=dev)
-        ctx.fill_m31_dev(3, d_cw.data_ptr(), (1 << dlog) * 4)
-        rng = np.random.default_rng(3)
-        alphas = rng.integers(0, Z.P, (dlog - rem_log, 4), dtype=np.uint32)
-        ctx.fri_commit_dev(d_cw.data_ptr(), dlog, 2, rem_log, alphas=alphas)
-        t0 = time.perf_counter()
-        reps = 3
-        for _ in range(reps):
-            ctx.fri_commit_dev(d_cw.data_ptr(), dlog, 2, rem_log, alphas=alphas)
-        fri_ms = (time.perf_counter() - t0) * 1e3 / reps
+    fri = None if args.no_fri else fri_leg(Z, ctx, torch, dev)
+    full = None if args.no_fri else full_commit_leg(ctx, d_cols, n_cols, lg, beta, d_lde, d_nodes)
+    batch = batch_leg(Z, torch, dev, stream, args.batch_traces) if args.batch_traces > 0 else None
+    serve = serve_leg(Z, args.serve_requests) if args.serve_requests > 0 else None
 
     elems = n_cols * n
     value = elems / (ms * 1e-3) / 1e6
@@ -215,7 +365,10 @@ def main():
         "plain": {"ms_per_step": round(plain_ms, 3) if plain_ms else None,
                   "speedup_fused_vs_plain": round(plain_ms / ms, 2) if plain_ms else None,
                   "bit_exact_root": bit_exact},
-        "fri_2p22_to_2p5_ms": round(fri_ms, 3) if fri_ms else None,
+        "fri": fri,
+        "full_commit": full,
+        "batch": batch,
+        "serve": serve,
         "root": root_fused.hex(),
     }
     if not args.no_cpu:

@@ -43,6 +43,7 @@ extern "C" {
 
 /* flags for zs_m31_lde_commit_dev / zs_fri_commit_dev */
 #define ZS_FLAG_PLAIN 1u /* reference-shaped per-column, per-layer launches (comparison path) */
+#define ZS_FLAG_NO_GRAPH 2u /* FRI with given challenges: launch each kernel instead of replaying a graph */
 
 typedef struct zs_hip_ctx zs_hip_ctx;
 typedef struct zs_tree zs_tree;
@@ -155,7 +156,10 @@ size_t zs_channel_sample_index(zs_channel *ch, size_t n);                     /*
  *           NULL the remainder is absorbed into `chan` afterwards.
  *   roots_out: L * 32 bytes; remainder_out: 4 * 2^(remainder_log - log_blowup) words.
  *   out_result: optional; keeps every layer and tree on the device for queries.
- * d_codeword is a device pointer; it is not modified. */
+ * d_codeword is a device pointer; it is not modified.  With alphas given and no
+ * out_result, the leaf, node-level and fold kernels of every layer are captured
+ * once per buffer set into a HIP graph and replayed on later calls
+ * (ZS_FLAG_NO_GRAPH launches them one by one instead). */
 int32_t zs_fri_commit_dev(zs_hip_ctx *ctx, const uint32_t *d_codeword, uint32_t dlog, uint32_t log_blowup,
                           uint32_t remainder_log, const uint32_t *alphas, zs_channel *chan, uint32_t flags,
                           uint8_t *roots_out, uint32_t *remainder_out, zs_fri_result **out_result);
@@ -208,9 +212,13 @@ typedef struct {
 int32_t zs_commit_batch_dev(zs_hip_ctx *ctx, zs_job *jobs, size_t n_jobs, uint32_t log_blowup,
                             uint32_t remainder_log, uint32_t n_workers, uint32_t flags);
 
-/* Serving loop: requests are admitted while their device memory fits the HBM
- * budget, run on worker streams concurrently with the ones already on the
- * device, and answered with their results and timestamps. */
+/* Serving loop (continuous batching): the HBM budget is allocated once when the
+ * server starts; each request's buffers (input, LDE, row tree, FRI layers and
+ * trees, scratch) are carved from it first-fit.  A queued request is admitted as
+ * soon as a worker is idle and its memory is free, and runs on that worker's
+ * stream concurrently with the requests already on the device.  Twiddle and
+ * FRI tables are shared by the workers and stay resident.  A request's result
+ * does not depend on what else is running. */
 typedef struct zs_server zs_server;
 typedef struct {
     uint32_t n_cols, log_n, log_blowup, remainder_log;
@@ -226,11 +234,14 @@ typedef struct {
     double t_submit, t_admit, t_done;  /* seconds, steady clock */
     uint64_t device_bytes;             /* the request's admitted footprint */
 } zs_response;
+/* NULL if the device, the budget allocation or a worker stream fails. */
 zs_server *zs_server_new(int32_t device, uint32_t n_workers, uint64_t hbm_budget_bytes);
 /* Enqueue; returns immediately with a ticket. ZS_ERR_OOM if the request alone exceeds the budget. */
 int32_t zs_server_submit(zs_server *srv, const zs_request *req, uint64_t *ticket);
 /* Block until the request is answered (the response is then released). */
 int32_t zs_server_wait(zs_server *srv, uint64_t ticket, zs_response *out);
+/* Non-blocking: *ready = 1 and the response (then released) if answered, else *ready = 0. */
+int32_t zs_server_poll(zs_server *srv, uint64_t ticket, zs_response *out, int32_t *ready);
 /* Device bytes admitted right now, and the largest value seen. */
 int32_t zs_server_memory(zs_server *srv, uint64_t *in_use, uint64_t *peak);
 void zs_server_free(zs_server *srv);

@@ -98,3 +98,28 @@ def test_fri_full_size_2p22(zctx):
     for i in range(L - 4, L):
         leaves = b"".join(O.hash_qm31(v) for v in res.layer(i))
         assert roots[i] == O.tree_root(O.merkle_build(leaves))
+
+
+@pytest.mark.parametrize("dlog,beta,rem_log", [(10, 2, 5), (15, 2, 5), (18, 1, 3)])
+def test_fri_graph_replays_match_eager_and_oracle(zctx, torch_dev, dlog, beta, rem_log):
+    """Challenges given up front: the captured graph is replayed on later calls.
+    Every replay (new challenges, new codeword contents in the same buffer, eager
+    work and synchronisations in between) equals the eager launches and the oracle."""
+    import torch
+    from conftest import to_dev
+    rng = np.random.default_rng(dlog * 7 + beta)
+    L = dlog - rem_log
+    d_cw = torch.empty((1 << dlog) * 4, dtype=torch.int32, device=torch_dev)
+    for rep in range(4):
+        cw = rng.integers(0, P, (1 << dlog, 4), dtype=np.uint32)
+        d_cw.copy_(to_dev(cw.reshape(-1), torch_dev))
+        alphas = rng.integers(0, P, (L, 4), dtype=np.uint32)
+        g_roots, g_rem, _ = zctx.fri_commit_dev(d_cw.data_ptr(), dlog, beta, rem_log, alphas=alphas)
+        e_roots, e_rem, _ = zctx.fri_commit_dev(d_cw.data_ptr(), dlog, beta, rem_log, alphas=alphas,
+                                                flags=Z.ZS_FLAG_NO_GRAPH)
+        assert g_roots == e_roots and np.array_equal(g_rem, e_rem), rep
+        if dlog <= 15:
+            oroots, _, orem, _ = O.fri_commit(cw, beta, rem_log, alphas=alphas, want_layers=False)
+            assert g_roots == oroots and np.array_equal(g_rem, orem), rep
+        torch.cuda.synchronize()
+        _ = (d_cw.view(-1)[:1024].long().sum()).item()  # eager work between replays

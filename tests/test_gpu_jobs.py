@@ -83,9 +83,17 @@ def test_server_results_same_alone_and_mixed(zctx, torch_dev):
     oroot, oroots, orem, _, _ = O.commit_job(cols, 2, 5)
     got = mixed[0].result(2, 5)
     assert got[0] == oroot and got[1] == oroots and np.array_equal(got[2], orem)
-    # host-buffer request path
+    # host-buffer request path, answered through poll
     t = srv.submit(c, lg, 2, 5, host_cols=cols)
-    assert srv.wait(t).result(2, 5)[0] == oroot
+    import time
+    for _ in range(6000):
+        resp = srv.poll(t)
+        if resp is not None:
+            break
+        time.sleep(0.005)
+    assert resp is not None and resp.result(2, 5)[0] == oroot
+    in_use, _ = srv.memory()
+    assert in_use == 0
     srv.close()
 
 

@@ -24,6 +24,7 @@ P = 2147483647
 
 ZS_OK, ZS_ERR_GENERIC, ZS_ERR_INPUT, ZS_ERR_OOM, ZS_ERR_PROTOCOL, ZS_ERR_DEVICE = 0, -1, -2, -3, -4, -5
 ZS_FLAG_PLAIN = 1
+ZS_FLAG_NO_GRAPH = 2
 
 _lib = None
 
@@ -116,6 +117,7 @@ def _declare(L):
     L.zs_server_new.restype = vp
     L.zs_server_submit.argtypes = [vp, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]
     L.zs_server_wait.argtypes = [vp, ctypes.c_uint64, ctypes.c_void_p]
+    L.zs_server_poll.argtypes = [vp, ctypes.c_uint64, ctypes.c_void_p, ctypes.POINTER(i32)]
     L.zs_server_memory.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]
     L.zs_server_free.argtypes = [vp]
     L.zs_hip_profile.argtypes = [vp, i32]
@@ -128,7 +130,7 @@ def _declare(L):
                  "zs_merkle_open", "zs_tree_read_level", "zs_fri_commit_dev", "zs_fri_commit", "zs_fri_query",
                  "zs_fri_read_layer", "zs_fill_m31_dev", "zs_hip_profile", "zs_hip_profile_read",
                  "zs_commit_job_dev", "zs_commit_batch_dev", "zs_server_submit", "zs_server_wait",
-                 "zs_server_memory"]:
+                 "zs_server_memory", "zs_server_poll"]:
         getattr(L, name).restype = i32
 
 
@@ -309,6 +311,14 @@ class Server:
             raise ZsError(rc, "zs_server_wait")
         return out
 
+    def poll(self, ticket):
+        """The response if the request is answered, else None."""
+        out, ready = Response(), i32()
+        rc = lib().zs_server_poll(self.h, ticket, ctypes.byref(out), ctypes.byref(ready))
+        if rc:
+            raise ZsError(rc, "zs_server_poll")
+        return out if ready.value else None
+
     def memory(self):
         a, b = ctypes.c_uint64(), ctypes.c_uint64()
         lib().zs_server_memory(self.h, ctypes.byref(a), ctypes.byref(b))
@@ -465,7 +475,8 @@ class Context:
         fr = FriResult(res, dlog, L, self) if keep else None
         return [roots.raw[32 * i:32 * (i + 1)] for i in range(L)], rem.reshape(-1, 4), fr
 
-    def fri_commit_dev(self, d_codeword, dlog, log_blowup, remainder_log, alphas=None, channel=None, keep=False):
+    def fri_commit_dev(self, d_codeword, dlog, log_blowup, remainder_log, alphas=None, channel=None, keep=False,
+                       flags=0):
         L = dlog - remainder_log
         roots = ctypes.create_string_buffer(32 * L)
         rem = np.zeros(4 << (remainder_log - log_blowup), np.uint32)
@@ -473,7 +484,7 @@ class Context:
         res = vp()
         self.check(lib().zs_fri_commit_dev(self.h, d_codeword, dlog, log_blowup, remainder_log,
                                            _p(al) if al is not None else None,
-                                           channel.h if channel is not None else None, 0, roots, _p(rem),
+                                           channel.h if channel is not None else None, flags, roots, _p(rem),
                                            ctypes.byref(res) if keep else None))
         fr = FriResult(res, dlog, L, self) if keep else None
         return [roots.raw[32 * i:32 * (i + 1)] for i in range(L)], rem.reshape(-1, 4), fr

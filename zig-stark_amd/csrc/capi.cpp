@@ -19,6 +19,11 @@ namespace zs {
 int32_t Ctx::ensure_scratch(int slot, size_t bytes) {
     DevBuf &b = scratch[slot];
     if (b.bytes >= bytes) return ZS_OK;
+    if (b.ptr && !b.owned) {
+        b.ptr = nullptr;
+        b.bytes = 0;
+        b.owned = true;
+    }
     if (b.ptr) {
         if (stream_ordered) {
             ZS_HIP(this, hipFreeAsync(b.ptr, stream));
@@ -42,12 +47,28 @@ int32_t Ctx::ensure_scratch(int slot, size_t bytes) {
 // give the scratch arenas back (stream-ordered contexts free them on their stream)
 void Ctx::release_scratch() {
     for (DevBuf &b : scratch) {
-        if (!b.ptr) continue;
-        if (stream_ordered) (void)hipFreeAsync(b.ptr, stream);
-        else (void)hipFree(b.ptr);
+        if (b.ptr && b.owned) {
+            if (stream_ordered) (void)hipFreeAsync(b.ptr, stream);
+            else (void)hipFree(b.ptr);
+        }
         b.ptr = nullptr;
         b.bytes = 0;
+        b.owned = true;
     }
+}
+
+void Ctx::lend_scratch(int slot, void *ptr, size_t bytes) {
+    DevBuf &b = scratch[slot];
+    if (b.ptr && b.owned) {
+        if (stream_ordered) (void)hipFreeAsync(b.ptr, stream);
+        else {
+            (void)hipStreamSynchronize(stream);
+            (void)hipFree(b.ptr);
+        }
+    }
+    b.ptr = ptr;
+    b.bytes = bytes;
+    b.owned = false;
 }
 }  // namespace zs
 
@@ -96,6 +117,9 @@ void zs_hip_destroy(zs_hip_ctx *ctx) {
     ctx->c.workers.clear();
     (void)hipStreamSynchronize(ctx->c.stream);
     prof_clear(&ctx->c);
+    for (auto &g : ctx->c.fri_graphs) (void)hipGraphExecDestroy(g.second);
+    ctx->c.fri_graphs.clear();
+    if (ctx->c.capture_stream) (void)hipStreamDestroy(ctx->c.capture_stream);
     ctx->c.release_scratch();
     (void)hipStreamSynchronize(ctx->c.stream);
     if (ctx->c.own_stream) (void)hipStreamDestroy(ctx->c.stream);
@@ -375,7 +399,6 @@ static void interpolate_host(const std::vector<qm31> &xs, const std::vector<qm31
 int32_t zs_fri_commit_dev(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t dlog, uint32_t log_blowup,
                           uint32_t remainder_log, const uint32_t *alphas, zs_channel *chan, uint32_t flags,
                           uint8_t *roots_out, uint32_t *remainder_out, zs_fri_result **out_result) {
-    (void)flags;
     if (!h || !d_codeword || !roots_out || !remainder_out) return ZS_ERR_INPUT;
     if (dlog > 28 || remainder_log > dlog || remainder_log < log_blowup || dlog - remainder_log < 1)
         return ZS_ERR_INPUT;
@@ -401,41 +424,96 @@ int32_t zs_fri_commit_dev(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t dl
         r->tree_off.push_back(tw);
         tw += tree_words(dlog - i);
     }
-    // a kept result owns its layers and trees; otherwise they live in the context's scratch
+    // A kept result owns its layers (layer 0 copied, for the query phase) and trees.
+    // Otherwise layer 0 is read in place and layers 1..L and the trees live in the
+    // context's scratch.
     uint32_t *layers, *trees, *alpha_dev;
+    const uint32_t *lay0;
     if (out_result) {
         ZS_TRY(dev_alloc(ctx, r->layers, lw * 4));
         ZS_TRY(dev_alloc(ctx, r->trees, tw * 4));
         layers = r->layers.as<uint32_t>();
         trees = r->trees.as<uint32_t>();
+        ZS_HIP(ctx, hipMemcpyAsync(layers, d_codeword, n0 * 16, hipMemcpyDeviceToDevice, ctx->stream));
+        lay0 = layers;
     } else {
-        ZS_TRY(ctx->ensure_scratch(4, lw * 4));
+        ZS_TRY(ctx->ensure_scratch(4, (lw - 4 * n0) * 4));
         ZS_TRY(ctx->ensure_scratch(5, tw * 4));
-        layers = ctx->scratch[4].as<uint32_t>();
+        layers = ctx->scratch[4].as<uint32_t>() - 4 * n0;  // layer i >= 1 at layers + layer_off[i]
         trees = ctx->scratch[5].as<uint32_t>();
+        lay0 = d_codeword;
     }
+    auto layer = [&](uint32_t i) -> uint32_t * {
+        return i == 0 ? const_cast<uint32_t *>(lay0) : layers + r->layer_off[i];
+    };
     ZS_TRY(ctx->ensure_scratch(6, size_t(L) * 16));
     alpha_dev = ctx->scratch[6].as<uint32_t>();
-    ZS_HIP(ctx, hipMemcpyAsync(layers, d_codeword, n0 * 16, hipMemcpyDeviceToDevice, ctx->stream));
+    const uint32_t *fri_tab;
+    ZS_TRY(get_fri_tables(ctx, dlog, &fri_tab));  // cached before any capture
+    (void)fri_tab;
     if (alphas)
         ZS_HIP(ctx, hipMemcpyAsync(alpha_dev, alphas, size_t(L) * 16, hipMemcpyHostToDevice, ctx->stream));
-    ZS_TRY(launch_qm31_leaves(ctx, layers, n0, trees));
     std::vector<uint32_t> host_alpha(size_t(L) * 4);
-    for (uint32_t i = 0; i < L; i++) {
-        uint32_t li = dlog - i;
-        ZS_TRY(launch_tree_levels(ctx, trees + r->tree_off[i], li, 0));
-        uint32_t *root = trees + r->tree_off[i] + tree_words(li) - 8;
-        if (!alphas) {
-            uint8_t rb[32];
-            ZS_HIP(ctx, hipMemcpyAsync(rb, root, 32, hipMemcpyDeviceToHost, ctx->stream));
-            ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
-            zs_channel_absorb(chan, rb, 32);
-            zs_channel_sample_qm31(chan, &host_alpha[4 * i]);
-            ZS_HIP(ctx, hipMemcpyAsync(alpha_dev + 4 * i, &host_alpha[4 * i], 16, hipMemcpyHostToDevice, ctx->stream));
+    // leaves of layer 0, then per layer: node levels, (challenge), fold + next layer's leaves
+    auto enqueue = [&]() -> int32_t {
+        ZS_TRY(launch_qm31_leaves(ctx, layer(0), n0, trees));
+        for (uint32_t i = 0; i < L; i++) {
+            uint32_t li = dlog - i;
+            ZS_TRY(launch_tree_levels(ctx, trees + r->tree_off[i], li, 0));
+            if (!alphas) {
+                uint32_t *root = trees + r->tree_off[i] + tree_words(li) - 8;
+                uint8_t rb[32];
+                ZS_HIP(ctx, hipMemcpyAsync(rb, root, 32, hipMemcpyDeviceToHost, ctx->stream));
+                ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+                zs_channel_absorb(chan, rb, 32);
+                zs_channel_sample_qm31(chan, &host_alpha[4 * i]);
+                ZS_HIP(ctx, hipMemcpyAsync(alpha_dev + 4 * i, &host_alpha[4 * i], 16, hipMemcpyHostToDevice,
+                                           ctx->stream));
+            }
+            uint32_t *next_leaves = (i + 1 < L) ? trees + r->tree_off[i + 1] : nullptr;
+            ZS_TRY(launch_fri_fold(ctx, layer(i), size_t(1) << li, dlog, i, alpha_dev + 4 * i, layer(i + 1),
+                                   next_leaves));
         }
-        uint32_t *next_leaves = (i + 1 < L) ? trees + r->tree_off[i + 1] : nullptr;
-        ZS_TRY(launch_fri_fold(ctx, layers + r->layer_off[i], size_t(1) << li, dlog, i,
-                               alpha_dev + 4 * i, layers + r->layer_off[i + 1], next_leaves));
+        return ZS_OK;
+    };
+    const bool graph = alphas && !out_result && !ctx->profiling && !(flags & ZS_FLAG_NO_GRAPH);
+    if (!graph) {
+        ZS_TRY(enqueue());
+    } else {
+        std::array<uint64_t, 6> key = {reinterpret_cast<uint64_t>(lay0), reinterpret_cast<uint64_t>(layers),
+                                       reinterpret_cast<uint64_t>(trees), reinterpret_cast<uint64_t>(alpha_dev),
+                                       dlog, L};
+        auto it = ctx->fri_graphs.find(key);
+        if (it == ctx->fri_graphs.end()) {
+            if (ctx->fri_graphs.size() >= 32) {
+                ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+                for (auto &g : ctx->fri_graphs) (void)hipGraphExecDestroy(g.second);
+                ctx->fri_graphs.clear();
+            }
+            if (!ctx->capture_stream)
+                ZS_HIP(ctx, hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking));
+            hipStream_t saved = ctx->stream;
+            ctx->stream = ctx->capture_stream;
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamBeginCapture(ctx->capture_stream, hipStreamCaptureModeThreadLocal);
+            int32_t rc = e == hipSuccess ? enqueue() : ZS_ERR_DEVICE;
+            hipError_t e2 = e == hipSuccess ? hipStreamEndCapture(ctx->capture_stream, &g) : e;
+            ctx->stream = saved;
+            if (rc != ZS_OK || e2 != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                if (rc == ZS_OK) ctx->set_error(std::string("FRI graph capture: ") + hipGetErrorString(e2));
+                return rc != ZS_OK ? rc : ZS_ERR_DEVICE;
+            }
+            hipGraphExec_t ex = nullptr;
+            e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                ctx->set_error(std::string("FRI graph instantiate: ") + hipGetErrorString(e));
+                return ZS_ERR_DEVICE;
+            }
+            it = ctx->fri_graphs.emplace(key, ex).first;
+        }
+        ZS_HIP(ctx, hipGraphLaunch(it->second, ctx->stream));
     }
     for (uint32_t i = 0; i < L; i++) {
         uint32_t li = dlog - i;
@@ -445,7 +523,7 @@ int32_t zs_fri_commit_dev(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t dl
     // remainder: interpolate the first 2^(remainder_log - log_blowup) points of layer L (fri.zig:228-233)
     size_t rc = size_t(1) << (remainder_log - log_blowup);
     std::vector<uint32_t> last(rc * 4);
-    ZS_HIP(ctx, hipMemcpyAsync(last.data(), layers + r->layer_off[L], rc * 16, hipMemcpyDeviceToHost, ctx->stream));
+    ZS_HIP(ctx, hipMemcpyAsync(last.data(), layer(L), rc * 16, hipMemcpyDeviceToHost, ctx->stream));
     ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     std::vector<qm31> xs(rc), ys(rc), co;
     qm31 w = qm31_root_of_unity(dlog - L);

@@ -77,9 +77,10 @@ cpoint coset_at_host(const zs_circle_coset &c, uint64_t i) {
 
 int32_t get_twiddles(Ctx *ctx, const zs_circle_coset &hc, const TwiddleTable **out) {
     auto key = std::make_tuple(hc.offset_x, hc.offset_y, hc.step_x, hc.step_y, hc.log_size);
-    std::lock_guard<std::mutex> lk(ctx->mu);
-    auto it = ctx->twiddles.find(key);
-    if (it != ctx->twiddles.end()) {
+    DeviceCaches &dc = *ctx->caches;
+    std::lock_guard<std::mutex> lk(dc.mu);
+    auto it = dc.twiddles.find(key);
+    if (it != dc.twiddles.end()) {
         *out = it->second.get();
         return ZS_OK;
     }
@@ -104,8 +105,9 @@ int32_t get_twiddles(Ctx *ctx, const zs_circle_coset &hc, const TwiddleTable **o
     hipLaunchKernelGGL(twiddle_gen_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, t->fwd.as<uint32_t>(),
                        t->inv.as<uint32_t>(), L, cp);
     ZS_HIP(ctx, hipGetLastError());
+    ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));  // complete before other streams read it
     *out = t.get();
-    ctx->twiddles[key] = std::move(t);
+    dc.twiddles[key] = std::move(t);
     return ZS_OK;
 }
 

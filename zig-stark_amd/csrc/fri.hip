@@ -32,9 +32,10 @@ qm31 qm31_root_of_unity(uint32_t log_size) {  // qm31.zig:80-89
 qm31 fri_offset_pow(uint32_t layer) { return qm_pow(qm_from_words(FRI_OFFSET_W), uint64_t(1) << layer); }
 
 int32_t get_fri_tables(Ctx *ctx, uint32_t dlog, const uint32_t **out) {
-    std::lock_guard<std::mutex> lk(ctx->mu);
-    auto it = ctx->fri_tables.find(dlog);
-    if (it != ctx->fri_tables.end()) {
+    DeviceCaches &dc = *ctx->caches;
+    std::lock_guard<std::mutex> lk(dc.mu);
+    auto it = dc.fri_tables.find(dlog);
+    if (it != dc.fri_tables.end()) {
         *out = it->second->as<uint32_t>();
         return ZS_OK;
     }
@@ -58,7 +59,7 @@ int32_t get_fri_tables(Ctx *ctx, uint32_t dlog, const uint32_t **out) {
     ZS_HIP(ctx, hipMemcpyAsync(buf->ptr, host.data(), host.size() * 4, hipMemcpyHostToDevice, ctx->stream));
     ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     *out = buf->as<uint32_t>();
-    ctx->fri_tables[dlog] = std::move(buf);
+    dc.fri_tables[dlog] = std::move(buf);
     return ZS_OK;
 }
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <array>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -50,11 +51,12 @@ namespace zs {
 struct DevBuf {
     void *ptr = nullptr;
     size_t bytes = 0;
+    bool owned = true;  // false: a slice of memory that someone else allocated and frees
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
     ~DevBuf() {
-        if (ptr) (void)hipFree(ptr);
+        if (ptr && owned) (void)hipFree(ptr);
     }
     template <class T>
     T *as() const { return static_cast<T *>(ptr); }
@@ -72,6 +74,15 @@ struct ProfRec {
     hipEvent_t a, b;
 };
 
+// Device-resident tables shared by the contexts of one device (a context and its
+// batch workers, or a server's workers): built once, read by every stream.
+struct DeviceCaches {
+    std::mutex mu;
+    std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t, uint32_t>, std::unique_ptr<TwiddleTable>> twiddles;
+    // FRI inverse-domain tables keyed by dlog: [lo table 2^10 | hi table] QM31 words
+    std::map<uint32_t, std::unique_ptr<DevBuf>> fri_tables;
+};
+
 struct Ctx {
     int device = 0;
     bool own_stream = false;          // the stream was created for this context
@@ -80,17 +91,20 @@ struct Ctx {
     bool profiling = false;           // record HIP events around the hot kernels
     std::vector<ProfRec> prof;
     hipStream_t stream = nullptr;  // nullptr = the null stream
-    std::mutex mu;                  // guards the caches below
     std::string last_error;
-    std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t, uint32_t>, std::unique_ptr<TwiddleTable>> twiddles;
-    // FRI inverse-domain tables keyed by dlog: [lo table 2^10 | hi table] QM31 words
-    std::map<uint32_t, std::unique_ptr<DevBuf>> fri_tables;
+    std::shared_ptr<DeviceCaches> caches = std::make_shared<DeviceCaches>();
     // grow-only scratch arenas
     DevBuf scratch[10];  // 0 transforms, 1-2 LDE, 3 plain path, 4-6 FRI, 7-9 commit jobs
+    // FRI commit phases with challenges given up front, captured once per buffer set and
+    // replayed (key: codeword, layers, trees, challenges, dlog, layer count)
+    hipStream_t capture_stream = nullptr;
+    std::map<std::array<uint64_t, 6>, hipGraphExec_t> fri_graphs;
 
     void set_error(const std::string &s) { last_error = s; }
     int32_t ensure_scratch(int slot, size_t bytes);
     void release_scratch();
+    // use [ptr, ptr + bytes) as scratch slot `slot` until release_scratch (not freed here)
+    void lend_scratch(int slot, void *ptr, size_t bytes);
 };
 
 // ---- twiddles (fft.hip) ----

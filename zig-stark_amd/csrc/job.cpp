@@ -68,21 +68,42 @@ int32_t commit_job(zs_hip_ctx *h, const uint32_t *d_cols, uint32_t n_cols, uint3
                              fri_roots, remainder, nullptr);
 }
 
-// device bytes one request holds while it runs (its buffers and the scratch it grows)
-uint64_t job_footprint(uint32_t n_cols, uint32_t log_n, uint32_t log_blowup, uint32_t remainder_log) {
-    const uint64_t n = uint64_t(1) << log_n, N = uint64_t(1) << (log_n + log_blowup), c = n_cols;
-    uint64_t fri_trees = 0, fri_layers = 0;
-    for (uint32_t i = 0; i + remainder_log < log_n + log_blowup; i++) fri_trees += ((2 * N) >> i) * 32;
-    for (uint32_t i = 0; i + remainder_log <= log_n + log_blowup; i++) fri_layers += (N >> i) * 16;
-    return 4 * c * n              // input
-           + 4 * c * N            // LDE
-           + 32 * (2 * N - 1)     // row tree
-           + 4 * c * n + 4 * c * N  // LDE scratch: coefficients and the multi-pass array
-           + 16 * N + 16 * c      // combination codeword and powers
-           + fri_layers + fri_trees;
+// Device memory of one commit job: its input, LDE and row tree, and every scratch
+// buffer commit_job uses (slot numbers of Ctx::scratch), each 256-byte aligned.
+struct JobLayout {
+    enum Part { IN, LDE, TREE, COEF, BIG, FRI_LAYERS, FRI_TREES, ALPHAS, COMB, POWERS, NPARTS };
+    uint64_t off[NPARTS], len[NPARTS], total;
+};
+constexpr int kSlotOf[JobLayout::NPARTS] = {-1, -1, -1, 1, 2, 4, 5, 6, 8, 9};
+
+JobLayout job_layout(uint32_t n_cols, uint32_t log_n, uint32_t log_blowup, uint32_t remainder_log) {
+    JobLayout j{};
+    const uint32_t LT = log_n + log_blowup, L = LT - remainder_log;
+    const uint64_t n = uint64_t(1) << log_n, N = uint64_t(1) << LT, c = n_cols;
+    size_t coef_words;
+    j.len[JobLayout::IN] = 4 * c * n;
+    j.len[JobLayout::LDE] = 4 * c * N;
+    j.len[JobLayout::TREE] = tree_words(LT) * 4;
+    j.len[JobLayout::BIG] = lde_scratch_words(n_cols, log_n, log_blowup, &coef_words) * 4;
+    j.len[JobLayout::COEF] = coef_words * 4;
+    uint64_t fl = 0, ft = 0;
+    for (uint32_t i = 1; i <= L; i++) fl += (N >> i) * 16;  // layers 1..L (layer 0 is read in place)
+    for (uint32_t i = 0; i < L; i++) ft += tree_words(LT - i) * 4;
+    j.len[JobLayout::FRI_LAYERS] = fl;
+    j.len[JobLayout::FRI_TREES] = ft;
+    j.len[JobLayout::ALPHAS] = uint64_t(L) * 16;
+    j.len[JobLayout::COMB] = 16 * N;
+    j.len[JobLayout::POWERS] = 16 * c;
+    uint64_t o = 0;
+    for (int k = 0; k < JobLayout::NPARTS; k++) {
+        j.off[k] = o;
+        o += (j.len[k] + 255) & ~uint64_t(255);
+    }
+    j.total = o;
+    return j;
 }
 
-zs_hip_ctx *new_worker(int device, bool stream_ordered) {
+zs_hip_ctx *new_worker(int device, const std::shared_ptr<DeviceCaches> &caches) {
     zs_hip_ctx *w = nullptr;
     if (zs_hip_init(device, 0, &w) != ZS_OK) return nullptr;
     hipStream_t s = nullptr;
@@ -92,9 +113,56 @@ zs_hip_ctx *new_worker(int device, bool stream_ordered) {
     }
     w->c.stream = s;
     w->c.own_stream = true;
-    w->c.stream_ordered = stream_ordered;
+    if (caches) w->c.caches = caches;
     return w;
 }
+
+// First-fit allocator over one device allocation (offsets, 256-byte granules);
+// free blocks are kept coalesced.
+class Arena {
+  public:
+    void reset(uint64_t size) {
+        free_.clear();
+        if (size) free_[0] = size;
+    }
+    bool alloc(uint64_t bytes, uint64_t *off) {
+        bytes = (bytes + 255) & ~uint64_t(255);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < bytes) continue;
+            *off = it->first;
+            uint64_t rest = it->second - bytes, at = it->first + bytes;
+            free_.erase(it);
+            if (rest) free_[at] = rest;
+            return true;
+        }
+        return false;
+    }
+    bool fits(uint64_t bytes) const {
+        bytes = (bytes + 255) & ~uint64_t(255);
+        for (auto &f : free_)
+            if (f.second >= bytes) return true;
+        return false;
+    }
+    void release(uint64_t off, uint64_t bytes) {
+        bytes = (bytes + 255) & ~uint64_t(255);
+        auto it = free_.emplace(off, bytes).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+
+  private:
+    std::map<uint64_t, uint64_t> free_;
+};
 
 }  // namespace
 
@@ -121,7 +189,7 @@ int32_t zs_commit_batch_dev(zs_hip_ctx *h, zs_job *jobs, size_t n_jobs, uint32_t
         ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));  // inputs written on the caller's stream are complete
         auto &ws = ctx->workers;
         while (ws.size() < n_workers) {
-            zs_hip_ctx *w = new_worker(ctx->device, false);
+            zs_hip_ctx *w = new_worker(ctx->device, ctx->caches);
             if (!w) return ZS_ERR_DEVICE;
             ws.push_back(w);
         }
@@ -152,7 +220,8 @@ struct ServReq {
     zs_request r;
     std::vector<uint32_t> host;
     zs_response resp;
-    uint64_t bytes;
+    JobLayout lay;
+    uint64_t bytes, off = 0;
     bool done = false;
 };
 
@@ -160,6 +229,8 @@ struct zs_server {
     int device = 0;
     uint64_t budget = 0;
     uint64_t in_use = 0, peak = 0;
+    DevBuf pool;  // the budget, allocated once; requests are carved from it
+    Arena arena;
     std::vector<zs_hip_ctx *> workers;
     std::vector<std::thread> threads;
     std::mutex mu;
@@ -170,37 +241,31 @@ struct zs_server {
     bool stop = false;
 };
 
-static void serve_one(zs_hip_ctx *w, ServReq *q) {
+static void serve_one(zs_server *s, zs_hip_ctx *w, ServReq *q) {
     Ctx *ctx = &w->c;
     const zs_request &r = q->r;
-    const uint32_t LT = r.log_n + r.log_blowup;
-    const size_t n = size_t(1) << r.log_n, N = size_t(1) << LT;
-    void *d_cols = nullptr, *d_lde = nullptr, *d_nodes = nullptr;
-    int32_t rc = ZS_OK;
-    if (hipMallocAsync(&d_cols, n * r.n_cols * 4, ctx->stream) != hipSuccess ||
-        hipMallocAsync(&d_lde, N * r.n_cols * 4, ctx->stream) != hipSuccess ||
-        hipMallocAsync(&d_nodes, tree_words(LT) * 4, ctx->stream) != hipSuccess)
-        rc = ZS_ERR_OOM;
-    if (!rc) {
-        if (r.host_cols)
-            rc = hipMemcpyAsync(d_cols, q->host.data(), n * r.n_cols * 4, hipMemcpyHostToDevice, ctx->stream) ==
-                         hipSuccess
-                     ? ZS_OK
-                     : ZS_ERR_DEVICE;
-        else
-            rc = zs_fill_m31_dev(w, r.seed, static_cast<uint32_t *>(d_cols), n * r.n_cols);
-    }
+    const JobLayout &j = q->lay;
+    char *base = s->pool.as<char>() + q->off;
+    auto part = [&](int k) { return reinterpret_cast<void *>(base + j.off[k]); };
+    for (int k = 0; k < JobLayout::NPARTS; k++)
+        if (kSlotOf[k] >= 0) ctx->lend_scratch(kSlotOf[k], part(k), j.len[k]);
+    uint32_t *d_cols = static_cast<uint32_t *>(part(JobLayout::IN));
+    const size_t elems = size_t(r.n_cols) << r.log_n;
+    int32_t rc;
+    if (r.host_cols)
+        rc = hipMemcpyAsync(d_cols, q->host.data(), elems * 4, hipMemcpyHostToDevice, ctx->stream) == hipSuccess
+                 ? ZS_OK
+                 : ZS_ERR_DEVICE;
+    else
+        rc = zs_fill_m31_dev(w, r.seed, d_cols, elems);
     if (!rc)
-        rc = commit_job(w, static_cast<uint32_t *>(d_cols), r.n_cols, r.log_n, r.log_blowup, r.remainder_log,
-                        static_cast<uint32_t *>(d_lde), static_cast<uint8_t *>(d_nodes), q->resp.trace_root,
-                        q->resp.fri_roots, q->resp.remainder, 0);
-    if (d_cols) (void)hipFreeAsync(d_cols, ctx->stream);
-    if (d_lde) (void)hipFreeAsync(d_lde, ctx->stream);
-    if (d_nodes) (void)hipFreeAsync(d_nodes, ctx->stream);
-    ctx->release_scratch();
+        rc = commit_job(w, d_cols, r.n_cols, r.log_n, r.log_blowup, r.remainder_log,
+                        static_cast<uint32_t *>(part(JobLayout::LDE)), static_cast<uint8_t *>(part(JobLayout::TREE)),
+                        q->resp.trace_root, q->resp.fri_roots, q->resp.remainder, 0);
     (void)hipStreamSynchronize(ctx->stream);
+    ctx->release_scratch();
     q->resp.status = rc;
-    q->resp.fri_layers = LT - r.remainder_log;
+    q->resp.fri_layers = r.log_n + r.log_blowup - r.remainder_log;
 }
 
 static void serve_loop(zs_server *s, zs_hip_ctx *w) {
@@ -212,13 +277,13 @@ static void serve_loop(zs_server *s, zs_hip_ctx *w) {
             s->cv_work.wait(lk, [&] {
                 if (s->stop && s->queue.empty()) return true;
                 for (ServReq *x : s->queue)
-                    if (s->in_use + x->bytes <= s->budget) return true;
+                    if (s->arena.fits(x->bytes)) return true;
                 return false;
             });
             if (s->stop && s->queue.empty()) return;
-            // first request that fits what is free (earlier large ones wait for memory)
+            // the first queued request whose memory is free (earlier large ones wait)
             for (auto it = s->queue.begin(); it != s->queue.end(); ++it) {
-                if (s->in_use + (*it)->bytes <= s->budget) {
+                if (s->arena.alloc((*it)->bytes, &(*it)->off)) {
                     q = *it;
                     s->queue.erase(it);
                     break;
@@ -229,12 +294,13 @@ static void serve_loop(zs_server *s, zs_hip_ctx *w) {
             s->peak = std::max(s->peak, s->in_use);
             q->resp.t_admit = now_s();
         }
-        serve_one(w, q);
+        serve_one(s, w, q);
         {
             std::lock_guard<std::mutex> lk(s->mu);
             q->resp.t_done = now_s();
             q->done = true;
             s->in_use -= q->bytes;
+            s->arena.release(q->off, q->bytes);
         }
         s->cv_done.notify_all();
         s->cv_work.notify_all();
@@ -245,11 +311,19 @@ extern "C" {
 
 zs_server *zs_server_new(int32_t device, uint32_t n_workers, uint64_t hbm_budget_bytes) {
     if (n_workers == 0 || n_workers > 64) return nullptr;
+    if (hipSetDevice(device) != hipSuccess) return nullptr;
     auto *s = new zs_server();
     s->device = device;
-    s->budget = hbm_budget_bytes;
+    s->budget = (hbm_budget_bytes + 255) & ~uint64_t(255);
+    if (hipMalloc(&s->pool.ptr, s->budget) != hipSuccess) {
+        s->pool.ptr = nullptr;
+        delete s;
+        return nullptr;
+    }
+    s->pool.bytes = s->budget;
+    s->arena.reset(s->budget);
     for (uint32_t i = 0; i < n_workers; i++) {
-        zs_hip_ctx *w = new_worker(device, true);
+        zs_hip_ctx *w = new_worker(device, s->workers.empty() ? nullptr : s->workers[0]->c.caches);
         if (!w) {
             zs_server_free(s);
             return nullptr;
@@ -271,7 +345,8 @@ int32_t zs_server_submit(zs_server *s, const zs_request *req, uint64_t *ticket) 
     std::memset(&q->resp, 0, sizeof q->resp);
     q->resp.t_submit = now_s();
     if (req->host_cols) q->host.assign(req->host_cols, req->host_cols + (size_t(req->n_cols) << req->log_n));
-    q->bytes = job_footprint(req->n_cols, req->log_n, req->log_blowup, req->remainder_log);
+    q->lay = job_layout(req->n_cols, req->log_n, req->log_blowup, req->remainder_log);
+    q->bytes = q->lay.total;
     q->resp.device_bytes = q->bytes;
     if (q->bytes > s->budget) return ZS_ERR_OOM;
     {
@@ -294,6 +369,19 @@ int32_t zs_server_wait(zs_server *s, uint64_t ticket, zs_response *out) {
     s->cv_done.wait(lk, [&] { return q->done; });
     *out = q->resp;
     s->reqs.erase(it);
+    return ZS_OK;
+}
+
+int32_t zs_server_poll(zs_server *s, uint64_t ticket, zs_response *out, int32_t *ready) {
+    if (!s || !out || !ready) return ZS_ERR_INPUT;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return ZS_ERR_INPUT;
+    *ready = it->second->done ? 1 : 0;
+    if (*ready) {
+        *out = it->second->resp;
+        s->reqs.erase(it);
+    }
     return ZS_OK;
 }
 
