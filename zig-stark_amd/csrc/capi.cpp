@@ -454,9 +454,24 @@ int32_t zs_fri_commit_dev(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t dl
     if (alphas)
         ZS_HIP(ctx, hipMemcpyAsync(alpha_dev, alphas, size_t(L) * 16, hipMemcpyHostToDevice, ctx->stream));
     std::vector<uint32_t> host_alpha(size_t(L) * 4);
-    // leaves of layer 0, then per layer: node levels, (challenge), fold + next layer's leaves
+    // Leaves of layer 0, then per layer: node levels, (challenge), fold + next
+    // layer's leaves.  With the challenges given, no fold waits for a root: the
+    // folds run first and the trees of all layers are built together after them.
     auto enqueue = [&]() -> int32_t {
         ZS_TRY(launch_qm31_leaves(ctx, layer(0), n0, trees));
+        if (alphas) {
+            std::vector<uint64_t> offs(L);
+            std::vector<uint32_t> logs(L);
+            for (uint32_t i = 0; i < L; i++) {
+                uint32_t li = dlog - i;
+                uint32_t *next_leaves = (i + 1 < L) ? trees + r->tree_off[i + 1] : nullptr;
+                ZS_TRY(launch_fri_fold(ctx, layer(i), size_t(1) << li, dlog, i, alpha_dev + 4 * i, layer(i + 1),
+                                       next_leaves));
+                offs[i] = r->tree_off[i];
+                logs[i] = li;
+            }
+            return launch_trees_multi(ctx, trees, offs.data(), logs.data(), L);
+        }
         for (uint32_t i = 0; i < L; i++) {
             uint32_t li = dlog - i;
             ZS_TRY(launch_tree_levels(ctx, trees + r->tree_off[i], li, 0));

@@ -142,6 +142,9 @@ int32_t launch_row_leaves(Ctx *ctx, const uint32_t *d_cols, uint64_t col_stride,
 // node levels above an already written level 0 of 2^log_leaves digests
 int32_t launch_tree_levels(Ctx *ctx, uint32_t *d_nodes, uint32_t log_leaves, uint32_t from_level);
 int32_t launch_qm31_leaves(Ctx *ctx, const uint32_t *d_vals, uint64_t n, uint32_t *d_nodes);
+// node levels of n_trees independent trees with written leaves, tree k at base + off_words[k]
+int32_t launch_trees_multi(Ctx *ctx, uint32_t *base, const uint64_t *off_words, const uint32_t *log_leaves,
+                           uint32_t n_trees);
 // Merkle path gather: for each request, log_leaves sibling digests written at out_off_words.
 struct PathReq {
     uint64_t tree_off_words;

@@ -123,9 +123,7 @@ __device__ __forceinline__ void st_digest(uint32_t *p, const uint32_t d[8]) {
 // NLEV (1..3) levels per launch with no idle lanes: lane i owns the subtree
 // over children 2^NLEV * i .. 2^NLEV * (i + 1) - 1 and writes all its nodes.
 template <int NLEV>
-__global__ __launch_bounds__(256) void tree_subtree_kernel(const uint32_t *child, uint32_t *lvl1, uint64_t n_lanes) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_lanes) return;
+__device__ __forceinline__ void subtree_lane(const uint32_t *child, uint32_t *lvl1, uint64_t n_lanes, uint64_t i) {
     constexpr int W = 1 << NLEV;
     uint32_t *lvl2 = lvl1 + n_lanes * (W / 2) * 8;
     uint32_t *lvl3 = lvl2 + n_lanes * (W / 4) * 8;
@@ -152,26 +150,72 @@ __global__ __launch_bounds__(256) void tree_subtree_kernel(const uint32_t *child
     }
 }
 
+template <int NLEV>
+__global__ __launch_bounds__(256) void tree_subtree_kernel(const uint32_t *child, uint32_t *lvl1, uint64_t n_lanes) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_lanes) return;
+    subtree_lane<NLEV>(child, lvl1, n_lanes, i);
+}
+
+// Several independent trees in one launch (the layers of an FRI commitment):
+// tree t owns lanes [lane_end[t-1], lane_end[t]) and its levels at word offsets
+// child[t], lvl1[t] from `base`.
+struct MultiTrees {
+    static constexpr int MAX = 32;
+    uint32_t n;
+    uint64_t child[MAX], lvl1[MAX], lane_end[MAX];
+};
+
+__global__ __launch_bounds__(256) void tree_subtree3_multi_kernel(uint32_t *base, const MultiTrees mt) {
+    uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= mt.lane_end[mt.n - 1]) return;
+    uint32_t t = 0;
+    while (g >= mt.lane_end[t]) t++;
+    uint64_t start = t ? mt.lane_end[t - 1] : 0;
+    subtree_lane<3>(base + mt.child[t], base + mt.lvl1[t], mt.lane_end[t] - start, g - start);
+}
+
 // the top levels in one workgroup: one node per thread per level, levels
 // separated by a device-scope fence and a barrier
-__global__ __launch_bounds__(1024) void tree_tail_kernel(uint32_t *child, uint32_t log_children) {
-    uint32_t *lvl = child;
+// Parents of each level are written to the tree and kept in LDS for the next
+// level (ping-pong: 2^(k-1) digests, then 2^(k-2), ...), so only the first level
+// reads global memory.  lds: 3 * 2^(log_children-1) * 16 words.
+__device__ __forceinline__ void tail_levels(uint32_t *child, uint32_t log_children, uint32_t *lds) {
     uint32_t cnt = 1u << log_children;
+    uint32_t *out = child + (uint64_t)cnt * 8;  // next level in the tree
+    uint32_t *bufs[2] = {lds, lds + (cnt >> 1) * 8};
     for (uint32_t l = 0; l < log_children; l++) {
-        uint32_t *next = lvl + (uint64_t)cnt * 8;
         uint32_t half = cnt >> 1;
+        const uint32_t *src = l == 0 ? child : bufs[(l - 1) & 1];
+        uint32_t *dst = bufs[l & 1];
         for (uint32_t i = threadIdx.x; i < half; i += blockDim.x) {
             uint32_t a[8], b[8], d[8];
-            ld_digest(lvl + (uint64_t)(2 * i) * 8, a);
-            ld_digest(lvl + (uint64_t)(2 * i + 1) * 8, b);
+            ld_digest(src + (uint64_t)(2 * i) * 8, a);
+            ld_digest(src + (uint64_t)(2 * i + 1) * 8, b);
             b3_hash2(a, b, d);
-            st_digest(next + (uint64_t)i * 8, d);
+            st_digest(out + (uint64_t)i * 8, d);
+            st_digest(dst + i * 8, d);
         }
-        __threadfence();
         __syncthreads();
-        lvl = next;
+        out += (uint64_t)half * 8;
         cnt = half;
     }
+}
+
+inline size_t tail_lds_bytes(uint32_t log_children) {
+    return log_children ? size_t(3) * (size_t(1) << (log_children - 1)) * 16 : 16;
+}
+
+__global__ __launch_bounds__(1024) void tree_tail_kernel(uint32_t *child, uint32_t log_children) {
+    extern __shared__ uint32_t tail_lds[];
+    tail_levels(child, log_children, tail_lds);
+}
+
+// one workgroup per tree: tree blockIdx.x has 2^lvl1[t] children at word offset child[t]
+__global__ __launch_bounds__(1024) void tree_tail_multi_kernel(uint32_t *base, const MultiTrees mt) {
+    extern __shared__ uint32_t tail_lds[];
+    uint32_t t = blockIdx.x;
+    tail_levels(base + mt.child[t], (uint32_t)mt.lvl1[t], tail_lds);
 }
 
 // Build levels from_level+1 .. top over level from_level (already written).
@@ -182,7 +226,8 @@ int32_t launch_tree_levels(Ctx *ctx, uint32_t *d_nodes, uint32_t log_leaves, uin
         uint32_t log_children = remaining;  // children at `level`: 2^remaining
         uint32_t *child = d_nodes + level_offset_words(log_leaves, level);
         if (log_children <= 12) {
-            ZS_LAUNCH(ctx, "tree_tail_kernel", tree_tail_kernel, dim3(1), dim3(1024), 0, ctx->stream, child, log_children);
+            ZS_LAUNCH(ctx, "tree_tail_kernel", tree_tail_kernel, dim3(1), dim3(1024), tail_lds_bytes(log_children),
+                      ctx->stream, child, log_children);
             ZS_HIP(ctx, hipGetLastError());
             break;
         }
@@ -201,6 +246,54 @@ int32_t launch_tree_levels(Ctx *ctx, uint32_t *d_nodes, uint32_t log_leaves, uin
                       n_lanes);
         ZS_HIP(ctx, hipGetLastError());
         level += nlev;
+    }
+    return ZS_OK;
+}
+
+// Every node level of several trees whose leaves are written: tree k has
+// 2^log_leaves[k] leaves at word offset off_words[k] from `base`.  Levels
+// above 2^12 children go 3 at a time in one launch across all trees; the top
+// levels of every tree then run in one launch, a workgroup per tree.
+int32_t launch_trees_multi(Ctx *ctx, uint32_t *base, const uint64_t *off_words, const uint32_t *log_leaves,
+                           uint32_t n_trees) {
+    std::vector<uint32_t> level(n_trees, 0);
+    for (uint32_t start = 0; start < n_trees; start += MultiTrees::MAX) {
+        uint32_t cnt = std::min<uint32_t>(MultiTrees::MAX, n_trees - start);
+        for (;;) {
+            MultiTrees mt{};
+            uint64_t lanes = 0;
+            std::vector<uint32_t> in;
+            for (uint32_t k = start; k < start + cnt; k++) {
+                uint32_t rem = log_leaves[k] - level[k];
+                if (rem <= 12) continue;
+                mt.child[mt.n] = off_words[k] + level_offset_words(log_leaves[k], level[k]);
+                mt.lvl1[mt.n] = off_words[k] + level_offset_words(log_leaves[k], level[k] + 1);
+                lanes += (uint64_t(1) << rem) >> 3;
+                mt.lane_end[mt.n] = lanes;
+                mt.n++;
+                in.push_back(k);
+            }
+            if (mt.n == 0) break;
+            ZS_LAUNCH(ctx, "tree_subtree3_multi_kernel", tree_subtree3_multi_kernel,
+                      dim3((uint32_t)((lanes + 255) / 256)), dim3(256), 0, ctx->stream, base, mt);
+            ZS_HIP(ctx, hipGetLastError());
+            for (uint32_t k : in) level[k] += 3;
+        }
+        MultiTrees mt{};
+        uint32_t max_rem = 0;
+        for (uint32_t k = start; k < start + cnt; k++) {
+            uint32_t rem = log_leaves[k] - level[k];
+            if (rem == 0) continue;
+            mt.child[mt.n] = off_words[k] + level_offset_words(log_leaves[k], level[k]);
+            mt.lvl1[mt.n] = rem;
+            mt.n++;
+            max_rem = std::max(max_rem, rem);
+        }
+        if (mt.n) {
+            ZS_LAUNCH(ctx, "tree_tail_multi_kernel", tree_tail_multi_kernel, dim3(mt.n), dim3(1024),
+                      tail_lds_bytes(max_rem), ctx->stream, base, mt);
+            ZS_HIP(ctx, hipGetLastError());
+        }
     }
     return ZS_OK;
 }
