@@ -121,9 +121,14 @@ def full_commit_leg(ctx, d_cols, n_cols, lg, beta, d_lde, d_nodes):
         root, roots, rem = ctx.commit_job_dev(d_cols.data_ptr(), n_cols, lg, beta, 5, d_lde.data_ptr(),
                                               d_nodes.data_ptr())
     ms = (time.perf_counter() - t0) * 1e3 / reps
+    ctx.profile(True)
+    ctx.commit_job_dev(d_cols.data_ptr(), n_cols, lg, beta, 5, d_lde.data_ptr(), d_nodes.data_ptr())
+    kern = ctx.profile_read()
+    ctx.profile(False)
     return {"workload": f"{n_cols} x 2^{lg}: LDE x{1 << beta}, row tree, gamma from the trace root, "
                         f"QM31 combination, FRI to 2^5 with channel challenges",
             "ms": round(ms, 3), "Melem_per_s": round(n_cols * (1 << lg) / (ms * 1e-3) / 1e6, 1),
+            "kernels_ms": {k: [round(v[0], 4), v[1]] for k, v in sorted(kern.items(), key=lambda kv: -kv[1][0])},
             "trace_root": root.hex(), "fri_root0": roots[0].hex()}
 
 

@@ -123,3 +123,23 @@ def test_fri_graph_replays_match_eager_and_oracle(zctx, torch_dev, dlog, beta, r
             assert g_roots == oroots and np.array_equal(g_rem, orem), rep
         torch.cuda.synchronize()
         _ = (d_cw.view(-1)[:1024].long().sum()).item()  # eager work between replays
+
+
+@pytest.mark.parametrize("dlog,beta,rem_log", [(9, 2, 5), (14, 2, 4)])
+def test_fri_device_channel_graph_replays(zctx, torch_dev, dlog, beta, rem_log):
+    """Challenges drawn on the device from the channel, no result kept: the
+    commit phase is captured once and replayed; roots, remainder and the
+    channel state afterwards equal the oracle's on every call."""
+    from conftest import to_dev
+    rng = np.random.default_rng(dlog + 100)
+    d_cw = to_dev(np.zeros((1 << dlog) * 4, np.uint32), torch_dev)
+    for rep in range(3):
+        cw = rng.integers(0, P, (1 << dlog, 4), dtype=np.uint32)
+        d_cw.copy_(to_dev(cw.reshape(-1), torch_dev))
+        for flags in (0, Z.ZS_FLAG_NO_GRAPH):
+            ch = Z.Channel(b"zig-stark:fri:dev")
+            roots, rem, _ = zctx.fri_commit_dev(d_cw.data_ptr(), dlog, beta, rem_log, channel=ch, flags=flags)
+            och = O.Channel(b"zig-stark:fri:dev")
+            oroots, _, orem, _ = O.fri_commit(cw, beta, rem_log, channel=och, want_layers=False)
+            assert roots == oroots and np.array_equal(rem, orem), (rep, flags)
+            assert ch.sample_bytes(32) == och.sample_bytes(32), (rep, flags)

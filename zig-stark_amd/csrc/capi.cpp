@@ -396,13 +396,15 @@ static void interpolate_host(const std::vector<qm31> &xs, const std::vector<qm31
     coeffs = acc;
 }
 
-int32_t zs_fri_commit_dev(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t dlog, uint32_t log_blowup,
-                          uint32_t remainder_log, const uint32_t *alphas, zs_channel *chan, uint32_t flags,
-                          uint8_t *roots_out, uint32_t *remainder_out, zs_fri_result **out_result) {
+}  // extern "C"
+
+int32_t zs::fri_commit_impl(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t dlog, uint32_t log_blowup,
+                            uint32_t remainder_log, const uint32_t *alphas, B3Hasher *d_chan, uint32_t flags,
+                            uint8_t *roots_out, uint32_t *remainder_out, zs_fri_result **out_result) {
     if (!h || !d_codeword || !roots_out || !remainder_out) return ZS_ERR_INPUT;
     if (dlog > 28 || remainder_log > dlog || remainder_log < log_blowup || dlog - remainder_log < 1)
         return ZS_ERR_INPUT;
-    if (!alphas && !chan) return ZS_ERR_INPUT;
+    if (!alphas && !d_chan) return ZS_ERR_INPUT;
     Ctx *ctx = &h->c;
     ZS_HIP(ctx, hipSetDevice(ctx->device));
     auto *r = new (std::nothrow) zs_fri_result();
@@ -453,7 +455,6 @@ int32_t zs_fri_commit_dev(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t dl
     (void)fri_tab;
     if (alphas)
         ZS_HIP(ctx, hipMemcpyAsync(alpha_dev, alphas, size_t(L) * 16, hipMemcpyHostToDevice, ctx->stream));
-    std::vector<uint32_t> host_alpha(size_t(L) * 4);
     // Leaves of layer 0, then per layer: node levels, (challenge), fold + next
     // layer's leaves.  With the challenges given, no fold waits for a root: the
     // folds run first and the trees of all layers are built together after them.
@@ -475,29 +476,21 @@ int32_t zs_fri_commit_dev(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t dl
         for (uint32_t i = 0; i < L; i++) {
             uint32_t li = dlog - i;
             ZS_TRY(launch_tree_levels(ctx, trees + r->tree_off[i], li, 0));
-            if (!alphas) {
-                uint32_t *root = trees + r->tree_off[i] + tree_words(li) - 8;
-                uint8_t rb[32];
-                ZS_HIP(ctx, hipMemcpyAsync(rb, root, 32, hipMemcpyDeviceToHost, ctx->stream));
-                ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
-                zs_channel_absorb(chan, rb, 32);
-                zs_channel_sample_qm31(chan, &host_alpha[4 * i]);
-                ZS_HIP(ctx, hipMemcpyAsync(alpha_dev + 4 * i, &host_alpha[4 * i], 16, hipMemcpyHostToDevice,
-                                           ctx->stream));
-            }
+            // the challenge is drawn on the device from this layer's root
+            ZS_TRY(launch_channel_draw(ctx, d_chan, trees + r->tree_off[i] + tree_words(li) - 8, alpha_dev + 4 * i));
             uint32_t *next_leaves = (i + 1 < L) ? trees + r->tree_off[i + 1] : nullptr;
             ZS_TRY(launch_fri_fold(ctx, layer(i), size_t(1) << li, dlog, i, alpha_dev + 4 * i, layer(i + 1),
                                    next_leaves));
         }
         return ZS_OK;
     };
-    const bool graph = alphas && !out_result && !ctx->profiling && !(flags & ZS_FLAG_NO_GRAPH);
+    const bool graph = !out_result && !ctx->profiling && !(flags & ZS_FLAG_NO_GRAPH);
     if (!graph) {
         ZS_TRY(enqueue());
     } else {
         std::array<uint64_t, 6> key = {reinterpret_cast<uint64_t>(lay0), reinterpret_cast<uint64_t>(layers),
                                        reinterpret_cast<uint64_t>(trees), reinterpret_cast<uint64_t>(alpha_dev),
-                                       dlog, L};
+                                       reinterpret_cast<uint64_t>(d_chan), (uint64_t(dlog) << 32) | L};
         auto it = ctx->fri_graphs.find(key);
         if (it == ctx->fri_graphs.end()) {
             if (ctx->fri_graphs.size() >= 32) {
@@ -550,18 +543,39 @@ int32_t zs_fri_commit_dev(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t dl
     }
     interpolate_host(xs, ys, co);
     for (size_t k = 0; k < rc; k++) qm_to_words(co[k], remainder_out + 4 * k);
-    if (!alphas) {
-        uint8_t b[16];
-        for (size_t k = 0; k < rc; k++) {
-            for (int q = 0; q < 4; q++) {
-                uint32_t v = remainder_out[4 * k + q];
-                b[4 * q] = (uint8_t)v; b[4 * q + 1] = (uint8_t)(v >> 8);
-                b[4 * q + 2] = (uint8_t)(v >> 16); b[4 * q + 3] = (uint8_t)(v >> 24);
-            }
-            zs_channel_absorb(chan, b, 16);
-        }
-    }
     if (out_result) *out_result = guard.release();
+    return ZS_OK;
+}
+
+extern "C" {
+
+int32_t zs_fri_commit_dev(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t dlog, uint32_t log_blowup,
+                          uint32_t remainder_log, const uint32_t *alphas, zs_channel *chan, uint32_t flags,
+                          uint8_t *roots_out, uint32_t *remainder_out, zs_fri_result **out_result) {
+    if (!h || (!alphas && !chan)) return ZS_ERR_INPUT;
+    if (alphas)
+        return fri_commit_impl(h, d_codeword, dlog, log_blowup, remainder_log, alphas, nullptr, flags, roots_out,
+                               remainder_out, out_result);
+    // the channel travels to the device, draws every challenge there, and comes back
+    Ctx *ctx = &h->c;
+    ZS_TRY(ctx->ensure_scratch(7, sizeof(B3Hasher)));
+    B3Hasher *d_chan = ctx->scratch[7].as<B3Hasher>();
+    ZS_HIP(ctx, hipMemcpyAsync(d_chan, &chan->h, sizeof(B3Hasher), hipMemcpyHostToDevice, ctx->stream));
+    ZS_TRY(fri_commit_impl(h, d_codeword, dlog, log_blowup, remainder_log, nullptr, d_chan, flags, roots_out,
+                           remainder_out, out_result));
+    ZS_HIP(ctx, hipMemcpyAsync(&chan->h, d_chan, sizeof(B3Hasher), hipMemcpyDeviceToHost, ctx->stream));
+    ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    // absorbMany(remainder) (fri.zig:228-233)
+    size_t rc = size_t(1) << (remainder_log - log_blowup);
+    uint8_t b[16];
+    for (size_t k = 0; k < rc; k++) {
+        for (int q = 0; q < 4; q++) {
+            uint32_t v = remainder_out[4 * k + q];
+            b[4 * q] = (uint8_t)v; b[4 * q + 1] = (uint8_t)(v >> 8);
+            b[4 * q + 2] = (uint8_t)(v >> 16); b[4 * q + 3] = (uint8_t)(v >> 24);
+        }
+        zs_channel_absorb(chan, b, 16);
+    }
     return ZS_OK;
 }
 

@@ -111,7 +111,15 @@ int32_t launch_fri_fold(Ctx *ctx, const uint32_t *d_cur, uint64_t n, uint32_t dl
 
 namespace zs {
 
-// g[i] = sum_c gamma^c * lde_c[i] (QM31 combination of M31 columns); powers: n_cols QM31 values
+// g[i] = sum_c gamma^c * lde_c[i] (QM31 combination of M31 columns); powers: n_cols QM31 values.
+// Products of two canonical values are < 2^62, so four of them are summed in 64 bits
+// (one multiply-add each) before a reduction.
+__device__ __forceinline__ uint32_t m31_reduce64(uint64_t x) {
+    uint64_t y = (x & P) + (x >> 31);                           // < 2^31 + 2^33
+    uint32_t z = (uint32_t)(y & P) + (uint32_t)(y >> 31);       // < 2^31 + 8
+    return m31_reduce_once(z);
+}
+
 __global__ __launch_bounds__(256) void combine_kernel(const uint32_t *lde, uint64_t cs, uint32_t n_cols, uint64_t n,
                                                       const uint32_t *powers, uint32_t *out) {
     extern __shared__ uint32_t pw[];
@@ -119,15 +127,26 @@ __global__ __launch_bounds__(256) void combine_kernel(const uint32_t *lde, uint6
     __syncthreads();
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    uint32_t a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-    for (uint32_t c = 0; c < n_cols; c++) {
-        uint32_t v = lde[(uint64_t)c * cs + i];
-        a0 = m31_add(a0, m31_mul(pw[4 * c], v));
-        a1 = m31_add(a1, m31_mul(pw[4 * c + 1], v));
-        a2 = m31_add(a2, m31_mul(pw[4 * c + 2], v));
-        a3 = m31_add(a3, m31_mul(pw[4 * c + 3], v));
+    uint32_t a[4] = {0, 0, 0, 0};
+    uint32_t c = 0;
+    for (; c + 4 <= n_cols; c += 4) {
+        uint32_t v[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) v[k] = lde[(uint64_t)(c + k) * cs + i];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            uint64_t acc = 0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) acc += (uint64_t)pw[4 * (c + k) + q] * v[k];
+            a[q] = m31_add(a[q], m31_reduce64(acc));
+        }
     }
-    *reinterpret_cast<uint4 *>(out + 4 * i) = make_uint4(a0, a1, a2, a3);
+    for (; c < n_cols; c++) {
+        uint32_t v = lde[(uint64_t)c * cs + i];
+#pragma unroll
+        for (int q = 0; q < 4; q++) a[q] = m31_add(a[q], m31_mul(pw[4 * c + q], v));
+    }
+    *reinterpret_cast<uint4 *>(out + 4 * i) = make_uint4(a[0], a[1], a[2], a[3]);
 }
 
 int32_t launch_combine(Ctx *ctx, const uint32_t *d_lde, uint64_t cs, uint32_t n_cols, uint64_t n,
@@ -136,6 +155,62 @@ int32_t launch_combine(Ctx *ctx, const uint32_t *d_lde, uint64_t cs, uint32_t n_
     if (lds > 64 * 1024) return ZS_ERR_INPUT;
     ZS_LAUNCH(ctx, "combine_kernel", combine_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), lds, ctx->stream,
               d_lde, cs, n_cols, n, d_powers, d_out);
+    ZS_HIP(ctx, hipGetLastError());
+    return ZS_OK;
+}
+
+}  // namespace zs
+
+namespace zs {
+
+// Fiat-Shamir on the device: one lane absorbs the 32-byte root at `digest` into
+// the channel state and draws a QM31 challenge (fri.zig:205-210 with
+// channel.zig:49-69), so a layer's fold never waits for the host.  The state is
+// staged in LDS (the byte-level hasher would otherwise make every access a
+// dependent global load) and written back.
+__global__ __launch_bounds__(64) void channel_draw_kernel(B3Hasher *st, const uint32_t *digest, uint32_t *out) {
+    __shared__ B3Hasher h, snap;
+    constexpr uint32_t W = sizeof(B3Hasher) / 4;
+    static_assert(sizeof(B3Hasher) % 4 == 0, "hasher state is word-sized");
+    uint32_t *hw = reinterpret_cast<uint32_t *>(&h);
+    const uint32_t *sw = reinterpret_cast<const uint32_t *>(st);
+    for (uint32_t k = threadIdx.x; k < W; k += blockDim.x) hw[k] = sw[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t d[8];
+        for (int i = 0; i < 8; i++) d[i] = digest[i];
+        channel_absorb_digest(h, d);
+        uint32_t q[4];
+        channel_sample_qm31(h, snap, q);
+        for (int i = 0; i < 4; i++) out[i] = q[i];
+    }
+    __syncthreads();
+    uint32_t *dw = reinterpret_cast<uint32_t *>(st);
+    for (uint32_t k = threadIdx.x; k < W; k += blockDim.x) dw[k] = hw[k];
+}
+
+int32_t launch_channel_draw(Ctx *ctx, B3Hasher *d_state, const uint32_t *d_digest, uint32_t *d_out) {
+    ZS_LAUNCH(ctx, "channel_draw_kernel", channel_draw_kernel, dim3(1), dim3(64), 0, ctx->stream, d_state, d_digest,
+              d_out);
+    ZS_HIP(ctx, hipGetLastError());
+    return ZS_OK;
+}
+
+// out[c] = x^c for c < n (square-and-multiply per lane)
+__global__ __launch_bounds__(256) void qm31_powers_kernel(const uint32_t *x, uint32_t *out, uint32_t n) {
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    qm31 b = ld_qm(x), r = {{1, 0}, {0, 0}};
+    for (uint32_t e = c; e; e >>= 1) {
+        if (e & 1u) r = qm_mul(r, b);
+        b = qm_mul(b, b);
+    }
+    st_qm(out + 4 * c, r);
+}
+
+int32_t launch_qm31_powers(Ctx *ctx, const uint32_t *d_x, uint32_t *d_out, uint32_t n) {
+    ZS_LAUNCH(ctx, "qm31_powers_kernel", qm31_powers_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, d_x,
+              d_out, n);
     ZS_HIP(ctx, hipGetLastError());
     return ZS_OK;
 }

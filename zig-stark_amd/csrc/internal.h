@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/zs_hip.h"
+#include "b3_hasher.h"
 #include "field.h"
 
 namespace zs {
@@ -165,6 +166,14 @@ int32_t launch_fri_fold(Ctx *ctx, const uint32_t *d_cur, uint64_t n, uint32_t dl
                         const uint32_t *d_alpha, uint32_t *d_next, uint32_t *d_next_leaves);
 
 // g[i] = sum_c powers[c] * lde_c[i]: QM31 combination of n_cols M31 columns of n values
+// device Fiat-Shamir: absorb the digest at d_digest, write a QM31 challenge to d_out
+int32_t launch_channel_draw(Ctx *ctx, B3Hasher *d_state, const uint32_t *d_digest, uint32_t *d_out);
+int32_t launch_qm31_powers(Ctx *ctx, const uint32_t *d_x, uint32_t *d_out, uint32_t n);
+// FRI commit phase on a device codeword; challenges from `alphas` (host) or drawn
+// on the device from the channel state at d_chan (left advanced past the last root)
+int32_t fri_commit_impl(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t dlog, uint32_t log_blowup,
+                        uint32_t remainder_log, const uint32_t *alphas, B3Hasher *d_chan, uint32_t flags,
+                        uint8_t *roots_out, uint32_t *remainder_out, zs_fri_result **out_result);
 int32_t launch_combine(Ctx *ctx, const uint32_t *d_lde, uint64_t cs, uint32_t n_cols, uint64_t n,
                        const uint32_t *d_powers, uint32_t *d_out);
 
@@ -203,4 +212,6 @@ struct zs_fri_result {
     std::vector<size_t> layer_off, tree_off;
 };
 
-
+struct zs_channel {
+    zs::B3Hasher h;
+};

@@ -32,6 +32,23 @@ double now_s() {
 
 constexpr char kCommitSep[] = "zig-stark:hip:commit";
 
+// The channel every commit job starts from: Channel.init("zig-stark:hip:commit")
+const B3Hasher &commit_channel() {
+    static const B3Hasher st = [] {
+        B3Hasher h;
+        h.init();
+        h.update(reinterpret_cast<const uint8_t *>("zig-stark:channel"), 17);
+        h.update(reinterpret_cast<const uint8_t *>(kCommitSep), sizeof(kCommitSep) - 1);
+        return h;
+    }();
+    return st;
+}
+
+// One trace's commitment, queued on the context's stream with no host round
+// trip until the end: LDE and row tree; the channel (on the device) absorbs the
+// trace root and draws gamma; g = sum_c gamma^c lde_c; FRI on g with challenges
+// drawn from the same channel.  Scratch: 7 channel state, 8 the codeword g,
+// 9 gamma and its powers; FRI uses 4-6 and the LDE 1-2.
 int32_t commit_job(zs_hip_ctx *h, const uint32_t *d_cols, uint32_t n_cols, uint32_t log_n, uint32_t log_blowup,
                    uint32_t remainder_log, uint32_t *d_lde, uint8_t *d_nodes, uint8_t trace_root[32],
                    uint8_t *fri_roots, uint32_t *remainder, uint32_t flags) {
@@ -41,40 +58,30 @@ int32_t commit_job(zs_hip_ctx *h, const uint32_t *d_cols, uint32_t n_cols, uint3
         return ZS_ERR_INPUT;
     Ctx *ctx = &h->c;
     const size_t N = size_t(1) << LT;
-    // 1. LDE and the row-leaf tree
     ZS_TRY(zs_m31_lde_commit_dev(h, d_cols, n_cols, log_n, log_blowup, d_lde, d_nodes, flags & ZS_FLAG_PLAIN));
-    ZS_HIP(ctx, hipMemcpyAsync(trace_root, d_nodes + (tree_words(LT) - 8) * 4, 32, hipMemcpyDeviceToHost,
-                               ctx->stream));
-    ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    // 2. gamma from the trace root
-    std::unique_ptr<zs_channel, void (*)(zs_channel *)> chan(
-        zs_channel_new(reinterpret_cast<const uint8_t *>(kCommitSep), sizeof(kCommitSep) - 1), zs_channel_free);
-    zs_channel_absorb(chan.get(), trace_root, 32);
-    uint32_t gw[4];
-    zs_channel_sample_qm31(chan.get(), gw);
-    // 3. g = sum_c gamma^c lde_c
-    std::vector<uint32_t> pw(size_t(n_cols) * 4);
-    qm31 gamma = qm_from_words(gw), acc = {{1, 0}, {0, 0}};
-    for (uint32_t c = 0; c < n_cols; c++) {
-        qm_to_words(acc, &pw[4 * c]);
-        acc = qm_mul(acc, gamma);
-    }
-    ZS_TRY(ctx->ensure_scratch(9, pw.size() * 4));
+    const uint32_t *d_root = reinterpret_cast<const uint32_t *>(d_nodes) + tree_words(LT) - 8;
+    ZS_TRY(ctx->ensure_scratch(7, sizeof(B3Hasher)));
     ZS_TRY(ctx->ensure_scratch(8, N * 16));
-    ZS_HIP(ctx, hipMemcpyAsync(ctx->scratch[9].ptr, pw.data(), pw.size() * 4, hipMemcpyHostToDevice, ctx->stream));
-    ZS_TRY(launch_combine(ctx, d_lde, N, n_cols, N, ctx->scratch[9].as<uint32_t>(), ctx->scratch[8].as<uint32_t>()));
-    // 4. FRI commit phase on g, challenges from the same channel
-    return zs_fri_commit_dev(h, ctx->scratch[8].as<uint32_t>(), LT, log_blowup, remainder_log, nullptr, chan.get(), 0,
-                             fri_roots, remainder, nullptr);
+    ZS_TRY(ctx->ensure_scratch(9, (size_t(n_cols) + 1) * 16));
+    B3Hasher *d_chan = ctx->scratch[7].as<B3Hasher>();
+    uint32_t *d_gamma = ctx->scratch[9].as<uint32_t>(), *d_powers = d_gamma + 4;
+    ZS_HIP(ctx, hipMemcpyAsync(d_chan, &commit_channel(), sizeof(B3Hasher), hipMemcpyHostToDevice, ctx->stream));
+    ZS_TRY(launch_channel_draw(ctx, d_chan, d_root, d_gamma));
+    ZS_TRY(launch_qm31_powers(ctx, d_gamma, d_powers, n_cols));
+    ZS_TRY(launch_combine(ctx, d_lde, N, n_cols, N, d_powers, ctx->scratch[8].as<uint32_t>()));
+    ZS_HIP(ctx, hipMemcpyAsync(trace_root, d_root, 32, hipMemcpyDeviceToHost, ctx->stream));
+    // the FRI call ends with a synchronisation that also covers the trace root copy
+    return fri_commit_impl(h, ctx->scratch[8].as<uint32_t>(), LT, log_blowup, remainder_log, nullptr, d_chan,
+                           flags & ZS_FLAG_NO_GRAPH, fri_roots, remainder, nullptr);
 }
 
 // Device memory of one commit job: its input, LDE and row tree, and every scratch
 // buffer commit_job uses (slot numbers of Ctx::scratch), each 256-byte aligned.
 struct JobLayout {
-    enum Part { IN, LDE, TREE, COEF, BIG, FRI_LAYERS, FRI_TREES, ALPHAS, COMB, POWERS, NPARTS };
+    enum Part { IN, LDE, TREE, COEF, BIG, FRI_LAYERS, FRI_TREES, ALPHAS, CHAN, COMB, POWERS, NPARTS };
     uint64_t off[NPARTS], len[NPARTS], total;
 };
-constexpr int kSlotOf[JobLayout::NPARTS] = {-1, -1, -1, 1, 2, 4, 5, 6, 8, 9};
+constexpr int kSlotOf[JobLayout::NPARTS] = {-1, -1, -1, 1, 2, 4, 5, 6, 7, 8, 9};
 
 JobLayout job_layout(uint32_t n_cols, uint32_t log_n, uint32_t log_blowup, uint32_t remainder_log) {
     JobLayout j{};
@@ -92,8 +99,9 @@ JobLayout job_layout(uint32_t n_cols, uint32_t log_n, uint32_t log_blowup, uint3
     j.len[JobLayout::FRI_LAYERS] = fl;
     j.len[JobLayout::FRI_TREES] = ft;
     j.len[JobLayout::ALPHAS] = uint64_t(L) * 16;
+    j.len[JobLayout::CHAN] = sizeof(B3Hasher);
     j.len[JobLayout::COMB] = 16 * N;
-    j.len[JobLayout::POWERS] = 16 * c;
+    j.len[JobLayout::POWERS] = 16 * (c + 1);
     uint64_t o = 0;
     for (int k = 0; k < JobLayout::NPARTS; k++) {
         j.off[k] = o;
@@ -261,9 +269,9 @@ static void serve_one(zs_server *s, zs_hip_ctx *w, ServReq *q) {
     if (!rc)
         rc = commit_job(w, d_cols, r.n_cols, r.log_n, r.log_blowup, r.remainder_log,
                         static_cast<uint32_t *>(part(JobLayout::LDE)), static_cast<uint8_t *>(part(JobLayout::TREE)),
-                        q->resp.trace_root, q->resp.fri_roots, q->resp.remainder, 0);
+                        q->resp.trace_root, q->resp.fri_roots, q->resp.remainder, ZS_FLAG_NO_GRAPH);
     (void)hipStreamSynchronize(ctx->stream);
-    ctx->release_scratch();
+    ctx->release_scratch();  // (each request's buffers differ, so its FRI is not captured as a graph)
     q->resp.status = rc;
     q->resp.fri_layers = r.log_n + r.log_blowup - r.remainder_log;
 }
