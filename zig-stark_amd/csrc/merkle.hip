@@ -12,7 +12,8 @@
 // several 1024-byte chunks and are merged with BLAKE3's chunk tree.  Node
 // levels are built bottom-up, three per launch: a lane owns an 8-child
 // subtree and hashes all 7 of its nodes, so no lane idles while the level
-// narrows; the last 12 levels run in one workgroup.
+// narrows; from 2^13 nodes down, workgroups keep 8-level subtrees in LDS and
+// hash the narrow levels with four lanes per compression.
 #include "blake3.h"
 #include "internal.h"
 
@@ -109,6 +110,13 @@ __global__ __launch_bounds__(256) void qm31_leaf_kernel(const uint32_t *vals, ui
     o[1] = make_uint4(d[4], d[5], d[6], d[7]);
 }
 
+// LDS writes of a wave must land before its lanes read each other's values
+__device__ __forceinline__ void wave_lds_sync_m() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 __device__ __forceinline__ void ld_digest(const uint32_t *p, uint32_t d[8]) {
     const uint4 *q = reinterpret_cast<const uint4 *>(p);
     uint4 a = q[0], b = q[1];
@@ -175,97 +183,174 @@ __global__ __launch_bounds__(256) void tree_subtree3_multi_kernel(uint32_t *base
     subtree_lane<3>(base + mt.child[t], base + mt.lvl1[t], mt.lane_end[t] - start, g - start);
 }
 
-// the top levels in one workgroup: one node per thread per level, levels
-// separated by a device-scope fence and a barrier
-// Parents of each level are written to the tree and kept in LDS for the next
-// level (ping-pong: 2^(k-1) digests, then 2^(k-2), ...), so only the first level
-// reads global memory.  lds: 3 * 2^(log_children-1) * 16 words.
-__device__ __forceinline__ void tail_levels(uint32_t *child, uint32_t log_children, uint32_t *lds) {
-    uint32_t cnt = 1u << log_children;
-    uint32_t *out = child + (uint64_t)cnt * 8;  // next level in the tree
-    uint32_t *bufs[2] = {lds, lds + (cnt >> 1) * 8};
-    for (uint32_t l = 0; l < log_children; l++) {
-        uint32_t half = cnt >> 1;
-        const uint32_t *src = l == 0 ? child : bufs[(l - 1) & 1];
+// ---------------------------------------------------------------------------
+// Narrow node levels: four lanes per compression.
+//
+// Near the root a level has too few nodes to fill the GPU, so its time is the
+// latency of one hash2 (two compressions, about 700 dependent instructions
+// each on one lane).  Here the four lanes of a quad share a compression: lane j
+// holds column j of the 4x4 state (s[j], s[4+j], s[8+j], s[12+j]) and runs
+// the column G on it; for the diagonal G, rows 1-3 are rotated across the quad
+// by 1, 2 and 3 lanes (DPP quad_perm) and rotated back afterwards.  The
+// message words a lane needs in round r are m[PERM^r(2j)], m[PERM^r(2j+1)]
+// (column) and m[PERM^r(8+2j)], m[PERM^r(9+2j)] (diagonal): the quad writes
+// the block's 16 words to LDS and each lane reads its 28 through a fixed
+// per-lane index table.
+// ---------------------------------------------------------------------------
+
+// kQuadMsg[j][4r + s]: message word of lane j, round r, slot s (column x, y, diagonal x, y)
+__constant__ uint8_t kQuadMsg[4][28] = {
+    {0, 1, 8, 9, 2, 6, 1, 11, 3, 4, 6, 5, 10, 7, 4, 0, 12, 13, 7, 2, 9, 14, 13, 3, 11, 15, 14, 10},
+    {2, 3, 10, 11, 3, 10, 12, 5, 10, 12, 9, 0, 12, 9, 11, 2, 9, 11, 5, 3, 11, 5, 0, 10, 5, 0, 2, 12},
+    {4, 5, 12, 13, 7, 0, 9, 14, 13, 2, 11, 15, 14, 3, 5, 8, 15, 10, 0, 1, 8, 12, 2, 6, 1, 9, 3, 4},
+    {6, 7, 14, 15, 4, 13, 15, 8, 7, 14, 8, 1, 13, 15, 1, 6, 14, 8, 6, 4, 15, 1, 4, 7, 8, 6, 7, 13},
+};
+
+__constant__ uint32_t kIV[8] = {0x6A09E667u, 0xBB67AE85u, 0x3C6EF372u, 0xA54FF53Au,
+                                0x510E527Fu, 0x9B05688Cu, 0x1F83D9ABu, 0x5BE0CD19u};
+
+// lane j of the quad reads lane p[j]: rotations by 1, 2, 3 lanes
+constexpr int QROT1 = 0x39, QROT2 = 0x4E, QROT3 = 0x93;
+template <int CTRL>
+__device__ __forceinline__ uint32_t qperm(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, true);
+}
+
+__device__ __forceinline__ void quad_compress(uint32_t &a, uint32_t &b, uint32_t &c, uint32_t &d, const uint32_t *msg,
+                                              const uint32_t (&mo)[28]) {
+#pragma unroll
+    for (int r = 0; r < 7; r++) {
+        const uint32_t m0 = msg[mo[4 * r]], m1 = msg[mo[4 * r + 1]];
+        const uint32_t m2 = msg[mo[4 * r + 2]], m3 = msg[mo[4 * r + 3]];
+        ZS_B3_G(a, b, c, d, m0, m1);
+        b = qperm<QROT1>(b);
+        c = qperm<QROT2>(c);
+        d = qperm<QROT3>(d);
+        ZS_B3_G(a, b, c, d, m2, m3);
+        b = qperm<QROT3>(b);
+        c = qperm<QROT2>(c);
+        d = qperm<QROT1>(d);
+    }
+}
+
+// hash2 of the 16 words at xp (left || right digest) by the quad; lane j gets
+// digest words j and 4 + j.  qm: the quad's 32 LDS words (block 0, then block 1
+// whose words 4..15 are zero).  iv_lo / iv_hi: IV[j], IV[4 + j].
+__device__ __forceinline__ void quad_hash2(const uint32_t *xp, uint32_t *qm, const uint32_t (&mo)[28], uint32_t j,
+                                           uint32_t iv_lo, uint32_t iv_hi, uint32_t &o_lo, uint32_t &o_hi) {
+    if (j == 0) {
+        qm[0] = 0x2d67697au;  // "zig-"
+        qm[1] = 0x72617473u;  // "star"
+        qm[2] = 0x61703a6bu;  // "k:pa"
+        qm[3] = 0x7269u | (xp[0] << 16);
+    } else {
+        const uint32_t xa = xp[4 * j - 4], xb = xp[4 * j - 3], xc = xp[4 * j - 2], xd = xp[4 * j - 1], xe = xp[4 * j];
+        qm[4 * j] = __builtin_amdgcn_alignbit(xb, xa, 16);
+        qm[4 * j + 1] = __builtin_amdgcn_alignbit(xc, xb, 16);
+        qm[4 * j + 2] = __builtin_amdgcn_alignbit(xd, xc, 16);
+        qm[4 * j + 3] = __builtin_amdgcn_alignbit(xe, xd, 16);
+    }
+    const uint32_t y0 = xp[12 + j], y1 = xp[j < 3 ? 13 + j : 15u];
+    qm[16 + j] = __builtin_amdgcn_alignbit(j < 3 ? y1 : 0u, y0, 16);
+    wave_lds_sync_m();
+    uint32_t a = iv_lo, b = iv_hi, c = iv_lo, d = j == 2 ? 64u : (j == 3 ? B3_CHUNK_START : 0u);
+    quad_compress(a, b, c, d, qm, mo);
+    const uint32_t h0 = a ^ c, h1 = b ^ d;
+    a = h0;
+    b = h1;
+    c = iv_lo;
+    d = j == 2 ? 14u : (j == 3 ? (B3_CHUNK_END | B3_ROOT) : 0u);
+    quad_compress(a, b, c, d, qm + 16, mo);
+    o_lo = a ^ c;
+    o_hi = b ^ d;
+    wave_lds_sync_m();  // every lane has read qm before the quad's next hash2 rewrites it
+}
+
+// Workgroups that each build K levels of a subtree of 2^K children, keeping
+// the levels in LDS.  Levels with at least one node per thread hash one node per
+// lane; narrower levels use a quad per node.  Several trees can share a launch:
+// tree t starts at word offset child[t] (its level with 2^r[t] children) and owns
+// workgroups [wg_end[t-1], wg_end[t]), 2^(r[t]-K[t]) of them.
+struct BlockTrees {
+    static constexpr int MAX = 32;
+    uint32_t n;
+    uint64_t child[MAX];
+    uint32_t r[MAX], K[MAX], wg_end[MAX];
+};
+
+constexpr uint32_t BLOCK_NT = 256;
+
+inline size_t block_lds_words(uint32_t K) { return (size_t(6) << K) + (BLOCK_NT / 4) * 32; }
+
+__global__ __launch_bounds__(BLOCK_NT) void tree_block_kernel(uint32_t *base, const BlockTrees bt) {
+    extern __shared__ uint32_t sm[];
+    const uint32_t b = blockIdx.x;
+    uint32_t t = 0;
+    while (b >= bt.wg_end[t]) t++;
+    const uint32_t s = b - (t ? bt.wg_end[t - 1] : 0u);
+    const uint32_t K = bt.K[t], r = bt.r[t];
+    const uint32_t tid = threadIdx.x, j = tid & 3u, quad = tid >> 2;
+    constexpr uint32_t NQ = BLOCK_NT / 4;
+    uint32_t *bufs[2] = {sm, sm + (4u << K)};
+    uint32_t *qm = sm + (6u << K) + quad * 32;
+    const uint32_t iv_lo = kIV[j], iv_hi = kIV[4 + j];
+    uint32_t mo[28];
+#pragma unroll
+    for (int k = 0; k < 28; k++) mo[k] = kQuadMsg[j][k];
+    qm[20 + j] = 0u;
+    qm[24 + j] = 0u;
+    qm[28 + j] = 0u;
+    uint64_t lvl = bt.child[t];  // word offset of the current child level
+    uint64_t width = uint64_t(1) << r;
+    const uint32_t *src = base + lvl + (uint64_t)s * (8u << K);
+    __syncthreads();
+    for (uint32_t l = 0; l < K; l++) {
+        const uint32_t P = 1u << (K - 1 - l);
+        uint32_t *out = base + lvl + width * 8 + (uint64_t)s * P * 8;
         uint32_t *dst = bufs[l & 1];
-        for (uint32_t i = threadIdx.x; i < half; i += blockDim.x) {
-            uint32_t a[8], b[8], d[8];
-            ld_digest(src + (uint64_t)(2 * i) * 8, a);
-            ld_digest(src + (uint64_t)(2 * i + 1) * 8, b);
-            b3_hash2(a, b, d);
-            st_digest(out + (uint64_t)i * 8, d);
-            st_digest(dst + i * 8, d);
+        if (P >= BLOCK_NT) {
+            for (uint32_t p = tid; p < P; p += BLOCK_NT) {
+                uint32_t x[8], y[8], d[8];
+                ld_digest(src + 16 * p, x);
+                ld_digest(src + 16 * p + 8, y);
+                b3_hash2(x, y, d);
+                st_digest(out + 8 * p, d);
+                st_digest(dst + 8 * p, d);
+            }
+        } else {
+            for (uint32_t p = quad; p < P; p += NQ) {
+                uint32_t lo, hi;
+                quad_hash2(src + 16 * p, qm, mo, j, iv_lo, iv_hi, lo, hi);
+                out[8 * p + j] = lo;
+                out[8 * p + 4 + j] = hi;
+                dst[8 * p + j] = lo;
+                dst[8 * p + 4 + j] = hi;
+            }
         }
         __syncthreads();
-        out += (uint64_t)half * 8;
-        cnt = half;
+        src = dst;
+        lvl += width * 8;
+        width >>= 1;
     }
 }
 
-inline size_t tail_lds_bytes(uint32_t log_children) {
-    return log_children ? size_t(3) * (size_t(1) << (log_children - 1)) * 16 : 16;
-}
-
-__global__ __launch_bounds__(1024) void tree_tail_kernel(uint32_t *child, uint32_t log_children) {
-    extern __shared__ uint32_t tail_lds[];
-    tail_levels(child, log_children, tail_lds);
-}
-
-// one workgroup per tree: tree blockIdx.x has 2^lvl1[t] children at word offset child[t]
-__global__ __launch_bounds__(1024) void tree_tail_multi_kernel(uint32_t *base, const MultiTrees mt) {
-    extern __shared__ uint32_t tail_lds[];
-    uint32_t t = blockIdx.x;
-    tail_levels(base + mt.child[t], (uint32_t)mt.lvl1[t], tail_lds);
-}
-
-// Build levels from_level+1 .. top over level from_level (already written).
-int32_t launch_tree_levels(Ctx *ctx, uint32_t *d_nodes, uint32_t log_leaves, uint32_t from_level) {
-    uint32_t level = from_level;
-    while (level < log_leaves) {
-        uint32_t remaining = log_leaves - level;
-        uint32_t log_children = remaining;  // children at `level`: 2^remaining
-        uint32_t *child = d_nodes + level_offset_words(log_leaves, level);
-        if (log_children <= 12) {
-            ZS_LAUNCH(ctx, "tree_tail_kernel", tree_tail_kernel, dim3(1), dim3(1024), tail_lds_bytes(log_children),
-                      ctx->stream, child, log_children);
-            ZS_HIP(ctx, hipGetLastError());
-            break;
-        }
-        uint32_t nlev = remaining < 3 ? remaining : 3;
-        uint64_t n_lanes = (uint64_t(1) << log_children) >> nlev;
-        uint32_t *lvl1 = d_nodes + level_offset_words(log_leaves, level + 1);
-        dim3 grid((uint32_t)((n_lanes + 255) / 256));
-        if (nlev == 3)
-            ZS_LAUNCH(ctx, "tree_subtree_kernel<3>", tree_subtree_kernel<3>, grid, dim3(256), 0, ctx->stream, child, lvl1,
-                      n_lanes);
-        else if (nlev == 2)
-            ZS_LAUNCH(ctx, "tree_subtree_kernel<2>", tree_subtree_kernel<2>, grid, dim3(256), 0, ctx->stream, child, lvl1,
-                      n_lanes);
-        else
-            ZS_LAUNCH(ctx, "tree_subtree_kernel<1>", tree_subtree_kernel<1>, grid, dim3(256), 0, ctx->stream, child, lvl1,
-                      n_lanes);
-        ZS_HIP(ctx, hipGetLastError());
-        level += nlev;
-    }
-    return ZS_OK;
-}
-
-// Every node level of several trees whose leaves are written: tree k has
-// 2^log_leaves[k] leaves at word offset off_words[k] from `base`.  Levels
-// above 2^12 children go 3 at a time in one launch across all trees; the top
-// levels of every tree then run in one launch, a workgroup per tree.
+// Every node level of several trees whose level `level[k]` is written: tree k
+// has 2^log_leaves[k] leaves at word offset off_words[k] from `base`.  Wide
+// levels go 3 at a time (one lane per 8-child subtree, all trees in one launch);
+// from 2^13 children down, workgroup subtrees of 8 levels, then one workgroup
+// per tree for the rest.
 int32_t launch_trees_multi(Ctx *ctx, uint32_t *base, const uint64_t *off_words, const uint32_t *log_leaves,
                            uint32_t n_trees) {
     std::vector<uint32_t> level(n_trees, 0);
     for (uint32_t start = 0; start < n_trees; start += MultiTrees::MAX) {
-        uint32_t cnt = std::min<uint32_t>(MultiTrees::MAX, n_trees - start);
+        const uint32_t cnt = std::min<uint32_t>(MultiTrees::MAX, n_trees - start);
         for (;;) {
             MultiTrees mt{};
             uint64_t lanes = 0;
             std::vector<uint32_t> in;
             for (uint32_t k = start; k < start + cnt; k++) {
                 uint32_t rem = log_leaves[k] - level[k];
-                if (rem <= 12) continue;
+                if (rem < 14) continue;
                 mt.child[mt.n] = off_words[k] + level_offset_words(log_leaves[k], level[k]);
                 mt.lvl1[mt.n] = off_words[k] + level_offset_words(log_leaves[k], level[k] + 1);
                 lanes += (uint64_t(1) << rem) >> 3;
@@ -279,23 +364,39 @@ int32_t launch_trees_multi(Ctx *ctx, uint32_t *base, const uint64_t *off_words, 
             ZS_HIP(ctx, hipGetLastError());
             for (uint32_t k : in) level[k] += 3;
         }
-        MultiTrees mt{};
-        uint32_t max_rem = 0;
-        for (uint32_t k = start; k < start + cnt; k++) {
-            uint32_t rem = log_leaves[k] - level[k];
-            if (rem == 0) continue;
-            mt.child[mt.n] = off_words[k] + level_offset_words(log_leaves[k], level[k]);
-            mt.lvl1[mt.n] = rem;
-            mt.n++;
-            max_rem = std::max(max_rem, rem);
-        }
-        if (mt.n) {
-            ZS_LAUNCH(ctx, "tree_tail_multi_kernel", tree_tail_multi_kernel, dim3(mt.n), dim3(1024),
-                      tail_lds_bytes(max_rem), ctx->stream, base, mt);
+        for (;;) {
+            BlockTrees bt{};
+            uint32_t wgs = 0, kmax = 0;
+            std::vector<uint32_t> in;
+            for (uint32_t k = start; k < start + cnt; k++) {
+                uint32_t rem = log_leaves[k] - level[k];
+                if (rem == 0) continue;
+                uint32_t K = rem > 8 ? 8 : rem;
+                bt.child[bt.n] = off_words[k] + level_offset_words(log_leaves[k], level[k]);
+                bt.r[bt.n] = rem;
+                bt.K[bt.n] = K;
+                wgs += 1u << (rem - K);
+                bt.wg_end[bt.n] = wgs;
+                bt.n++;
+                kmax = std::max(kmax, K);
+                in.push_back(k);
+            }
+            if (bt.n == 0) break;
+            ZS_LAUNCH(ctx, "tree_block_kernel", tree_block_kernel, dim3(wgs), dim3(BLOCK_NT),
+                      block_lds_words(kmax) * 4, ctx->stream, base, bt);
             ZS_HIP(ctx, hipGetLastError());
+            for (uint32_t i = 0; i < bt.n; i++) level[in[i]] += bt.K[i];
         }
     }
     return ZS_OK;
+}
+
+// Build levels from_level+1 .. top over level from_level (already written): the
+// levels from there up are laid out like a tree of 2^(log_leaves - from_level) leaves.
+int32_t launch_tree_levels(Ctx *ctx, uint32_t *d_nodes, uint32_t log_leaves, uint32_t from_level) {
+    const uint64_t off = level_offset_words(log_leaves, from_level);
+    const uint32_t lg = log_leaves - from_level;
+    return launch_trees_multi(ctx, d_nodes, &off, &lg, 1);
 }
 
 int32_t launch_row_commit(Ctx *ctx, const uint32_t *d_cols, uint64_t col_stride, uint32_t n_cols, uint32_t log_rows,
