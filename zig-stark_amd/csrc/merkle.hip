@@ -12,7 +12,7 @@
 // several 1024-byte chunks and are merged with BLAKE3's chunk tree.  Node
 // levels are built bottom-up, three per launch: a lane owns an 8-child
 // subtree and hashes all 7 of its nodes, so no lane idles while the level
-// narrows; from 2^13 nodes down, workgroups keep 8-level subtrees in LDS and
+// narrows; from 2^16 nodes down, workgroups keep 8-level subtrees in LDS and
 // hash the narrow levels with four lanes per compression.
 #include "blake3.h"
 #include "internal.h"
@@ -334,10 +334,14 @@ __global__ __launch_bounds__(BLOCK_NT) void tree_block_kernel(uint32_t *base, co
     }
 }
 
+// levels with at least 2^17 children are hashed 3 at a time by one lane per
+// 8-child subtree (throughput); below that, by workgroup subtrees (latency)
+constexpr uint32_t kSubtreeMinLog = 17;
+
 // Every node level of several trees whose level `level[k]` is written: tree k
 // has 2^log_leaves[k] leaves at word offset off_words[k] from `base`.  Wide
 // levels go 3 at a time (one lane per 8-child subtree, all trees in one launch);
-// from 2^13 children down, workgroup subtrees of 8 levels, then one workgroup
+// from 2^16 children down, workgroup subtrees of 8 levels, then one workgroup
 // per tree for the rest.
 int32_t launch_trees_multi(Ctx *ctx, uint32_t *base, const uint64_t *off_words, const uint32_t *log_leaves,
                            uint32_t n_trees) {
@@ -350,7 +354,7 @@ int32_t launch_trees_multi(Ctx *ctx, uint32_t *base, const uint64_t *off_words, 
             std::vector<uint32_t> in;
             for (uint32_t k = start; k < start + cnt; k++) {
                 uint32_t rem = log_leaves[k] - level[k];
-                if (rem < 14) continue;
+                if (rem < kSubtreeMinLog) continue;
                 mt.child[mt.n] = off_words[k] + level_offset_words(log_leaves[k], level[k]);
                 mt.lvl1[mt.n] = off_words[k] + level_offset_words(log_leaves[k], level[k] + 1);
                 lanes += (uint64_t(1) << rem) >> 3;
