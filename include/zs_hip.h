@@ -246,6 +246,28 @@ int32_t zs_server_poll(zs_server *srv, uint64_t ticket, zs_response *out, int32_
 int32_t zs_server_memory(zs_server *srv, uint64_t *in_use, uint64_t *peak);
 void zs_server_free(zs_server *srv);
 
+/* ---- GenericStark trace commitment (src/m31/stark.zig:343-414) -------------- */
+/* The trace half of proveWithPreprocessed for AIRs without preprocessed
+ * columns: each QM31 column of 2^trace_log values is interpolated over
+ * H = <w> (w = primitiveRootOfUnity(trace_log)), evaluated on
+ * D = FRI_OFFSET * <w_ev> (w_ev = primitiveRootOfUnity(trace_log + log_blowup)),
+ * committed with hashQM31 leaves (hashCodeword, stark.zig:786-790) and
+ * MerkleTree.init, and each root is absorbed into `chan` in column order
+ * (absorbDigest).  The interpolant is unique, so the DFTs used here give the
+ * reference's Lagrange/Horner values exactly.
+ *   d_cols:      n_cols * 2^trace_log QM31 values, column-major
+ *   d_codewords: out, n_cols * 2^(trace_log+log_blowup) QM31 values
+ *   d_nodes:     out, n_cols trees; tree c at c * (2^(trace_log+log_blowup+1) - 1) * 32 bytes
+ *   roots_out:   n_cols * 32 bytes;  chan: may be NULL
+ * 1 <= trace_log, trace_log + log_blowup <= 24. */
+int32_t zs_qm31_trace_commit_dev(zs_hip_ctx *ctx, const uint32_t *d_cols, uint32_t n_cols, uint32_t trace_log,
+                                 uint32_t log_blowup, uint32_t *d_codewords, uint8_t *d_nodes, uint8_t *roots_out,
+                                 zs_channel *chan);
+/* FibAir.generateTrace (stark.zig:918-935) with seeds a0, a1 in place of 0, 1:
+ * out = two QM31 columns of 2^log_n values, [a_0 .. a_{n-1}] and [a_1 .. a_n],
+ * a_{i+2} = a_i + a_{i+1}, each M31 value in the first QM31 coordinate.  Host buffers. */
+int32_t zs_fib_trace(uint32_t a0, uint32_t a1, uint32_t log_n, uint32_t *out);
+
 /* ---- synthetic input ---------------------------------------------------------- */
 /* Seeded uniform M31 values: element i is the top 31 bits of the (i+1)-th
  * SplitMix64 output from `seed` (2^31 - 1 rejected). */

@@ -1092,3 +1092,58 @@ void zo_fill_m31(uint64_t seed, uint32_t *out, size_t n) {
         out[i] = v;
     }
 }
+
+/* ======================================================================== */
+/* GenericStark trace commitment: src/m31/stark.zig:343-414 (no preprocessed  */
+/* columns), with FibAir's trace (stark.zig:877-931)                         */
+/* ======================================================================== */
+/* stark.zig:343-414: per column, interpolate over H = {w^i} (O(n^2) Lagrange,
+ * univariate.zig:79-139), evaluate at D = {FRI_OFFSET * w_ev^i} by Horner,
+ * hashQM31 leaves (hashCodeword, stark.zig:786-790), MerkleTree.init, and
+ * absorbDigest(root) in column order. */
+int zo_trace_commit(const uint32_t *cols, size_t n_cols, uint32_t trace_log, uint32_t log_blowup, zo_channel *chan,
+                    uint32_t *codewords, uint8_t *roots) {
+    const size_t n = (size_t)1 << trace_log, N = (size_t)1 << (trace_log + log_blowup);
+    zo_qm31 w = zo_qm31_root_of_unity(trace_log), w_ev = zo_qm31_root_of_unity(trace_log + log_blowup);
+    uint32_t *h = (uint32_t *)malloc(n * 16), *d = (uint32_t *)malloc(N * 16), *co = (uint32_t *)malloc(n * 16);
+    uint32_t *cw = (uint32_t *)malloc(N * 16);
+    uint8_t *leaves = (uint8_t *)malloc(N * 32), *nodes = (uint8_t *)malloc((2 * N - 1) * 32);
+    if (!h || !d || !co || !cw || !leaves || !nodes) {
+        free(h); free(d); free(co); free(cw); free(leaves); free(nodes);
+        return -3;
+    }
+    zo_qm31 x = Q_ONE;
+    for (size_t i = 0; i < n; i++) { st4(h + 4 * i, x); x = zo_qm31_mul(x, w); }
+    x = ld4(ZO_FRI_OFFSET);
+    for (size_t i = 0; i < N; i++) { st4(d + 4 * i, x); x = zo_qm31_mul(x, w_ev); }
+    int rc = 0;
+    for (size_t c = 0; c < n_cols && !rc; c++) {
+        rc = zo_poly_interpolate(h, cols + 4 * n * c, n, co);
+        if (rc) break;
+        for (size_t i = 0; i < N; i++) {
+            zo_poly_eval(co, n, d + 4 * i, cw + 4 * i);
+            zo_hash_qm31(cw + 4 * i, leaves + 32 * i);
+        }
+        if (codewords) memcpy(codewords + 4 * N * c, cw, N * 16);
+        zo_merkle_build(leaves, N, nodes);
+        memcpy(roots + 32 * c, nodes + (2 * N - 2) * 32, 32);
+        if (chan) zo_channel_absorb(chan, roots + 32 * c, 32);
+    }
+    free(h); free(d); free(co); free(cw); free(leaves); free(nodes);
+    return rc;
+}
+
+/* FibAir.generateTrace (stark.zig:918-935) with seeds a0, a1 instead of 0, 1:
+ * column 0 holds a_i, column 1 holds a_{i+1}, a_{i+2} = a_i + a_{i+1}; values
+ * are QM31 with the M31 seed in the first coordinate. */
+void zo_fib_trace(uint32_t a0, uint32_t a1, size_t n, uint32_t *cols) {
+    uint32_t a = a0, b = a1;
+    for (size_t i = 0; i < n; i++) {
+        uint32_t *p = cols + 4 * i, *q = cols + 4 * (n + i);
+        p[0] = a; p[1] = p[2] = p[3] = 0;
+        q[0] = b; q[1] = q[2] = q[3] = 0;
+        uint32_t nb = zo_m31_add(a, b);
+        a = b;
+        b = nb;
+    }
+}

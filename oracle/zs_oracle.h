@@ -114,6 +114,15 @@ size_t zo_channel_transcript_len(const zo_channel *c);
 void zo_poly_eval(const uint32_t *coeffs, size_t n, const uint32_t *x, uint32_t *out);
 int zo_poly_interpolate(const uint32_t *xs, const uint32_t *ys, size_t n, uint32_t *coeffs);
 
+/* ---- GenericStark trace commitment (src/m31/stark.zig:343-414) ------------- */
+/* cols: n_cols columns of 2^trace_log QM31 values; codewords (may be NULL): out,
+ * n_cols * 2^(trace_log+log_blowup) QM31 values on D = FRI_OFFSET * <w_ev>;
+ * roots: out, n_cols digests, each absorbed into chan (if not NULL) in order. */
+int zo_trace_commit(const uint32_t *cols, size_t n_cols, uint32_t trace_log, uint32_t log_blowup, zo_channel *chan,
+                    uint32_t *codewords, uint8_t *roots);
+/* FibAir trace with seeds: cols = [a_0..a_{n-1}], [a_1..a_n] as QM31 (M31 in the first word) */
+void zo_fib_trace(uint32_t a0, uint32_t a1, size_t n, uint32_t *cols);
+
 /* ---- FRI commit phase (src/m31/fri.zig) ------------------------------------ */
 extern const uint32_t ZO_FRI_OFFSET[4];
 void zo_fri_domain_point(uint32_t dlog, uint32_t layer, size_t j, uint32_t out[4]);

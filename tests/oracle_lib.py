@@ -100,6 +100,8 @@ def _declare(L):
     L.zo_fri_verify.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, sz, ctypes.c_void_p,
                                 u8p, u32p, ctypes.POINTER(sz), u32p, u8p]
     L.zo_fill_m31.argtypes = [ctypes.c_uint64, u32p, sz]
+    L.zo_trace_commit.argtypes = [u32p, sz, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p, u32p, ctypes.c_char_p]
+    L.zo_fib_trace.argtypes = [ctypes.c_uint32, ctypes.c_uint32, sz, u32p]
 
 
 def _p(a):
@@ -459,3 +461,25 @@ def commit_job(cols, log_blowup, remainder_log):
     g = qm31_combine(lde_, gamma)
     roots, _, rem, _ = fri_commit(g, log_blowup, remainder_log, channel=ch, want_layers=False)
     return root, roots, rem, lde_, nodes
+
+
+def trace_commit(cols, log_blowup, channel=None, want_codewords=True):
+    """GenericStark trace commitment (stark.zig:343-414): cols (n_cols, n, 4) QM31.
+    Returns (roots list, codewords (n_cols, N, 4) or None)."""
+    cols = u32(cols)
+    n_cols, n = cols.shape[0], cols.shape[1]
+    tl = n.bit_length() - 1
+    N = n << log_blowup
+    cw = np.zeros((n_cols, N, 4), np.uint32) if want_codewords else None
+    roots = ctypes.create_string_buffer(32 * n_cols)
+    rc = lib().zo_trace_commit(_p(cols.reshape(-1)), n_cols, tl, log_blowup,
+                               channel.h if channel is not None else None,
+                               _p(cw) if cw is not None else None, roots)
+    assert rc == 0, rc
+    return [roots.raw[32 * c:32 * (c + 1)] for c in range(n_cols)], cw
+
+
+def fib_trace(a0, a1, n):
+    out = np.zeros((2, n, 4), np.uint32)
+    lib().zo_fib_trace(a0, a1, n, _p(out))
+    return out

@@ -113,6 +113,8 @@ def _declare(L):
     L.zs_fill_m31_dev.argtypes = [vp, ctypes.c_uint64, vp, sz]
     L.zs_commit_job_dev.argtypes = [vp, vp, u32, u32, u32, u32, vp, vp, ctypes.c_char_p, ctypes.c_char_p, u32p, u32]
     L.zs_commit_batch_dev.argtypes = [vp, ctypes.c_void_p, sz, u32, u32, u32, u32]
+    L.zs_qm31_trace_commit_dev.argtypes = [vp, vp, u32, u32, u32, vp, vp, ctypes.c_char_p, vp]
+    L.zs_fib_trace.argtypes = [u32, u32, u32, u32p]
     L.zs_server_new.argtypes = [i32, u32, ctypes.c_uint64]
     L.zs_server_new.restype = vp
     L.zs_server_submit.argtypes = [vp, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]
@@ -130,7 +132,7 @@ def _declare(L):
                  "zs_merkle_open", "zs_tree_read_level", "zs_fri_commit_dev", "zs_fri_commit", "zs_fri_query",
                  "zs_fri_read_layer", "zs_fill_m31_dev", "zs_hip_profile", "zs_hip_profile_read",
                  "zs_commit_job_dev", "zs_commit_batch_dev", "zs_server_submit", "zs_server_wait",
-                 "zs_server_memory", "zs_server_poll"]:
+                 "zs_server_memory", "zs_server_poll", "zs_qm31_trace_commit_dev", "zs_fib_trace"]:
         getattr(L, name).restype = i32
 
 
@@ -284,6 +286,15 @@ class Response(ctypes.Structure):
                 np.array(self.remainder[:4 * rc], np.uint32).reshape(rc, 4))
 
 
+def fib_trace(a0, a1, log_n):
+    """Seeded Fibonacci AIR trace: (2, 2^log_n, 4) QM31 columns (stark.zig:918-935)."""
+    out = np.zeros((2, 1 << log_n, 4), np.uint32)
+    rc = lib().zs_fib_trace(a0, a1, log_n, _p(out))
+    if rc:
+        raise ZsError(rc, "zs_fib_trace")
+    return out
+
+
 class Server:
     """Continuous-batching request loop with HBM-budget admission (zs_server_*)."""
 
@@ -427,6 +438,14 @@ class Context:
 
     def fill_m31_dev(self, seed, d_out, n):
         self.check(lib().zs_fill_m31_dev(self.h, seed, d_out, n))
+
+    # ---- GenericStark trace commitment (stark.zig:343-414) ----
+    def trace_commit_dev(self, d_cols, n_cols, trace_log, log_blowup, d_codewords, d_nodes, channel=None):
+        """Returns the n_cols column roots (absorbed into `channel` in order if given)."""
+        roots = ctypes.create_string_buffer(32 * n_cols)
+        self.check(lib().zs_qm31_trace_commit_dev(self.h, d_cols, n_cols, trace_log, log_blowup, d_codewords,
+                                                  d_nodes, roots, channel.h if channel is not None else None))
+        return [roots.raw[32 * c:32 * (c + 1)] for c in range(n_cols)]
 
     # ---- one trace's commitment, and batches of uneven traces ----
     def commit_job_dev(self, d_cols, n_cols, log_n, log_blowup, remainder_log, d_lde, d_nodes, flags=0):

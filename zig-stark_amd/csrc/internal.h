@@ -82,6 +82,8 @@ struct DeviceCaches {
     std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t, uint32_t>, std::unique_ptr<TwiddleTable>> twiddles;
     // FRI inverse-domain tables keyed by dlog: [lo table 2^10 | hi table] QM31 words
     std::map<uint32_t, std::unique_ptr<DevBuf>> fri_tables;
+    // QM31 power tables of the trace-commitment DFTs, keyed by (kind << 32) | log length
+    std::map<uint64_t, std::unique_ptr<DevBuf>> qtables;
 };
 
 struct Ctx {
