@@ -132,6 +132,58 @@ def full_commit_leg(ctx, d_cols, n_cols, lg, beta, d_lde, d_nodes):
             "trace_root": root.hex(), "fri_root0": roots[0].hex()}
 
 
+def stark_trace_leg(Z, ctx, torch, dev, with_cpu):
+    """GenericStark trace commitment of the seeded Fibonacci AIR: 2 QM31 columns x 2^12,
+    blowup 4 (codewords of 2^14, one hashQM31 tree per column, roots into the channel)."""
+    import numpy as np
+    tl, beta = 12, 2
+    a1 = int(fill_seed_value(1))
+    trace = Z.fib_trace(1, a1, tl)
+    n, N = 1 << tl, 1 << (tl + beta)
+    d_cols = torch.from_numpy(trace.reshape(-1).view(np.int32)).to(dev)
+    d_cw = torch.empty(2 * N * 4, dtype=torch.int32, device=dev)
+    d_nodes = torch.empty(2 * (2 * N - 1) * 8, dtype=torch.int32, device=dev)
+
+    def call():
+        ch = Z.Channel(b"zig-stark:fib")
+        return ctx.trace_commit_dev(d_cols.data_ptr(), 2, tl, beta, d_cw.data_ptr(), d_nodes.data_ptr(), ch)
+
+    roots = call()
+    reps = 20
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        call()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    out = {"workload": f"seeded Fibonacci AIR trace, 2 QM31 columns x 2^{tl} (a0 = 1, a1 = seed-1 M31), "
+                       f"interpolation over H, evaluation on D (2^{tl + beta}), hashQM31 column trees",
+           "ms": round(ms, 3), "roots": [r.hex() for r in roots]}
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+        t0 = time.perf_counter()
+        oroots, _ = O.trace_commit(trace, beta, want_codewords=False)
+        cpu_s = time.perf_counter() - t0
+        out["cpu_oracle_s"] = round(cpu_s, 2)
+        out["cpu_kind"] = "port (O(n^2) Lagrange + O(nN) Horner as stark.zig:343-414, 1 thread)"
+        out["bit_exact_vs_cpu"] = oroots == roots
+        out["speedup_vs_cpu"] = round(cpu_s * 1e3 / ms, 1)
+    return out
+
+
+def fill_seed_value(seed):
+    """First value of the seeded uniform M31 stream (the library's zs_fill_m31_dev)."""
+    import torch
+    import __graft_entry__
+    Z = __graft_entry__.load_package()
+    t = torch.empty(1, dtype=torch.int32, device="cuda:0")
+    c = Z.Context(0)
+    c.fill_m31_dev(seed, t.data_ptr(), 1)
+    c.synchronize()
+    v = int(t.cpu().numpy().view("uint32")[0])
+    c.close()
+    return v
+
+
 def batch_leg(Z, torch, dev, stream, n_traces):
     """Uneven batch: per-trace buffers resident in HBM, seeded inputs (seed 4, stream k)."""
     import numpy as np
@@ -336,6 +388,7 @@ def main():
 
     fri = None if args.no_fri else fri_leg(Z, ctx, torch, dev)
     full = None if args.no_fri else full_commit_leg(ctx, d_cols, n_cols, lg, beta, d_lde, d_nodes)
+    stark = stark_trace_leg(Z, ctx, torch, dev, not args.no_cpu)
     batch = batch_leg(Z, torch, dev, stream, args.batch_traces) if args.batch_traces > 0 else None
     serve = serve_leg(Z, args.serve_requests) if args.serve_requests > 0 else None
 
@@ -372,6 +425,7 @@ def main():
                   "bit_exact_root": bit_exact},
         "fri": fri,
         "full_commit": full,
+        "stark_trace": stark,
         "batch": batch,
         "serve": serve,
         "root": root_fused.hex(),
