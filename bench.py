@@ -184,7 +184,7 @@ def fill_seed_value(seed):
     return v
 
 
-def batch_leg(Z, torch, dev, stream, n_traces):
+def batch_leg(Z, torch, dev, stream, n_traces, autotune):
     """Uneven batch: per-trace buffers resident in HBM, seeded inputs (seed 4, stream k)."""
     import numpy as np
     rng = np.random.default_rng(4)
@@ -210,6 +210,14 @@ def batch_leg(Z, torch, dev, stream, n_traces):
         out = bctx.commit_batch_dev(specs, 2, 5, n_workers=workers, flags=flags)
         return (time.perf_counter() - t0) * 1e3, out
 
+    tuned, t_tune = 0, 0.0
+    if autotune:
+        # one launch configuration per shape (log rows, columns rounded up to a power of two)
+        t0 = time.perf_counter()
+        shapes = sorted({(int(lg), 1 << (int(c) - 1).bit_length()) for c, lg in zip(cols, lgs)})
+        for lg, cb in shapes:
+            bctx.autotune_lde(cb, lg, 2)
+        tuned, t_tune = len(shapes), time.perf_counter() - t0
     run(4)  # warm: twiddles, FRI tables, worker scratch
     tune = {}
     for w in (2, 4, 8):
@@ -230,7 +238,8 @@ def batch_leg(Z, torch, dev, stream, n_traces):
             "elements": elems, "ms": round(best_ms, 2), "Melem_per_s": round(elems / (best_ms * 1e-3) / 1e6, 1),
             "workers": best_w, "workers_ms": tune,
             "one_at_a_time_ms": round(seq_ms, 2), "plain_loop_ms": round(plain_ms, 2),
-            "speedup_vs_plain_loop": round(plain_ms / best_ms, 2), "bit_exact_all_paths": same}
+            "speedup_vs_plain_loop": round(plain_ms / best_ms, 2), "bit_exact_all_paths": same,
+            "autotuned_shapes": tuned, "autotune_s": round(t_tune, 2)}
 
 
 def serve_leg(Z, n_requests, n_workers=4, budget=200 * 10**9):
@@ -272,11 +281,12 @@ def serve_leg(Z, n_requests, n_workers=4, budget=200 * 10**9):
 
 
 def cpu_baseline(n_cols, log_n, log_blowup):
-    """The oracle (C restatement of the reference algorithm, one thread) on a bounded
-    sample: two full columns' LDE and the row leaves + tree of 2^13 rows x n_cols,
-    extrapolated to the whole job."""
+    """The oracle (C restatement of the reference algorithm) on a bounded sample:
+    two full columns' LDE and the row leaves + tree of 2^13 rows x n_cols,
+    extrapolated to the whole job, on one thread (the reference's M31 stack is
+    single-threaded); then the same columns spread over the host's cores."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import numpy as np
+    from concurrent.futures import ThreadPoolExecutor
     import oracle_lib as O
     n, N = 1 << log_n, 1 << (log_n + log_blowup)
     cols = O.fill_m31(SEED, 2 * n).reshape(2, n)
@@ -291,6 +301,14 @@ def cpu_baseline(n_cols, log_n, log_blowup):
     t_rows = time.perf_counter() - t0
     total = n_cols * t_col + (N / rows) * t_rows
     elems = n_cols * n
+    # column-parallel: one wave of `threads` full columns at once (ctypes releases the GIL)
+    threads = max(1, min(os.cpu_count() or 1, 16))
+    par_cols = O.fill_m31(SEED + 2, threads * n).reshape(threads, n)
+    with ThreadPoolExecutor(threads) as ex:
+        t0 = time.perf_counter()
+        list(ex.map(lambda c: O.lde(c, log_blowup), par_cols))
+        t_wave = time.perf_counter() - t0
+    total_par = -(-n_cols // threads) * t_wave + (N / rows) * t_rows / threads
     return {
         "value": round(elems / total / 1e6, 4),
         "unit": "Melem/s",
@@ -299,6 +317,9 @@ def cpu_baseline(n_cols, log_n, log_blowup):
         "sample": f"oracle (C, -O3, 1 thread): LDE of 2 of {n_cols} columns at full size "
                   f"({t_col:.2f} s/column) + row leaves and tree of {rows} rows x {n_cols} columns "
                   f"({t_rows:.3f} s), extrapolated to {n_cols} columns and {N} rows = {total:.1f} s per job",
+        "parallel": {"value": round(elems / total_par / 1e6, 4), "unit": "Melem/s", "cores": threads,
+                     "sample": f"{threads} full columns' LDE at once on {threads} threads "
+                               f"({t_wave:.2f} s), extrapolated to {n_cols} columns = {total_par:.1f} s per job"},
     }
 
 
@@ -310,6 +331,7 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fri", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch configuration")
     ap.add_argument("--batch-traces", type=int, default=256, help="uneven-batch leg size (0 = skip)")
     ap.add_argument("--serve-requests", type=int, default=200, help="serving leg size (0 = skip)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -339,6 +361,13 @@ def main():
 
     def step(flags=0):
         ctx.lde_commit_dev(d_cols.data_ptr(), n_cols, lg, beta, d_lde.data_ptr(), d_nodes.data_ptr(), flags)
+
+    tune = None
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        el, cpw, us = ctx.autotune_lde(n_cols, lg, beta)
+        tune = {"tile_elems_log": el, "cols_per_wg": cpw, "lde_us": us,
+                "seconds": round(time.perf_counter() - t0, 2)}
 
     def timed(fn, reps):
         torch.cuda.synchronize()
@@ -389,7 +418,8 @@ def main():
     fri = None if args.no_fri else fri_leg(Z, ctx, torch, dev)
     full = None if args.no_fri else full_commit_leg(ctx, d_cols, n_cols, lg, beta, d_lde, d_nodes)
     stark = stark_trace_leg(Z, ctx, torch, dev, not args.no_cpu)
-    batch = batch_leg(Z, torch, dev, stream, args.batch_traces) if args.batch_traces > 0 else None
+    batch = (batch_leg(Z, torch, dev, stream, args.batch_traces, not args.no_autotune)
+             if args.batch_traces > 0 else None)
     serve = serve_leg(Z, args.serve_requests) if args.serve_requests > 0 else None
 
     elems = n_cols * n
@@ -419,6 +449,7 @@ def main():
         "step_hbm": {"algorithmic_bytes": alg, "GBps": round(alg / (ms * 1e-3) / 1e9, 1),
                      "frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "stages_ms": {"lde": round(lde_ms, 4), "commit": round(commit_ms, 4)},
+        "autotune": tune,
         "kernels_ms_per_step": {k: round(v[0] / 3, 4) for k, v in sorted(kern.items(), key=lambda kv: -kv[1][0])},
         "plain": {"ms_per_step": round(plain_ms, 3) if plain_ms else None,
                   "speedup_fused_vs_plain": round(plain_ms / ms, 2) if plain_ms else None,

@@ -117,6 +117,15 @@ int32_t zs_m31_lde_commit_dev(zs_hip_ctx *ctx, const uint32_t *d_cols, uint32_t 
 /* The LDE alone (no commitment); same layouts as zs_m31_lde_commit_dev. */
 int32_t zs_m31_lde_dev(zs_hip_ctx *ctx, const uint32_t *d_cols, uint32_t n_cols, uint32_t log_n, uint32_t log_blowup,
                        uint32_t *d_lde);
+/* Launch-configuration autotuner (no reference counterpart; the reference has
+ * no GPU LDE).  Times zs_m31_lde_dev for one shape under each candidate
+ * configuration of the register-group passes (2^12..2^14 elements per
+ * workgroup x 1, 2, 4, 8 columns per workgroup) on synthetic device data, checks
+ * that every candidate yields the same row-tree root, and keeps the fastest for
+ * that shape (log_n, log_blowup, n_cols rounded up to a power of two): later
+ * LDE calls of the shape on this context and its batch workers use it.
+ * out (may be NULL): tile elements log2, columns per workgroup, microseconds. */
+int32_t zs_autotune_lde(zs_hip_ctx *ctx, uint32_t n_cols, uint32_t log_n, uint32_t log_blowup, uint32_t out[3]);
 /* Row leaves alone: d_leaves[i] = hashM31s(row i), 2^log_rows digests (level 0 only). */
 int32_t zs_m31_row_leaves_dev(zs_hip_ctx *ctx, const uint32_t *d_cols, uint32_t n_cols, uint32_t log_rows,
                               size_t col_stride, uint8_t *d_leaves);

@@ -125,3 +125,21 @@ def test_lde_commit_register_group_path(zctx, n_cols, lg, beta):
     assert b"".join(got[32 * i: 32 * i + 32] for i in range(0, N, step)) == leaves
     if N <= 1 << 17:
         assert root == O.tree_root(O.row_commit(ref_lde))
+
+
+@pytest.mark.parametrize("n_cols,lg,beta", [(8, 10, 2), (24, 13, 2)])
+def test_autotuned_lde_is_bit_exact(zctx, torch_dev, n_cols, lg, beta):
+    """The autotuner keeps a configuration per shape; LDE and root stay equal to the oracle's."""
+    import torch
+    cols = O.fill_m31(lg * 31 + n_cols, n_cols << lg).reshape(n_cols, -1)
+    el, cpw, us = zctx.autotune_lde(n_cols, lg, beta)
+    assert 12 <= el <= 14 and cpw in (1, 2, 4, 8) and us > 0
+    d_cols = to_dev(cols.reshape(-1), torch_dev)
+    N = 1 << (lg + beta)
+    d_lde = torch.empty(n_cols * N, dtype=torch.int32, device=torch_dev)
+    d_nodes = torch.empty((2 * N - 1) * 8, dtype=torch.int32, device=torch_dev)
+    zctx.lde_commit_dev(d_cols.data_ptr(), n_cols, lg, beta, d_lde.data_ptr(), d_nodes.data_ptr())
+    lde = from_dev(d_lde).reshape(n_cols, N)
+    for c in (0, n_cols - 1):
+        assert np.array_equal(lde[c], O.lde(cols[c], beta))
+    assert from_dev(d_nodes)[-8:].view(np.uint32).tobytes() == O.tree_root(O.row_commit(lde))

@@ -113,6 +113,7 @@ def _declare(L):
     L.zs_fill_m31_dev.argtypes = [vp, ctypes.c_uint64, vp, sz]
     L.zs_commit_job_dev.argtypes = [vp, vp, u32, u32, u32, u32, vp, vp, ctypes.c_char_p, ctypes.c_char_p, u32p, u32]
     L.zs_commit_batch_dev.argtypes = [vp, ctypes.c_void_p, sz, u32, u32, u32, u32]
+    L.zs_autotune_lde.argtypes = [vp, u32, u32, u32, u32p]
     L.zs_qm31_trace_commit_dev.argtypes = [vp, vp, u32, u32, u32, vp, vp, ctypes.c_char_p, vp]
     L.zs_fib_trace.argtypes = [u32, u32, u32, u32p]
     L.zs_server_new.argtypes = [i32, u32, ctypes.c_uint64]
@@ -132,7 +133,7 @@ def _declare(L):
                  "zs_merkle_open", "zs_tree_read_level", "zs_fri_commit_dev", "zs_fri_commit", "zs_fri_query",
                  "zs_fri_read_layer", "zs_fill_m31_dev", "zs_hip_profile", "zs_hip_profile_read",
                  "zs_commit_job_dev", "zs_commit_batch_dev", "zs_server_submit", "zs_server_wait",
-                 "zs_server_memory", "zs_server_poll", "zs_qm31_trace_commit_dev", "zs_fib_trace"]:
+                 "zs_server_memory", "zs_server_poll", "zs_qm31_trace_commit_dev", "zs_fib_trace", "zs_autotune_lde"]:
         getattr(L, name).restype = i32
 
 
@@ -438,6 +439,13 @@ class Context:
 
     def fill_m31_dev(self, seed, d_out, n):
         self.check(lib().zs_fill_m31_dev(self.h, seed, d_out, n))
+
+    def autotune_lde(self, n_cols, log_n, log_blowup):
+        """Pick the fastest launch configuration for this LDE shape (kept on the
+        context); returns (tile elements log2, columns per workgroup, microseconds)."""
+        out = np.zeros(3, np.uint32)
+        self.check(lib().zs_autotune_lde(self.h, n_cols, log_n, log_blowup, _p(out)))
+        return tuple(int(x) for x in out)
 
     # ---- GenericStark trace commitment (stark.zig:343-414) ----
     def trace_commit_dev(self, d_cols, n_cols, trace_log, log_blowup, d_codewords, d_nodes, channel=None):

@@ -271,6 +271,13 @@ int32_t zs_m31_lde_dev(zs_hip_ctx *h, const uint32_t *d_cols, uint32_t n_cols, u
                        uint32_t *d_lde) {
     if (!h || !d_cols || !d_lde || n_cols == 0 || log_n < 1 || log_n + log_blowup > 28) return ZS_ERR_INPUT;
     Ctx *ctx = &h->c;
+    {
+        // the launch configuration chosen for this shape, if it was tuned
+        DeviceCaches &dc = *ctx->caches;
+        std::lock_guard<std::mutex> lk(dc.mu);
+        auto it = dc.tuned.find(lde_shape_key(n_cols, log_n, log_blowup));
+        ctx->tile = it != dc.tuned.end() ? it->second : default_tile_config();
+    }
     size_t n = size_t(1) << log_n, N = size_t(1) << (log_n + log_blowup);
     size_t coef_words;
     size_t big_words = lde_scratch_words(n_cols, log_n, log_blowup, &coef_words);
@@ -278,6 +285,91 @@ int32_t zs_m31_lde_dev(zs_hip_ctx *h, const uint32_t *d_cols, uint32_t n_cols, u
     if (big_words) ZS_TRY(ctx->ensure_scratch(2, big_words * 4));
     return launch_lde(ctx, d_cols, n, n_cols, log_n, log_blowup, d_lde, N, ctx->scratch[1].as<uint32_t>(),
                       big_words ? ctx->scratch[2].as<uint32_t>() : nullptr);
+}
+
+int32_t zs_autotune_lde(zs_hip_ctx *h, uint32_t n_cols, uint32_t log_n, uint32_t log_blowup, uint32_t out[3]) {
+    if (!h || n_cols == 0 || log_n < 1 || log_n + log_blowup > 28) return ZS_ERR_INPUT;
+    Ctx *ctx = &h->c;
+    ZS_HIP(ctx, hipSetDevice(ctx->device));
+    const uint32_t LT = log_n + log_blowup;
+    const size_t n = size_t(1) << log_n, N = size_t(1) << LT;
+    DevBuf cols, lde, nodes;
+    ZS_TRY(dev_alloc(ctx, cols, n * n_cols * 4));
+    ZS_TRY(dev_alloc(ctx, lde, N * n_cols * 4));
+    ZS_TRY(dev_alloc(ctx, nodes, tree_words(LT) * 4));
+    ZS_TRY(launch_fill_m31(ctx, 0x5eed, cols.as<uint32_t>(), n * n_cols));
+    const uint64_t key = lde_shape_key(n_cols, log_n, log_blowup);
+    hipEvent_t a, b;
+    ZS_HIP(ctx, hipEventCreate(&a));
+    ZS_HIP(ctx, hipEventCreate(&b));
+    TileConfig best = default_tile_config();
+    float best_ms = 0;
+    uint8_t ref_root[32];
+    bool have_ref = false;
+    int32_t rc = ZS_OK;
+    for (uint32_t el = 12; el <= 14 && rc == ZS_OK; el++) {
+        for (uint32_t cpw = 1; cpw <= 8 && rc == ZS_OK; cpw *= 2) {
+            TileConfig cand;
+            cand.elems_log = el;
+            cand.cols_per_wg = cpw;
+            {
+                std::lock_guard<std::mutex> lk(ctx->caches->mu);
+                ctx->caches->tuned[key] = cand;
+            }
+            // one run for the root (every candidate must give the same one), then the timed runs
+            rc = zs_m31_lde_commit_dev(h, cols.as<uint32_t>(), n_cols, log_n, log_blowup, lde.as<uint32_t>(),
+                                       nodes.as<uint8_t>(), 0);
+            if (rc == ZS_ERR_INPUT) {  // a configuration this shape cannot launch (LDS): not a candidate
+                rc = ZS_OK;
+                continue;
+            }
+            if (rc) break;
+            uint8_t root[32];
+            if (hipMemcpyAsync(root, nodes.as<uint8_t>() + (tree_words(LT) - 8) * 4, 32, hipMemcpyDeviceToHost,
+                               ctx->stream) != hipSuccess ||
+                hipStreamSynchronize(ctx->stream) != hipSuccess) {
+                rc = ZS_ERR_DEVICE;
+                break;
+            }
+            if (!have_ref) {
+                std::memcpy(ref_root, root, 32);
+                have_ref = true;
+            } else if (std::memcmp(ref_root, root, 32) != 0) {
+                ctx->set_error("autotune: launch configurations disagree on the LDE root");
+                rc = ZS_ERR_GENERIC;
+                break;
+            }
+            const int reps = 3;
+            (void)hipEventRecord(a, ctx->stream);
+            for (int r = 0; r < reps && rc == ZS_OK; r++)
+                rc = zs_m31_lde_dev(h, cols.as<uint32_t>(), n_cols, log_n, log_blowup, lde.as<uint32_t>());
+            (void)hipEventRecord(b, ctx->stream);
+            if (rc || hipEventSynchronize(b) != hipSuccess) {
+                rc = rc ? rc : ZS_ERR_DEVICE;
+                break;
+            }
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, a, b);
+            ms /= reps;
+            if (!best_ms || ms < best_ms) {
+                best_ms = ms;
+                best = cand;
+            }
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    {
+        std::lock_guard<std::mutex> lk(ctx->caches->mu);
+        if (rc == ZS_OK) ctx->caches->tuned[key] = best;
+        else ctx->caches->tuned.erase(key);
+    }
+    if (rc == ZS_OK && out) {
+        out[0] = best.elems_log;
+        out[1] = best.cols_per_wg;
+        out[2] = (uint32_t)(best_ms * 1000.0f);
+    }
+    return rc;
 }
 
 int32_t zs_m31_row_leaves_dev(zs_hip_ctx *h, const uint32_t *d_cols, uint32_t n_cols, uint32_t log_rows,

@@ -458,7 +458,7 @@ static int32_t launch_lde_tiled(Ctx *ctx, const uint32_t *d_in, uint64_t in_stri
         p.tw = tf->fwd.as<uint32_t>();
         p.L = log_n; p.LT = LT; p.beta = log_blowup;
         p.s_lo = pf[i].s_lo; p.C = pf[i].C;
-        p.Q_log = last ? std::min<uint32_t>(log_blowup, PASS_ELEMS_LOG > p.C ? PASS_ELEMS_LOG - p.C : 0) : 0;
+        p.Q_log = last ? std::min<uint32_t>(log_blowup, ctx->tile.elems_log > p.C ? ctx->tile.elems_log - p.C : 0) : 0;
         p.in_mode = IN_KSPACE;
         p.out_mode = last ? OUT_NATURAL : OUT_KSPACE;
         p.natural_tiles = last ? 1 : 0;
@@ -535,7 +535,7 @@ int32_t launch_lde(Ctx *ctx, const uint32_t *d_in, uint64_t in_stride, uint32_t 
         p.s_lo = plan[i].s_lo; p.C = plan[i].C;
         // the last pass keeps 2^Q_log of the 2^beta replicas in one workgroup so its natural-order
         // stores are contiguous; capped so the LDS tile stays at 2^13 elements
-        p.Q_log = last ? std::min<uint32_t>(log_blowup, PASS_ELEMS_LOG > p.C ? PASS_ELEMS_LOG - p.C : 0) : 0;
+        p.Q_log = last ? std::min<uint32_t>(log_blowup, ctx->tile.elems_log > p.C ? ctx->tile.elems_log - p.C : 0) : 0;
         p.in_mode = first ? IN_SHARED : IN_KSPACE;
         p.out_mode = last ? OUT_NATURAL : OUT_KSPACE;
         p.natural_tiles = last ? 1 : 0;

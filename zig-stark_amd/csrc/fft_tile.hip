@@ -386,7 +386,7 @@ static size_t tile_lds_bytes(uint32_t ntiles, uint32_t nsets) {
     return (size_t(ntiles) * TC<C>::TS + (size_t(nsets) << C)) * 4;
 }
 
-// Launch-shape knobs (read once from the environment; defaults from measurement):
+// Launch-shape defaults, read once from the environment (defaults from measurement):
 //   ZS_TILE_ELEMS_LOG: elements of the tiles one workgroup transforms (default 2^13)
 //   ZS_COLS_PER_WG:    columns a workgroup runs with one set of staged twiddles (default 4)
 static int env_int(const char *name, int dflt, int lo, int hi) {
@@ -394,31 +394,32 @@ static int env_int(const char *name, int dflt, int lo, int hi) {
     int x = e ? atoi(e) : dflt;
     return std::min(hi, std::max(lo, x));
 }
-static uint32_t tile_elems_log() {
-    static uint32_t v = (uint32_t)env_int("ZS_TILE_ELEMS_LOG", 13, 11, 15);
-    return v;
-}
-static uint32_t cols_per_wg() {
-    static uint32_t v = (uint32_t)env_int("ZS_COLS_PER_WG", 4, 1, 64);
-    return v;
+TileConfig default_tile_config() {
+    static const TileConfig c = [] {
+        TileConfig t;
+        t.elems_log = (uint32_t)env_int("ZS_TILE_ELEMS_LOG", 13, 11, 15);
+        t.cols_per_wg = (uint32_t)env_int("ZS_COLS_PER_WG", 4, 1, 64);
+        return t;
+    }();
+    return c;
 }
 // columns per workgroup, reduced so the grid still has >= 2048 workgroups
-static uint32_t pick_ncb(uint32_t n_cols, uint32_t wgs_per_col) {
-    uint32_t ncb = std::min(cols_per_wg(), std::max<uint32_t>(n_cols, 1));
+static uint32_t pick_ncb(const Ctx *ctx, uint32_t n_cols, uint32_t wgs_per_col) {
+    uint32_t ncb = std::min(ctx->tile.cols_per_wg, std::max<uint32_t>(n_cols, 1));
     while (ncb > 1 && uint64_t(wgs_per_col) * ((n_cols + ncb - 1) / ncb) < 2048) ncb >>= 1;
     return ncb;
 }
 
 int32_t run_tile_pass(Ctx *ctx, FftPass p, bool inverse, uint32_t n_cols) {
     uint32_t avail = p.s_lo > 0 ? p.s_lo : (p.L - p.C);
-    int32_t want = (int32_t)tile_elems_log() - (int32_t)p.C - (int32_t)p.Q_log;
+    int32_t want = (int32_t)ctx->tile.elems_log - (int32_t)p.C - (int32_t)p.Q_log;
     p.G_log = (uint32_t)std::max<int32_t>(0, std::min<int32_t>((int32_t)avail, want));
     uint32_t tiles = 1u << (p.L - p.C);
     uint32_t ntile = 1u << (p.G_log + p.Q_log);
     uint32_t nsets = p.s_lo == 0 ? ntile : (1u << p.Q_log);
     uint32_t gx = tiles >> p.G_log, gz = (1u << p.beta) >> p.Q_log;
     p.n_cols = n_cols;
-    p.ncb = pick_ncb(n_cols, gx * gz);
+    p.ncb = pick_ncb(ctx, n_cols, gx * gz);
     dim3 grid(gx, (n_cols + p.ncb - 1) / p.ncb, gz);
     uint32_t maxw = p.C >= 11 ? 8 : 16;
     dim3 block(64 * std::min<uint32_t>(ntile, maxw));
@@ -595,13 +596,13 @@ int32_t run_lde_fused(Ctx *ctx, const uint32_t *d_in, uint64_t in_cs, uint32_t *
         p.out_mode = OUT_NATURAL;
         p.natural_tiles = 1;
         p.n_cols = n_cols;
-        p.ncb = pick_ncb(n_cols, 1);
+        p.ncb = pick_ncb(ctx, n_cols, 1);
         grid = dim3(1, (n_cols + p.ncb - 1) / p.ncb, 1);
         block = dim3(64 * std::min<uint32_t>(1u << beta, 4));
         lds = ((size_t(1) << beta) + 1) * TC_TS(C) + (((size_t(1) << beta) + 1) << C);
     } else {
         // one wave per tile, at most 8 (C = 11) or 16 tiles per workgroup
-        int32_t want = std::min<int32_t>((int32_t)tile_elems_log() - (int32_t)C, C >= 11 ? 3 : 4);
+        int32_t want = std::min<int32_t>((int32_t)ctx->tile.elems_log - (int32_t)C, C >= 11 ? 3 : 4);
         p.G_log = (uint32_t)std::max<int32_t>(0, std::min<int32_t>((int32_t)s_lo, want));
         p.Q_log = 0;
         p.in_mode = IN_KSPACE;
@@ -609,7 +610,7 @@ int32_t run_lde_fused(Ctx *ctx, const uint32_t *d_in, uint64_t in_cs, uint32_t *
         p.natural_tiles = 0;
         uint32_t gx = (1u << (L - C)) >> p.G_log;
         p.n_cols = n_cols;
-        p.ncb = pick_ncb(n_cols, gx);
+        p.ncb = pick_ncb(ctx, n_cols, gx);
         grid = dim3(gx, (n_cols + p.ncb - 1) / p.ncb, 1);
         block = dim3(64u << p.G_log);
         lds = (size_t(1) << p.G_log) * TC_TS(C) + (((size_t(1) << beta) + 1) << C);

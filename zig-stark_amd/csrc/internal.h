@@ -75,6 +75,21 @@ struct ProfRec {
     hipEvent_t a, b;
 };
 
+// Launch configuration of the register-group FFT passes: elements of the tile
+// set one workgroup transforms (log2) and columns it runs with one set of staged
+// twiddles.  Defaults come from the environment (ZS_TILE_ELEMS_LOG,
+// ZS_COLS_PER_WG) or measurement; zs_autotune_lde picks one per LDE shape.
+struct TileConfig {
+    uint32_t elems_log = 13, cols_per_wg = 4;
+};
+TileConfig default_tile_config();
+// key of an LDE shape in the tuned table: columns rounded up to a power of two
+inline uint64_t lde_shape_key(uint32_t n_cols, uint32_t log_n, uint32_t log_blowup) {
+    uint32_t cl = 0;
+    while ((1u << cl) < n_cols) cl++;
+    return (uint64_t(log_n) << 16) | (uint64_t(log_blowup) << 8) | cl;
+}
+
 // Device-resident tables shared by the contexts of one device (a context and its
 // batch workers, or a server's workers): built once, read by every stream.
 struct DeviceCaches {
@@ -84,6 +99,8 @@ struct DeviceCaches {
     std::map<uint32_t, std::unique_ptr<DevBuf>> fri_tables;
     // QM31 power tables of the trace-commitment DFTs, keyed by (kind << 32) | log length
     std::map<uint64_t, std::unique_ptr<DevBuf>> qtables;
+    // launch configurations chosen by zs_autotune_lde, keyed by lde_shape_key
+    std::map<uint64_t, TileConfig> tuned;
 };
 
 struct Ctx {
@@ -96,6 +113,7 @@ struct Ctx {
     hipStream_t stream = nullptr;  // nullptr = the null stream
     std::string last_error;
     std::shared_ptr<DeviceCaches> caches = std::make_shared<DeviceCaches>();
+    TileConfig tile = default_tile_config();  // used by the next transform launches
     // grow-only scratch arenas
     DevBuf scratch[10];  // 0 transforms, 1-2 LDE, 3 plain path, 4-6 FRI, 7-9 commit jobs
     // FRI commit phases with challenges given up front, captured once per buffer set and
