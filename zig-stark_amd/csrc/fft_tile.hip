@@ -121,6 +121,25 @@ __device__ __forceinline__ void tile_run(uint32_t *tl, uint32_t lane, const uint
     if constexpr (GI + 1 < TC<C>::NG) tile_run<INV, C, GI + 1>(tl, lane, twl);
 }
 
+// count of i >= 0 with first + i * step < end
+__device__ __forceinline__ uint32_t strided_count(uint32_t first, uint32_t step, uint32_t end) {
+    return first < end ? (end - first + step - 1) / step : 0u;
+}
+
+// dst(i) <- src(i) for i < count, in batches: LOAD(i) reads global memory, STORE(i, v) writes LDS
+template <uint32_t B, typename LOAD, typename STORE>
+__device__ __forceinline__ void batched_copy(uint32_t count, LOAD load, STORE store) {
+    for (uint32_t i0 = 0; i0 < count; i0 += B) {
+        uint32_t v[B];
+#pragma unroll
+        for (uint32_t k = 0; k < B; k++)
+            if (i0 + k < count) v[k] = load(i0 + k);
+#pragma unroll
+        for (uint32_t k = 0; k < B; k++)
+            if (i0 + k < count) store(i0 + k, v[k]);
+    }
+}
+
 // Cooperatively copy the 2^C - 1 twiddles a tile set needs into LDS (heap
 // layout).  Stage s = s_lo + ls of the global table (log size LT) is read at
 // (Kbase >> (s+1)) + i: the tile's fixed bits select one contiguous segment
@@ -128,13 +147,14 @@ __device__ __forceinline__ void tile_run(uint32_t *tl, uint32_t lane, const uint
 template <int C>
 __device__ __forceinline__ void stage_twiddles(uint32_t *twl, const uint32_t *tw, uint32_t LT, uint32_t s_lo,
                                                uint32_t Kbase) {
-    for (uint32_t e = threadIdx.x; e + 1 < (1u << C); e += blockDim.x) {
-        uint32_t idx = e + 1;
-        uint32_t lvl = 31 - __builtin_clz(idx);
-        uint32_t ls = C - 1 - lvl;
-        uint32_t s = s_lo + ls;
-        twl[idx] = tw[(1u << (LT - 1 - s)) + (Kbase >> (s + 1)) + (idx - (1u << lvl))];
-    }
+    const uint32_t t0 = threadIdx.x + 1, step = blockDim.x;
+    batched_copy<4>(
+        strided_count(t0, step, 1u << C),
+        [&](uint32_t i) {
+            const uint32_t idx = t0 + i * step, lvl = 31 - __builtin_clz(idx), s = s_lo + (C - 1 - lvl);
+            return tw[(1u << (LT - 1 - s)) + (Kbase >> (s + 1)) + (idx - (1u << lvl))];
+        },
+        [&](uint32_t i, uint32_t v) { twl[t0 + i * step] = v; });
 }
 
 // workgroup -> tile group, remapped so consecutive tile groups share an XCD
@@ -251,8 +271,14 @@ __device__ __forceinline__ void coop_store_generic(const FftPass &p, const TileG
 
 // Cooperative copies with the hot layouts specialised: the per-thread part of
 // each address (its run position) is computed once, and k-space tiles are
-// streamed with t fastest so every wave access is one contiguous run.
-template <int C>
+// streamed with t fastest so every wave access is one contiguous run.  Each
+// thread moves its elements in batches of CP_BATCH: all of a batch's global
+// loads (or LDS reads) are issued before the first is waited for, so a column's
+// copy costs a few memory latencies instead of one per element.  The batch is
+// 8 values in the pass kernels and 4 in the fused LDE kernels, whose registers
+// are fuller (measured: larger batches spill there).
+
+template <int C, uint32_t B = 8>
 __device__ __forceinline__ void coop_load(const FftPass &p, const TileGeom &tg, uint32_t *lds, uint32_t col,
                                           uint32_t qblk, uint32_t nq_log, bool replicate_q) {
     constexpr uint32_t T = 1u << C, TS = TC<C>::TS;
@@ -260,45 +286,58 @@ __device__ __forceinline__ void coop_load(const FftPass &p, const TileGeom &tg, 
     const uint32_t *in_col = p.in + (uint64_t)col * p.in_cs;
     if (p.in_mode == IN_NATURAL && p.s_lo == 0 && p.natural_tiles && nq_log == 0 && G <= nt) {
         // element (g, t) sits at natural (bitrev_C(t) << (L - C)) | (tau0 + g): runs of G along g
-        const uint32_t g = tid & (G - 1), tstep = nt >> p.G_log;
+        const uint32_t g = tid & (G - 1), tstep = nt >> p.G_log, t0 = tid >> p.G_log;
         const uint32_t *src = in_col + tg.tau0 + g;
         uint32_t *dst = lds + g * TS;
-        for (uint32_t t = tid >> p.G_log; t < T; t += tstep) dst[lidx(t)] = src[(uint64_t)bitrev32(t, C) << (p.L - C)];
+        const uint32_t sh = p.L - C;
+        batched_copy<B>(
+            strided_count(t0, tstep, T),
+            [&](uint32_t i) { return src[(uint64_t)bitrev32(t0 + i * tstep, C) << sh]; },
+            [&](uint32_t i, uint32_t v) { dst[lidx(t0 + i * tstep)] = v; });
         return;
     }
     if ((p.in_mode == IN_KSPACE || p.in_mode == IN_SHARED) && p.s_lo > 0 && G <= nt) {
         // strided tiles: (q, hi, t, lo0 + g) -> runs of G along g
-        const uint32_t g = tid & (G - 1), tstep = nt >> p.G_log;
+        const uint32_t g = tid & (G - 1), tstep = nt >> p.G_log, t0 = tid >> p.G_log;
         const uint32_t nq = (p.in_mode == IN_SHARED) ? 1u : (1u << nq_log);
-        for (uint32_t ql = 0; ql < nq; ql++) {
-            uint64_t base = (p.in_mode == IN_SHARED ? 0ull : ((uint64_t)(qblk + ql) << p.L)) |
-                            ((uint64_t)tg.hi << (p.s_lo + C)) | (tg.lo0 + g);
-            const uint32_t *src = in_col + base;
-            for (uint32_t t = tid >> p.G_log; t < T; t += tstep) {
-                uint32_t v = src[(uint64_t)t << p.s_lo];
+        const uint32_t per_q = strided_count(t0, tstep, T);
+        const uint64_t hi_part = ((uint64_t)tg.hi << (p.s_lo + C)) | (tg.lo0 + g);
+        batched_copy<B>(
+            per_q * nq,
+            [&](uint32_t i) {
+                const uint32_t ql = i / per_q, t = t0 + (i - ql * per_q) * tstep;
+                const uint64_t base = (p.in_mode == IN_SHARED ? 0ull : ((uint64_t)(qblk + ql) << p.L)) | hi_part;
+                return in_col[base + ((uint64_t)t << p.s_lo)];
+            },
+            [&](uint32_t i, uint32_t v) {
+                const uint32_t ql = i / per_q, t = t0 + (i - ql * per_q) * tstep;
                 if (replicate_q) {
                     for (uint32_t qq = 0; qq < (1u << p.Q_log); qq++) lds[(qq * G + g) * TS + lidx(t)] = v;
                 } else {
                     lds[(ql * G + g) * TS + lidx(t)] = v;
                 }
-            }
-        }
+            });
         return;
     }
     if (p.in_mode == IN_KSPACE && p.s_lo == 0 && !replicate_q) {
-        // contiguous tiles: stream each tile with t fastest
-        for (uint32_t tile = 0; tile < (G << nq_log); tile++) {
-            const uint32_t g = tile & (G - 1), ql = tile >> p.G_log;
-            const uint32_t *src = in_col + (((uint64_t)(qblk + ql) << p.L) | tile_fixed(p, tg, g));
-            uint32_t *dst = lds + tile * TS;
-            for (uint32_t t = tid; t < T; t += nt) dst[lidx(t)] = src[t];
-        }
+        // contiguous tiles: element e = tid + i * nt is (tile = e >> C, t = e & (T - 1))
+        const uint32_t total = (G << nq_log) << C;
+        batched_copy<B>(
+            strided_count(tid, nt, total),
+            [&](uint32_t i) {
+                const uint32_t e = tid + i * nt, tile = e >> C, g = tile & (G - 1), ql = tile >> p.G_log;
+                return in_col[(((uint64_t)(qblk + ql) << p.L) | tile_fixed(p, tg, g)) + (e & (T - 1))];
+            },
+            [&](uint32_t i, uint32_t v) {
+                const uint32_t e = tid + i * nt;
+                lds[(e >> C) * TS + lidx(e & (T - 1))] = v;
+            });
         return;
     }
     coop_load_generic<C>(p, tg, lds, col, qblk, nq_log, replicate_q);
 }
 
-template <int C>
+template <int C, uint32_t B = 8>
 __device__ __forceinline__ void coop_store(const FftPass &p, const TileGeom &tg, const uint32_t *lds, uint32_t col,
                                            uint32_t qblk, uint32_t nq_log) {
     constexpr uint32_t T = 1u << C, TS = TC<C>::TS;
@@ -308,33 +347,45 @@ __device__ __forceinline__ void coop_store(const FftPass &p, const TileGeom &tg,
     if (p.out_mode == OUT_NATURAL && p.s_lo == 0 && p.natural_tiles && (1u << run_log) <= nt && !p.scale) {
         // element (q, g, t) goes to (bitrev_C(t) << (L-C+beta)) | ((tau0 + g) << beta) | bitrev_beta(q):
         // a thread keeps one run position rp = (g, bitrev(q)) and walks t
-        const uint32_t rp = tid & ((1u << run_log) - 1), tstep = nt >> run_log;
+        const uint32_t rp = tid & ((1u << run_log) - 1), tstep = nt >> run_log, t0 = tid >> run_log;
         const uint32_t rq = rp & ((1u << nq_log) - 1), g = rp >> nq_log;
         const uint32_t ql = bitrev32(rq, nq_log), q = qblk + ql;
         const uint64_t low = ((uint64_t)(tg.tau0 + g) << p.beta) | bitrev32(q, p.beta);
         const uint32_t sh = p.L - C + p.beta;
         const uint32_t *src = lds + (ql * G + g) * TS;
-        for (uint32_t t = tid >> run_log; t < T; t += tstep)
-            out_col[((uint64_t)bitrev32(t, C) << sh) | low] = src[lidx(t)];
+        batched_copy<B>(
+            strided_count(t0, tstep, T), [&](uint32_t i) { return src[lidx(t0 + i * tstep)]; },
+            [&](uint32_t i, uint32_t v) { out_col[((uint64_t)bitrev32(t0 + i * tstep, C) << sh) | low] = v; });
         return;
     }
     if (p.out_mode == OUT_KSPACE && p.s_lo > 0 && G <= nt && !p.scale) {
-        const uint32_t g = tid & (G - 1), tstep = nt >> p.G_log;
-        for (uint32_t ql = 0; ql < (1u << nq_log); ql++) {
-            uint64_t base = ((uint64_t)(qblk + ql) << p.L) | ((uint64_t)tg.hi << (p.s_lo + C)) | (tg.lo0 + g);
-            uint32_t *dst = out_col + base;
-            const uint32_t *src = lds + (ql * G + g) * TS;
-            for (uint32_t t = tid >> p.G_log; t < T; t += tstep) dst[(uint64_t)t << p.s_lo] = src[lidx(t)];
-        }
+        const uint32_t g = tid & (G - 1), tstep = nt >> p.G_log, t0 = tid >> p.G_log;
+        const uint32_t per_q = strided_count(t0, tstep, T);
+        const uint64_t hi_part = ((uint64_t)tg.hi << (p.s_lo + C)) | (tg.lo0 + g);
+        batched_copy<B>(
+            per_q << nq_log,
+            [&](uint32_t i) {
+                const uint32_t ql = i / per_q, t = t0 + (i - ql * per_q) * tstep;
+                return lds[(ql * G + g) * TS + lidx(t)];
+            },
+            [&](uint32_t i, uint32_t v) {
+                const uint32_t ql = i / per_q, t = t0 + (i - ql * per_q) * tstep;
+                out_col[(((uint64_t)(qblk + ql) << p.L) | hi_part) + ((uint64_t)t << p.s_lo)] = v;
+            });
         return;
     }
     if (p.out_mode == OUT_KSPACE && p.s_lo == 0 && !p.scale) {
-        for (uint32_t tile = 0; tile < (G << nq_log); tile++) {
-            const uint32_t g = tile & (G - 1), ql = tile >> p.G_log;
-            uint32_t *dst = out_col + (((uint64_t)(qblk + ql) << p.L) | tile_fixed(p, tg, g));
-            const uint32_t *src = lds + tile * TS;
-            for (uint32_t t = tid; t < T; t += nt) dst[t] = src[lidx(t)];
-        }
+        const uint32_t total = (G << nq_log) << C;
+        batched_copy<B>(
+            strided_count(tid, nt, total),
+            [&](uint32_t i) {
+                const uint32_t e = tid + i * nt;
+                return lds[(e >> C) * TS + lidx(e & (T - 1))];
+            },
+            [&](uint32_t i, uint32_t v) {
+                const uint32_t e = tid + i * nt, tile = e >> C, g = tile & (G - 1), ql = tile >> p.G_log;
+                out_col[(((uint64_t)(qblk + ql) << p.L) | tile_fixed(p, tg, g)) + (e & (T - 1))] = v;
+            });
         return;
     }
     coop_store_generic<C>(p, tg, lds, col, qblk, nq_log);
@@ -491,7 +542,7 @@ __global__ __launch_bounds__(tile_block_max<C>()) void lde_mid_kernel(LdeFused f
     const uint32_t c0 = blockIdx.y * p.ncb;
     const uint32_t c1 = min(c0 + p.ncb, p.n_cols);
     for (uint32_t col = c0; col < c1; col++) {
-        coop_load<C>(p, tg, lds, col, 0, 0, false);
+        coop_load<C, 4>(p, tg, lds, col, 0, 0, false);
         __syncthreads();
         uint32_t c[TC<C>::R];
         if (wave < G) {
@@ -509,7 +560,7 @@ __global__ __launch_bounds__(tile_block_max<C>()) void lde_mid_kernel(LdeFused f
                 fwd_from_regs<C>(v, tl, lane, twl + ((1 + q) << C));
             }
             __syncthreads();
-            coop_store<C>(p, tg, lds, col, q, 0);
+            coop_store<C, 4>(p, tg, lds, col, q, 0);
         }
         __syncthreads();
     }
@@ -533,7 +584,7 @@ __global__ __launch_bounds__(256) void lde_single_kernel(LdeFused f) {
     const uint32_t c0 = blockIdx.y * p.ncb;
     const uint32_t c1 = min(c0 + p.ncb, p.n_cols);
     for (uint32_t col = c0; col < c1; col++) {
-        coop_load<C>(p, tg, src, col, 0, 0, false);
+        coop_load<C, 4>(p, tg, src, col, 0, 0, false);
         __syncthreads();
         if (wave == 0) tile_run<true, C, 0>(src, lane, twl);
         __syncthreads();
@@ -548,7 +599,7 @@ __global__ __launch_bounds__(256) void lde_single_kernel(LdeFused f) {
             fwd_from_regs<C>(v, lds + q * TS, lane, twl + ((1 + q) << C));
         }
         __syncthreads();
-        coop_store<C>(p, tg, lds, col, 0, p.beta);
+        coop_store<C, 4>(p, tg, lds, col, 0, p.beta);
         __syncthreads();
     }
 }
