@@ -278,59 +278,71 @@ __device__ __forceinline__ void coop_store_generic(const FftPass &p, const TileG
 // 8 values in the pass kernels and 4 in the fused LDE kernels, whose registers
 // are fuller (measured: larger batches spill there).
 
+// In the fast paths a thread's element i sits at t = t0 + i * tstep (or at
+// t = tid + j * nt within tile i >> per_log) with t0 < tstep, so every address
+// splits into a part fixed per thread, computed once, and a part that depends
+// only on i and is the same for the whole workgroup (scalar registers): the
+// index bits of t0 and of i * tstep are disjoint, which also makes
+// bitrev_C(t) = bitrev_C(t0) + bitrev_{C - log2 tstep}(i) and
+// lidx(t) = lidx(t0) + lidx(i * tstep).
 template <int C, uint32_t B = 8>
 __device__ __forceinline__ void coop_load(const FftPass &p, const TileGeom &tg, uint32_t *lds, uint32_t col,
                                           uint32_t qblk, uint32_t nq_log, bool replicate_q) {
     constexpr uint32_t T = 1u << C, TS = TC<C>::TS;
     const uint32_t G = 1u << p.G_log, nt = blockDim.x, tid = threadIdx.x;
+    const uint32_t lnt = __builtin_ctz(nt);  // block sizes are powers of two
     const uint32_t *in_col = p.in + (uint64_t)col * p.in_cs;
-    if (p.in_mode == IN_NATURAL && p.s_lo == 0 && p.natural_tiles && nq_log == 0 && G <= nt) {
+    if (p.in_mode == IN_NATURAL && p.s_lo == 0 && p.natural_tiles && nq_log == 0 && G <= nt &&
+        lnt - p.G_log <= (uint32_t)C) {
         // element (g, t) sits at natural (bitrev_C(t) << (L - C)) | (tau0 + g): runs of G along g
-        const uint32_t g = tid & (G - 1), tstep = nt >> p.G_log, t0 = tid >> p.G_log;
-        const uint32_t *src = in_col + tg.tau0 + g;
-        uint32_t *dst = lds + g * TS;
-        const uint32_t sh = p.L - C;
+        const uint32_t g = tid & (G - 1), lts = lnt - p.G_log, t0 = tid >> p.G_log, sh = p.L - C;
+        const uint32_t *src = in_col + tg.tau0 + g + ((uint64_t)bitrev32(t0, C) << sh);
+        uint32_t *dst = lds + g * TS + lidx(t0);
         batched_copy<B>(
-            strided_count(t0, tstep, T),
-            [&](uint32_t i) { return src[(uint64_t)bitrev32(t0 + i * tstep, C) << sh]; },
-            [&](uint32_t i, uint32_t v) { dst[lidx(t0 + i * tstep)] = v; });
+            T >> lts, [&](uint32_t i) { return src[(uint64_t)bitrev32(i, C - lts) << sh]; },
+            [&](uint32_t i, uint32_t v) { dst[lidx(i << lts)] = v; });
         return;
     }
-    if ((p.in_mode == IN_KSPACE || p.in_mode == IN_SHARED) && p.s_lo > 0 && G <= nt) {
+    if ((p.in_mode == IN_KSPACE || p.in_mode == IN_SHARED) && p.s_lo > 0 && G <= nt &&
+        lnt - p.G_log <= (uint32_t)C) {
         // strided tiles: (q, hi, t, lo0 + g) -> runs of G along g
-        const uint32_t g = tid & (G - 1), tstep = nt >> p.G_log, t0 = tid >> p.G_log;
+        const uint32_t g = tid & (G - 1), lts = lnt - p.G_log, t0 = tid >> p.G_log, per_log = C - lts;
         const uint32_t nq = (p.in_mode == IN_SHARED) ? 1u : (1u << nq_log);
-        const uint32_t per_q = strided_count(t0, tstep, T);
-        const uint64_t hi_part = ((uint64_t)tg.hi << (p.s_lo + C)) | (tg.lo0 + g);
+        const uint32_t *src = in_col + tg.lo0 + g + ((uint64_t)t0 << p.s_lo);
+        const uint64_t hi_part = (uint64_t)tg.hi << (p.s_lo + C);
+        uint32_t *dst = lds + g * TS + lidx(t0);
         batched_copy<B>(
-            per_q * nq,
+            nq << per_log,
             [&](uint32_t i) {
-                const uint32_t ql = i / per_q, t = t0 + (i - ql * per_q) * tstep;
-                const uint64_t base = (p.in_mode == IN_SHARED ? 0ull : ((uint64_t)(qblk + ql) << p.L)) | hi_part;
-                return in_col[base + ((uint64_t)t << p.s_lo)];
+                const uint32_t ql = i >> per_log, j = i & ((1u << per_log) - 1);
+                const uint64_t q_part = p.in_mode == IN_SHARED ? 0ull : ((uint64_t)(qblk + ql) << p.L);
+                return src[q_part + hi_part + ((uint64_t)j << (lts + p.s_lo))];
             },
             [&](uint32_t i, uint32_t v) {
-                const uint32_t ql = i / per_q, t = t0 + (i - ql * per_q) * tstep;
+                const uint32_t ql = i >> per_log, j = i & ((1u << per_log) - 1);
                 if (replicate_q) {
-                    for (uint32_t qq = 0; qq < (1u << p.Q_log); qq++) lds[(qq * G + g) * TS + lidx(t)] = v;
+                    for (uint32_t qq = 0; qq < (1u << p.Q_log); qq++) dst[qq * G * TS + lidx(j << lts)] = v;
                 } else {
-                    lds[(ql * G + g) * TS + lidx(t)] = v;
+                    dst[ql * G * TS + lidx(j << lts)] = v;
                 }
             });
         return;
     }
-    if (p.in_mode == IN_KSPACE && p.s_lo == 0 && !replicate_q) {
-        // contiguous tiles: element e = tid + i * nt is (tile = e >> C, t = e & (T - 1))
-        const uint32_t total = (G << nq_log) << C;
+    if (p.in_mode == IN_KSPACE && p.s_lo == 0 && !replicate_q && nt <= T) {
+        // contiguous tiles, t fastest: element i of the thread is tile i >> per_log, t = tid + j * nt
+        const uint32_t per_log = C - lnt;
+        const uint32_t *src = in_col + tid;
+        uint32_t *dst = lds + lidx(tid);
         batched_copy<B>(
-            strided_count(tid, nt, total),
+            (G << nq_log) << per_log,
             [&](uint32_t i) {
-                const uint32_t e = tid + i * nt, tile = e >> C, g = tile & (G - 1), ql = tile >> p.G_log;
-                return in_col[(((uint64_t)(qblk + ql) << p.L) | tile_fixed(p, tg, g)) + (e & (T - 1))];
+                const uint32_t tile = i >> per_log, j = i & ((1u << per_log) - 1);
+                const uint32_t g = tile & (G - 1), ql = tile >> p.G_log;
+                return src[(((uint64_t)(qblk + ql) << p.L) | tile_fixed(p, tg, g)) + (j << lnt)];
             },
             [&](uint32_t i, uint32_t v) {
-                const uint32_t e = tid + i * nt;
-                lds[(e >> C) * TS + lidx(e & (T - 1))] = v;
+                const uint32_t tile = i >> per_log, j = i & ((1u << per_log) - 1);
+                dst[tile * TS + lidx(j << lnt)] = v;
             });
         return;
     }
@@ -342,49 +354,56 @@ __device__ __forceinline__ void coop_store(const FftPass &p, const TileGeom &tg,
                                            uint32_t qblk, uint32_t nq_log) {
     constexpr uint32_t T = 1u << C, TS = TC<C>::TS;
     const uint32_t G = 1u << p.G_log, nt = blockDim.x, tid = threadIdx.x;
+    const uint32_t lnt = __builtin_ctz(nt);
     uint32_t *out_col = p.out + (uint64_t)col * p.out_cs;
     const uint32_t run_log = p.G_log + nq_log;
-    if (p.out_mode == OUT_NATURAL && p.s_lo == 0 && p.natural_tiles && (1u << run_log) <= nt && !p.scale) {
+    if (p.out_mode == OUT_NATURAL && p.s_lo == 0 && p.natural_tiles && run_log <= lnt && lnt - run_log <= (uint32_t)C &&
+        !p.scale) {
         // element (q, g, t) goes to (bitrev_C(t) << (L-C+beta)) | ((tau0 + g) << beta) | bitrev_beta(q):
         // a thread keeps one run position rp = (g, bitrev(q)) and walks t
-        const uint32_t rp = tid & ((1u << run_log) - 1), tstep = nt >> run_log, t0 = tid >> run_log;
+        const uint32_t rp = tid & ((1u << run_log) - 1), lts = lnt - run_log, t0 = tid >> run_log;
         const uint32_t rq = rp & ((1u << nq_log) - 1), g = rp >> nq_log;
         const uint32_t ql = bitrev32(rq, nq_log), q = qblk + ql;
-        const uint64_t low = ((uint64_t)(tg.tau0 + g) << p.beta) | bitrev32(q, p.beta);
         const uint32_t sh = p.L - C + p.beta;
-        const uint32_t *src = lds + (ql * G + g) * TS;
+        const uint64_t low = ((uint64_t)(tg.tau0 + g) << p.beta) | bitrev32(q, p.beta);
+        const uint32_t *src = lds + (ql * G + g) * TS + lidx(t0);
+        uint32_t *dst = out_col + low + ((uint64_t)bitrev32(t0, C) << sh);
         batched_copy<B>(
-            strided_count(t0, tstep, T), [&](uint32_t i) { return src[lidx(t0 + i * tstep)]; },
-            [&](uint32_t i, uint32_t v) { out_col[((uint64_t)bitrev32(t0 + i * tstep, C) << sh) | low] = v; });
+            T >> lts, [&](uint32_t i) { return src[lidx(i << lts)]; },
+            [&](uint32_t i, uint32_t v) { dst[(uint64_t)bitrev32(i, C - lts) << sh] = v; });
         return;
     }
-    if (p.out_mode == OUT_KSPACE && p.s_lo > 0 && G <= nt && !p.scale) {
-        const uint32_t g = tid & (G - 1), tstep = nt >> p.G_log, t0 = tid >> p.G_log;
-        const uint32_t per_q = strided_count(t0, tstep, T);
-        const uint64_t hi_part = ((uint64_t)tg.hi << (p.s_lo + C)) | (tg.lo0 + g);
+    if (p.out_mode == OUT_KSPACE && p.s_lo > 0 && G <= nt && lnt - p.G_log <= (uint32_t)C && !p.scale) {
+        const uint32_t g = tid & (G - 1), lts = lnt - p.G_log, t0 = tid >> p.G_log, per_log = C - lts;
+        const uint32_t *src = lds + g * TS + lidx(t0);
+        uint32_t *dst = out_col + tg.lo0 + g + ((uint64_t)t0 << p.s_lo);
+        const uint64_t hi_part = (uint64_t)tg.hi << (p.s_lo + C);
         batched_copy<B>(
-            per_q << nq_log,
+            (1u << nq_log) << per_log,
             [&](uint32_t i) {
-                const uint32_t ql = i / per_q, t = t0 + (i - ql * per_q) * tstep;
-                return lds[(ql * G + g) * TS + lidx(t)];
+                const uint32_t ql = i >> per_log, j = i & ((1u << per_log) - 1);
+                return src[ql * G * TS + lidx(j << lts)];
             },
             [&](uint32_t i, uint32_t v) {
-                const uint32_t ql = i / per_q, t = t0 + (i - ql * per_q) * tstep;
-                out_col[(((uint64_t)(qblk + ql) << p.L) | hi_part) + ((uint64_t)t << p.s_lo)] = v;
+                const uint32_t ql = i >> per_log, j = i & ((1u << per_log) - 1);
+                dst[((uint64_t)(qblk + ql) << p.L) + hi_part + ((uint64_t)j << (lts + p.s_lo))] = v;
             });
         return;
     }
-    if (p.out_mode == OUT_KSPACE && p.s_lo == 0 && !p.scale) {
-        const uint32_t total = (G << nq_log) << C;
+    if (p.out_mode == OUT_KSPACE && p.s_lo == 0 && !p.scale && nt <= T) {
+        const uint32_t per_log = C - lnt;
+        const uint32_t *src = lds + lidx(tid);
+        uint32_t *dst = out_col + tid;
         batched_copy<B>(
-            strided_count(tid, nt, total),
+            (G << nq_log) << per_log,
             [&](uint32_t i) {
-                const uint32_t e = tid + i * nt;
-                return lds[(e >> C) * TS + lidx(e & (T - 1))];
+                const uint32_t tile = i >> per_log, j = i & ((1u << per_log) - 1);
+                return src[tile * TS + lidx(j << lnt)];
             },
             [&](uint32_t i, uint32_t v) {
-                const uint32_t e = tid + i * nt, tile = e >> C, g = tile & (G - 1), ql = tile >> p.G_log;
-                out_col[(((uint64_t)(qblk + ql) << p.L) | tile_fixed(p, tg, g)) + (e & (T - 1))] = v;
+                const uint32_t tile = i >> per_log, j = i & ((1u << per_log) - 1);
+                const uint32_t g = tile & (G - 1), ql = tile >> p.G_log;
+                dst[(((uint64_t)(qblk + ql) << p.L) | tile_fixed(p, tg, g)) + (j << lnt)] = v;
             });
         return;
     }
