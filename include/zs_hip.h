@@ -183,6 +183,17 @@ int32_t zs_fri_commit(zs_hip_ctx *ctx, const uint32_t *codeword, uint32_t dlog, 
 int32_t zs_fri_query(zs_fri_result *res, zs_channel *chan, size_t num_queries, size_t *idx_out, uint32_t *vals_out,
                      uint8_t *paths_out);
 /* Copy FRI layer `layer` (2^(dlog - layer) QM31 values) to host memory. */
+/* Fri.proveCodeword (src/m31/fri.zig:157-273) on a device codeword of
+ * 2^(log_size + log_blowup) QM31 values, with every challenge and query index
+ * drawn from `chan`, returned as the reference's wire bytes
+ * (src/core/serialization.zig, docs/wire.md) of its Fri.Proof: FriParams
+ * {log_size u8, log_blowup u8, num_queries u64, remainder_log u8}, roots
+ * []Digest, remainder []QM31, queries []Query{index u64, pairs []Pair{value0,
+ * value1 QM31, path0, path1 []Digest}}, every slice after its u64 length.
+ * The bytes are allocated with `host` and released with zs_free. */
+int32_t zs_fri_prove_dev(zs_hip_ctx *ctx, const uint32_t *d_codeword, uint32_t log_size, uint32_t log_blowup,
+                         uint32_t remainder_log, size_t num_queries, zs_channel *chan, zs_host_allocator host,
+                         uint8_t **proof_out, size_t *proof_len);
 int32_t zs_fri_read_layer(const zs_fri_result *res, uint32_t layer, uint32_t *out);
 void zs_fri_result_free(zs_fri_result *res);
 

@@ -1147,3 +1147,81 @@ void zo_fib_trace(uint32_t a0, uint32_t a1, size_t n, uint32_t *cols) {
         b = nb;
     }
 }
+
+/* ======================================================================== */
+/* Wire format: src/core/serialization.zig:37-140, docs/wire.md              */
+/* Little-endian unsigned ints (usize as 8 bytes), [N]u8 verbatim, slices as */
+/* a u64 length then the elements, ?T as a presence byte then T, structs as  */
+/* their fields in declaration order (allocator fields skipped), field       */
+/* elements as their toBytes (QM31: a.c0, a.c1, b.c0, b.c1 as u32 LE).       */
+/* ======================================================================== */
+typedef struct { uint8_t *p; size_t n; } wire_buf;
+static void w_bytes(wire_buf *w, const void *d, size_t n) {
+    if (w->p) memcpy(w->p + w->n, d, n);
+    w->n += n;
+}
+static void w_uint(wire_buf *w, uint64_t v, unsigned bytes) {
+    uint8_t t[8];
+    for (unsigned b = 0; b < bytes; b++) t[b] = (uint8_t)(v >> (8 * b));
+    w_bytes(w, t, bytes);
+}
+static void w_qm31(wire_buf *w, const uint32_t v[4]) {
+    for (int k = 0; k < 4; k++) w_uint(w, v[k], 4);
+}
+
+/* serialization.zig:219-247: the golden layout of
+ * struct { name: [3]u8, count: u32, rows: []const u16, flag: bool, maybe: ?[2]u8 }
+ * with ("abc", 0x01020304, {0x0102, 0x0304}, true, {0xde, 0xad}), written with the
+ * writers above (pins them) */
+size_t zo_wire_golden_example(uint8_t *out) {
+    wire_buf w = {out, 0};
+    w_bytes(&w, "abc", 3);
+    w_uint(&w, 0x01020304, 4);
+    w_uint(&w, 2, 8);
+    w_uint(&w, 0x0102, 2);
+    w_uint(&w, 0x0304, 2);
+    w_uint(&w, 1, 1);
+    w_uint(&w, 1, 1);
+    const uint8_t maybe[2] = {0xde, 0xad};
+    w_bytes(&w, maybe, 2);
+    return w.n;
+}
+
+/* Fri.Proof (fri.zig:38-80): params {log_size u8, log_blowup u8, num_queries usize,
+ * remainder_log u8}, roots []Digest, remainder []QM31, queries []Query{index usize,
+ * pairs []Pair{value0 QM31, value1 QM31, path0 []Digest, path1 []Digest}}.
+ * Inputs in zo_fri_prove's layouts; out may be NULL (returns the length). */
+size_t zo_fri_proof_serialize(uint32_t log_size, uint32_t log_blowup, uint32_t remainder_log, size_t num_queries,
+                              const uint8_t *roots, const uint32_t *remainder, const size_t *query_idx,
+                              const uint32_t *query_vals, const uint8_t *query_paths, uint8_t *out) {
+    wire_buf w = {out, 0};
+    const uint32_t dlog = log_size + log_blowup, L = dlog - remainder_log;
+    const size_t rc = (size_t)1 << (remainder_log - log_blowup);
+    w_uint(&w, log_size, 1);
+    w_uint(&w, log_blowup, 1);
+    w_uint(&w, num_queries, 8);
+    w_uint(&w, remainder_log, 1);
+    w_uint(&w, L, 8);
+    w_bytes(&w, roots, (size_t)L * 32);
+    w_uint(&w, rc, 8);
+    for (size_t k = 0; k < rc; k++) w_qm31(&w, remainder + 4 * k);
+    w_uint(&w, num_queries, 8);
+    const size_t per_q = zo_fri_query_path_bytes(dlog, remainder_log);
+    for (size_t q = 0; q < num_queries; q++) {
+        w_uint(&w, query_idx[q], 8);
+        w_uint(&w, L, 8);
+        const uint8_t *pp = query_paths + q * per_q;
+        for (uint32_t i = 0; i < L; i++) {
+            const size_t plen = dlog - i;
+            w_qm31(&w, query_vals + 4 * ((q * L + i) * 2));
+            w_qm31(&w, query_vals + 4 * ((q * L + i) * 2 + 1));
+            w_uint(&w, plen, 8);
+            w_bytes(&w, pp, plen * 32);
+            pp += plen * 32;
+            w_uint(&w, plen, 8);
+            w_bytes(&w, pp, plen * 32);
+            pp += plen * 32;
+        }
+    }
+    return w.n;
+}

@@ -159,6 +159,14 @@ int zo_fri_verify(uint32_t dlog, uint32_t log_blowup, uint32_t remainder_log, si
                   zo_channel *chan, const uint8_t *roots, const uint32_t *remainder,
                   const size_t *query_idx, const uint32_t *query_vals, const uint8_t *query_paths);
 
+/* ---- wire format (src/core/serialization.zig, docs/wire.md) ------------------ */
+/* the golden example of serialization.zig:219-247 written with the oracle's writers */
+size_t zo_wire_golden_example(uint8_t *out);
+/* Fri.Proof bytes from zo_fri_prove's outputs; out may be NULL (length only) */
+size_t zo_fri_proof_serialize(uint32_t log_size, uint32_t log_blowup, uint32_t remainder_log, size_t num_queries,
+                              const uint8_t *roots, const uint32_t *remainder, const size_t *query_idx,
+                              const uint32_t *query_vals, const uint8_t *query_paths, uint8_t *out);
+
 /* ---- seeded synthetic inputs ------------------------------------------------ */
 /* SplitMix64 stream; M31 = 31 random bits with 0x7FFFFFFF rejected */
 void zo_fill_m31(uint64_t seed, uint32_t *out, size_t n);

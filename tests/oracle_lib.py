@@ -102,6 +102,11 @@ def _declare(L):
     L.zo_fill_m31.argtypes = [ctypes.c_uint64, u32p, sz]
     L.zo_trace_commit.argtypes = [u32p, sz, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p, u32p, ctypes.c_char_p]
     L.zo_fib_trace.argtypes = [ctypes.c_uint32, ctypes.c_uint32, sz, u32p]
+    L.zo_wire_golden_example.restype = sz
+    L.zo_wire_golden_example.argtypes = [ctypes.c_char_p]
+    L.zo_fri_proof_serialize.restype = sz
+    L.zo_fri_proof_serialize.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, sz, u8p, u32p,
+                                         ctypes.POINTER(sz), u32p, u8p, ctypes.c_char_p]
 
 
 def _p(a):
@@ -483,3 +488,22 @@ def fib_trace(a0, a1, n):
     out = np.zeros((2, n, 4), np.uint32)
     lib().zo_fib_trace(a0, a1, n, _p(out))
     return out
+
+
+def wire_golden_example():
+    out = ctypes.create_string_buffer(64)
+    n = lib().zo_wire_golden_example(out)
+    return out.raw[:n]
+
+
+def fri_proof_bytes(proof, log_blowup, remainder_log):
+    """Fri.Proof wire bytes (serialization.zig) of a fri_prove() result."""
+    dlog = proof["dlog"]
+    q = proof["num_queries"]
+    qidx = (sz * q)(*proof["qidx"])
+    args = (dlog - log_blowup, log_blowup, remainder_log, q, bytes(proof["roots"]), _p(u32(proof["remainder"])),
+            qidx, _p(u32(proof["qvals"])), bytes(proof["qpaths"]))
+    n = lib().zo_fri_proof_serialize(*args, None)
+    out = ctypes.create_string_buffer(n)
+    lib().zo_fri_proof_serialize(*args, out)
+    return out.raw

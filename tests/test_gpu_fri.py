@@ -143,3 +143,19 @@ def test_fri_device_channel_graph_replays(zctx, torch_dev, dlog, beta, rem_log):
             oroots, _, orem, _ = O.fri_commit(cw, beta, rem_log, channel=och, want_layers=False)
             assert roots == oroots and np.array_equal(rem, orem), (rep, flags)
             assert ch.sample_bytes(32) == och.sample_bytes(32), (rep, flags)
+
+
+@pytest.mark.parametrize("log_size,beta,rem_log,queries", [(8, 2, 5, 20), (12, 2, 3, 20), (6, 3, 4, 7)])
+def test_fri_prove_wire_bytes_match_oracle(zctx, torch_dev, log_size, beta, rem_log, queries):
+    """proveCodeword's Fri.Proof in the reference's wire format, byte for byte."""
+    from conftest import to_dev
+    from wire import parse_fri_proof
+    rng = np.random.default_rng(log_size * 3 + rem_log)
+    dlog = log_size + beta
+    cw = rng.integers(0, P, (1 << dlog, 4), dtype=np.uint32)
+    d_cw = to_dev(cw.reshape(-1), torch_dev)
+    got = zctx.fri_prove_dev(d_cw.data_ptr(), log_size, beta, rem_log, queries, Z.Channel(b"zig-stark:wire"))
+    proof = O.fri_prove(cw, beta, rem_log, queries, O.Channel(b"zig-stark:wire"))
+    assert got == O.fri_proof_bytes(proof, beta, rem_log)
+    p = parse_fri_proof(got)
+    assert len(p["queries"]) == queries and p["log_size"] == log_size

@@ -113,6 +113,10 @@ def _declare(L):
     L.zs_fill_m31_dev.argtypes = [vp, ctypes.c_uint64, vp, sz]
     L.zs_commit_job_dev.argtypes = [vp, vp, u32, u32, u32, u32, vp, vp, ctypes.c_char_p, ctypes.c_char_p, u32p, u32]
     L.zs_commit_batch_dev.argtypes = [vp, ctypes.c_void_p, sz, u32, u32, u32, u32]
+    L.zs_fri_prove_dev.argtypes = [vp, vp, u32, u32, u32, sz, vp, HostAllocator, ctypes.POINTER(ctypes.c_void_p),
+                                   ctypes.POINTER(sz)]
+    L.zs_free.argtypes = [HostAllocator, ctypes.c_void_p, sz]
+    L.zs_free.restype = None
     L.zs_autotune_lde.argtypes = [vp, u32, u32, u32, u32p]
     L.zs_qm31_trace_commit_dev.argtypes = [vp, vp, u32, u32, u32, vp, vp, ctypes.c_char_p, vp]
     L.zs_fib_trace.argtypes = [u32, u32, u32, u32p]
@@ -133,7 +137,7 @@ def _declare(L):
                  "zs_merkle_open", "zs_tree_read_level", "zs_fri_commit_dev", "zs_fri_commit", "zs_fri_query",
                  "zs_fri_read_layer", "zs_fill_m31_dev", "zs_hip_profile", "zs_hip_profile_read",
                  "zs_commit_job_dev", "zs_commit_batch_dev", "zs_server_submit", "zs_server_wait",
-                 "zs_server_memory", "zs_server_poll", "zs_qm31_trace_commit_dev", "zs_fib_trace", "zs_autotune_lde"]:
+                 "zs_server_memory", "zs_server_poll", "zs_qm31_trace_commit_dev", "zs_fib_trace", "zs_autotune_lde", "zs_fri_prove_dev"]:
         getattr(L, name).restype = i32
 
 
@@ -346,6 +350,24 @@ class Server:
             self.close()
 
 
+_ALLOC_FN = ctypes.CFUNCTYPE(ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t)
+_FREE_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t)
+
+
+class HostAllocator(ctypes.Structure):
+    """zs_host_allocator (zig-capi.h:35-39) backed by the C library's malloc/free."""
+    _fields_ = [("ctx", ctypes.c_void_p), ("alloc", _ALLOC_FN), ("free", _FREE_FN)]
+
+
+_libc = ctypes.CDLL(None)
+_libc.malloc.restype = ctypes.c_void_p
+_libc.malloc.argtypes = [ctypes.c_size_t]
+_libc.free.argtypes = [ctypes.c_void_p]
+_alloc_cb = _ALLOC_FN(lambda ctx, n: _libc.malloc(max(int(n), 1)))
+_free_cb = _FREE_FN(lambda ctx, ptr, n: _libc.free(ptr))
+HOST = HostAllocator(None, _alloc_cb, _free_cb)
+
+
 class Context:
     """A zs_hip_ctx on one device (circlefft_gpu.zig:20-39's lazily-initialised context)."""
 
@@ -484,6 +506,16 @@ class Context:
         self.check(lib().zs_commit_batch_dev(self.h, jobs, len(specs), log_blowup, remainder_log, n_workers, flags))
         return [(bytes(jobs[k].trace_root), [keep[k][0].raw[32 * i:32 * (i + 1)] for i in range(keep[k][2])],
                  keep[k][1].reshape(-1, 4)) for k in range(len(specs))]
+
+    def fri_prove_dev(self, d_codeword, log_size, log_blowup, remainder_log, num_queries, channel):
+        """proveCodeword on a device codeword; returns the Fri.Proof wire bytes."""
+        ptr, n = ctypes.c_void_p(), sz()
+        self.check(lib().zs_fri_prove_dev(self.h, d_codeword, log_size, log_blowup, remainder_log, num_queries,
+                                          channel.h, HOST, ctypes.byref(ptr), ctypes.byref(n)))
+        try:
+            return ctypes.string_at(ptr.value, n.value)
+        finally:
+            lib().zs_free(HOST, ptr, n)
 
     # ---- FRI (fri.zig:157-233) ----
     def fri_commit(self, codeword, log_blowup, remainder_log, alphas=None, channel=None, keep=True):

@@ -7,6 +7,7 @@
 // missing device or a HIP failure is returned as ZS_ERR_DEVICE.
 #include <cstring>
 #include <map>
+#include <memory>
 #include <new>
 #include <vector>
 
@@ -725,6 +726,82 @@ int32_t zs_fri_query(zs_fri_result *r, zs_channel *chan, size_t num_queries, siz
     ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ZS_HIP(ctx, hipMemcpyAsync(paths_out, dp.ptr, num_queries * per_q_digests * 32, hipMemcpyDeviceToHost, ctx->stream));
     ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return ZS_OK;
+}
+
+// Fri.Proof in the wire format of src/core/serialization.zig (docs/wire.md):
+// unsigned ints little-endian (usize as 8 bytes), digests verbatim, slices as a
+// u64 length then their elements, structs as their fields in order, QM31 as
+// its four u32 words.
+namespace {
+struct WireWriter {
+    std::vector<uint8_t> b;
+    void bytes(const void *d, size_t n) {
+        const uint8_t *p = static_cast<const uint8_t *>(d);
+        b.insert(b.end(), p, p + n);
+    }
+    void uint(uint64_t v, unsigned n) {
+        for (unsigned k = 0; k < n; k++) b.push_back((uint8_t)(v >> (8 * k)));
+    }
+    void qm31(const uint32_t *v) {
+        for (int k = 0; k < 4; k++) uint(v[k], 4);
+    }
+};
+}  // namespace
+
+int32_t zs_fri_prove_dev(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t log_size, uint32_t log_blowup,
+                         uint32_t remainder_log, size_t num_queries, zs_channel *chan, zs_host_allocator host,
+                         uint8_t **proof_out, size_t *proof_len) {
+    if (!h || !d_codeword || !chan || !proof_out || !proof_len || !host.alloc || log_size > 255 ||
+        log_blowup > 255 || remainder_log > 255)
+        return ZS_ERR_INPUT;
+    const uint32_t dlog = log_size + log_blowup;
+    if (dlog > 28 || remainder_log < log_blowup || remainder_log >= dlog) return ZS_ERR_INPUT;
+    const uint32_t L = dlog - remainder_log;
+    const size_t rc = size_t(1) << (remainder_log - log_blowup);
+    std::vector<uint8_t> roots(size_t(L) * 32);
+    std::vector<uint32_t> rem(4 * rc);
+    zs_fri_result *res = nullptr;
+    ZS_TRY(zs_fri_commit_dev(h, d_codeword, dlog, log_blowup, remainder_log, nullptr, chan, 0, roots.data(),
+                             rem.data(), &res));
+    std::unique_ptr<zs_fri_result, void (*)(zs_fri_result *)> guard(res, zs_fri_result_free);
+    size_t per_q = 0;
+    for (uint32_t i = 0; i < L; i++) per_q += 2 * size_t(dlog - i) * 32;
+    std::vector<size_t> idx(num_queries);
+    std::vector<uint32_t> vals(num_queries * L * 8);
+    std::vector<uint8_t> paths(num_queries * per_q);
+    ZS_TRY(zs_fri_query(res, chan, num_queries, idx.data(), vals.data(), paths.data()));
+    WireWriter w;
+    w.uint(log_size, 1);  // FriParams (fri.zig:38-46)
+    w.uint(log_blowup, 1);
+    w.uint(num_queries, 8);
+    w.uint(remainder_log, 1);
+    w.uint(L, 8);  // roots
+    w.bytes(roots.data(), roots.size());
+    w.uint(rc, 8);  // remainder
+    for (size_t k = 0; k < rc; k++) w.qm31(&rem[4 * k]);
+    w.uint(num_queries, 8);  // queries
+    for (size_t q = 0; q < num_queries; q++) {
+        w.uint(idx[q], 8);
+        w.uint(L, 8);
+        const uint8_t *pp = paths.data() + q * per_q;
+        for (uint32_t i = 0; i < L; i++) {
+            const size_t plen = dlog - i;
+            w.qm31(&vals[8 * (q * L + i)]);
+            w.qm31(&vals[8 * (q * L + i) + 4]);
+            w.uint(plen, 8);
+            w.bytes(pp, plen * 32);
+            pp += plen * 32;
+            w.uint(plen, 8);
+            w.bytes(pp, plen * 32);
+            pp += plen * 32;
+        }
+    }
+    uint8_t *out = static_cast<uint8_t *>(host.alloc(host.ctx, w.b.size()));
+    if (!out) return ZS_ERR_OOM;
+    std::memcpy(out, w.b.data(), w.b.size());
+    *proof_out = out;
+    *proof_len = w.b.size();
     return ZS_OK;
 }
 
