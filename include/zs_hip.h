@@ -288,6 +288,19 @@ int32_t zs_qm31_trace_commit_dev(zs_hip_ctx *ctx, const uint32_t *d_cols, uint32
  * a_{i+2} = a_i + a_{i+1}, each M31 value in the first QM31 coordinate.  Host buffers. */
 int32_t zs_fib_trace(uint32_t a0, uint32_t a1, uint32_t log_n, uint32_t *out);
 
+/* GenericStark(FibAir).prove (src/m31/stark.zig:303-657) with the Fibonacci AIR
+ * of stark.zig:877-904 and the boundary assertions (0,0) = a0, (1,0) = a1,
+ * (0,n-1) = claimed, on a device trace of two QM31 columns of 2^trace_log
+ * values: trace commitment, quotient and DEEP codewords and FRI on the device,
+ * transcript on the host.  The proof is returned as the reference's wire bytes
+ * of GenericStark.Proof (serialization.zig), allocated with `host`; the
+ * channel ends where the reference prover's does.
+ * 1 <= log_blowup <= remainder_log < trace_log + log_blowup <= 24. */
+int32_t zs_fib_stark_prove_dev(zs_hip_ctx *ctx, const uint32_t *d_trace, uint32_t trace_log, uint32_t log_blowup,
+                               size_t num_queries, uint32_t remainder_log, const uint32_t a0[4], const uint32_t a1[4],
+                               const uint32_t claimed[4], zs_channel *chan, zs_host_allocator host,
+                               uint8_t **proof_out, size_t *proof_len);
+
 /* ---- synthetic input ---------------------------------------------------------- */
 /* Seeded uniform M31 values: element i is the top 31 bits of the (i+1)-th
  * SplitMix64 output from `seed` (2^31 - 1 rejected). */

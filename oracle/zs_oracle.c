@@ -1225,3 +1225,296 @@ size_t zo_fri_proof_serialize(uint32_t log_size, uint32_t log_blowup, uint32_t r
     }
     return w.n;
 }
+
+/* ======================================================================== */
+/* GenericStark(FibSeededAir): src/m31/stark.zig:303-657 (prove) and        */
+/* :659-786 (verify), without preprocessed columns or lookups, for FibAir    */
+/* (stark.zig:877-904) with seeded boundary assertions (0,0) = a0,          */
+/* (1,0) = a1, (0,n-1) = claimed.  Proofs are GenericStark.Proof in the     */
+/* wire format (serialization.zig).                                          */
+/* ======================================================================== */
+#define FIB_M 2
+#define FIB_NDEEP 5
+static void fib_boundary(const uint32_t *a0, const uint32_t *a1, const uint32_t *claimed, size_t n, size_t col[3],
+                         size_t step[3], zo_qm31 val[3]) {
+    col[0] = 0; step[0] = 0; val[0] = ld4(a0);
+    col[1] = 1; step[1] = 0; val[1] = ld4(a1);
+    col[2] = 0; step[2] = n - 1; val[2] = ld4(claimed);
+}
+/* Hc(x) of stark.zig:504-528 (without lookups) for FibAir's transitions:
+ * next[0] - cur[1] and next[1] - cur[0] - cur[1] (stark.zig:886-891) */
+static zo_qm31 fib_composition(zo_qm31 x, const zo_qm31 cur[2], const zo_qm31 nxt[2], zo_qm31 last, zo_qm31 zh,
+                               const zo_qm31 al[2], const zo_qm31 be[3], const size_t col[3], const zo_qm31 pk[3],
+                               const zo_qm31 val[3]) {
+    zo_qm31 res[2] = {zo_qm31_sub(nxt[0], cur[1]), zo_qm31_sub(zo_qm31_sub(nxt[1], cur[0]), cur[1])};
+    zo_qm31 h = Q_ZERO, xl = zo_qm31_sub(x, last);
+    for (int k = 0; k < 2; k++) h = zo_qm31_add(h, zo_qm31_mul(al[k], zo_qm31_mul(res[k], xl)));
+    for (int k = 0; k < 3; k++) {
+        zo_qm31 term = zo_qm31_mul(zo_qm31_mul(zo_qm31_sub(cur[col[k]], val[k]), zh), zo_qm31_inv(zo_qm31_sub(x, pk[k])));
+        h = zo_qm31_add(h, zo_qm31_mul(be[k], term));
+    }
+    return h;
+}
+static void absorb_qm31(zo_channel *c, zo_qm31 x) {
+    uint8_t b[16];
+    qm31_to_bytes(x, b);
+    zo_channel_absorb(c, b, 16);
+}
+static zo_qm31 sample_q(zo_channel *c) {
+    zo_qm31 r;
+    zo_channel_sample_qm31(c, r.v);
+    return r;
+}
+static void w_stark_params(wire_buf *w, uint32_t tl, uint32_t lb, size_t nq, uint32_t rl) {
+    w_uint(w, tl, 1);
+    w_uint(w, lb, 1);
+    w_uint(w, nq, 8);
+    w_uint(w, rl, 1);
+}
+
+uint8_t *zo_fib_stark_prove(uint32_t tl, uint32_t lb, size_t nq, uint32_t rl, const uint32_t *trace,
+                            const uint32_t a0[4], const uint32_t a1[4], const uint32_t claimed[4], zo_channel *chan,
+                            size_t *out_len) {
+    const size_t n = (size_t)1 << tl, N = (size_t)1 << (tl + lb), shift = (size_t)1 << lb;
+    const uint32_t dlog = tl + lb;
+    if (!chan || tl < 1 || rl < lb || rl >= dlog) return NULL;
+    zo_qm31 w = zo_qm31_root_of_unity(tl), w_ev = zo_qm31_root_of_unity(dlog);
+    uint32_t *h = malloc(n * 16), *co = malloc(FIB_M * n * 16), *cw = malloc(FIB_M * N * 16), *qc = malloc(N * 16);
+    uint32_t *g = malloc(N * 16);
+    zo_qm31 *d = malloc(N * sizeof(zo_qm31));
+    uint8_t *leaves = malloc(N * 32), *trees[FIB_M + 1];
+    for (int t = 0; t <= FIB_M; t++) trees[t] = malloc((2 * N - 1) * 32);
+    /* 1. trace: interpolate over H, evaluate on D, commit (stark.zig:343-414) */
+    zo_qm31 x = Q_ONE;
+    for (size_t i = 0; i < n; i++) { st4(h + 4 * i, x); x = zo_qm31_mul(x, w); }
+    d[0] = ld4(ZO_FRI_OFFSET);
+    for (size_t i = 1; i < N; i++) d[i] = zo_qm31_mul(d[i - 1], w_ev);
+    uint8_t roots[FIB_M][32], qroot[32];
+    for (int j = 0; j < FIB_M; j++) {
+        zo_poly_interpolate(h, trace + 4 * n * j, n, co + 4 * n * j);
+        for (size_t i = 0; i < N; i++) {
+            zo_poly_eval(co + 4 * n * j, n, d[i].v, cw + 4 * (N * j + i));
+            zo_hash_qm31(cw + 4 * (N * j + i), leaves + 32 * i);
+        }
+        zo_merkle_build(leaves, N, trees[j]);
+        memcpy(roots[j], trees[j] + (2 * N - 2) * 32, 32);
+        zo_channel_absorb(chan, roots[j], 32);
+    }
+    /* 3. transition weights, boundary assertions, boundary weights (stark.zig:470-483) */
+    zo_qm31 al[2], be[3], val[3], pk[3];
+    for (int k = 0; k < 2; k++) al[k] = sample_q(chan);
+    size_t col[3], step[3];
+    fib_boundary(a0, a1, claimed, n, col, step, val);
+    for (int k = 0; k < 3; k++) be[k] = sample_q(chan);
+    for (int k = 0; k < 3; k++) pk[k] = zo_qm31_pow(w, step[k]);
+    const zo_qm31 last = ld4(h + 4 * (n - 1));
+    /* 4. quotient on D (stark.zig:485-531) */
+    for (size_t i = 0; i < N; i++) {
+        zo_qm31 zh = zo_qm31_sub(zo_qm31_pow(d[i], n), Q_ONE);
+        size_t ni = (i + shift) % N;
+        zo_qm31 cur[2] = {ld4(cw + 4 * i), ld4(cw + 4 * (N + i))}, nxt[2] = {ld4(cw + 4 * ni), ld4(cw + 4 * (N + ni))};
+        zo_qm31 hv = fib_composition(d[i], cur, nxt, last, zh, al, be, col, pk, val);
+        st4(qc + 4 * i, zo_qm31_mul(hv, zo_qm31_inv(zh)));
+        zo_hash_qm31(qc + 4 * i, leaves + 32 * i);
+    }
+    zo_merkle_build(leaves, N, trees[FIB_M]);
+    memcpy(qroot, trees[FIB_M] + (2 * N - 2) * 32, 32);
+    zo_channel_absorb(chan, qroot, 32);
+    /* 5. DEEP evaluations and weights (stark.zig:540-566, computeQuotientAt :793-846) */
+    zo_qm31 z = sample_q(chan), wz = zo_qm31_mul(z, w), de[FIB_NDEEP], ga[FIB_NDEEP];
+    for (int j = 0; j < FIB_M; j++) {
+        zo_poly_eval(co + 4 * n * j, n, z.v, de[j].v);
+        zo_poly_eval(co + 4 * n * j, n, wz.v, de[FIB_M + j].v);
+    }
+    {
+        zo_qm31 zh = zo_qm31_sub(zo_qm31_pow(z, n), Q_ONE);
+        zo_qm31 cur[2] = {de[0], de[1]}, nxt[2] = {de[2], de[3]};
+        de[4] = zo_qm31_mul(fib_composition(z, cur, nxt, last, zh, al, be, col, pk, val), zo_qm31_inv(zh));
+    }
+    for (int k = 0; k < FIB_NDEEP; k++) absorb_qm31(chan, de[k]);
+    for (int k = 0; k < FIB_NDEEP; k++) ga[k] = sample_q(chan);
+    /* DEEP combined codeword (stark.zig:568-580) */
+    for (size_t i = 0; i < N; i++) {
+        zo_qm31 inv_dz = zo_qm31_inv(zo_qm31_sub(d[i], z)), gv = Q_ZERO;
+        size_t ni = (i + shift) % N;
+        for (int j = 0; j < FIB_M; j++) {
+            gv = zo_qm31_add(gv, zo_qm31_mul(ga[j], zo_qm31_mul(zo_qm31_sub(ld4(cw + 4 * (N * j + i)), de[j]), inv_dz)));
+            gv = zo_qm31_add(gv, zo_qm31_mul(ga[FIB_M + j],
+                                             zo_qm31_mul(zo_qm31_sub(ld4(cw + 4 * (N * j + ni)), de[FIB_M + j]), inv_dz)));
+        }
+        gv = zo_qm31_add(gv, zo_qm31_mul(ga[4], zo_qm31_mul(zo_qm31_sub(ld4(qc + 4 * i), de[4]), inv_dz)));
+        st4(g + 4 * i, gv);
+    }
+    /* FRI on g: friParams = {log_size = trace_log, log_blowup, num_queries, remainder_log} for FibAir */
+    const uint32_t L = dlog - rl;
+    const size_t rc = (size_t)1 << (rl - lb), per_q = zo_fri_query_path_bytes(dlog, rl);
+    uint8_t *froots = malloc(32 * L), *fpaths = malloc(per_q * nq + 1);
+    uint32_t *frem = malloc(16 * rc), *fvals = malloc(16 * 2 * L * nq + 16);
+    size_t *fidx = malloc(sizeof(size_t) * (nq + 1));
+    int rcode = zo_fri_prove(g, dlog, lb, rl, nq, chan, froots, frem, fidx, fvals, fpaths);
+    uint8_t *out = NULL;
+    if (rcode == 0) {
+        /* serialize GenericStark.Proof (stark.zig:110-123) */
+        for (int pass = 0; pass < 2; pass++) {
+            wire_buf wb = {pass ? out : NULL, 0};
+            w_stark_params(&wb, tl, lb, nq, rl);
+            w_uint(&wb, FIB_M, 8);
+            for (int j = 0; j < FIB_M; j++) w_bytes(&wb, roots[j], 32);
+            w_uint(&wb, 0, 1); /* preprocessed_roots = null */
+            w_uint(&wb, 0, 1); /* accumulator_roots = null */
+            w_bytes(&wb, qroot, 32);
+            w_uint(&wb, FIB_NDEEP, 8);
+            for (int k = 0; k < FIB_NDEEP; k++) w_qm31(&wb, de[k].v);
+            size_t fl = zo_fri_proof_serialize(tl, lb, rl, nq, froots, frem, fidx, fvals, fpaths,
+                                               pass ? out + wb.n : NULL);
+            wb.n += fl;
+            w_uint(&wb, nq, 8);
+            for (size_t q = 0; q < nq; q++) {
+                size_t p0 = fidx[q], pn = (p0 + shift) % N;
+                const size_t at[FIB_NDEEP] = {p0, p0, pn, pn, p0};
+                const int tr[FIB_NDEEP] = {0, 1, 0, 1, FIB_M};
+                w_uint(&wb, p0, 8);
+                w_uint(&wb, FIB_NDEEP, 8);
+                for (int k = 0; k < FIB_NDEEP; k++) {
+                    const uint32_t *src = tr[k] == FIB_M ? qc + 4 * at[k] : cw + 4 * (N * tr[k] + at[k]);
+                    w_qm31(&wb, src);
+                }
+                w_uint(&wb, FIB_NDEEP, 8);
+                uint8_t path[32 * 32];
+                for (int k = 0; k < FIB_NDEEP; k++) {
+                    zo_merkle_open(trees[tr[k]], N, at[k], path);
+                    w_uint(&wb, dlog, 8);
+                    w_bytes(&wb, path, (size_t)dlog * 32);
+                }
+            }
+            if (!pass) { out = malloc(wb.n); *out_len = wb.n; }
+        }
+    }
+    free(h); free(co); free(cw); free(qc); free(g); free(d); free(leaves);
+    for (int t = 0; t <= FIB_M; t++) free(trees[t]);
+    free(froots); free(fpaths); free(frem); free(fvals); free(fidx);
+    return out;
+}
+
+void zo_free(void *p) { free(p); }
+
+/* ---- verify (stark.zig:659-786) over the wire bytes ---- */
+typedef struct { const uint8_t *b; size_t n, pos; int bad; } wire_rd;
+static const uint8_t *r_take(wire_rd *r, size_t k) {
+    if (r->bad || r->pos + k > r->n) { r->bad = 1; return NULL; }
+    const uint8_t *p = r->b + r->pos;
+    r->pos += k;
+    return p;
+}
+static uint64_t r_uint(wire_rd *r, unsigned k) {
+    const uint8_t *p = r_take(r, k);
+    uint64_t v = 0;
+    if (p) for (unsigned i = 0; i < k; i++) v |= (uint64_t)p[i] << (8 * i);
+    return v;
+}
+static zo_qm31 r_qm31(wire_rd *r) {
+    zo_qm31 x;
+    for (int k = 0; k < 4; k++) x.v[k] = (uint32_t)r_uint(r, 4);
+    return x;
+}
+
+int zo_fib_stark_verify(const uint8_t *proof, size_t len, const uint32_t a0[4], const uint32_t a1[4],
+                        const uint32_t claimed[4], zo_channel *chan) {
+    wire_rd r = {proof, len, 0, 0};
+    const uint32_t tl = (uint32_t)r_uint(&r, 1), lb = (uint32_t)r_uint(&r, 1);
+    const size_t nq = (size_t)r_uint(&r, 8);
+    const uint32_t rl = (uint32_t)r_uint(&r, 1);
+    if (r.bad || tl < 1 || tl + lb > 24 || rl < lb || rl >= tl + lb || nq > 4096) return 0;
+    const size_t n = (size_t)1 << tl, N = (size_t)1 << (tl + lb), shift = (size_t)1 << lb;
+    const uint32_t dlog = tl + lb, L = dlog - rl;
+    const size_t rc = (size_t)1 << (rl - lb);
+    if (r_uint(&r, 8) != FIB_M) return 0;
+    const uint8_t *roots = r_take(&r, 32 * FIB_M);
+    if (r_uint(&r, 1) != 0 || r_uint(&r, 1) != 0) return 0; /* no preprocessed / accumulator roots */
+    const uint8_t *qroot = r_take(&r, 32);
+    if (r_uint(&r, 8) != FIB_NDEEP) return 0;
+    zo_qm31 de[FIB_NDEEP];
+    for (int k = 0; k < FIB_NDEEP; k++) de[k] = r_qm31(&r);
+    /* nested Fri.Proof */
+    if (r_uint(&r, 1) != tl || r_uint(&r, 1) != lb || r_uint(&r, 8) != nq || r_uint(&r, 1) != rl) return 0;
+    if (r_uint(&r, 8) != L) return 0;
+    const uint8_t *froots = r_take(&r, 32 * (size_t)L);
+    if (r_uint(&r, 8) != rc) return 0;
+    uint32_t *frem = malloc(16 * rc), *fvals = malloc(16 * 2 * (size_t)L * nq + 16);
+    const size_t per_q = zo_fri_query_path_bytes(dlog, rl);
+    uint8_t *fpaths = malloc(per_q * nq + 1);
+    size_t *fidx = malloc(sizeof(size_t) * (nq + 1));
+    for (size_t k = 0; k < rc; k++) st4(frem + 4 * k, r_qm31(&r));
+    if (r_uint(&r, 8) != nq) r.bad = 1;
+    for (size_t q = 0; q < nq && !r.bad; q++) {
+        fidx[q] = (size_t)r_uint(&r, 8);
+        if (r_uint(&r, 8) != L) { r.bad = 1; break; }
+        uint8_t *pp = fpaths + q * per_q;
+        for (uint32_t i = 0; i < L && !r.bad; i++) {
+            st4(fvals + 4 * ((q * L + i) * 2), r_qm31(&r));
+            st4(fvals + 4 * ((q * L + i) * 2 + 1), r_qm31(&r));
+            for (int s = 0; s < 2 && !r.bad; s++) {
+                if (r_uint(&r, 8) != dlog - i) { r.bad = 1; break; }
+                const uint8_t *p = r_take(&r, 32 * (size_t)(dlog - i));
+                if (p) memcpy(pp, p, 32 * (size_t)(dlog - i));
+                pp += 32 * (size_t)(dlog - i);
+            }
+        }
+    }
+    int ok = !r.bad;
+    /* replay the transcript (stark.zig:676-705) */
+    zo_qm31 al[2], be[3], val[3], pk[3], ga[FIB_NDEEP];
+    size_t col[3], step[3];
+    const zo_qm31 w = zo_qm31_root_of_unity(tl), w_ev = zo_qm31_root_of_unity(dlog);
+    if (ok) {
+        for (int j = 0; j < FIB_M; j++) zo_channel_absorb(chan, roots + 32 * j, 32);
+        for (int k = 0; k < 2; k++) al[k] = sample_q(chan);
+        fib_boundary(a0, a1, claimed, n, col, step, val);
+        for (int k = 0; k < 3; k++) be[k] = sample_q(chan);
+        for (int k = 0; k < 3; k++) pk[k] = zo_qm31_pow(w, step[k]);
+        zo_channel_absorb(chan, qroot, 32);
+        zo_qm31 z = sample_q(chan);
+        for (int k = 0; k < FIB_NDEEP; k++) absorb_qm31(chan, de[k]);
+        for (int k = 0; k < FIB_NDEEP; k++) ga[k] = sample_q(chan);
+        ok = zo_fri_verify(dlog, lb, rl, nq, chan, froots, frem, fidx, fvals, fpaths);
+        const zo_qm31 last = zo_qm31_pow(w, n - 1);
+        if (ok && r_uint(&r, 8) != nq) ok = 0;
+        for (size_t q = 0; q < nq && ok; q++) {
+            size_t p0 = (size_t)r_uint(&r, 8);
+            if (p0 != fidx[q] || r_uint(&r, 8) != FIB_NDEEP) { ok = 0; break; }
+            zo_qm31 v[FIB_NDEEP];
+            for (int k = 0; k < FIB_NDEEP; k++) v[k] = r_qm31(&r);
+            if (r_uint(&r, 8) != FIB_NDEEP) { ok = 0; break; }
+            const size_t pn = (p0 + shift) % N, at[FIB_NDEEP] = {p0, p0, pn, pn, p0};
+            for (int k = 0; k < FIB_NDEEP && ok; k++) {
+                if (r_uint(&r, 8) != dlog) { ok = 0; break; }
+                const uint8_t *path = r_take(&r, 32 * (size_t)dlog);
+                uint8_t leaf[32];
+                zo_hash_qm31(v[k].v, leaf);
+                const uint8_t *root = k == 4 ? qroot : roots + 32 * (k % 2);
+                if (!path || !zo_merkle_verify(root, at[k], leaf, path, dlog)) ok = 0;
+            }
+            if (!ok) break;
+            zo_qm31 x0 = zo_qm31_mul(ld4(ZO_FRI_OFFSET), zo_qm31_pow(w_ev, p0));
+            /* DEEP identity against the FRI layer-0 leaf (stark.zig:736-751) */
+            zo_qm31 inv_dz = zo_qm31_inv(zo_qm31_sub(x0, z)), gv = Q_ZERO;
+            for (int j = 0; j < FIB_M; j++) {
+                gv = zo_qm31_add(gv, zo_qm31_mul(ga[j], zo_qm31_mul(zo_qm31_sub(v[j], de[j]), inv_dz)));
+                gv = zo_qm31_add(gv, zo_qm31_mul(ga[FIB_M + j], zo_qm31_mul(zo_qm31_sub(v[FIB_M + j], de[FIB_M + j]), inv_dz)));
+            }
+            gv = zo_qm31_add(gv, zo_qm31_mul(ga[4], zo_qm31_mul(zo_qm31_sub(v[4], de[4]), inv_dz)));
+            const size_t p1 = p0 % (N / 2);
+            zo_qm31 leafv = ld4(fvals + 4 * ((q * L) * 2 + (p0 == p1 ? 0 : 1)));
+            if (memcmp(gv.v, leafv.v, 16) != 0) { ok = 0; break; }
+            /* Hc(x0) == Z_H(x0) * Q(x0) (stark.zig:753-779) */
+            zo_qm31 zh = zo_qm31_sub(zo_qm31_pow(x0, n), Q_ONE);
+            zo_qm31 cur[2] = {v[0], v[1]}, nxt[2] = {v[2], v[3]};
+            zo_qm31 hv = fib_composition(x0, cur, nxt, last, zh, al, be, col, pk, val);
+            zo_qm31 rhs = zo_qm31_mul(zh, v[4]);
+            if (memcmp(hv.v, rhs.v, 16) != 0) ok = 0;
+        }
+        if (ok && (r.bad || r.pos != r.n)) ok = 0; /* truncated or trailing bytes */
+    }
+    free(frem); free(fvals); free(fpaths); free(fidx);
+    return ok;
+}

@@ -167,6 +167,18 @@ size_t zo_fri_proof_serialize(uint32_t log_size, uint32_t log_blowup, uint32_t r
                               const uint8_t *roots, const uint32_t *remainder, const size_t *query_idx,
                               const uint32_t *query_vals, const uint8_t *query_paths, uint8_t *out);
 
+/* ---- GenericStark(FibSeededAir) prove / verify (src/m31/stark.zig) ------------ */
+/* trace: 2 QM31 columns of 2^trace_log values; a0, a1, claimed: boundary values of
+ * (0,0), (1,0), (0,n-1).  Returns malloc'd GenericStark.Proof wire bytes (free with
+ * zo_free), NULL on error.  Channel state advances as the prover's. */
+uint8_t *zo_fib_stark_prove(uint32_t trace_log, uint32_t log_blowup, size_t num_queries, uint32_t remainder_log,
+                            const uint32_t *trace, const uint32_t a0[4], const uint32_t a1[4],
+                            const uint32_t claimed[4], zo_channel *chan, size_t *out_len);
+/* 1 accept, 0 reject (including malformed, truncated or trailing bytes) */
+int zo_fib_stark_verify(const uint8_t *proof, size_t len, const uint32_t a0[4], const uint32_t a1[4],
+                        const uint32_t claimed[4], zo_channel *chan);
+void zo_free(void *p);
+
 /* ---- seeded synthetic inputs ------------------------------------------------ */
 /* SplitMix64 stream; M31 = 31 random bits with 0x7FFFFFFF rejected */
 void zo_fill_m31(uint64_t seed, uint32_t *out, size_t n);

@@ -102,6 +102,11 @@ def _declare(L):
     L.zo_fill_m31.argtypes = [ctypes.c_uint64, u32p, sz]
     L.zo_trace_commit.argtypes = [u32p, sz, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p, u32p, ctypes.c_char_p]
     L.zo_fib_trace.argtypes = [ctypes.c_uint32, ctypes.c_uint32, sz, u32p]
+    L.zo_fib_stark_prove.restype = ctypes.c_void_p
+    L.zo_fib_stark_prove.argtypes = [ctypes.c_uint32, ctypes.c_uint32, sz, ctypes.c_uint32, u32p, u32p, u32p, u32p,
+                                     ctypes.c_void_p, ctypes.POINTER(sz)]
+    L.zo_fib_stark_verify.argtypes = [u8p, sz, u32p, u32p, u32p, ctypes.c_void_p]
+    L.zo_free.argtypes = [ctypes.c_void_p]
     L.zo_wire_golden_example.restype = sz
     L.zo_wire_golden_example.argtypes = [ctypes.c_char_p]
     L.zo_fri_proof_serialize.restype = sz
@@ -507,3 +512,23 @@ def fri_proof_bytes(proof, log_blowup, remainder_log):
     out = ctypes.create_string_buffer(n)
     lib().zo_fri_proof_serialize(*args, out)
     return out.raw
+
+
+def fib_stark_prove(trace, log_blowup, num_queries, remainder_log, a0, a1, claimed, channel):
+    """GenericStark(FibSeededAir).prove -> GenericStark.Proof wire bytes."""
+    trace = u32(trace)
+    tl = trace.shape[1].bit_length() - 1
+    q = lambda v: _p(u32(v).reshape(4))
+    n = sz()
+    ptr = lib().zo_fib_stark_prove(tl, log_blowup, num_queries, remainder_log, _p(trace.reshape(-1)), q(a0), q(a1),
+                                   q(claimed), channel.h, ctypes.byref(n))
+    assert ptr, "prove failed"
+    try:
+        return ctypes.string_at(ptr, n.value)
+    finally:
+        lib().zo_free(ptr)
+
+
+def fib_stark_verify(proof, a0, a1, claimed, channel):
+    q = lambda v: _p(u32(v).reshape(4))
+    return bool(lib().zo_fib_stark_verify(bytes(proof), len(proof), q(a0), q(a1), q(claimed), channel.h))

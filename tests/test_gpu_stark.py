@@ -81,3 +81,39 @@ def test_trace_commit_large_size_property(zctx, torch_dev):
         assert np.array_equal(cw[0, i], O.poly_eval(coeffs, d)), i
     leaves = b"".join(O.hash_qm31(v) for v in cw[0])
     assert O.tree_root(O.merkle_build(leaves)) == roots[0]
+
+
+@pytest.mark.parametrize("tl,beta,nq,rl", [(4, 3, 12, 3), (6, 2, 8, 3), (9, 1, 5, 2)])
+def test_fib_stark_proof_matches_oracle(zctx, torch_dev, tl, beta, nq, rl):
+    """The whole GenericStark(FibAir) proof, byte for byte: trace roots, quotient
+    root, DEEP evaluations, every FRI layer root, remainder, query indices,
+    revealed values and paths; the oracle's verifier accepts it."""
+    a1 = 5 + tl
+    trace = Z.fib_trace(1, a1, tl)
+    a0v, a1v = np.array([1, 0, 0, 0], np.uint32), np.array([a1, 0, 0, 0], np.uint32)
+    claimed = trace[0, -1]
+    d_tr = to_dev(trace.reshape(-1), torch_dev)
+    ch = Z.Channel(b"zig-stark:fib")
+    got = zctx.fib_stark_prove_dev(d_tr.data_ptr(), tl, beta, nq, rl, a0v, a1v, claimed, ch)
+    och = O.Channel(b"zig-stark:fib")
+    want = O.fib_stark_prove(trace, beta, nq, rl, a0v, a1v, claimed, och)
+    assert got == want
+    assert ch.sample_bytes(32) == och.sample_bytes(32)
+    assert O.fib_stark_verify(got, a0v, a1v, claimed, O.Channel(b"zig-stark:fib"))
+
+
+def test_fib_stark_configuration_proof(zctx, torch_dev):
+    """The STARK configuration: seeded Fibonacci AIR, 2 columns x 2^12, blowup 4,
+    20 queries, remainder 2^3: the GPU proof equals the oracle's and verifies."""
+    a1 = int(O.fill_m31(1, 1)[0])
+    trace = Z.fib_trace(1, a1, 12)
+    a0v, a1v = np.array([1, 0, 0, 0], np.uint32), np.array([a1, 0, 0, 0], np.uint32)
+    claimed = trace[0, -1]
+    d_tr = to_dev(trace.reshape(-1), torch_dev)
+    got = zctx.fib_stark_prove_dev(d_tr.data_ptr(), 12, 2, 20, 3, a0v, a1v, claimed, Z.Channel(b"zig-stark:fib"))
+    want = O.fib_stark_prove(trace, 2, 20, 3, a0v, a1v, claimed, O.Channel(b"zig-stark:fib"))
+    assert got == want
+    assert O.fib_stark_verify(got, a0v, a1v, claimed, O.Channel(b"zig-stark:fib"))
+    bad = np.array(claimed)
+    bad[0] = (int(bad[0]) + 1) % P
+    assert not O.fib_stark_verify(got, a0v, a1v, bad, O.Channel(b"zig-stark:fib"))

@@ -115,6 +115,8 @@ def _declare(L):
     L.zs_commit_batch_dev.argtypes = [vp, ctypes.c_void_p, sz, u32, u32, u32, u32]
     L.zs_fri_prove_dev.argtypes = [vp, vp, u32, u32, u32, sz, vp, HostAllocator, ctypes.POINTER(ctypes.c_void_p),
                                    ctypes.POINTER(sz)]
+    L.zs_fib_stark_prove_dev.argtypes = [vp, vp, u32, u32, sz, u32, u32p, u32p, u32p, vp, HostAllocator,
+                                         ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(sz)]
     L.zs_free.argtypes = [HostAllocator, ctypes.c_void_p, sz]
     L.zs_free.restype = None
     L.zs_autotune_lde.argtypes = [vp, u32, u32, u32, u32p]
@@ -137,7 +139,7 @@ def _declare(L):
                  "zs_merkle_open", "zs_tree_read_level", "zs_fri_commit_dev", "zs_fri_commit", "zs_fri_query",
                  "zs_fri_read_layer", "zs_fill_m31_dev", "zs_hip_profile", "zs_hip_profile_read",
                  "zs_commit_job_dev", "zs_commit_batch_dev", "zs_server_submit", "zs_server_wait",
-                 "zs_server_memory", "zs_server_poll", "zs_qm31_trace_commit_dev", "zs_fib_trace", "zs_autotune_lde", "zs_fri_prove_dev"]:
+                 "zs_server_memory", "zs_server_poll", "zs_qm31_trace_commit_dev", "zs_fib_trace", "zs_autotune_lde", "zs_fri_prove_dev", "zs_fib_stark_prove_dev"]:
         getattr(L, name).restype = i32
 
 
@@ -512,6 +514,20 @@ class Context:
         ptr, n = ctypes.c_void_p(), sz()
         self.check(lib().zs_fri_prove_dev(self.h, d_codeword, log_size, log_blowup, remainder_log, num_queries,
                                           channel.h, HOST, ctypes.byref(ptr), ctypes.byref(n)))
+        try:
+            return ctypes.string_at(ptr.value, n.value)
+        finally:
+            lib().zs_free(HOST, ptr, n)
+
+    def fib_stark_prove_dev(self, d_trace, trace_log, log_blowup, num_queries, remainder_log, a0, a1, claimed,
+                            channel):
+        """GenericStark(FibAir) proof of a device trace (2 QM31 columns); returns the wire bytes."""
+        q = lambda v: _p(_u32(v).reshape(4))
+        av, bv, cv = _u32(a0).reshape(4), _u32(a1).reshape(4), _u32(claimed).reshape(4)
+        ptr, n = ctypes.c_void_p(), sz()
+        self.check(lib().zs_fib_stark_prove_dev(self.h, d_trace, trace_log, log_blowup, num_queries, remainder_log,
+                                                _p(av), _p(bv), _p(cv), channel.h, HOST, ctypes.byref(ptr),
+                                                ctypes.byref(n)))
         try:
             return ctypes.string_at(ptr.value, n.value)
         finally:
