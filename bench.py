@@ -379,8 +379,8 @@ def main():
     tune = None
     if not args.no_autotune:
         t0 = time.perf_counter()
-        el, cpw, us = ctx.autotune_lde(n_cols, lg, beta)
-        tune = {"tile_elems_log": el, "cols_per_wg": cpw, "lde_us": us,
+        t = ctx.autotune_lde(n_cols, lg, beta)
+        tune = {"interp": list(t["interp"]), "mid": list(t["mid"]), "eval": list(t["eval"]), "lde_us": t["us"],
                 "seconds": round(time.perf_counter() - t0, 2)}
 
     def timed(fn, reps):

@@ -118,14 +118,18 @@ int32_t zs_m31_lde_commit_dev(zs_hip_ctx *ctx, const uint32_t *d_cols, uint32_t 
 int32_t zs_m31_lde_dev(zs_hip_ctx *ctx, const uint32_t *d_cols, uint32_t n_cols, uint32_t log_n, uint32_t log_blowup,
                        uint32_t *d_lde);
 /* Launch-configuration autotuner (no reference counterpart; the reference has
- * no GPU LDE).  Times zs_m31_lde_dev for one shape under each candidate
+ * no GPU LDE).  Runs zs_m31_lde_dev for one shape under each candidate
  * configuration of the register-group passes (2^12..2^14 elements per
  * workgroup x 1, 2, 4, 8 columns per workgroup) on synthetic device data, checks
- * that every candidate yields the same row-tree root, and keeps the fastest for
- * that shape (log_n, log_blowup, n_cols rounded up to a power of two): later
- * LDE calls of the shape on this context and its batch workers use it.
- * out (may be NULL): tile elements log2, columns per workgroup, microseconds. */
-int32_t zs_autotune_lde(zs_hip_ctx *ctx, uint32_t n_cols, uint32_t log_n, uint32_t log_blowup, uint32_t out[3]);
+ * that every candidate yields the same row-tree root, times each pass kernel
+ * with events, and keeps for each kind of pass (interpolation passes, the fused
+ * middle kernel, evaluation passes) the candidate it ran fastest under, for that
+ * shape (log_n, log_blowup, n_cols rounded up to a power of two): later LDE
+ * calls of the shape on this context and its batch workers use it.
+ * out (may be NULL): tile elements log2 and columns per workgroup of the
+ * interpolation, middle and evaluation passes, then the microseconds of one
+ * LDE under the chosen combination. */
+int32_t zs_autotune_lde(zs_hip_ctx *ctx, uint32_t n_cols, uint32_t log_n, uint32_t log_blowup, uint32_t out[7]);
 /* Row leaves alone: d_leaves[i] = hashM31s(row i), 2^log_rows digests (level 0 only). */
 int32_t zs_m31_row_leaves_dev(zs_hip_ctx *ctx, const uint32_t *d_cols, uint32_t n_cols, uint32_t log_rows,
                               size_t col_stride, uint8_t *d_leaves);

@@ -132,8 +132,11 @@ def test_autotuned_lde_is_bit_exact(zctx, torch_dev, n_cols, lg, beta):
     """The autotuner keeps a configuration per shape; LDE and root stay equal to the oracle's."""
     import torch
     cols = O.fill_m31(lg * 31 + n_cols, n_cols << lg).reshape(n_cols, -1)
-    el, cpw, us = zctx.autotune_lde(n_cols, lg, beta)
-    assert 12 <= el <= 14 and cpw in (1, 2, 4, 8) and us > 0
+    tuned = zctx.autotune_lde(n_cols, lg, beta)
+    for role in ("interp", "mid", "eval"):
+        el, cpw = tuned[role]
+        assert 12 <= el <= 14 and cpw in (1, 2, 4, 8)
+    assert tuned["us"] > 0
     d_cols = to_dev(cols.reshape(-1), torch_dev)
     N = 1 << (lg + beta)
     d_lde = torch.empty(n_cols * N, dtype=torch.int32, device=torch_dev)

@@ -465,11 +465,13 @@ class Context:
         self.check(lib().zs_fill_m31_dev(self.h, seed, d_out, n))
 
     def autotune_lde(self, n_cols, log_n, log_blowup):
-        """Pick the fastest launch configuration for this LDE shape (kept on the
-        context); returns (tile elements log2, columns per workgroup, microseconds)."""
-        out = np.zeros(3, np.uint32)
+        """Pick the fastest launch configuration of each kind of LDE pass for this
+        shape (kept on the context); returns {"interp": (tile elements log2,
+        columns per workgroup), "mid": (...), "eval": (...), "us": one LDE}."""
+        out = np.zeros(7, np.uint32)
         self.check(lib().zs_autotune_lde(self.h, n_cols, log_n, log_blowup, _p(out)))
-        return tuple(int(x) for x in out)
+        o = [int(x) for x in out]
+        return {"interp": (o[0], o[1]), "mid": (o[2], o[3]), "eval": (o[4], o[5]), "us": o[6]}
 
     # ---- GenericStark trace commitment (stark.zig:343-414) ----
     def trace_commit_dev(self, d_cols, n_cols, trace_log, log_blowup, d_codewords, d_nodes, channel=None):

@@ -277,7 +277,7 @@ int32_t zs_m31_lde_dev(zs_hip_ctx *h, const uint32_t *d_cols, uint32_t n_cols, u
         DeviceCaches &dc = *ctx->caches;
         std::lock_guard<std::mutex> lk(dc.mu);
         auto it = dc.tuned.find(lde_shape_key(n_cols, log_n, log_blowup));
-        ctx->tile = it != dc.tuned.end() ? it->second : default_tile_config();
+        ctx->lde_tiles = it != dc.tuned.end() ? it->second : default_lde_tiles();
     }
     size_t n = size_t(1) << log_n, N = size_t(1) << (log_n + log_blowup);
     size_t coef_words;
@@ -288,10 +288,14 @@ int32_t zs_m31_lde_dev(zs_hip_ctx *h, const uint32_t *d_cols, uint32_t n_cols, u
                       big_words ? ctx->scratch[2].as<uint32_t>() : nullptr);
 }
 
-int32_t zs_autotune_lde(zs_hip_ctx *h, uint32_t n_cols, uint32_t log_n, uint32_t log_blowup, uint32_t out[3]) {
+// Times each candidate configuration with per-kernel events and keeps, for each
+// kind of LDE pass, the candidate under which that pass's kernels ran fastest: the
+// passes exchange data in one fixed layout, so their choices are independent.
+int32_t zs_autotune_lde(zs_hip_ctx *h, uint32_t n_cols, uint32_t log_n, uint32_t log_blowup, uint32_t out[7]) {
     if (!h || n_cols == 0 || log_n < 1 || log_n + log_blowup > 28) return ZS_ERR_INPUT;
     Ctx *ctx = &h->c;
     ZS_HIP(ctx, hipSetDevice(ctx->device));
+    ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     const uint32_t LT = log_n + log_blowup;
     const size_t n = size_t(1) << log_n, N = size_t(1) << LT;
     DevBuf cols, lde, nodes;
@@ -300,24 +304,29 @@ int32_t zs_autotune_lde(zs_hip_ctx *h, uint32_t n_cols, uint32_t log_n, uint32_t
     ZS_TRY(dev_alloc(ctx, nodes, tree_words(LT) * 4));
     ZS_TRY(launch_fill_m31(ctx, 0x5eed, cols.as<uint32_t>(), n * n_cols));
     const uint64_t key = lde_shape_key(n_cols, log_n, log_blowup);
-    hipEvent_t a, b;
-    ZS_HIP(ctx, hipEventCreate(&a));
-    ZS_HIP(ctx, hipEventCreate(&b));
-    TileConfig best = default_tile_config();
-    float best_ms = 0;
+    // the caller's profiling state is set aside while the candidates are timed
+    const bool was_profiling = ctx->profiling;
+    std::vector<ProfRec> saved_prof;
+    saved_prof.swap(ctx->prof);
+    LdeTiles best = default_lde_tiles();
+    float best_ms[3] = {0, 0, 0};
     uint8_t ref_root[32];
     bool have_ref = false;
     int32_t rc = ZS_OK;
+    auto set_tuned = [&](const LdeTiles &t) {
+        std::lock_guard<std::mutex> lk(ctx->caches->mu);
+        ctx->caches->tuned[key] = t;
+    };
     for (uint32_t el = 12; el <= 14 && rc == ZS_OK; el++) {
         for (uint32_t cpw = 1; cpw <= 8 && rc == ZS_OK; cpw *= 2) {
             TileConfig cand;
             cand.elems_log = el;
             cand.cols_per_wg = cpw;
-            {
-                std::lock_guard<std::mutex> lk(ctx->caches->mu);
-                ctx->caches->tuned[key] = cand;
-            }
+            LdeTiles all;
+            for (auto &r : all.role) r = cand;
+            set_tuned(all);
             // one run for the root (every candidate must give the same one), then the timed runs
+            ctx->profiling = false;
             rc = zs_m31_lde_commit_dev(h, cols.as<uint32_t>(), n_cols, log_n, log_blowup, lde.as<uint32_t>(),
                                        nodes.as<uint8_t>(), 0);
             if (rc == ZS_ERR_INPUT) {  // a configuration this shape cannot launch (LDS): not a candidate
@@ -341,34 +350,64 @@ int32_t zs_autotune_lde(zs_hip_ctx *h, uint32_t n_cols, uint32_t log_n, uint32_t
                 break;
             }
             const int reps = 3;
-            (void)hipEventRecord(a, ctx->stream);
+            ctx->profiling = true;
             for (int r = 0; r < reps && rc == ZS_OK; r++)
                 rc = zs_m31_lde_dev(h, cols.as<uint32_t>(), n_cols, log_n, log_blowup, lde.as<uint32_t>());
-            (void)hipEventRecord(b, ctx->stream);
-            if (rc || hipEventSynchronize(b) != hipSuccess) {
+            ctx->profiling = false;
+            if (rc || hipStreamSynchronize(ctx->stream) != hipSuccess) {
                 rc = rc ? rc : ZS_ERR_DEVICE;
                 break;
             }
-            float ms = 0;
-            (void)hipEventElapsedTime(&ms, a, b);
-            ms /= reps;
-            if (!best_ms || ms < best_ms) {
-                best_ms = ms;
-                best = cand;
+            float role_ms[3] = {0, 0, 0};
+            for (auto &r : ctx->prof) {
+                float t = 0;
+                if (r.role >= 0 && r.role < 3 && hipEventElapsedTime(&t, r.a, r.b) == hipSuccess) role_ms[r.role] += t;
+            }
+            prof_clear(ctx);
+            for (int k = 0; k < 3; k++) {
+                if (!best_ms[k] || role_ms[k] < best_ms[k]) {
+                    best_ms[k] = role_ms[k];
+                    best.role[k] = cand;
+                }
             }
         }
     }
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
+    // the chosen combination, timed as a whole
+    float total_ms = 0;
+    if (rc == ZS_OK) {
+        set_tuned(best);
+        hipEvent_t a, b;
+        ZS_HIP(ctx, hipEventCreate(&a));
+        ZS_HIP(ctx, hipEventCreate(&b));
+        const int reps = 3;
+        rc = zs_m31_lde_dev(h, cols.as<uint32_t>(), n_cols, log_n, log_blowup, lde.as<uint32_t>());
+        (void)hipEventRecord(a, ctx->stream);
+        for (int r = 0; r < reps && rc == ZS_OK; r++)
+            rc = zs_m31_lde_dev(h, cols.as<uint32_t>(), n_cols, log_n, log_blowup, lde.as<uint32_t>());
+        (void)hipEventRecord(b, ctx->stream);
+        if (rc == ZS_OK && hipEventSynchronize(b) == hipSuccess) {
+            (void)hipEventElapsedTime(&total_ms, a, b);
+            total_ms /= reps;
+        } else if (rc == ZS_OK) {
+            rc = ZS_ERR_DEVICE;
+        }
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+    }
+    prof_clear(ctx);
+    ctx->prof.swap(saved_prof);
+    ctx->profiling = was_profiling;
     {
         std::lock_guard<std::mutex> lk(ctx->caches->mu);
         if (rc == ZS_OK) ctx->caches->tuned[key] = best;
         else ctx->caches->tuned.erase(key);
     }
     if (rc == ZS_OK && out) {
-        out[0] = best.elems_log;
-        out[1] = best.cols_per_wg;
-        out[2] = (uint32_t)(best_ms * 1000.0f);
+        for (int k = 0; k < 3; k++) {
+            out[2 * k] = best.role[k].elems_log;
+            out[2 * k + 1] = best.role[k].cols_per_wg;
+        }
+        out[6] = (uint32_t)(total_ms * 1000.0f);
     }
     return rc;
 }

@@ -427,9 +427,14 @@ static int32_t launch_lde_tiled(Ctx *ctx, const uint32_t *d_in, uint64_t in_stri
     const uint32_t LT = log_n + log_blowup;
     const size_t n = size_t(1) << log_n, N = size_t(1) << LT;
     const size_t np = pf.size();
-    if (np == 1)
+    if (np == 1) {
+        ctx->tile = ctx->lde_tiles.role[LDE_MID];
+        ctx->lde_role = LDE_MID;
         return run_lde_fused(ctx, d_in, in_stride, d_out, out_stride, ti->inv.as<uint32_t>(), tf->fwd.as<uint32_t>(),
                              log_n, log_blowup, 0, log_n, n_cols);
+    }
+    ctx->tile = ctx->lde_tiles.role[LDE_INTERP];
+    ctx->lde_role = LDE_INTERP;
     for (size_t i = 0; i + 1 < np; i++) {
         FftPass p{};
         bool first = i == 0;
@@ -446,8 +451,12 @@ static int32_t launch_lde_tiled(Ctx *ctx, const uint32_t *d_in, uint64_t in_stri
         p.scale = 0;
         ZS_TRY(run_tile_pass(ctx, p, true, n_cols));
     }
+    ctx->tile = ctx->lde_tiles.role[LDE_MID];
+    ctx->lde_role = LDE_MID;
     ZS_TRY(run_lde_fused(ctx, d_coef, n, d_big, N, ti->inv.as<uint32_t>(), tf->fwd.as<uint32_t>(), log_n,
                          log_blowup, pf[0].s_lo, pf[0].C, n_cols));
+    ctx->tile = ctx->lde_tiles.role[LDE_EVAL];
+    ctx->lde_role = LDE_EVAL;
     for (size_t i = 1; i < np; i++) {
         FftPass p{};
         bool last = i + 1 == np;
@@ -468,8 +477,9 @@ static int32_t launch_lde_tiled(Ctx *ctx, const uint32_t *d_in, uint64_t in_stri
     return ZS_OK;
 }
 
-int32_t launch_lde(Ctx *ctx, const uint32_t *d_in, uint64_t in_stride, uint32_t n_cols, uint32_t log_n,
-                   uint32_t log_blowup, uint32_t *d_out, uint64_t out_stride, uint32_t *d_coef, uint32_t *d_big) {
+static int32_t launch_lde_impl(Ctx *ctx, const uint32_t *d_in, uint64_t in_stride, uint32_t n_cols, uint32_t log_n,
+                               uint32_t log_blowup, uint32_t *d_out, uint64_t out_stride, uint32_t *d_coef,
+                               uint32_t *d_big) {
     uint32_t LT = log_n + log_blowup;
     if (log_n < 1 || LT > 28) return ZS_ERR_INPUT;
     if (n_cols == 0) return ZS_OK;
@@ -495,6 +505,8 @@ int32_t launch_lde(Ctx *ctx, const uint32_t *d_in, uint64_t in_stride, uint32_t 
                                            d_coef, d_big, hc_small, hc_big, pi, pf);
     }
     // 1) interpolate: natural evaluations -> interleaved coefficients scaled by 1/n (k-space)
+    ctx->tile = ctx->lde_tiles.role[LDE_INTERP];
+    ctx->lde_role = LDE_INTERP;
     if (log_n < 3) {
         ZS_TRY(run_small(ctx, true, hc_small, d_in, in_stride, d_coef, n, n_cols, true));
     } else {
@@ -519,6 +531,8 @@ int32_t launch_lde(Ctx *ctx, const uint32_t *d_in, uint64_t in_stride, uint32_t 
         }
     }
     // 2) evaluate on the 2^beta quarter transforms of the big canonic domain
+    ctx->tile = ctx->lde_tiles.role[LDE_EVAL];
+    ctx->lde_role = LDE_EVAL;
     const TwiddleTable *tf;
     ZS_TRY(get_twiddles(ctx, hc_big, &tf));
     auto plan = plan_passes(log_n, false, PASS_CMAX);
@@ -577,6 +591,18 @@ __global__ void plain_deinterleave_kernel(const uint32_t *c, uint32_t *ab, uint3
     if (i >= half) return;
     ab[i] = m31_mul(c[2 * i], inv_n);
     ab[half + i] = m31_mul(c[2 * i + 1], inv_n);
+}
+
+// the LDE's passes take their launch shapes from ctx->lde_tiles; the context's
+// transform configuration is left as it was
+int32_t launch_lde(Ctx *ctx, const uint32_t *d_in, uint64_t in_stride, uint32_t n_cols, uint32_t log_n,
+                   uint32_t log_blowup, uint32_t *d_out, uint64_t out_stride, uint32_t *d_coef, uint32_t *d_big) {
+    const TileConfig saved = ctx->tile;
+    const int32_t rc = launch_lde_impl(ctx, d_in, in_stride, n_cols, log_n, log_blowup, d_out, out_stride, d_coef,
+                                       d_big);
+    ctx->tile = saved;
+    ctx->lde_role = -1;
+    return rc;
 }
 
 int32_t launch_lde_plain(Ctx *ctx, const uint32_t *d_in, uint64_t in_stride, uint32_t n_cols, uint32_t log_n,

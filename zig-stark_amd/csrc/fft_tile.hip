@@ -29,6 +29,31 @@ struct TC {
     static constexpr int NG = (C + RB - 1) / RB;  // register groups per pass
 };
 
+// Twiddle placement in LDS.  Within stage ls's region of the heap, the index
+// a lane reads is (lane bits, then k bits that come from its own registers):
+// consecutive lanes would read 2^k words apart, up to 8-way bank conflicts.
+// The k low bits are rotated to the top of the region's index instead, so the
+// lanes of a wave read consecutive words (or the same word).  k is the number of
+// the stage's group register bits above ls; see t_of for the lane/register split.
+template <int C>
+constexpr int tw_reg_bits(int ls) {
+    constexpr int RB = C - 6, NG = (C + RB - 1) / RB;
+    for (int gf = 0; gf < NG; gf++) {
+        const int hi = C - 1 - gf * RB, lo = (C - (gf + 1) * RB) > 0 ? (C - (gf + 1) * RB) : 0;
+        if (lo <= ls && ls <= hi) {
+            const int top = (hi - lo + 1 < RB && lo == 0) ? RB - 1 : hi;
+            return top - ls;
+        }
+    }
+    return 0;
+}
+template <int C>
+__host__ __device__ __forceinline__ uint32_t tw_pos(int ls, uint32_t i) {
+    const int k = tw_reg_bits<C>(ls), S = C - 1 - ls;
+    if (k <= 0 || k >= S) return i;
+    return ((i & ((1u << k) - 1)) << (S - k)) | (i >> k);
+}
+
 // LDS index of tile element t: one pad word per 16 keeps the group access
 // patterns (stride 1, 4 and 16 across lanes) free of bank conflicts
 __device__ __forceinline__ uint32_t lidx(uint32_t t) { return t + (t >> 4); }
@@ -62,7 +87,7 @@ __device__ __forceinline__ void g_store(const uint32_t (&v)[TC<C>::R], uint32_t 
 
 // one butterfly stage on local bit LO + b.  twl is the tile set's twiddles in
 // LDS, heap layout: local stage ls at [2^(C-1-ls), 2^(C-ls)), entry i being the
-// twiddle of the pairs whose bits above ls are i.
+// twiddle of the pairs whose bits above ls are i, stored doubled (m31_mul_dbl).
 template <bool INV, int C, int LO, int r, int b>
 __device__ __forceinline__ void g_stage(uint32_t (&v)[TC<C>::R], uint32_t lane, const uint32_t *twl) {
     constexpr int RB = TC<C>::RB;
@@ -74,16 +99,16 @@ __device__ __forceinline__ void g_stage(uint32_t (&v)[TC<C>::R], uint32_t lane, 
         for (int u = 0; u < (1 << (r - 1 - b)); u++) {
             const uint32_t jb = ((uint32_t)o << r) | ((uint32_t)u << (b + 1));
             const uint32_t t0 = t_of<RB, LO, r>(lane, jb);
-            const uint32_t w = tws[t0 >> (ls + 1)];
+            const uint32_t w = tws[tw_pos<C>(ls, t0 >> (ls + 1))];
 #pragma unroll
             for (int l = 0; l < (1 << b); l++) {
                 const int j0 = (int)jb | l, j1 = j0 | (1 << b);
                 const uint32_t a = v[j0];
                 if (INV) {
                     v[j0] = m31_add(a, v[j1]);
-                    v[j1] = m31_mul(m31_sub(a, v[j1]), w);
+                    v[j1] = m31_mul_dbl(m31_sub(a, v[j1]), w);
                 } else {
-                    const uint32_t m = m31_mul(v[j1], w);
+                    const uint32_t m = m31_mul_dbl(v[j1], w);
                     v[j0] = m31_add(a, m);
                     v[j1] = m31_sub(a, m);
                 }
@@ -143,7 +168,8 @@ __device__ __forceinline__ void batched_copy(uint32_t count, LOAD load, STORE st
 // Cooperatively copy the 2^C - 1 twiddles a tile set needs into LDS (heap
 // layout).  Stage s = s_lo + ls of the global table (log size LT) is read at
 // (Kbase >> (s+1)) + i: the tile's fixed bits select one contiguous segment
-// per stage, so the copy is coalesced.
+// per stage, so the copy is coalesced.  Entries are stored doubled, as
+// m31_mul_dbl takes them, at the rotated positions tw_pos gives.
 template <int C>
 __device__ __forceinline__ void stage_twiddles(uint32_t *twl, const uint32_t *tw, uint32_t LT, uint32_t s_lo,
                                                uint32_t Kbase) {
@@ -154,7 +180,10 @@ __device__ __forceinline__ void stage_twiddles(uint32_t *twl, const uint32_t *tw
             const uint32_t idx = t0 + i * step, lvl = 31 - __builtin_clz(idx), s = s_lo + (C - 1 - lvl);
             return tw[(1u << (LT - 1 - s)) + (Kbase >> (s + 1)) + (idx - (1u << lvl))];
         },
-        [&](uint32_t i, uint32_t v) { twl[t0 + i * step] = v; });
+        [&](uint32_t i, uint32_t v) {
+            const uint32_t idx = t0 + i * step, lvl = 31 - __builtin_clz(idx);
+            twl[(1u << lvl) + tw_pos<C>(C - 1 - (int)lvl, idx - (1u << lvl))] = v << 1;
+        });
 }
 
 // workgroup -> tile group, remapped so consecutive tile groups share an XCD

@@ -39,6 +39,14 @@ ZS_HD uint32_t m31_mul(uint32_t a, uint32_t b) {
     return m31_reduce_once(((uint32_t)prod & P) + hi);
 }
 
+// a * b mod p with the second factor given doubled (w2 = 2b, b < 2^31): the
+// 64-bit product 2ab has ab >> 31 as its high word and (ab mod 2^31) << 1 as
+// its low word, so the Mersenne fold needs no bit-align or mask.
+ZS_HD uint32_t m31_mul_dbl(uint32_t a, uint32_t w2) {
+    const uint64_t prod = (uint64_t)a * (uint64_t)w2;
+    return m31_reduce_once((uint32_t)(prod >> 32) + ((uint32_t)prod >> 1));
+}
+
 // x / 2 mod p without a multiply (p is odd)
 ZS_HD uint32_t m31_half(uint32_t x) { return (x & 1u) ? (uint32_t)((x + P) >> 1) : (x >> 1); }
 

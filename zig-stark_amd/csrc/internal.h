@@ -38,7 +38,7 @@ namespace zs {
         hipLaunchKernelGGL(__VA_ARGS__);                                           \
         if (_za && hipEventCreate(&_zb) == hipSuccess) {                           \
             (void)hipEventRecord(_zb, (ctx)->stream);                              \
-            (ctx)->prof.push_back({std::string(label), _za, _zb});                 \
+            (ctx)->prof.push_back({std::string(label), _za, _zb, (ctx)->lde_role}); \
         }                                                                          \
     } while (0)
 
@@ -73,6 +73,7 @@ struct TwiddleTable {
 struct ProfRec {
     std::string name;
     hipEvent_t a, b;
+    int role;  // LdeRole of the LDE pass that launched it, -1 outside an LDE
 };
 
 // Launch configuration of the register-group FFT passes: elements of the tile
@@ -83,6 +84,17 @@ struct TileConfig {
     uint32_t elems_log = 13, cols_per_wg = 4;
 };
 TileConfig default_tile_config();
+// An LDE's three kinds of pass (interpolation passes, the fused middle kernel,
+// evaluation passes) move different data patterns, so each has its own configuration.
+enum LdeRole { LDE_INTERP = 0, LDE_MID = 1, LDE_EVAL = 2 };
+struct LdeTiles {
+    TileConfig role[3];
+};
+inline LdeTiles default_lde_tiles() {
+    LdeTiles t;
+    for (auto &r : t.role) r = default_tile_config();
+    return t;
+}
 // key of an LDE shape in the tuned table: columns rounded up to a power of two
 inline uint64_t lde_shape_key(uint32_t n_cols, uint32_t log_n, uint32_t log_blowup) {
     uint32_t cl = 0;
@@ -100,7 +112,7 @@ struct DeviceCaches {
     // QM31 power tables of the trace-commitment DFTs, keyed by (kind << 32) | log length
     std::map<uint64_t, std::unique_ptr<DevBuf>> qtables;
     // launch configurations chosen by zs_autotune_lde, keyed by lde_shape_key
-    std::map<uint64_t, TileConfig> tuned;
+    std::map<uint64_t, LdeTiles> tuned;
 };
 
 struct Ctx {
@@ -114,6 +126,8 @@ struct Ctx {
     std::string last_error;
     std::shared_ptr<DeviceCaches> caches = std::make_shared<DeviceCaches>();
     TileConfig tile = default_tile_config();  // used by the next transform launches
+    LdeTiles lde_tiles = default_lde_tiles();  // used by the next LDE, per kind of pass
+    int lde_role = -1;                         // the kind of LDE pass being launched (profiling)
     // grow-only scratch arenas
     DevBuf scratch[10];  // 0 transforms, 1-2 LDE, 3 plain path, 4-6 FRI, 7-9 commit jobs
     // FRI commit phases with challenges given up front, captured once per buffer set and
