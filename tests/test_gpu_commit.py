@@ -127,9 +127,11 @@ def test_lde_commit_register_group_path(zctx, n_cols, lg, beta):
         assert root == O.tree_root(O.row_commit(ref_lde))
 
 
-@pytest.mark.parametrize("n_cols,lg,beta", [(8, 10, 2), (24, 13, 2)])
+@pytest.mark.parametrize("n_cols,lg,beta", [(8, 10, 2), (24, 13, 2), (3, 19, 2), (2, 20, 2)])
 def test_autotuned_lde_is_bit_exact(zctx, torch_dev, n_cols, lg, beta):
-    """The autotuner keeps a configuration per shape; LDE and root stay equal to the oracle's."""
+    """The autotuner keeps a configuration per shape; LDE and root stay equal to the oracle's.
+    Every candidate launch shape must give the same root (the autotuner checks), so the
+    two-pass shapes (lg 19, 20) cover each tile size of the pipelined last pass."""
     import torch
     cols = O.fill_m31(lg * 31 + n_cols, n_cols << lg).reshape(n_cols, -1)
     tuned = zctx.autotune_lde(n_cols, lg, beta)

@@ -22,6 +22,7 @@ struct FftPass {
     uint32_t natural_tiles;  // tiles chosen so bitrev addresses are adjacent
     uint32_t n_cols;         // columns in the batch
     uint32_t ncb;            // columns one workgroup processes with the same staged twiddles
+    uint32_t TS;             // LDS words from one tile to the next (register-group kernels)
 };
 
 struct PassRange {

@@ -25,7 +25,6 @@ struct TC {
     static constexpr int RB = C - 6;  // log2 elements per lane
     static constexpr int R = 1 << RB;
     static constexpr int T = 1 << C;
-    static constexpr int TS = T + (T >> 4) + 1;  // padded tile stride in LDS (words)
     static constexpr int NG = (C + RB - 1) / RB;  // register groups per pass
 };
 
@@ -219,7 +218,8 @@ __device__ __forceinline__ uint32_t tile_fixed(const FftPass &p, const TileGeom 
 template <int C>
 __device__ __forceinline__ void coop_load_generic(const FftPass &p, const TileGeom &tg, uint32_t *lds, uint32_t col,
                                           uint32_t qblk, uint32_t nq_log, bool replicate_q) {
-    constexpr uint32_t T = 1u << C, TS = TC<C>::TS;
+    constexpr uint32_t T = 1u << C;
+    const uint32_t TS = p.TS;
     const uint32_t G = 1u << p.G_log, Q = 1u << p.Q_log;
     const bool g_fast = (p.s_lo > 0) || p.natural_tiles;
     const uint32_t half_n = 1u << (p.L - 1);
@@ -259,7 +259,8 @@ __device__ __forceinline__ void coop_load_generic(const FftPass &p, const TileGe
 template <int C>
 __device__ __forceinline__ void coop_store_generic(const FftPass &p, const TileGeom &tg, const uint32_t *lds, uint32_t col,
                                            uint32_t qblk, uint32_t nq_log) {
-    constexpr uint32_t T = 1u << C, TS = TC<C>::TS;
+    constexpr uint32_t T = 1u << C;
+    const uint32_t TS = p.TS;
     const uint32_t G = 1u << p.G_log, NQ = 1u << nq_log;
     const bool g_fast = (p.s_lo > 0) || p.natural_tiles;
     const uint32_t half_n = 1u << (p.L - 1);
@@ -317,7 +318,8 @@ __device__ __forceinline__ void coop_store_generic(const FftPass &p, const TileG
 template <int C, uint32_t B = 8>
 __device__ __forceinline__ void coop_load(const FftPass &p, const TileGeom &tg, uint32_t *lds, uint32_t col,
                                           uint32_t qblk, uint32_t nq_log, bool replicate_q) {
-    constexpr uint32_t T = 1u << C, TS = TC<C>::TS;
+    constexpr uint32_t T = 1u << C;
+    const uint32_t TS = p.TS;
     const uint32_t G = 1u << p.G_log, nt = blockDim.x, tid = threadIdx.x;
     const uint32_t lnt = __builtin_ctz(nt);  // block sizes are powers of two
     const uint32_t *in_col = p.in + (uint64_t)col * p.in_cs;
@@ -381,7 +383,8 @@ __device__ __forceinline__ void coop_load(const FftPass &p, const TileGeom &tg, 
 template <int C, uint32_t B = 8>
 __device__ __forceinline__ void coop_store(const FftPass &p, const TileGeom &tg, const uint32_t *lds, uint32_t col,
                                            uint32_t qblk, uint32_t nq_log) {
-    constexpr uint32_t T = 1u << C, TS = TC<C>::TS;
+    constexpr uint32_t T = 1u << C;
+    const uint32_t TS = p.TS;
     const uint32_t G = 1u << p.G_log, nt = blockDim.x, tid = threadIdx.x;
     const uint32_t lnt = __builtin_ctz(nt);
     uint32_t *out_col = p.out + (uint64_t)col * p.out_cs;
@@ -448,7 +451,7 @@ constexpr int tile_block_max() { return C >= 11 ? 512 : 1024; }
 template <bool INV, int C>
 __global__ __launch_bounds__(tile_block_max<C>()) void fft_tile_kernel(FftPass p) {
     extern __shared__ uint32_t lds[];
-    constexpr uint32_t TS = TC<C>::TS;
+    const uint32_t TS = p.TS;
     const uint32_t G = 1u << p.G_log, Q = 1u << p.Q_log;
     const uint32_t qblk = blockIdx.z << p.Q_log;
     const TileGeom tg = tile_geom(p, xcd_block(blockIdx.x, gridDim.x));
@@ -480,9 +483,102 @@ __global__ __launch_bounds__(tile_block_max<C>()) void fft_tile_kernel(FftPass p
     }
 }
 
+// The LDE's last evaluation pass (contiguous k-space tiles in, natural order
+// out; s_lo == 0, no scale), one wave per tile of the workgroup's G tiles x Q
+// cosets.  Each wave loads and transforms its own tile, so only the natural-order
+// store, whose runs interleave every tile of the workgroup, needs the workgroup.
+// The next column's tile is loaded into registers while the current column is
+// transformed, and a column's stores are issued after the next column's tile has
+// landed in LDS, so they drain during its butterflies: HBM latency stays off the
+// critical path.  A thread's store run position is (g, bitrev(q)) with
+// t = t0 + 64 i, as in coop_store's natural-order path.
 template <int C>
-static size_t tile_lds_bytes(uint32_t ntiles, uint32_t nsets) {
-    return (size_t(ntiles) * TC<C>::TS + (size_t(nsets) << C)) * 4;
+__global__ __launch_bounds__(512, 2) void lde_eval_last_kernel(FftPass p) {
+    constexpr uint32_t PER = (1u << C) >> 6;  // tile elements per lane
+    extern __shared__ uint32_t lds[];
+    const uint32_t TS = p.TS, G = 1u << p.G_log, Q = 1u << p.Q_log;
+    const uint32_t qblk = blockIdx.z << p.Q_log;
+    const uint32_t tau0 = xcd_block(blockIdx.x, gridDim.x) << p.G_log;
+    const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    uint32_t *twl = lds + Q * G * TS;
+    for (uint32_t set = 0; set < Q * G; set++) {
+        const uint32_t h = bitrev32(tau0 + (set & (G - 1)), p.L - C);
+        stage_twiddles<C>(twl + (set << C), p.tw, p.LT, 0, ((qblk + (set >> p.G_log)) << p.L) | (h << C));
+    }
+    // this wave's tile: coset qblk + (wave >> G_log), k-space tile bitrev(tau0 + (wave & (G - 1)))
+    const uint64_t in_off = ((uint64_t)(qblk + (wave >> p.G_log)) << p.L) |
+                            ((uint64_t)bitrev32(tau0 + (wave & (G - 1)), p.L - C) << C);
+    uint32_t *tl = lds + wave * TS;
+    const uint32_t *twt = twl + (wave << C);
+    // this thread's natural-order store run position
+    const uint32_t run_log = p.G_log + p.Q_log;
+    const uint32_t rp = tid & ((1u << run_log) - 1), rq = rp & (Q - 1), g = rp >> p.Q_log;
+    const uint32_t ql = bitrev32(rq, p.Q_log), t0 = tid >> run_log;
+    const uint32_t sh = p.L - C + p.beta;
+    const uint64_t out_off = (((uint64_t)(tau0 + g)) << p.beta) | bitrev32(qblk + ql, p.beta) |
+                             ((uint64_t)bitrev32(t0, C) << sh);
+    const uint32_t *rd = lds + (ql * G + g) * TS + lidx(t0);
+    const uint32_t c0 = blockIdx.y * p.ncb, c1 = min(c0 + p.ncb, p.n_cols);
+    uint32_t pf[PER], res[PER];
+    if (c0 < c1) {
+        const uint32_t *src = p.in + (uint64_t)c0 * p.in_cs + in_off + lane;
+#pragma unroll
+        for (uint32_t j = 0; j < PER; j++) pf[j] = src[j * 64];
+    }
+    __syncthreads();  // twiddle sets staged
+    for (uint32_t col = c0; col < c1; col++) {
+#pragma unroll
+        for (uint32_t j = 0; j < PER; j++) tl[lidx(lane + j * 64)] = pf[j];
+        if (col > c0) {
+            uint32_t *dst = p.out + (uint64_t)(col - 1) * p.out_cs + out_off;
+#pragma unroll
+            for (uint32_t i = 0; i < PER; i++) dst[(uint64_t)bitrev32(i, C - 6) << sh] = res[i];
+        }
+        if (col + 1 < c1) {
+            const uint32_t *src = p.in + (uint64_t)(col + 1) * p.in_cs + in_off + lane;
+#pragma unroll
+            for (uint32_t j = 0; j < PER; j++) pf[j] = src[j * 64];
+        }
+        wave_lds_sync();
+        tile_run<false, C, 0>(tl, lane, twt);
+        __syncthreads();  // every tile of the workgroup transformed
+#pragma unroll
+        for (uint32_t i = 0; i < PER; i++) res[i] = rd[lidx(i << 6)];
+        __syncthreads();  // every read done: the tiles may take the next column
+    }
+    if (c0 < c1) {
+        uint32_t *dst = p.out + (uint64_t)(c1 - 1) * p.out_cs + out_off;
+#pragma unroll
+        for (uint32_t i = 0; i < PER; i++) dst[(uint64_t)bitrev32(i, C - 6) << sh] = res[i];
+    }
+}
+
+template <int C>
+static size_t tile_lds_bytes(uint32_t ntiles, uint32_t nsets, uint32_t TS) {
+    return (size_t(ntiles) * TS + (size_t(nsets) << C)) * 4;
+}
+
+// LDS words from one tile to the next: 2^C plus one pad word per 16 (which keeps
+// the register groups' accesses conflict-free), plus a pad that spreads the tiles
+// of one copy instruction over the 32 LDS banks.  The copies interleave `il` tiles
+// across consecutive lanes (tile index fastest, then the position in the tile),
+// so a stride of 32 / il banks (mod 32) gives every lane of a half-wave its own bank.
+uint32_t tile_stride(uint32_t C, uint32_t il) {
+    const uint32_t base = (1u << C) + (1u << (C - 4));
+    const uint32_t target = (il >= 2 && il <= 32) ? 32 / il : 1;
+    return base + (target + 32 - base % 32) % 32;
+}
+
+// tiles one copy instruction of the pass interleaves across lanes (see coop_load, coop_store)
+static uint32_t copy_interleave(const FftPass &p, uint32_t q_in_wg) {
+    const uint32_t G = 1u << p.G_log;
+    uint32_t il = 1;
+    if ((p.in_mode == IN_NATURAL && p.s_lo == 0 && p.natural_tiles) ||
+        ((p.in_mode == IN_KSPACE || p.in_mode == IN_SHARED) && p.s_lo > 0))
+        il = std::max(il, G);
+    if (p.out_mode == OUT_NATURAL && p.s_lo == 0 && p.natural_tiles) il = std::max(il, G * q_in_wg);
+    if (p.out_mode == OUT_KSPACE && p.s_lo > 0) il = std::max(il, G);
+    return il;
 }
 
 // Launch-shape defaults, read once from the environment (defaults from measurement):
@@ -519,12 +615,27 @@ int32_t run_tile_pass(Ctx *ctx, FftPass p, bool inverse, uint32_t n_cols) {
     uint32_t gx = tiles >> p.G_log, gz = (1u << p.beta) >> p.Q_log;
     p.n_cols = n_cols;
     p.ncb = pick_ncb(ctx, n_cols, gx * gz);
+    p.TS = tile_stride(p.C, copy_interleave(p, 1u << p.Q_log));
     dim3 grid(gx, (n_cols + p.ncb - 1) / p.ncb, gz);
     uint32_t maxw = p.C >= 11 ? 8 : 16;
     dim3 block(64 * std::min<uint32_t>(ntile, maxw));
+    if (!inverse && p.s_lo == 0 && p.in_mode == IN_KSPACE && p.out_mode == OUT_NATURAL && p.natural_tiles &&
+        !p.scale && ntile <= 8 && (p.C == 9 || p.C == 10)) {
+        // the LDE's last evaluation pass: the pipelined one-wave-per-tile kernel
+        const size_t lds = (size_t(ntile) * p.TS + (size_t(ntile) << p.C)) * 4;
+        if (lds <= 160 * 1024) {
+            const dim3 blk(64 * ntile);
+            if (p.C == 10)
+                ZS_LAUNCH(ctx, "lde_eval_last_kernel<10>", (lde_eval_last_kernel<10>), grid, blk, lds, ctx->stream, p);
+            else
+                ZS_LAUNCH(ctx, "lde_eval_last_kernel<9>", (lde_eval_last_kernel<9>), grid, blk, lds, ctx->stream, p);
+            ZS_HIP(ctx, hipGetLastError());
+            return ZS_OK;
+        }
+    }
 #define ZS_TILE_CASE(CC)                                                                                         \
     case CC: {                                                                                                   \
-        size_t lds = tile_lds_bytes<CC>(ntile, nsets);                                                           \
+        size_t lds = tile_lds_bytes<CC>(ntile, nsets, p.TS);                                                         \
         if (lds > 160 * 1024) return run_pass(ctx, p, inverse, n_cols);                                          \
         if (inverse)                                                                                             \
             ZS_LAUNCH(ctx, p.s_lo == 0 ? "fft_tile_kernel<inv," #CC ",natural-in>" : "fft_tile_kernel<inv," #CC ">", \
@@ -574,8 +685,8 @@ struct LdeFused {
 template <int C>
 __global__ __launch_bounds__(tile_block_max<C>()) void lde_mid_kernel(LdeFused f) {
     extern __shared__ uint32_t lds[];
-    constexpr uint32_t TS = TC<C>::TS;
     const FftPass &p = f.p;
+    const uint32_t TS = p.TS;
     const TileGeom tg = tile_geom(p, xcd_block(blockIdx.x, gridDim.x));
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t G = 1u << p.G_log;  // == number of waves
@@ -619,8 +730,8 @@ __global__ __launch_bounds__(tile_block_max<C>()) void lde_mid_kernel(LdeFused f
 template <int C>
 __global__ __launch_bounds__(256) void lde_single_kernel(LdeFused f) {
     extern __shared__ uint32_t lds[];
-    constexpr uint32_t TS = TC<C>::TS;
     const FftPass &p = f.p;
+    const uint32_t TS = p.TS;
     const TileGeom tg{0, 0, 0};
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nwaves = blockDim.x >> 6;
     const uint32_t NQ = 1u << p.beta;
@@ -652,14 +763,12 @@ __global__ __launch_bounds__(256) void lde_single_kernel(LdeFused f) {
     }
 }
 
-static size_t TC_TS(uint32_t C) { return (size_t(1) << C) + (size_t(1) << (C - 4)) + 1; }
-
 bool lde_fused_ok(uint32_t C, uint32_t beta, bool single) {
     if (!tile_pass_ok(C) || beta > 8) return false;
     if (!single) return true;
     // the single-workgroup form keeps 2^beta output tiles, the interpolation tile and
     // 1 + 2^beta twiddle sets in LDS
-    size_t ts = (size_t(1) << C) + (size_t(1) << (C - 4)) + 1;
+    size_t ts = tile_stride(C, 1u << beta);
     size_t nq = size_t(1) << beta;
     return ((nq + 1) * ts + ((nq + 1) << C)) * 4 <= 160 * 1024;
 }
@@ -698,7 +807,8 @@ int32_t run_lde_fused(Ctx *ctx, const uint32_t *d_in, uint64_t in_cs, uint32_t *
         p.ncb = pick_ncb(ctx, n_cols, 1);
         grid = dim3(1, (n_cols + p.ncb - 1) / p.ncb, 1);
         block = dim3(64 * std::min<uint32_t>(1u << beta, 4));
-        lds = ((size_t(1) << beta) + 1) * TC_TS(C) + (((size_t(1) << beta) + 1) << C);
+        p.TS = tile_stride(C, 1u << beta);
+        lds = ((size_t(1) << beta) + 1) * p.TS + (((size_t(1) << beta) + 1) << C);
     } else {
         // one wave per tile, at most 8 (C = 11) or 16 tiles per workgroup
         int32_t want = std::min<int32_t>((int32_t)ctx->tile.elems_log - (int32_t)C, C >= 11 ? 3 : 4);
@@ -712,7 +822,8 @@ int32_t run_lde_fused(Ctx *ctx, const uint32_t *d_in, uint64_t in_cs, uint32_t *
         p.ncb = pick_ncb(ctx, n_cols, gx);
         grid = dim3(gx, (n_cols + p.ncb - 1) / p.ncb, 1);
         block = dim3(64u << p.G_log);
-        lds = (size_t(1) << p.G_log) * TC_TS(C) + (((size_t(1) << beta) + 1) << C);
+        p.TS = tile_stride(C, 1u << p.G_log);
+        lds = (size_t(1) << p.G_log) * p.TS + (((size_t(1) << beta) + 1) << C);
         if (lds * 4 > 160 * 1024) return ZS_ERR_INPUT;
     }
 #define ZS_FUSED_CASE(CC)                                                                                          \
