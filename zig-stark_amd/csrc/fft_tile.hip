@@ -681,14 +681,18 @@ struct LdeFused {
 
 // s_lo > 0: G tiles per workgroup (one wave each).  Twiddle sets in LDS: the
 // interpolation's (shared by the G tiles) and one per q for the evaluation.
-// Per q the workgroup's G output tiles are stored together (k-space).
+// Per q the workgroup's G output tiles are stored together (k-space).  The next
+// column's tiles are loaded into registers while the current column is
+// transformed; the last coset's stores are issued after the next column has
+// landed in LDS, so the wait for that load never covers fresh stores.
 template <int C>
-__global__ __launch_bounds__(tile_block_max<C>()) void lde_mid_kernel(LdeFused f) {
+__global__ __launch_bounds__(512, 2) void lde_mid_kernel(LdeFused f) {
+    constexpr uint32_t PER = (1u << C) >> 6;  // elements per thread of one column's G tiles
     extern __shared__ uint32_t lds[];
     const FftPass &p = f.p;
     const uint32_t TS = p.TS;
     const TileGeom tg = tile_geom(p, xcd_block(blockIdx.x, gridDim.x));
-    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const uint32_t G = 1u << p.G_log;  // == number of waves
     const uint32_t NQ = 1u << p.beta;
     uint32_t *twl = lds + G * TS;  // set 0: interpolation, set 1 + q: evaluation of q
@@ -698,30 +702,62 @@ __global__ __launch_bounds__(tile_block_max<C>()) void lde_mid_kernel(LdeFused f
         stage_twiddles<C>(twl + ((1 + q) << C), f.tw_fwd, p.L + p.beta, p.s_lo, (q << p.L) | hi_bits);
     using Top = Grp<false, C, 0>;
     uint32_t *tl = lds + wave * TS;
+    // the strided copies: thread (g, t0) moves elements t = t0 + 64 j of tile g, which sit
+    // at (hi, t, lo0 + g) of a column (runs of G words along g)
+    const uint32_t g = tid & (G - 1), t0 = tid >> p.G_log;
+    const uint64_t off = ((uint64_t)tg.hi << (p.s_lo + C)) + tg.lo0 + g + ((uint64_t)t0 << p.s_lo);
+    const uint32_t jsh = 6 + p.s_lo;
+    uint32_t *cp = lds + g * TS + lidx(t0);
     const uint32_t c0 = blockIdx.y * p.ncb;
     const uint32_t c1 = min(c0 + p.ncb, p.n_cols);
+    uint32_t pf[PER], res[PER];
+    if (c0 < c1) {
+        const uint32_t *src = p.in + (uint64_t)c0 * p.in_cs + off;
+#pragma unroll
+        for (uint32_t j = 0; j < PER; j++) pf[j] = src[(uint64_t)j << jsh];
+    }
+    __syncthreads();  // twiddle sets staged
     for (uint32_t col = c0; col < c1; col++) {
-        coop_load<C, 4>(p, tg, lds, col, 0, 0, false);
+#pragma unroll
+        for (uint32_t j = 0; j < PER; j++) cp[lidx(j << 6)] = pf[j];
+        if (col > c0) {  // the previous column's last coset, held back until this column's tiles landed
+            uint32_t *dst = p.out + (uint64_t)(col - 1) * p.out_cs + ((uint64_t)(NQ - 1) << p.L) + off;
+#pragma unroll
+            for (uint32_t j = 0; j < PER; j++) dst[(uint64_t)j << jsh] = res[j];
+        }
+        if (col + 1 < c1) {
+            const uint32_t *src = p.in + (uint64_t)(col + 1) * p.in_cs + off;
+#pragma unroll
+            for (uint32_t j = 0; j < PER; j++) pf[j] = src[(uint64_t)j << jsh];
+        }
         __syncthreads();
         uint32_t c[TC<C>::R];
-        if (wave < G) {
-            tile_run<true, C, 0>(tl, lane, twl);
-            g_load<C, Top::LO, Top::r>(c, tl, lane);
+        tile_run<true, C, 0>(tl, lane, twl);
+        g_load<C, Top::LO, Top::r>(c, tl, lane);
 #pragma unroll
-            for (int j = 0; j < TC<C>::R; j++) c[j] = m31_mul(c[j], f.inv_n);
-        }
+        for (int j = 0; j < TC<C>::R; j++) c[j] = m31_mul(c[j], f.inv_n);
+        uint32_t *out_col = p.out + (uint64_t)col * p.out_cs + off;
         for (uint32_t q = 0; q < NQ; q++) {
-            __syncthreads();  // the previous store (or the interpolation's reads) are done with the tiles
-            if (wave < G) {
-                uint32_t v[TC<C>::R];
+            __syncthreads();  // the previous coset's reads (or the interpolation's) are done with the tiles
+            uint32_t v[TC<C>::R];
 #pragma unroll
-                for (int j = 0; j < TC<C>::R; j++) v[j] = c[j];
-                fwd_from_regs<C>(v, tl, lane, twl + ((1 + q) << C));
-            }
+            for (int j = 0; j < TC<C>::R; j++) v[j] = c[j];
+            fwd_from_regs<C>(v, tl, lane, twl + ((1 + q) << C));
             __syncthreads();
-            coop_store<C, 4>(p, tg, lds, col, q, 0);
+#pragma unroll
+            for (uint32_t j = 0; j < PER; j++) res[j] = cp[lidx(j << 6)];
+            if (q + 1 < NQ) {
+                uint32_t *dst = out_col + ((uint64_t)q << p.L);
+#pragma unroll
+                for (uint32_t j = 0; j < PER; j++) dst[(uint64_t)j << jsh] = res[j];
+            }
         }
-        __syncthreads();
+        __syncthreads();  // every read done: the tiles may take the next column
+    }
+    if (c0 < c1) {
+        uint32_t *dst = p.out + (uint64_t)(c1 - 1) * p.out_cs + ((uint64_t)(NQ - 1) << p.L) + off;
+#pragma unroll
+        for (uint32_t j = 0; j < PER; j++) dst[(uint64_t)j << jsh] = res[j];
     }
 }
 
@@ -810,8 +846,8 @@ int32_t run_lde_fused(Ctx *ctx, const uint32_t *d_in, uint64_t in_cs, uint32_t *
         p.TS = tile_stride(C, 1u << beta);
         lds = ((size_t(1) << beta) + 1) * p.TS + (((size_t(1) << beta) + 1) << C);
     } else {
-        // one wave per tile, at most 8 (C = 11) or 16 tiles per workgroup
-        int32_t want = std::min<int32_t>((int32_t)ctx->tile.elems_log - (int32_t)C, C >= 11 ? 3 : 4);
+        // one wave per tile, at most 8 tiles per workgroup (the kernel's launch bound)
+        int32_t want = std::min<int32_t>((int32_t)ctx->tile.elems_log - (int32_t)C, 3);
         p.G_log = (uint32_t)std::max<int32_t>(0, std::min<int32_t>((int32_t)s_lo, want));
         p.Q_log = 0;
         p.in_mode = IN_KSPACE;
