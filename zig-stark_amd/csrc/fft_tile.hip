@@ -553,6 +553,65 @@ __global__ __launch_bounds__(512, 2) void lde_eval_last_kernel(FftPass p) {
     }
 }
 
+// The LDE's first interpolation pass (natural-order input, contiguous k-space
+// tiles out; s_lo == 0, no scale), one wave per tile of the workgroup's G tiles.
+// The natural-order load interleaves the G tiles (runs of G words), so it goes
+// through the workgroup; each wave then transforms and stores its own tile.  The
+// next column is loaded into registers during the butterflies, and a column's
+// stores are issued after the next column has landed in LDS.
+template <int C>
+__global__ __launch_bounds__(512, 2) void lde_interp_first_kernel(FftPass p) {
+    constexpr uint32_t PER = (1u << C) >> 6;
+    extern __shared__ uint32_t lds[];
+    const uint32_t TS = p.TS, G = 1u << p.G_log;
+    const uint32_t tau0 = xcd_block(blockIdx.x, gridDim.x) << p.G_log;
+    const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    uint32_t *twl = lds + G * TS;
+    for (uint32_t set = 0; set < G; set++)
+        stage_twiddles<C>(twl + (set << C), p.tw, p.LT, 0, bitrev32(tau0 + set, p.L - C) << C);
+    // load: thread (g, t0) reads elements t = t0 + 64 j of tile g, natural index
+    // (bitrev_C(t) << (L - C)) | (tau0 + g), bitrev_C(t) = (bitrev_6(t0) << (C - 6)) | bitrev_{C-6}(j)
+    const uint32_t g = tid & (G - 1), t0 = tid >> p.G_log, sh = p.L - C;
+    const uint64_t in_off = (uint64_t)(tau0 + g) + ((uint64_t)bitrev32(t0, C) << sh);
+    uint32_t *cp = lds + g * TS + lidx(t0);
+    // store: this wave's tile, k-space tile bitrev(tau0 + wave), contiguous
+    const uint64_t out_off = ((uint64_t)bitrev32(tau0 + wave, p.L - C) << C) + lane;
+    uint32_t *tl = lds + wave * TS;
+    const uint32_t *twt = twl + (wave << C);
+    const uint32_t c0 = blockIdx.y * p.ncb, c1 = min(c0 + p.ncb, p.n_cols);
+    uint32_t pf[PER], res[PER];
+    if (c0 < c1) {
+        const uint32_t *src = p.in + (uint64_t)c0 * p.in_cs + in_off;
+#pragma unroll
+        for (uint32_t j = 0; j < PER; j++) pf[j] = src[(uint64_t)bitrev32(j, C - 6) << sh];
+    }
+    __syncthreads();  // twiddle sets staged
+    for (uint32_t col = c0; col < c1; col++) {
+#pragma unroll
+        for (uint32_t j = 0; j < PER; j++) cp[lidx(j << 6)] = pf[j];
+        if (col > c0) {
+            uint32_t *dst = p.out + (uint64_t)(col - 1) * p.out_cs + out_off;
+#pragma unroll
+            for (uint32_t j = 0; j < PER; j++) dst[j * 64] = res[j];
+        }
+        if (col + 1 < c1) {
+            const uint32_t *src = p.in + (uint64_t)(col + 1) * p.in_cs + in_off;
+#pragma unroll
+            for (uint32_t j = 0; j < PER; j++) pf[j] = src[(uint64_t)bitrev32(j, C - 6) << sh];
+        }
+        __syncthreads();  // every tile loaded
+        tile_run<true, C, 0>(tl, lane, twt);
+#pragma unroll
+        for (uint32_t j = 0; j < PER; j++) res[j] = tl[lidx(lane + j * 64)];
+        __syncthreads();  // every read done: the tiles may take the next column
+    }
+    if (c0 < c1) {
+        uint32_t *dst = p.out + (uint64_t)(c1 - 1) * p.out_cs + out_off;
+#pragma unroll
+        for (uint32_t j = 0; j < PER; j++) dst[j * 64] = res[j];
+    }
+}
+
 template <int C>
 static size_t tile_lds_bytes(uint32_t ntiles, uint32_t nsets, uint32_t TS) {
     return (size_t(ntiles) * TS + (size_t(nsets) << C)) * 4;
@@ -619,6 +678,22 @@ int32_t run_tile_pass(Ctx *ctx, FftPass p, bool inverse, uint32_t n_cols) {
     dim3 grid(gx, (n_cols + p.ncb - 1) / p.ncb, gz);
     uint32_t maxw = p.C >= 11 ? 8 : 16;
     dim3 block(64 * std::min<uint32_t>(ntile, maxw));
+    if (inverse && p.s_lo == 0 && p.in_mode == IN_NATURAL && p.out_mode == OUT_KSPACE && p.natural_tiles &&
+        !p.scale && p.Q_log == 0 && p.beta == 0 && ntile <= 8 && (p.C == 9 || p.C == 10)) {
+        // the LDE's first interpolation pass: the pipelined one-wave-per-tile kernel
+        const size_t lds = (size_t(ntile) * p.TS + (size_t(ntile) << p.C)) * 4;
+        if (lds <= 160 * 1024) {
+            const dim3 blk(64 * ntile);
+            if (p.C == 10)
+                ZS_LAUNCH(ctx, "lde_interp_first_kernel<10>", (lde_interp_first_kernel<10>), grid, blk, lds,
+                          ctx->stream, p);
+            else
+                ZS_LAUNCH(ctx, "lde_interp_first_kernel<9>", (lde_interp_first_kernel<9>), grid, blk, lds,
+                          ctx->stream, p);
+            ZS_HIP(ctx, hipGetLastError());
+            return ZS_OK;
+        }
+    }
     if (!inverse && p.s_lo == 0 && p.in_mode == IN_KSPACE && p.out_mode == OUT_NATURAL && p.natural_tiles &&
         !p.scale && ntile <= 8 && (p.C == 9 || p.C == 10)) {
         // the LDE's last evaluation pass: the pipelined one-wave-per-tile kernel
