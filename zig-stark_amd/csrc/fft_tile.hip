@@ -35,16 +35,12 @@ struct TC {
 // lanes of a wave read consecutive words (or the same word).  k is the number of
 // the stage's group register bits above ls; see t_of for the lane/register split.
 template <int C>
-constexpr int tw_reg_bits(int ls) {
-    constexpr int RB = C - 6, NG = (C + RB - 1) / RB;
-    for (int gf = 0; gf < NG; gf++) {
-        const int hi = C - 1 - gf * RB, lo = (C - (gf + 1) * RB) > 0 ? (C - (gf + 1) * RB) : 0;
-        if (lo <= ls && ls <= hi) {
-            const int top = (hi - lo + 1 < RB && lo == 0) ? RB - 1 : hi;
-            return top - ls;
-        }
-    }
-    return 0;
+__host__ __device__ constexpr int tw_reg_bits(int ls) {
+    constexpr int RB = C - 6;
+    const int gf = (C - 1 - ls) / RB;  // the group holding stage ls
+    const int hi = C - 1 - gf * RB, lo = C - (gf + 1) * RB;
+    const int top = (lo <= 0 && hi + 1 < RB) ? RB - 1 : hi;  // a short last group sits in the low bits
+    return top - ls;
 }
 template <int C>
 __host__ __device__ __forceinline__ uint32_t tw_pos(int ls, uint32_t i) {
@@ -182,6 +178,30 @@ __device__ __forceinline__ void stage_twiddles(uint32_t *twl, const uint32_t *tw
         [&](uint32_t i, uint32_t v) {
             const uint32_t idx = t0 + i * step, lvl = 31 - __builtin_clz(idx);
             twl[(1u << lvl) + tw_pos<C>(C - 1 - (int)lvl, idx - (1u << lvl))] = v << 1;
+        });
+}
+
+// Stage nsets twiddle sets at once (set k at twl + (k << C), same layout as
+// stage_twiddles): the loads of the whole batch are in flight together, so a
+// workgroup pays one memory latency for its twiddles.  entry(set, ls, i) reads
+// entry i of local stage ls of the set's global table.
+template <int C, uint32_t B, typename ENTRY>
+__device__ __forceinline__ void stage_twiddle_sets(uint32_t *twl, uint32_t nsets, ENTRY entry) {
+    constexpr uint32_t T = 1u << C;
+    const uint32_t nt = blockDim.x, tid = threadIdx.x;
+    batched_copy<B>(
+        strided_count(tid, nt, nsets << C),
+        [&](uint32_t i) {
+            const uint32_t e = tid + i * nt, idx = e & (T - 1);
+            if (idx == 0) return 0u;  // heap slot 0 is unused
+            const uint32_t lvl = 31 - __builtin_clz(idx);
+            return entry(e >> C, C - 1 - lvl, idx - (1u << lvl));
+        },
+        [&](uint32_t i, uint32_t v) {
+            const uint32_t e = tid + i * nt, idx = e & (T - 1);
+            if (idx == 0) return;
+            const uint32_t lvl = 31 - __builtin_clz(idx);
+            twl[((e >> C) << C) + (1u << lvl) + tw_pos<C>(C - 1 - (int)lvl, idx - (1u << lvl))] = v << 1;
         });
 }
 
@@ -501,10 +521,6 @@ __global__ __launch_bounds__(512, 2) void lde_eval_last_kernel(FftPass p) {
     const uint32_t tau0 = xcd_block(blockIdx.x, gridDim.x) << p.G_log;
     const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     uint32_t *twl = lds + Q * G * TS;
-    for (uint32_t set = 0; set < Q * G; set++) {
-        const uint32_t h = bitrev32(tau0 + (set & (G - 1)), p.L - C);
-        stage_twiddles<C>(twl + (set << C), p.tw, p.LT, 0, ((qblk + (set >> p.G_log)) << p.L) | (h << C));
-    }
     // this wave's tile: coset qblk + (wave >> G_log), k-space tile bitrev(tau0 + (wave & (G - 1)))
     const uint64_t in_off = ((uint64_t)(qblk + (wave >> p.G_log)) << p.L) |
                             ((uint64_t)bitrev32(tau0 + (wave & (G - 1)), p.L - C) << C);
@@ -525,6 +541,11 @@ __global__ __launch_bounds__(512, 2) void lde_eval_last_kernel(FftPass p) {
 #pragma unroll
         for (uint32_t j = 0; j < PER; j++) pf[j] = src[j * 64];
     }
+    // tile set (q, g) covers k-space ((qblk + q) << L) | (bitrev(tau0 + g) << C)
+    stage_twiddle_sets<C, 16>(twl, Q * G, [&](uint32_t set, uint32_t ls, uint32_t i) {
+        const uint32_t kb = ((qblk + (set >> p.G_log)) << p.L) | (bitrev32(tau0 + (set & (G - 1)), p.L - C) << C);
+        return p.tw[(1u << (p.LT - 1 - ls)) + (kb >> (ls + 1)) + i];
+    });
     __syncthreads();  // twiddle sets staged
     for (uint32_t col = c0; col < c1; col++) {
 #pragma unroll
@@ -567,8 +588,6 @@ __global__ __launch_bounds__(512, 2) void lde_interp_first_kernel(FftPass p) {
     const uint32_t tau0 = xcd_block(blockIdx.x, gridDim.x) << p.G_log;
     const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     uint32_t *twl = lds + G * TS;
-    for (uint32_t set = 0; set < G; set++)
-        stage_twiddles<C>(twl + (set << C), p.tw, p.LT, 0, bitrev32(tau0 + set, p.L - C) << C);
     // load: thread (g, t0) reads elements t = t0 + 64 j of tile g, natural index
     // (bitrev_C(t) << (L - C)) | (tau0 + g), bitrev_C(t) = (bitrev_6(t0) << (C - 6)) | bitrev_{C-6}(j)
     const uint32_t g = tid & (G - 1), t0 = tid >> p.G_log, sh = p.L - C;
@@ -585,6 +604,11 @@ __global__ __launch_bounds__(512, 2) void lde_interp_first_kernel(FftPass p) {
 #pragma unroll
         for (uint32_t j = 0; j < PER; j++) pf[j] = src[(uint64_t)bitrev32(j, C - 6) << sh];
     }
+    // tile set g covers k-space bitrev(tau0 + g) << C
+    stage_twiddle_sets<C, 16>(twl, G, [&](uint32_t set, uint32_t ls, uint32_t i) {
+        const uint32_t kb = bitrev32(tau0 + set, p.L - C) << C;
+        return p.tw[(1u << (p.LT - 1 - ls)) + (kb >> (ls + 1)) + i];
+    });
     __syncthreads();  // twiddle sets staged
     for (uint32_t col = c0; col < c1; col++) {
 #pragma unroll
@@ -772,9 +796,6 @@ __global__ __launch_bounds__(512, 2) void lde_mid_kernel(LdeFused f) {
     const uint32_t NQ = 1u << p.beta;
     uint32_t *twl = lds + G * TS;  // set 0: interpolation, set 1 + q: evaluation of q
     const uint32_t hi_bits = tile_fixed(p, tg, 0) & ~((1u << p.s_lo) - 1);
-    stage_twiddles<C>(twl, f.tw_inv, p.L, p.s_lo, hi_bits);
-    for (uint32_t q = 0; q < NQ; q++)
-        stage_twiddles<C>(twl + ((1 + q) << C), f.tw_fwd, p.L + p.beta, p.s_lo, (q << p.L) | hi_bits);
     using Top = Grp<false, C, 0>;
     uint32_t *tl = lds + wave * TS;
     // the strided copies: thread (g, t0) moves elements t = t0 + 64 j of tile g, which sit
@@ -791,6 +812,14 @@ __global__ __launch_bounds__(512, 2) void lde_mid_kernel(LdeFused f) {
 #pragma unroll
         for (uint32_t j = 0; j < PER; j++) pf[j] = src[(uint64_t)j << jsh];
     }
+    // set 0: the interpolation's stages s_lo.. of the size-2^L table; set 1 + q: coset q's of the
+    // size-2^(L + beta) table
+    stage_twiddle_sets<C, 16>(twl, 1 + NQ, [&](uint32_t set, uint32_t ls, uint32_t i) {
+        const uint32_t s = p.s_lo + ls;
+        if (set == 0) return f.tw_inv[(1u << (p.L - 1 - s)) + (hi_bits >> (s + 1)) + i];
+        const uint32_t kb = ((set - 1) << p.L) | hi_bits;
+        return f.tw_fwd[(1u << (p.L + p.beta - 1 - s)) + (kb >> (s + 1)) + i];
+    });
     __syncthreads();  // twiddle sets staged
     for (uint32_t col = c0; col < c1; col++) {
 #pragma unroll
