@@ -182,27 +182,40 @@ __device__ __forceinline__ void stage_twiddles(uint32_t *twl, const uint32_t *tw
 }
 
 // Stage nsets twiddle sets at once (set k at twl + (k << C), same layout as
-// stage_twiddles): the loads of the whole batch are in flight together, so a
-// workgroup pays one memory latency for its twiddles.  entry(set, ls, i) reads
-// entry i of local stage ls of the set's global table.
-template <int C, uint32_t B, typename ENTRY>
+// stage_twiddles).  A thread covers heap slots idx = tid + m * blockDim.x of
+// every set, so their stage, entry and LDS position are worked out once per m
+// and reused for all sets; the loads of up to 8 sets x 4 slots are in flight
+// together.  entry(set, ls, i) reads entry i of local stage ls of the set's table.
+template <int C, typename ENTRY>
 __device__ __forceinline__ void stage_twiddle_sets(uint32_t *twl, uint32_t nsets, ENTRY entry) {
-    constexpr uint32_t T = 1u << C;
+    constexpr uint32_t T = 1u << C, KM = 4, SB = 8;
     const uint32_t nt = blockDim.x, tid = threadIdx.x;
-    batched_copy<B>(
-        strided_count(tid, nt, nsets << C),
-        [&](uint32_t i) {
-            const uint32_t e = tid + i * nt, idx = e & (T - 1);
-            if (idx == 0) return 0u;  // heap slot 0 is unused
-            const uint32_t lvl = 31 - __builtin_clz(idx);
-            return entry(e >> C, C - 1 - lvl, idx - (1u << lvl));
-        },
-        [&](uint32_t i, uint32_t v) {
-            const uint32_t e = tid + i * nt, idx = e & (T - 1);
-            if (idx == 0) return;
-            const uint32_t lvl = 31 - __builtin_clz(idx);
-            twl[((e >> C) << C) + (1u << lvl) + tw_pos<C>(C - 1 - (int)lvl, idx - (1u << lvl))] = v << 1;
-        });
+    for (uint32_t m0 = 0; m0 * nt < T; m0 += KM) {
+        uint32_t ls[KM], ent[KM], pos[KM];
+        bool ok[KM];
+#pragma unroll
+        for (uint32_t k = 0; k < KM; k++) {
+            const uint32_t idx = tid + (m0 + k) * nt;
+            ok[k] = idx != 0 && idx < T;  // heap slot 0 is unused
+            const uint32_t x = ok[k] ? idx : 1u, lvl = 31 - __builtin_clz(x);
+            ls[k] = C - 1 - lvl;
+            ent[k] = x - (1u << lvl);
+            pos[k] = (1u << lvl) + tw_pos<C>((int)ls[k], ent[k]);
+        }
+        for (uint32_t s0 = 0; s0 < nsets; s0 += SB) {
+            uint32_t v[SB][KM];
+#pragma unroll
+            for (uint32_t b = 0; b < SB; b++)
+#pragma unroll
+                for (uint32_t k = 0; k < KM; k++)
+                    v[b][k] = (s0 + b < nsets && ok[k]) ? entry(s0 + b, ls[k], ent[k]) : 0u;
+#pragma unroll
+            for (uint32_t b = 0; b < SB; b++)
+#pragma unroll
+                for (uint32_t k = 0; k < KM; k++)
+                    if (s0 + b < nsets && ok[k]) twl[((s0 + b) << C) + pos[k]] = v[b][k] << 1;
+        }
+    }
 }
 
 // workgroup -> tile group, remapped so consecutive tile groups share an XCD
@@ -542,7 +555,7 @@ __global__ __launch_bounds__(512, 2) void lde_eval_last_kernel(FftPass p) {
         for (uint32_t j = 0; j < PER; j++) pf[j] = src[j * 64];
     }
     // tile set (q, g) covers k-space ((qblk + q) << L) | (bitrev(tau0 + g) << C)
-    stage_twiddle_sets<C, 16>(twl, Q * G, [&](uint32_t set, uint32_t ls, uint32_t i) {
+    stage_twiddle_sets<C>(twl, Q * G, [&](uint32_t set, uint32_t ls, uint32_t i) {
         const uint32_t kb = ((qblk + (set >> p.G_log)) << p.L) | (bitrev32(tau0 + (set & (G - 1)), p.L - C) << C);
         return p.tw[(1u << (p.LT - 1 - ls)) + (kb >> (ls + 1)) + i];
     });
@@ -605,7 +618,7 @@ __global__ __launch_bounds__(512, 2) void lde_interp_first_kernel(FftPass p) {
         for (uint32_t j = 0; j < PER; j++) pf[j] = src[(uint64_t)bitrev32(j, C - 6) << sh];
     }
     // tile set g covers k-space bitrev(tau0 + g) << C
-    stage_twiddle_sets<C, 16>(twl, G, [&](uint32_t set, uint32_t ls, uint32_t i) {
+    stage_twiddle_sets<C>(twl, G, [&](uint32_t set, uint32_t ls, uint32_t i) {
         const uint32_t kb = bitrev32(tau0 + set, p.L - C) << C;
         return p.tw[(1u << (p.LT - 1 - ls)) + (kb >> (ls + 1)) + i];
     });
@@ -814,7 +827,7 @@ __global__ __launch_bounds__(512, 2) void lde_mid_kernel(LdeFused f) {
     }
     // set 0: the interpolation's stages s_lo.. of the size-2^L table; set 1 + q: coset q's of the
     // size-2^(L + beta) table
-    stage_twiddle_sets<C, 16>(twl, 1 + NQ, [&](uint32_t set, uint32_t ls, uint32_t i) {
+    stage_twiddle_sets<C>(twl, 1 + NQ, [&](uint32_t set, uint32_t ls, uint32_t i) {
         const uint32_t s = p.s_lo + ls;
         if (set == 0) return f.tw_inv[(1u << (p.L - 1 - s)) + (hi_bits >> (s + 1)) + i];
         const uint32_t kb = ((set - 1) << p.L) | hi_bits;
