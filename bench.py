@@ -50,9 +50,9 @@ def kernel_bytes(n_cols, log_n, log_blowup):
       row leaves: read every LDE value, write one 32-byte digest per row."""
     n, N = 1 << log_n, 1 << (log_n + log_blowup)
     return {
-        "fft_tile_kernel<inv,10,natural-in>": 8 * n_cols * n,
+        "lde_interp_first_kernel<10>": 8 * n_cols * n,
         "lde_mid_kernel<10>": 4 * n_cols * n + 4 * n_cols * N,
-        "fft_tile_kernel<fwd,10,natural-out>": 8 * n_cols * N,
+        "lde_eval_last_kernel<10>": 8 * n_cols * N,
         "row_leaf_kernel": 4 * n_cols * N + 32 * N,
     }
 

@@ -694,6 +694,22 @@ TileConfig default_tile_config() {
     }();
     return c;
 }
+// LDE passes without a tuned entry: per kind of pass, the shape the autotuner
+// picks for 64 x 2^20 columns (interpolation 2^12 x 4, middle 2^13 x 8,
+// evaluation 2^13 x 4); ZS_TILE_ELEMS_LOG / ZS_COLS_PER_WG, when set, apply to all.
+LdeTiles default_lde_tiles() {
+    static const LdeTiles t = [] {
+        LdeTiles d;
+        d.role[LDE_INTERP] = {12, 4};
+        d.role[LDE_MID] = {13, 8};
+        d.role[LDE_EVAL] = {13, 4};
+        if (getenv("ZS_TILE_ELEMS_LOG") || getenv("ZS_COLS_PER_WG"))
+            for (auto &r : d.role) r = default_tile_config();
+        return d;
+    }();
+    return t;
+}
+
 // columns per workgroup, reduced so the grid still has >= 2048 workgroups
 static uint32_t pick_ncb(const Ctx *ctx, uint32_t n_cols, uint32_t wgs_per_col) {
     uint32_t ncb = std::min(ctx->tile.cols_per_wg, std::max<uint32_t>(n_cols, 1));

@@ -90,11 +90,7 @@ enum LdeRole { LDE_INTERP = 0, LDE_MID = 1, LDE_EVAL = 2 };
 struct LdeTiles {
     TileConfig role[3];
 };
-inline LdeTiles default_lde_tiles() {
-    LdeTiles t;
-    for (auto &r : t.role) r = default_tile_config();
-    return t;
-}
+LdeTiles default_lde_tiles();
 // key of an LDE shape in the tuned table: columns rounded up to a power of two
 inline uint64_t lde_shape_key(uint32_t n_cols, uint32_t log_n, uint32_t log_blowup) {
     uint32_t cl = 0;
