@@ -718,8 +718,16 @@ static uint32_t pick_ncb(const Ctx *ctx, uint32_t n_cols, uint32_t wgs_per_col) 
 }
 
 int32_t run_tile_pass(Ctx *ctx, FftPass p, bool inverse, uint32_t n_cols) {
+    // the LDE's first interpolation and last evaluation passes have pipelined
+    // one-wave-per-tile kernels (8..10 stages, at most 8 tiles per workgroup)
+    const bool pipe_c = p.C >= 8 && p.C <= 10 && p.s_lo == 0 && p.natural_tiles && !p.scale;
+    const bool pipe_interp = pipe_c && inverse && p.in_mode == IN_NATURAL && p.out_mode == OUT_KSPACE &&
+                             p.Q_log == 0 && p.beta == 0;
+    const bool pipe_eval = pipe_c && !inverse && p.in_mode == IN_KSPACE && p.out_mode == OUT_NATURAL;
+    if (pipe_eval) p.Q_log = std::min<uint32_t>(p.Q_log, 3);
     uint32_t avail = p.s_lo > 0 ? p.s_lo : (p.L - p.C);
     int32_t want = (int32_t)ctx->tile.elems_log - (int32_t)p.C - (int32_t)p.Q_log;
+    if (pipe_interp || pipe_eval) want = std::min<int32_t>(want, 3 - (int32_t)p.Q_log);
     p.G_log = (uint32_t)std::max<int32_t>(0, std::min<int32_t>((int32_t)avail, want));
     uint32_t tiles = 1u << (p.L - p.C);
     uint32_t ntile = 1u << (p.G_log + p.Q_log);
@@ -731,32 +739,25 @@ int32_t run_tile_pass(Ctx *ctx, FftPass p, bool inverse, uint32_t n_cols) {
     dim3 grid(gx, (n_cols + p.ncb - 1) / p.ncb, gz);
     uint32_t maxw = p.C >= 11 ? 8 : 16;
     dim3 block(64 * std::min<uint32_t>(ntile, maxw));
-    if (inverse && p.s_lo == 0 && p.in_mode == IN_NATURAL && p.out_mode == OUT_KSPACE && p.natural_tiles &&
-        !p.scale && p.Q_log == 0 && p.beta == 0 && ntile <= 8 && (p.C == 9 || p.C == 10)) {
-        // the LDE's first interpolation pass: the pipelined one-wave-per-tile kernel
+    if ((pipe_interp || pipe_eval) && ntile <= 8) {
         const size_t lds = (size_t(ntile) * p.TS + (size_t(ntile) << p.C)) * 4;
         if (lds <= 160 * 1024) {
             const dim3 blk(64 * ntile);
-            if (p.C == 10)
-                ZS_LAUNCH(ctx, "lde_interp_first_kernel<10>", (lde_interp_first_kernel<10>), grid, blk, lds,
-                          ctx->stream, p);
-            else
-                ZS_LAUNCH(ctx, "lde_interp_first_kernel<9>", (lde_interp_first_kernel<9>), grid, blk, lds,
-                          ctx->stream, p);
-            ZS_HIP(ctx, hipGetLastError());
-            return ZS_OK;
-        }
-    }
-    if (!inverse && p.s_lo == 0 && p.in_mode == IN_KSPACE && p.out_mode == OUT_NATURAL && p.natural_tiles &&
-        !p.scale && ntile <= 8 && (p.C == 9 || p.C == 10)) {
-        // the LDE's last evaluation pass: the pipelined one-wave-per-tile kernel
-        const size_t lds = (size_t(ntile) * p.TS + (size_t(ntile) << p.C)) * 4;
-        if (lds <= 160 * 1024) {
-            const dim3 blk(64 * ntile);
-            if (p.C == 10)
-                ZS_LAUNCH(ctx, "lde_eval_last_kernel<10>", (lde_eval_last_kernel<10>), grid, blk, lds, ctx->stream, p);
-            else
-                ZS_LAUNCH(ctx, "lde_eval_last_kernel<9>", (lde_eval_last_kernel<9>), grid, blk, lds, ctx->stream, p);
+#define ZS_PIPE_CASE(CC)                                                                                        \
+    case CC:                                                                                                    \
+        if (pipe_interp)                                                                                        \
+            ZS_LAUNCH(ctx, "lde_interp_first_kernel<" #CC ">", (lde_interp_first_kernel<CC>), grid, blk, lds,   \
+                      ctx->stream, p);                                                                          \
+        else                                                                                                    \
+            ZS_LAUNCH(ctx, "lde_eval_last_kernel<" #CC ">", (lde_eval_last_kernel<CC>), grid, blk, lds,         \
+                      ctx->stream, p);                                                                          \
+        break;
+            switch (p.C) {
+                ZS_PIPE_CASE(8)
+                ZS_PIPE_CASE(9)
+                ZS_PIPE_CASE(10)
+            }
+#undef ZS_PIPE_CASE
             ZS_HIP(ctx, hipGetLastError());
             return ZS_OK;
         }
