@@ -1,0 +1,67 @@
+// b3_quad.h -- BLAKE3 compression shared by the four lanes of a quad (device only).
+//
+// Used where a compression's latency matters more than its throughput: the
+// narrow levels of Merkle trees near the root and the Fiat-Shamir channel.
+#pragma once
+#include "blake3.h"
+
+namespace zs {
+
+// LDS writes of a wave must land before its lanes read each other's values
+__device__ __forceinline__ void wave_lds_sync_m() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// ---------------------------------------------------------------------------
+// Four lanes per compression.
+//
+// One compression on one lane is about 700 dependent instructions; where only
+// a few compressions are in flight (tree levels near the root, the channel),
+// that latency is the cost.  Here the four lanes of a quad share a compression: lane j
+// holds column j of the 4x4 state (s[j], s[4+j], s[8+j], s[12+j]) and runs
+// the column G on it; for the diagonal G, rows 1-3 are rotated across the quad
+// by 1, 2 and 3 lanes (DPP quad_perm) and rotated back afterwards.  The
+// message words a lane needs in round r are m[PERM^r(2j)], m[PERM^r(2j+1)]
+// (column) and m[PERM^r(8+2j)], m[PERM^r(9+2j)] (diagonal): the quad writes
+// the block's 16 words to LDS and each lane reads its 28 through a fixed
+// per-lane index table.
+// ---------------------------------------------------------------------------
+
+// kQuadMsg[j][4r + s]: message word of lane j, round r, slot s (column x, y, diagonal x, y)
+static __constant__ uint8_t kQuadMsg[4][28] = {
+    {0, 1, 8, 9, 2, 6, 1, 11, 3, 4, 6, 5, 10, 7, 4, 0, 12, 13, 7, 2, 9, 14, 13, 3, 11, 15, 14, 10},
+    {2, 3, 10, 11, 3, 10, 12, 5, 10, 12, 9, 0, 12, 9, 11, 2, 9, 11, 5, 3, 11, 5, 0, 10, 5, 0, 2, 12},
+    {4, 5, 12, 13, 7, 0, 9, 14, 13, 2, 11, 15, 14, 3, 5, 8, 15, 10, 0, 1, 8, 12, 2, 6, 1, 9, 3, 4},
+    {6, 7, 14, 15, 4, 13, 15, 8, 7, 14, 8, 1, 13, 15, 1, 6, 14, 8, 6, 4, 15, 1, 4, 7, 8, 6, 7, 13},
+};
+
+static __constant__ uint32_t kIV[8] = {0x6A09E667u, 0xBB67AE85u, 0x3C6EF372u, 0xA54FF53Au,
+                                       0x510E527Fu, 0x9B05688Cu, 0x1F83D9ABu, 0x5BE0CD19u};
+
+// lane j of the quad reads lane p[j]: rotations by 1, 2, 3 lanes
+constexpr int QROT1 = 0x39, QROT2 = 0x4E, QROT3 = 0x93;
+template <int CTRL>
+__device__ __forceinline__ uint32_t qperm(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, true);
+}
+
+__device__ __forceinline__ void quad_compress(uint32_t &a, uint32_t &b, uint32_t &c, uint32_t &d, const uint32_t *msg,
+                                              const uint32_t (&mo)[28]) {
+#pragma unroll
+    for (int r = 0; r < 7; r++) {
+        const uint32_t m0 = msg[mo[4 * r]], m1 = msg[mo[4 * r + 1]];
+        const uint32_t m2 = msg[mo[4 * r + 2]], m3 = msg[mo[4 * r + 3]];
+        ZS_B3_G(a, b, c, d, m0, m1);
+        b = qperm<QROT1>(b);
+        c = qperm<QROT2>(c);
+        d = qperm<QROT3>(d);
+        ZS_B3_G(a, b, c, d, m2, m3);
+        b = qperm<QROT3>(b);
+        c = qperm<QROT2>(c);
+        d = qperm<QROT1>(d);
+    }
+}
+
+}  // namespace zs

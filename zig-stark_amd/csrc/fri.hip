@@ -11,7 +11,7 @@
 // with no inversion.  Field results are canonical, so the output equals the
 // reference's value for value.  The kernel also writes the hashQM31 leaf of
 // each new value, fusing the next layer's leaf pass into the fold.
-#include "blake3.h"
+#include "b3_quad.h"
 #include "internal.h"
 
 namespace zs {
@@ -163,26 +163,57 @@ int32_t launch_combine(Ctx *ctx, const uint32_t *d_lde, uint64_t cs, uint32_t n_
 
 namespace zs {
 
-// Fiat-Shamir on the device: one lane absorbs the 32-byte root at `digest` into
-// the channel state and draws a QM31 challenge (fri.zig:205-210 with
-// channel.zig:49-69), so a layer's fold never waits for the host.  The state is
-// staged in LDS (the byte-level hasher would otherwise make every access a
-// dependent global load) and written back.
+// The hasher's compression run by one quad of lanes (b3_quad.h): every lane of
+// the quad holds the same chaining value and message; lane j computes column j
+// of the state and the quad trades the result through LDS.
+struct B3QuadCF {
+    uint32_t *qs;  // the quad's 24 LDS words: message, then chaining value
+    uint32_t j;    // lane in the quad
+    uint32_t mo[28];
+    __device__ void operator()(uint32_t cv[8], const uint32_t m[16], uint32_t counter_lo, uint32_t counter_hi,
+                               uint32_t block_len, uint32_t flags) {
+        for (int i = 0; i < 16; i++) qs[i] = m[i];
+        for (int i = 0; i < 8; i++) qs[16 + i] = cv[i];
+        wave_lds_sync_m();
+        uint32_t a = qs[16 + j], b = qs[20 + j], c = kIV[j];
+        uint32_t d = j == 0 ? counter_lo : (j == 1 ? counter_hi : (j == 2 ? block_len : flags));
+        quad_compress(a, b, c, d, qs, mo);
+        wave_lds_sync_m();  // every lane has read the message and the chaining value
+        qs[16 + j] = a ^ c;
+        qs[20 + j] = b ^ d;
+        wave_lds_sync_m();
+        for (int i = 0; i < 8; i++) cv[i] = qs[16 + i];
+        wave_lds_sync_m();  // read before the next compression rewrites qs
+    }
+};
+
+// Fiat-Shamir on the device: absorb the 32-byte root at `digest` and draw a
+// QM31 challenge (fri.zig:205-210 with channel.zig:31-43, 49-69 and
+// qm31.zig:95-101), so a layer's fold never waits for the host.  The state is staged in LDS; the four lanes
+// of quad 0 run the hasher's control flow in lockstep (identical state, so
+// identical writes) and share each compression.
 __global__ __launch_bounds__(64) void channel_draw_kernel(B3Hasher *st, const uint32_t *digest, uint32_t *out) {
     __shared__ B3Hasher h, snap;
+    __shared__ uint32_t qs[24], work[16];
     constexpr uint32_t W = sizeof(B3Hasher) / 4;
     static_assert(sizeof(B3Hasher) % 4 == 0, "hasher state is word-sized");
     uint32_t *hw = reinterpret_cast<uint32_t *>(&h);
     const uint32_t *sw = reinterpret_cast<const uint32_t *>(st);
     for (uint32_t k = threadIdx.x; k < W; k += blockDim.x) hw[k] = sw[k];
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < 4) {
+        B3QuadCF cf;
+        cf.qs = qs;
+        cf.j = threadIdx.x;
+#pragma unroll
+        for (int k = 0; k < 28; k++) cf.mo[k] = kQuadMsg[cf.j][k];
         uint32_t d[8];
         for (int i = 0; i < 8; i++) d[i] = digest[i];
-        channel_absorb_digest(h, d);
+        channel_absorb_digest(h, d, work, cf);
         uint32_t q[4];
-        channel_sample_qm31(h, snap, q);
-        for (int i = 0; i < 4; i++) out[i] = q[i];
+        channel_sample_qm31(h, snap, q, work, cf);
+        if (cf.j == 0)
+            for (int i = 0; i < 4; i++) out[i] = q[i];
     }
     __syncthreads();
     uint32_t *dw = reinterpret_cast<uint32_t *>(st);
