@@ -257,41 +257,54 @@ def batch_leg(Z, torch, dev, stream, n_traces, autotune):
 
 
 def serve_leg(Z, n_requests, n_workers=4, budget=200 * 10**9):
+    """Continuous batching under Poisson arrivals: 200/s (more than the server
+    completes, so it shows the throughput and a growing queue), then 100/s
+    (below capacity, so the latencies are service times)."""
     import numpy as np
-    rng = np.random.default_rng(5)
-    gaps = rng.exponential(1 / 200.0, n_requests)
-    lgs = rng.integers(14, 23, n_requests)
-    cols = rng.integers(16, 257, n_requests)
     srv = Z.Server(0, n_workers=n_workers, hbm_budget=budget)
     # warm every size on the workers (twiddle tables and FRI tables stay resident)
     warm = [srv.submit(16, lg, 2, 5, seed=lg) for lg in range(14, 23) for _ in range(n_workers)]
     for t in warm:
         srv.wait(t)
-    t0 = time.perf_counter()
-    at = np.cumsum(gaps)
-    tickets = []
-    for k in range(n_requests):
-        d = t0 + at[k] - time.perf_counter()
-        if d > 0:
-            time.sleep(d)
-        tickets.append(srv.submit(int(cols[k]), int(lgs[k]), 2, 5, seed=(5 << 32) + k))
-    resp = [srv.wait(t) for t in tickets]
-    _, peak = srv.memory()
-    srv.close()
-    ok = all(r.status == 0 for r in resp)
-    lat = np.array([r.t_done - r.t_submit for r in resp]) * 1e3
-    qwait = np.array([r.t_admit - r.t_submit for r in resp]) * 1e3
-    span = max(r.t_done for r in resp) - min(r.t_submit for r in resp)
-    elems = int(sum(int(c) << int(lg) for c, lg in zip(cols, lgs)))
-    return {"workload": f"{n_requests} requests, Poisson arrivals mean 200/s, log rows U{{14..22}}, "
-                        f"columns U{{16..256}} (seed 5), blowup 4, LDE + row tree + FRI to 2^5, "
-                        f"inputs generated on the device from the request seed",
-            "workers": n_workers, "budget_bytes": budget, "peak_admitted_bytes": peak, "all_ok": ok,
-            "offered_rate_per_s": round(n_requests / at[-1], 1),
-            "completed_per_s": round(n_requests / span, 1), "Melem_per_s": round(elems / span / 1e6, 1),
+
+    def run_at(rate, n, seed):
+        rng = np.random.default_rng(seed)
+        gaps = rng.exponential(1 / rate, n)
+        lgs = rng.integers(14, 23, n)
+        cols = rng.integers(16, 257, n)
+        t0 = time.perf_counter()
+        at = np.cumsum(gaps)
+        tickets = []
+        for k in range(n):
+            d = t0 + at[k] - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            tickets.append(srv.submit(int(cols[k]), int(lgs[k]), 2, 5, seed=(seed << 32) + k))
+        resp = [srv.wait(t) for t in tickets]
+        lat = np.array([r.t_done - r.t_submit for r in resp]) * 1e3
+        qwait = np.array([r.t_admit - r.t_submit for r in resp]) * 1e3
+        span = max(r.t_done for r in resp) - min(r.t_submit for r in resp)
+        elems = int(sum(int(c) << int(lg) for c, lg in zip(cols, lgs)))
+        return resp, {
+            "offered_rate_per_s": round(n / at[-1], 1),
+            "completed_per_s": round(n / span, 1), "Melem_per_s": round(elems / span / 1e6, 1),
             "latency_ms_p50": round(float(np.percentile(lat, 50)), 2),
             "latency_ms_p99": round(float(np.percentile(lat, 99)), 2),
             "queue_wait_ms_p50": round(float(np.percentile(qwait, 50)), 2)}
+
+    resp, over = run_at(200.0, n_requests, 5)
+    resp2, under = run_at(100.0, max(1, (3 * n_requests) // 4), 6)
+    _, peak = srv.memory()
+    srv.close()
+    ok = all(r.status == 0 for r in resp + resp2)
+    out = {"workload": f"{n_requests} requests, Poisson arrivals mean 200/s, log rows U{{14..22}}, "
+                       f"columns U{{16..256}} (seed 5), blowup 4, LDE + row tree + FRI to 2^5, "
+                       f"inputs generated on the device from the request seed",
+           "workers": n_workers, "budget_bytes": budget, "peak_admitted_bytes": peak, "all_ok": ok}
+    out.update(over)
+    out["at_100_per_s"] = dict(under, workload=f"{max(1, (3 * n_requests) // 4)} requests, Poisson mean 100/s, "
+                                                f"same size distribution (seed 6)")
+    return out
 
 
 def cpu_baseline(n_cols, log_n, log_blowup):
