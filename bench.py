@@ -336,6 +336,12 @@ def cpu_baseline(n_cols, log_n, log_blowup):
         list(ex.map(lambda c: O.lde(c, log_blowup), par_cols))
         t_wave = time.perf_counter() - t0
     total_par = -(-n_cols // threads) * t_wave + (N / rows) * t_rows / threads
+    # sanity check of the port against the only CPU numbers the reference publishes for this
+    # path (README.md:114-129): circleFFT / circleIFFT of 2^16 on one Ryzen 7 5800H thread,
+    # twiddles computed per call, as the oracle does
+    x = O.fill_m31(SEED + 3, 1 << 16)
+    t_fft = min(_timed_cpu(lambda: O.circle_fft(x)) for _ in range(3))
+    t_ifft = min(_timed_cpu(lambda: O.circle_ifft(x)) for _ in range(3))
     return {
         "value": round(elems / total / 1e6, 4),
         "unit": "Melem/s",
@@ -347,7 +353,16 @@ def cpu_baseline(n_cols, log_n, log_blowup):
         "parallel": {"value": round(elems / total_par / 1e6, 4), "unit": "Melem/s", "cores": threads,
                      "sample": f"{threads} full columns' LDE at once on {threads} threads "
                                f"({t_wave:.2f} s), extrapolated to {n_cols} columns = {total_par:.1f} s per job"},
+        "circle_fft_2p16_ms": {"oracle_fft": round(t_fft * 1e3, 2), "oracle_ifft": round(t_ifft * 1e3, 2),
+                               "reference_published_fft": 21.786, "reference_published_ifft": 24.592,
+                               "reference_hardware": "Ryzen 7 5800H, 1 thread, Zig 0.16 ReleaseFast"},
     }
+
+
+def _timed_cpu(fn):
+    t0 = time.perf_counter()
+    fn()
+    return time.perf_counter() - t0
 
 
 def main():
