@@ -57,6 +57,21 @@ def kernel_bytes(n_cols, log_n, log_blowup):
     }
 
 
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # MI355X: 256 CUs x 64 lanes x 2.4 GHz, 32-bit integer lane-ops
+
+
+def load_valu(path):
+    """Measured VALU instructions per dispatch (rocprofv3 SQ_INSTS_VALU, summed over
+    waves) from the summary committed under profiles/: lines 'valu <kernel> <count>'."""
+    out = {}
+    if os.path.exists(path):
+        for line in open(path):
+            f = line.split()
+            if len(f) == 3 and f[0] == "valu":
+                out[f[1]] = int(f[2])
+    return out
+
+
 def load_traffic(path):
     """Measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated),
     from the summary committed under profiles/: lines 'traffic <kernel> <bytes>'."""
@@ -469,6 +484,8 @@ def main():
     alg = algorithmic_bytes(n_cols, lg, beta)
     achieved = kbytes[dom] / (dom_ms * 1e-3) / 1e9
     traffic = load_traffic(os.path.join(ROOT, "profiles", "hbm_traffic_lde_commit_64x2p20.txt")).get(dom)
+    valu_insts = load_valu(os.path.join(ROOT, "profiles", "valu_lde_commit_64x2p20.txt")).get(dom)
+    valu_tops = valu_insts * 64 / (dom_ms * 1e-3) / 1e12 if valu_insts else None
     out = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -488,6 +505,11 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "algorithmic_bytes": kbytes[dom], "avg_launch_ms": round(dom_ms, 4)},
+        "valu_roofline": {"kernel": dom, "achieved": round(valu_tops, 2) if valu_tops else None,
+                          "peak": round(VALU_PEAK_TOPS, 2), "unit": "T lane-ops/s",
+                          "frac": round(valu_tops / VALU_PEAK_TOPS, 4) if valu_tops else None,
+                          "insts_per_launch": valu_insts,
+                          "note": "SQ_INSTS_VALU x 64 lanes / live launch time; the bound this path meets"},
         "step_hbm": {"algorithmic_bytes": alg, "GBps": round(alg / (ms * 1e-3) / 1e9, 1),
                      "frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "stages_ms": {"lde": round(lde_ms, 4), "commit": round(commit_ms, 4)},
