@@ -376,23 +376,28 @@ int32_t zs_autotune_lde(zs_hip_ctx *h, uint32_t n_cols, uint32_t log_n, uint32_t
     float total_ms = 0;
     if (rc == ZS_OK) {
         set_tuned(best);
-        hipEvent_t a, b;
-        ZS_HIP(ctx, hipEventCreate(&a));
-        ZS_HIP(ctx, hipEventCreate(&b));
-        const int reps = 3;
-        rc = zs_m31_lde_dev(h, cols.as<uint32_t>(), n_cols, log_n, log_blowup, lde.as<uint32_t>());
-        (void)hipEventRecord(a, ctx->stream);
-        for (int r = 0; r < reps && rc == ZS_OK; r++)
-            rc = zs_m31_lde_dev(h, cols.as<uint32_t>(), n_cols, log_n, log_blowup, lde.as<uint32_t>());
-        (void)hipEventRecord(b, ctx->stream);
-        if (rc == ZS_OK && hipEventSynchronize(b) == hipSuccess) {
-            (void)hipEventElapsedTime(&total_ms, a, b);
-            total_ms /= reps;
-        } else if (rc == ZS_OK) {
+        // no early return from here on: the caller's profiling state is restored below
+        hipEvent_t a = nullptr, b = nullptr;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+            ctx->set_error("autotune: hipEventCreate failed");
             rc = ZS_ERR_DEVICE;
         }
-        (void)hipEventDestroy(a);
-        (void)hipEventDestroy(b);
+        const int reps = 3;
+        if (rc == ZS_OK) rc = zs_m31_lde_dev(h, cols.as<uint32_t>(), n_cols, log_n, log_blowup, lde.as<uint32_t>());
+        if (rc == ZS_OK) {
+            (void)hipEventRecord(a, ctx->stream);
+            for (int r = 0; r < reps && rc == ZS_OK; r++)
+                rc = zs_m31_lde_dev(h, cols.as<uint32_t>(), n_cols, log_n, log_blowup, lde.as<uint32_t>());
+            (void)hipEventRecord(b, ctx->stream);
+            if (rc == ZS_OK && hipEventSynchronize(b) == hipSuccess) {
+                (void)hipEventElapsedTime(&total_ms, a, b);
+                total_ms /= reps;
+            } else if (rc == ZS_OK) {
+                rc = ZS_ERR_DEVICE;
+            }
+        }
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
     }
     prof_clear(ctx);
     ctx->prof.swap(saved_prof);
