@@ -589,12 +589,31 @@ int32_t zs::fri_commit_impl(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t 
     alpha_dev = ctx->scratch[6].as<uint32_t>();
     const uint32_t *fri_tab;
     ZS_TRY(get_fri_tables(ctx, dlog, &fri_tab));  // cached before any capture
-    (void)fri_tab;
     if (alphas)
         ZS_HIP(ctx, hipMemcpyAsync(alpha_dev, alphas, size_t(L) * 16, hipMemcpyHostToDevice, ctx->stream));
     // Leaves of layer 0, then per layer: node levels, (challenge), fold + next
     // layer's leaves.  With the challenges given, no fold waits for a root: the
     // folds run first and the trees of all layers are built together after them.
+    // layers of at most 2^FRI_TAIL_LOG values go to one single-workgroup launch
+    const uint32_t i_tail = std::min(L, dlog > FRI_TAIL_LOG ? dlog - FRI_TAIL_LOG : 0u);
+    auto tail = [&]() -> int32_t {
+        FriTail t{};
+        t.in = layer(i_tail);
+        t.layers = layers;
+        t.trees = trees;
+        t.tables = fri_tab;
+        t.alphas = alpha_dev;
+        t.chan = alphas ? nullptr : d_chan;
+        t.dlog = dlog;
+        t.i0 = i_tail;
+        t.L = L;
+        for (uint32_t i = i_tail + 1; i <= L; i++) t.lay_off[i] = r->layer_off[i];
+        for (uint32_t i = i_tail; i < L; i++) {
+            t.tree_off[i] = r->tree_off[i];
+            t.inv_off[i] = qm_inv(fri_offset_pow(i));
+        }
+        return launch_fri_tail(ctx, t);
+    };
     auto enqueue = [&]() -> int32_t {
         ZS_TRY(launch_qm31_leaves(ctx, layer(0), n0, trees));
         if (alphas) {
@@ -602,15 +621,18 @@ int32_t zs::fri_commit_impl(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t 
             std::vector<uint32_t> logs(L);
             for (uint32_t i = 0; i < L; i++) {
                 uint32_t li = dlog - i;
-                uint32_t *next_leaves = (i + 1 < L) ? trees + r->tree_off[i + 1] : nullptr;
-                ZS_TRY(launch_fri_fold(ctx, layer(i), size_t(1) << li, dlog, i, alpha_dev + 4 * i, layer(i + 1),
-                                       next_leaves));
+                if (i < i_tail) {
+                    uint32_t *next_leaves = (i + 1 < L) ? trees + r->tree_off[i + 1] : nullptr;
+                    ZS_TRY(launch_fri_fold(ctx, layer(i), size_t(1) << li, dlog, i, alpha_dev + 4 * i, layer(i + 1),
+                                           next_leaves));
+                }
                 offs[i] = r->tree_off[i];
                 logs[i] = li;
             }
+            if (i_tail < L) ZS_TRY(tail());
             return launch_trees_multi(ctx, trees, offs.data(), logs.data(), L);
         }
-        for (uint32_t i = 0; i < L; i++) {
+        for (uint32_t i = 0; i < i_tail; i++) {
             uint32_t li = dlog - i;
             ZS_TRY(launch_tree_levels(ctx, trees + r->tree_off[i], li, 0));
             // the challenge is drawn on the device from this layer's root
@@ -619,7 +641,7 @@ int32_t zs::fri_commit_impl(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t 
             ZS_TRY(launch_fri_fold(ctx, layer(i), size_t(1) << li, dlog, i, alpha_dev + 4 * i, layer(i + 1),
                                    next_leaves));
         }
-        return ZS_OK;
+        return i_tail < L ? tail() : ZS_OK;
     };
     const bool graph = !out_result && !ctx->profiling && !(flags & ZS_FLAG_NO_GRAPH);
     if (!graph) {

@@ -167,9 +167,9 @@ namespace zs {
 // the quad holds the same chaining value and message; lane j computes column j
 // of the state and the quad trades the result through LDS.
 struct B3QuadCF {
-    uint32_t *qs;  // the quad's 24 LDS words: message, then chaining value
-    uint32_t j;    // lane in the quad
-    uint32_t mo[28];
+    uint32_t *qs;               // the quad's 24 LDS words: message, then chaining value
+    uint32_t j;                 // lane in the quad
+    const uint32_t (&mo)[28];   // the lane's message-word table (kQuadMsg[j])
     __device__ void operator()(uint32_t cv[8], const uint32_t m[16], uint32_t counter_lo, uint32_t counter_hi,
                                uint32_t block_len, uint32_t flags) {
         for (int i = 0; i < 16; i++) qs[i] = m[i];
@@ -202,11 +202,10 @@ __global__ __launch_bounds__(64) void channel_draw_kernel(B3Hasher *st, const ui
     for (uint32_t k = threadIdx.x; k < W; k += blockDim.x) hw[k] = sw[k];
     __syncthreads();
     if (threadIdx.x < 4) {
-        B3QuadCF cf;
-        cf.qs = qs;
-        cf.j = threadIdx.x;
+        uint32_t mo[28];
 #pragma unroll
-        for (int k = 0; k < 28; k++) cf.mo[k] = kQuadMsg[cf.j][k];
+        for (int k = 0; k < 28; k++) mo[k] = kQuadMsg[threadIdx.x][k];
+        B3QuadCF cf{qs, threadIdx.x, mo};
         uint32_t d[8];
         for (int i = 0; i < 8; i++) d[i] = digest[i];
         channel_absorb_digest(h, d, work, cf);
@@ -223,6 +222,145 @@ __global__ __launch_bounds__(64) void channel_draw_kernel(B3Hasher *st, const ui
 int32_t launch_channel_draw(Ctx *ctx, B3Hasher *d_state, const uint32_t *d_digest, uint32_t *d_out) {
     ZS_LAUNCH(ctx, "channel_draw_kernel", channel_draw_kernel, dim3(1), dim3(64), 0, ctx->stream, d_state, d_digest,
               d_out);
+    ZS_HIP(ctx, hipGetLastError());
+    return ZS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// FRI tail: every layer of at most 2^FRI_TAIL_LOG values in one workgroup.
+//
+// Small layers are latency-bound: each costs a tree launch or two, a channel
+// launch and a fold launch, most of it launch latency.  Here one workgroup keeps
+// the layer values and the tree levels in LDS and, per layer, builds the tree
+// (quad-lane hash2 per node, each level also written to its place in HBM),
+// draws the challenge from the root on one quad (channel mode) or reads it
+// (given challenges: the trees are then built with the other layers'), folds,
+// and hashes the next layer's leaves (fri.zig:199-219).
+// ---------------------------------------------------------------------------
+constexpr uint32_t FRI_TAIL_NT = 512;
+
+__global__ __launch_bounds__(FRI_TAIL_NT) void fri_tail_kernel(FriTail t) {
+    extern __shared__ uint32_t sm[];
+    constexpr uint32_t NV = 1u << FRI_TAIL_LOG;
+    uint32_t *const val0 = sm, *const val1 = sm + 4 * NV;       // layer values (QM31 words), ping-pong
+    uint32_t *const lv0 = sm + 8 * NV, *const lv1 = sm + 16 * NV;  // tree levels: NV digests, then NV / 2
+    auto val = [&](uint32_t k) { return k ? val1 : val0; };
+    auto lv = [&](uint32_t k) { return k ? lv1 : lv0; };
+    uint32_t *qmb = sm + 20 * NV;                   // 32 words per quad (hash2 blocks)
+    B3Hasher *h = reinterpret_cast<B3Hasher *>(qmb + 32 * (FRI_TAIL_NT / 4));
+    B3Hasher *snap = h + 1;
+    uint32_t *qs = reinterpret_cast<uint32_t *>(snap + 1);  // 24 words: the channel's compressions
+    uint32_t *work = qs + 24;                               // 16 words: channel byte staging
+    uint32_t *al = work + 16;                               // 4 words: this layer's challenge
+    const uint32_t tid = threadIdx.x, j = tid & 3u, quad = tid >> 2;
+    const bool chan = t.chan != nullptr;
+    uint32_t *qm = qmb + 32 * quad;
+    uint32_t mo[28];
+#pragma unroll
+    for (int k = 0; k < 28; k++) mo[k] = kQuadMsg[j][k];
+    const uint32_t iv_lo = kIV[j], iv_hi = kIV[4 + j];
+    qm[20 + j] = 0u;
+    qm[24 + j] = 0u;
+    qm[28 + j] = 0u;
+    constexpr uint32_t HW = sizeof(B3Hasher) / 4;
+    {
+        const uint32_t n = 1u << (t.dlog - t.i0);
+        for (uint32_t k = tid; k < 4 * n; k += FRI_TAIL_NT) val0[k] = t.in[k];
+        if (chan) {
+            const uint32_t *leaves = t.trees + t.tree_off[t.i0];
+            for (uint32_t k = tid; k < 8 * n; k += FRI_TAIL_NT) lv0[k] = leaves[k];
+            const uint32_t *cw = reinterpret_cast<const uint32_t *>(t.chan);
+            uint32_t *hw = reinterpret_cast<uint32_t *>(h);
+            for (uint32_t k = tid; k < HW; k += FRI_TAIL_NT) hw[k] = cw[k];
+        }
+    }
+    __syncthreads();
+    uint32_t cur = 0;
+    for (uint32_t i = t.i0; i < t.L; i++) {
+        const uint32_t li = t.dlog - i, n = 1u << li;
+        if (chan) {
+            // the layer's tree from its leaves in lv[0]
+            uint32_t *tg = t.trees + t.tree_off[i];
+            uint64_t off = 0;  // words before the child level in tg
+            uint32_t P = n, src = 0;
+            for (uint32_t l = 1; l <= li; l++) {
+                const uint32_t *cs = lv(src);
+                uint32_t *ds = lv(src ^ 1u);
+                const uint64_t out = off + (uint64_t)P * 8;
+                P >>= 1;
+                for (uint32_t p = quad; p < P; p += FRI_TAIL_NT / 4) {
+                    uint32_t lo, hi;
+                    quad_hash2(cs + 16 * p, qm, mo, j, iv_lo, iv_hi, lo, hi);
+                    ds[8 * p + j] = lo;
+                    ds[8 * p + 4 + j] = hi;
+                    tg[out + 8 * p + j] = lo;
+                    tg[out + 8 * p + 4 + j] = hi;
+                }
+                __syncthreads();
+                off = out;
+                src ^= 1u;
+            }
+            // absorb the root (lv(src)) and draw the challenge (fri.zig:205-210)
+            if (tid < 4) {
+                B3QuadCF cf{qs, j, mo};
+                uint32_t d[8];
+                for (int w = 0; w < 8; w++) d[w] = lv(src)[w];
+                channel_absorb_digest(*h, d, work, cf);
+                uint32_t q[4];
+                channel_sample_qm31(*h, *snap, q, work, cf);
+                if (j == 0)
+                    for (int k = 0; k < 4; k++) {
+                        al[k] = q[k];
+                        t.alphas[4 * i + k] = q[k];
+                    }
+            }
+        } else if (tid < 4) {
+            al[tid] = t.alphas[4 * i + tid];
+        }
+        __syncthreads();
+        // fold layer i into layer i + 1 (fri.zig:211-219), hashing the new layer's leaves
+        const qm31 alpha = {{al[0], al[1]}, {al[2], al[3]}};
+        const uint32_t half = n >> 1;
+        const uint32_t *vin = val(cur);
+        uint32_t *vout = val(cur ^ 1u);
+        uint32_t *gout = t.layers + t.lay_off[i + 1];
+        uint32_t *gleaves = (i + 1 < t.L) ? t.trees + t.tree_off[i + 1] : nullptr;
+        const qm31 inv_off = t.inv_off[i];
+        for (uint32_t k = tid; k < half; k += FRI_TAIL_NT) {
+            const qm31 y0 = ld_qm(vin + 4 * k), y1 = ld_qm(vin + 4 * (k + half));
+            const uint64_t e = (uint64_t)k << i;
+            const qm31 xinv =
+                qm_mul(inv_off, qm_mul(ld_qm(t.tables + 4 * (e & 1023)), ld_qm(t.tables + 4 * (1024 + (e >> 10)))));
+            const qm31 hs = qm_half(qm_add(y0, y1));
+            const qm31 hd = qm_half(qm_mul(qm_sub(y0, y1), xinv));
+            const qm31 v = qm_add(hs, qm_mul(alpha, hd));
+            st_qm(vout + 4 * k, v);
+            st_qm(gout + 4 * k, v);
+            if (gleaves) {
+                uint32_t w[4] = {v.a.c0, v.a.c1, v.b.c0, v.b.c1};
+                uint32_t d[8];
+                b3_hash_qm31(w, d);
+                uint4 *o = reinterpret_cast<uint4 *>(gleaves + 8 * k);
+                o[0] = make_uint4(d[0], d[1], d[2], d[3]);
+                o[1] = make_uint4(d[4], d[5], d[6], d[7]);
+                if (chan)
+                    for (int w8 = 0; w8 < 8; w8++) lv0[8 * k + w8] = d[w8];
+            }
+        }
+        __syncthreads();
+        cur ^= 1u;
+    }
+    if (chan) {
+        uint32_t *cw = reinterpret_cast<uint32_t *>(t.chan);
+        const uint32_t *hw = reinterpret_cast<const uint32_t *>(h);
+        for (uint32_t k = tid; k < HW; k += FRI_TAIL_NT) cw[k] = hw[k];
+    }
+}
+
+int32_t launch_fri_tail(Ctx *ctx, const FriTail &t) {
+    const size_t lds = (size_t(20) << FRI_TAIL_LOG) * 4 + size_t(32) * (FRI_TAIL_NT / 4) * 4 + 2 * sizeof(B3Hasher) +
+                       (24 + 16 + 4) * 4;
+    ZS_LAUNCH(ctx, "fri_tail_kernel", fri_tail_kernel, dim3(1), dim3(FRI_TAIL_NT), lds, ctx->stream, t);
     ZS_HIP(ctx, hipGetLastError());
     return ZS_OK;
 }

@@ -198,6 +198,20 @@ int32_t launch_fri_fold(Ctx *ctx, const uint32_t *d_cur, uint64_t n, uint32_t dl
 // g[i] = sum_c powers[c] * lde_c[i]: QM31 combination of n_cols M31 columns of n values
 // device Fiat-Shamir: absorb the digest at d_digest, write a QM31 challenge to d_out
 int32_t launch_channel_draw(Ctx *ctx, B3Hasher *d_state, const uint32_t *d_digest, uint32_t *d_out);
+// FRI layers i0..L-1 of at most 2^FRI_TAIL_LOG values in one workgroup (fri.hip)
+constexpr uint32_t FRI_TAIL_LOG = 10;
+struct FriTail {
+    const uint32_t *in;      // layer i0 values (4 words per QM31)
+    uint32_t *layers;        // layer i > i0 at layers + lay_off[i]
+    uint32_t *trees;         // tree of layer i at trees + tree_off[i], leaves first (layer i0's written)
+    const uint32_t *tables;  // get_fri_tables(dlog)
+    uint32_t *alphas;        // challenge of layer i at alphas + 4 i: read, or written when chan is set
+    B3Hasher *chan;          // device channel (the trees are built and the challenges drawn here), or null
+    uint32_t dlog, i0, L;
+    uint64_t lay_off[29], tree_off[28];
+    qm31 inv_off[28];  // (FRI_OFFSET^(2^i))^-1
+};
+int32_t launch_fri_tail(Ctx *ctx, const FriTail &t);
 int32_t launch_qm31_powers(Ctx *ctx, const uint32_t *d_x, uint32_t *d_out, uint32_t n);
 // FRI commit phase on a device codeword; challenges from `alphas` (host) or drawn
 // on the device from the channel state at d_chan (left advanced past the last root)

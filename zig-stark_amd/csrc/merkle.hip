@@ -177,38 +177,6 @@ __global__ __launch_bounds__(256) void tree_subtree3_multi_kernel(uint32_t *base
     subtree_lane<3>(base + mt.child[t], base + mt.lvl1[t], mt.lane_end[t] - start, g - start);
 }
 
-// hash2 of the 16 words at xp (left || right digest) by the quad; lane j gets
-// digest words j and 4 + j.  qm: the quad's 32 LDS words (block 0, then block 1
-// whose words 4..15 are zero).  iv_lo / iv_hi: IV[j], IV[4 + j].
-__device__ __forceinline__ void quad_hash2(const uint32_t *xp, uint32_t *qm, const uint32_t (&mo)[28], uint32_t j,
-                                           uint32_t iv_lo, uint32_t iv_hi, uint32_t &o_lo, uint32_t &o_hi) {
-    if (j == 0) {
-        qm[0] = 0x2d67697au;  // "zig-"
-        qm[1] = 0x72617473u;  // "star"
-        qm[2] = 0x61703a6bu;  // "k:pa"
-        qm[3] = 0x7269u | (xp[0] << 16);
-    } else {
-        const uint32_t xa = xp[4 * j - 4], xb = xp[4 * j - 3], xc = xp[4 * j - 2], xd = xp[4 * j - 1], xe = xp[4 * j];
-        qm[4 * j] = __builtin_amdgcn_alignbit(xb, xa, 16);
-        qm[4 * j + 1] = __builtin_amdgcn_alignbit(xc, xb, 16);
-        qm[4 * j + 2] = __builtin_amdgcn_alignbit(xd, xc, 16);
-        qm[4 * j + 3] = __builtin_amdgcn_alignbit(xe, xd, 16);
-    }
-    const uint32_t y0 = xp[12 + j], y1 = xp[j < 3 ? 13 + j : 15u];
-    qm[16 + j] = __builtin_amdgcn_alignbit(j < 3 ? y1 : 0u, y0, 16);
-    wave_lds_sync_m();
-    uint32_t a = iv_lo, b = iv_hi, c = iv_lo, d = j == 2 ? 64u : (j == 3 ? B3_CHUNK_START : 0u);
-    quad_compress(a, b, c, d, qm, mo);
-    const uint32_t h0 = a ^ c, h1 = b ^ d;
-    a = h0;
-    b = h1;
-    c = iv_lo;
-    d = j == 2 ? 14u : (j == 3 ? (B3_CHUNK_END | B3_ROOT) : 0u);
-    quad_compress(a, b, c, d, qm + 16, mo);
-    o_lo = a ^ c;
-    o_hi = b ^ d;
-    wave_lds_sync_m();  // every lane has read qm before the quad's next hash2 rewrites it
-}
 
 // Workgroups that each build K levels of a subtree of 2^K children, keeping
 // the levels in LDS.  Levels with at least one node per thread hash one node per
