@@ -119,7 +119,7 @@ int32_t zs_m31_lde_dev(zs_hip_ctx *ctx, const uint32_t *d_cols, uint32_t n_cols,
                        uint32_t *d_lde);
 /* Launch-configuration autotuner (no reference counterpart; the reference has
  * no GPU LDE).  Runs zs_m31_lde_dev for one shape under each candidate
- * configuration of the register-group passes (2^12..2^14 elements per
+ * configuration of the register-group passes (2^11..2^14 elements per
  * workgroup x 1, 2, 4, 8 columns per workgroup) on synthetic device data, checks
  * that every candidate yields the same row-tree root, times each pass kernel
  * with events, and keeps for each kind of pass (interpolation passes, the fused

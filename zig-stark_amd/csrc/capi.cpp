@@ -317,7 +317,7 @@ int32_t zs_autotune_lde(zs_hip_ctx *h, uint32_t n_cols, uint32_t log_n, uint32_t
         std::lock_guard<std::mutex> lk(ctx->caches->mu);
         ctx->caches->tuned[key] = t;
     };
-    for (uint32_t el = 12; el <= 14 && rc == ZS_OK; el++) {
+    for (uint32_t el = 11; el <= 14 && rc == ZS_OK; el++) {
         for (uint32_t cpw = 1; cpw <= 8 && rc == ZS_OK; cpw *= 2) {
             TileConfig cand;
             cand.elems_log = el;
