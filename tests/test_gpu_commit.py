@@ -20,7 +20,7 @@ def oracle_commit(cols, beta):
 
 @pytest.mark.parametrize("n_cols,lg,beta", [
     (1, 1, 1), (2, 1, 2), (3, 2, 2), (5, 3, 1), (64, 6, 2), (16, 10, 2), (7, 11, 2),
-    (4, 12, 3), (17, 8, 1), (300, 4, 2), (16, 9, 4),
+    (4, 12, 3), (17, 8, 1), (300, 4, 2), (16, 9, 4), (4, 10, 0), (2, 16, 0),
 ])
 def test_lde_commit_matches_oracle(zctx, n_cols, lg, beta):
     rng = np.random.default_rng(n_cols * 1000 + lg * 10 + beta)
