@@ -15,7 +15,11 @@
 // then the 2^beta forward transforms' top pass straight from the registers
 // (the interpolation's last group and the evaluation's first group cover the
 // same index bits), so the interleaved coefficients of those stages never go
-// to HBM.
+// to HBM.  The LDE's first interpolation pass, middle (fused) pass and last
+// evaluation pass have pipelined kernels of their own (lde_interp_first_kernel,
+// lde_mid_kernel, lde_eval_last_kernel): one wave per tile, the next column
+// loaded into registers during the current column's butterflies.
+// fft_tile_kernel serves every other pass shape.
 #include "fft_common.h"
 
 namespace zs {
@@ -50,7 +54,8 @@ __host__ __device__ __forceinline__ uint32_t tw_pos(int ls, uint32_t i) {
 }
 
 // LDS index of tile element t: one pad word per 16 keeps the group access
-// patterns (stride 1, 4 and 16 across lanes) free of bank conflicts
+// patterns with stride 4 and 16 across lanes free of bank conflicts (32 banks);
+// a contiguous run of 32 lanes spans 33 words, one 2-way conflict per half-wave
 __device__ __forceinline__ uint32_t lidx(uint32_t t) { return t + (t >> 4); }
 
 // element index of register j of `lane` in a group covering bits [LO, LO + r):
@@ -335,11 +340,11 @@ __device__ __forceinline__ void coop_store_generic(const FftPass &p, const TileG
 // Cooperative copies with the hot layouts specialised: the per-thread part of
 // each address (its run position) is computed once, and k-space tiles are
 // streamed with t fastest so every wave access is one contiguous run.  Each
-// thread moves its elements in batches of CP_BATCH: all of a batch's global
-// loads (or LDS reads) are issued before the first is waited for, so a column's
-// copy costs a few memory latencies instead of one per element.  The batch is
-// 8 values in the pass kernels and 4 in the fused LDE kernels, whose registers
-// are fuller (measured: larger batches spill there).
+// thread moves its elements in batches of B: all of a batch's global loads
+// (or LDS reads) are issued before the first is waited for, so a column's copy
+// costs a few memory latencies instead of one per element.  The batch is 8
+// values in fft_tile_kernel and 4 in lde_single_kernel, whose registers are
+// fuller (measured: larger batches spill there).
 
 // In the fast paths a thread's element i sits at t = t0 + i * tstep (or at
 // t = tid + j * nt within tile i >> per_log) with t0 < tstep, so every address
@@ -654,8 +659,8 @@ static size_t tile_lds_bytes(uint32_t ntiles, uint32_t nsets, uint32_t TS) {
     return (size_t(ntiles) * TS + (size_t(nsets) << C)) * 4;
 }
 
-// LDS words from one tile to the next: 2^C plus one pad word per 16 (which keeps
-// the register groups' accesses conflict-free), plus a pad that spreads the tiles
+// LDS words from one tile to the next: 2^C plus one pad word per 16 (see lidx),
+// plus a pad that spreads the tiles
 // of one copy instruction over the 32 LDS banks.  The copies interleave `il` tiles
 // across consecutive lanes (tile index fastest, then the position in the tile),
 // so a stride of 32 / il banks (mod 32) gives every lane of a half-wave its own bank.
