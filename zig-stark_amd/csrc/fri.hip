@@ -114,11 +114,6 @@ namespace zs {
 // g[i] = sum_c gamma^c * lde_c[i] (QM31 combination of M31 columns); powers: n_cols QM31 values.
 // Products of two canonical values are < 2^62, so four of them are summed in 64 bits
 // (one multiply-add each) before a reduction.
-__device__ __forceinline__ uint32_t m31_reduce64(uint64_t x) {
-    uint64_t y = (x & P) + (x >> 31);                           // < 2^31 + 2^33
-    uint32_t z = (uint32_t)(y & P) + (uint32_t)(y >> 31);       // < 2^31 + 8
-    return m31_reduce_once(z);
-}
 
 __global__ __launch_bounds__(256) void combine_kernel(const uint32_t *lde, uint64_t cs, uint32_t n_cols, uint64_t n,
                                                       const uint32_t *powers, uint32_t *out) {
