@@ -585,8 +585,10 @@ int32_t zs::fri_commit_impl(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t 
     auto layer = [&](uint32_t i) -> uint32_t * {
         return i == 0 ? const_cast<uint32_t *>(lay0) : layers + r->layer_off[i];
     };
-    ZS_TRY(ctx->ensure_scratch(6, size_t(L) * 16));
+    // slot 6: the challenges (16 B per layer), then the layer roots gathered for one copy (32 B per layer)
+    ZS_TRY(ctx->ensure_scratch(6, size_t(L) * 48));
     alpha_dev = ctx->scratch[6].as<uint32_t>();
+    uint32_t *roots_dev = alpha_dev + 4 * size_t(L);
     const uint32_t *fri_tab;
     ZS_TRY(get_fri_tables(ctx, dlog, &fri_tab));  // cached before any capture
     if (alphas)
@@ -643,9 +645,16 @@ int32_t zs::fri_commit_impl(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t 
         }
         return i_tail < L ? tail() : ZS_OK;
     };
+    // every layer's root into roots_dev, so the host reads them with one copy
+    auto enqueue_all = [&]() -> int32_t {
+        ZS_TRY(enqueue());
+        std::vector<uint64_t> root_off(L);
+        for (uint32_t i = 0; i < L; i++) root_off[i] = r->tree_off[i] + tree_words(dlog - i) - 8;
+        return launch_gather_roots(ctx, trees, root_off.data(), L, roots_dev);
+    };
     const bool graph = !out_result && !ctx->profiling && !(flags & ZS_FLAG_NO_GRAPH);
     if (!graph) {
-        ZS_TRY(enqueue());
+        ZS_TRY(enqueue_all());
     } else {
         std::array<uint64_t, 6> key = {reinterpret_cast<uint64_t>(lay0), reinterpret_cast<uint64_t>(layers),
                                        reinterpret_cast<uint64_t>(trees), reinterpret_cast<uint64_t>(alpha_dev),
@@ -663,7 +672,7 @@ int32_t zs::fri_commit_impl(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t 
             ctx->stream = ctx->capture_stream;
             hipGraph_t g = nullptr;
             hipError_t e = hipStreamBeginCapture(ctx->capture_stream, hipStreamCaptureModeThreadLocal);
-            int32_t rc = e == hipSuccess ? enqueue() : ZS_ERR_DEVICE;
+            int32_t rc = e == hipSuccess ? enqueue_all() : ZS_ERR_DEVICE;
             hipError_t e2 = e == hipSuccess ? hipStreamEndCapture(ctx->capture_stream, &g) : e;
             ctx->stream = saved;
             if (rc != ZS_OK || e2 != hipSuccess) {
@@ -682,11 +691,7 @@ int32_t zs::fri_commit_impl(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t 
         }
         ZS_HIP(ctx, hipGraphLaunch(it->second, ctx->stream));
     }
-    for (uint32_t i = 0; i < L; i++) {
-        uint32_t li = dlog - i;
-        ZS_HIP(ctx, hipMemcpyAsync(roots_out + 32 * i, trees + r->tree_off[i] + tree_words(li) - 8, 32,
-                                   hipMemcpyDeviceToHost, ctx->stream));
-    }
+    ZS_HIP(ctx, hipMemcpyAsync(roots_out, roots_dev, size_t(L) * 32, hipMemcpyDeviceToHost, ctx->stream));
     // remainder: interpolate the first 2^(remainder_log - log_blowup) points of layer L (fri.zig:228-233)
     size_t rc = size_t(1) << (remainder_log - log_blowup);
     std::vector<uint32_t> last(rc * 4);

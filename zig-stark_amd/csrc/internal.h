@@ -184,6 +184,9 @@ struct PathReq {
     uint32_t index;
 };
 int32_t launch_gather_paths(Ctx *ctx, const uint32_t *d_trees, const std::vector<PathReq> &reqs, uint32_t *d_out);
+// the roots (last 8 words) at word offsets off_words[k] from d_base, k < n <= 28, into d_out (8 n words)
+int32_t launch_gather_roots(Ctx *ctx, const uint32_t *d_base, const uint64_t *off_words, uint32_t n,
+                            uint32_t *d_out);
 // copies `words` words from src + pairs[2r] to out + pairs[2r+1] for each r; synchronous
 int32_t launch_gather_words(Ctx *ctx, const uint32_t *d_src, const std::vector<uint64_t> &pairs, uint32_t words,
                             uint32_t *d_out);

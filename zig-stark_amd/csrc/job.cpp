@@ -98,7 +98,7 @@ JobLayout job_layout(uint32_t n_cols, uint32_t log_n, uint32_t log_blowup, uint3
     for (uint32_t i = 0; i < L; i++) ft += tree_words(LT - i) * 4;
     j.len[JobLayout::FRI_LAYERS] = fl;
     j.len[JobLayout::FRI_TREES] = ft;
-    j.len[JobLayout::ALPHAS] = uint64_t(L) * 16;
+    j.len[JobLayout::ALPHAS] = uint64_t(L) * 48;  // challenges, then the gathered layer roots
     j.len[JobLayout::CHAN] = sizeof(B3Hasher);
     j.len[JobLayout::COMB] = 16 * N;
     j.len[JobLayout::POWERS] = 16 * (c + 1);

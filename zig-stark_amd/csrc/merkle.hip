@@ -394,6 +394,27 @@ int32_t launch_gather_paths(Ctx *ctx, const uint32_t *d_trees, const std::vector
     return ZS_OK;
 }
 
+// roots of up to 28 trees (8 words each at off[k]) gathered into out, one word per thread
+struct RootOffsets {
+    uint64_t off[28];
+};
+__global__ __launch_bounds__(256) void gather_roots_kernel(const uint32_t *base, RootOffsets ro, uint32_t n,
+                                                           uint32_t *out) {
+    const uint32_t t = threadIdx.x;
+    if (t < 8 * n) out[t] = base[ro.off[t >> 3] + (t & 7u)];
+}
+
+int32_t launch_gather_roots(Ctx *ctx, const uint32_t *d_base, const uint64_t *off_words, uint32_t n,
+                            uint32_t *d_out) {
+    if (n == 0) return ZS_OK;
+    if (n > 28) return ZS_ERR_INPUT;
+    RootOffsets ro{};
+    for (uint32_t k = 0; k < n; k++) ro.off[k] = off_words[k];
+    hipLaunchKernelGGL(gather_roots_kernel, dim3(1), dim3(256), 0, ctx->stream, d_base, ro, n, d_out);
+    ZS_HIP(ctx, hipGetLastError());
+    return ZS_OK;
+}
+
 int32_t launch_gather_words(Ctx *ctx, const uint32_t *d_src, const std::vector<uint64_t> &pairs, uint32_t words,
                             uint32_t *d_out) {
     if (pairs.empty()) return ZS_OK;
