@@ -3,7 +3,7 @@
 // Used where a compression's latency matters more than its throughput: the
 // narrow levels of Merkle trees near the root and the Fiat-Shamir channel.
 #pragma once
-#include "blake3.h"
+#include "b3_hasher.h"
 
 namespace zs {
 
@@ -100,6 +100,42 @@ __device__ __forceinline__ void quad_hash2(const uint32_t *xp, uint32_t *qm, con
     o_lo = a ^ c;
     o_hi = b ^ d;
     wave_lds_sync_m();  // every lane has read qm before the quad's next hash2 rewrites it
+}
+
+// A hasher's compression (B3Hasher's CF) run by one quad of lanes: every lane of
+// the quad holds the same chaining value and message; lane j computes column j
+// of the state and the quad trades the result through LDS.
+struct B3QuadCF {
+    uint32_t *qs;               // the quad's 24 LDS words: message, then chaining value
+    uint32_t j;                 // lane in the quad
+    const uint32_t (&mo)[28];   // the lane's message-word table (kQuadMsg[j])
+    __device__ void operator()(uint32_t cv[8], const uint32_t m[16], uint32_t counter_lo, uint32_t counter_hi,
+                               uint32_t block_len, uint32_t flags) {
+        for (int i = 0; i < 16; i++) qs[i] = m[i];
+        for (int i = 0; i < 8; i++) qs[16 + i] = cv[i];
+        wave_lds_sync_m();
+        uint32_t a = qs[16 + j], b = qs[20 + j], c = kIV[j];
+        uint32_t d = j == 0 ? counter_lo : (j == 1 ? counter_hi : (j == 2 ? block_len : flags));
+        quad_compress(a, b, c, d, qs, mo);
+        wave_lds_sync_m();  // every lane has read the message and the chaining value
+        qs[16 + j] = a ^ c;
+        qs[20 + j] = b ^ d;
+        wave_lds_sync_m();
+        for (int i = 0; i < 8; i++) cv[i] = qs[16 + i];
+        wave_lds_sync_m();  // read before the next compression rewrites qs
+    }
+};
+
+// Fiat-Shamir step on the device (fri.zig:205-210 with channel.zig:31-43,
+// 49-69, qm31.zig:95-101): the four lanes of one quad absorb the digest d into
+// the LDS-resident hasher h and draw a QM31 challenge into q (every lane gets
+// it).  qs: 24 LDS words, work: 16 LDS words, snap: LDS working storage.
+__device__ __forceinline__ void quad_channel_draw(B3Hasher &h, B3Hasher &snap, uint32_t *qs, uint32_t *work,
+                                                  const uint32_t d[8], uint32_t q[4], uint32_t j,
+                                                  const uint32_t (&mo)[28]) {
+    B3QuadCF cf{qs, j, mo};
+    channel_absorb_digest(h, d, work, cf);
+    channel_sample_qm31(h, snap, q, work, cf);
 }
 
 }  // namespace zs

@@ -636,9 +636,8 @@ int32_t zs::fri_commit_impl(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t 
         }
         for (uint32_t i = 0; i < i_tail; i++) {
             uint32_t li = dlog - i;
-            ZS_TRY(launch_tree_levels(ctx, trees + r->tree_off[i], li, 0));
-            // the challenge is drawn on the device from this layer's root
-            ZS_TRY(launch_channel_draw(ctx, d_chan, trees + r->tree_off[i] + tree_words(li) - 8, alpha_dev + 4 * i));
+            // the launch that computes this layer's root also draws its challenge on the device
+            ZS_TRY(launch_tree_levels(ctx, trees + r->tree_off[i], li, 0, d_chan, alpha_dev + 4 * i));
             uint32_t *next_leaves = (i + 1 < L) ? trees + r->tree_off[i + 1] : nullptr;
             ZS_TRY(launch_fri_fold(ctx, layer(i), size_t(1) << li, dlog, i, alpha_dev + 4 * i, layer(i + 1),
                                    next_leaves));

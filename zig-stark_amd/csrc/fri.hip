@@ -158,29 +158,6 @@ int32_t launch_combine(Ctx *ctx, const uint32_t *d_lde, uint64_t cs, uint32_t n_
 
 namespace zs {
 
-// The hasher's compression run by one quad of lanes (b3_quad.h): every lane of
-// the quad holds the same chaining value and message; lane j computes column j
-// of the state and the quad trades the result through LDS.
-struct B3QuadCF {
-    uint32_t *qs;               // the quad's 24 LDS words: message, then chaining value
-    uint32_t j;                 // lane in the quad
-    const uint32_t (&mo)[28];   // the lane's message-word table (kQuadMsg[j])
-    __device__ void operator()(uint32_t cv[8], const uint32_t m[16], uint32_t counter_lo, uint32_t counter_hi,
-                               uint32_t block_len, uint32_t flags) {
-        for (int i = 0; i < 16; i++) qs[i] = m[i];
-        for (int i = 0; i < 8; i++) qs[16 + i] = cv[i];
-        wave_lds_sync_m();
-        uint32_t a = qs[16 + j], b = qs[20 + j], c = kIV[j];
-        uint32_t d = j == 0 ? counter_lo : (j == 1 ? counter_hi : (j == 2 ? block_len : flags));
-        quad_compress(a, b, c, d, qs, mo);
-        wave_lds_sync_m();  // every lane has read the message and the chaining value
-        qs[16 + j] = a ^ c;
-        qs[20 + j] = b ^ d;
-        wave_lds_sync_m();
-        for (int i = 0; i < 8; i++) cv[i] = qs[16 + i];
-        wave_lds_sync_m();  // read before the next compression rewrites qs
-    }
-};
 
 // Fiat-Shamir on the device: absorb the 32-byte root at `digest` and draw a
 // QM31 challenge (fri.zig:205-210 with channel.zig:31-43, 49-69 and
@@ -200,13 +177,10 @@ __global__ __launch_bounds__(64) void channel_draw_kernel(B3Hasher *st, const ui
         uint32_t mo[28];
 #pragma unroll
         for (int k = 0; k < 28; k++) mo[k] = kQuadMsg[threadIdx.x][k];
-        B3QuadCF cf{qs, threadIdx.x, mo};
-        uint32_t d[8];
+        uint32_t d[8], q[4];
         for (int i = 0; i < 8; i++) d[i] = digest[i];
-        channel_absorb_digest(h, d, work, cf);
-        uint32_t q[4];
-        channel_sample_qm31(h, snap, q, work, cf);
-        if (cf.j == 0)
+        quad_channel_draw(h, snap, qs, work, d, q, threadIdx.x, mo);
+        if (threadIdx.x == 0)
             for (int i = 0; i < 4; i++) out[i] = q[i];
     }
     __syncthreads();
@@ -297,12 +271,9 @@ __global__ __launch_bounds__(FRI_TAIL_NT) void fri_tail_kernel(FriTail t) {
             }
             // absorb the root (lv(src)) and draw the challenge (fri.zig:205-210)
             if (tid < 4) {
-                B3QuadCF cf{qs, j, mo};
-                uint32_t d[8];
+                uint32_t d[8], q[4];
                 for (int w = 0; w < 8; w++) d[w] = lv(src)[w];
-                channel_absorb_digest(*h, d, work, cf);
-                uint32_t q[4];
-                channel_sample_qm31(*h, *snap, q, work, cf);
+                quad_channel_draw(*h, *snap, qs, work, d, q, j, mo);
                 if (j == 0)
                     for (int k = 0; k < 4; k++) {
                         al[k] = q[k];

@@ -170,12 +170,15 @@ int32_t launch_row_commit_plain(Ctx *ctx, const uint32_t *d_cols, uint64_t col_s
                                 uint32_t log_rows, uint32_t *d_nodes);
 int32_t launch_row_leaves(Ctx *ctx, const uint32_t *d_cols, uint64_t col_stride, uint32_t n_cols, uint64_t n,
                           uint32_t *d_nodes);
-// node levels above an already written level 0 of 2^log_leaves digests
-int32_t launch_tree_levels(Ctx *ctx, uint32_t *d_nodes, uint32_t log_leaves, uint32_t from_level);
+// node levels above an already written level 0 of 2^log_leaves digests; with
+// draw_chan, the launch that computes the root also absorbs it into that device
+// channel and draws a QM31 challenge into draw_alpha (log_leaves - from_level >= 1)
+int32_t launch_tree_levels(Ctx *ctx, uint32_t *d_nodes, uint32_t log_leaves, uint32_t from_level,
+                           B3Hasher *draw_chan = nullptr, uint32_t *draw_alpha = nullptr);
 int32_t launch_qm31_leaves(Ctx *ctx, const uint32_t *d_vals, uint64_t n, uint32_t *d_nodes);
 // node levels of n_trees independent trees with written leaves, tree k at base + off_words[k]
 int32_t launch_trees_multi(Ctx *ctx, uint32_t *base, const uint64_t *off_words, const uint32_t *log_leaves,
-                           uint32_t n_trees);
+                           uint32_t n_trees, B3Hasher *draw_chan = nullptr, uint32_t *draw_alpha = nullptr);
 // Merkle path gather: for each request, log_leaves sibling digests written at out_off_words.
 struct PathReq {
     uint64_t tree_off_words;

@@ -58,15 +58,24 @@ int32_t commit_job(zs_hip_ctx *h, const uint32_t *d_cols, uint32_t n_cols, uint3
         return ZS_ERR_INPUT;
     Ctx *ctx = &h->c;
     const size_t N = size_t(1) << LT;
-    ZS_TRY(zs_m31_lde_commit_dev(h, d_cols, n_cols, log_n, log_blowup, d_lde, d_nodes, flags & ZS_FLAG_PLAIN));
-    const uint32_t *d_root = reinterpret_cast<const uint32_t *>(d_nodes) + tree_words(LT) - 8;
+    uint32_t *nodes = reinterpret_cast<uint32_t *>(d_nodes);
+    const uint32_t *d_root = nodes + tree_words(LT) - 8;
     ZS_TRY(ctx->ensure_scratch(7, sizeof(B3Hasher)));
     ZS_TRY(ctx->ensure_scratch(8, N * 16));
     ZS_TRY(ctx->ensure_scratch(9, (size_t(n_cols) + 1) * 16));
     B3Hasher *d_chan = ctx->scratch[7].as<B3Hasher>();
     uint32_t *d_gamma = ctx->scratch[9].as<uint32_t>(), *d_powers = d_gamma + 4;
-    ZS_HIP(ctx, hipMemcpyAsync(d_chan, &commit_channel(), sizeof(B3Hasher), hipMemcpyHostToDevice, ctx->stream));
-    ZS_TRY(launch_channel_draw(ctx, d_chan, d_root, d_gamma));
+    if (flags & ZS_FLAG_PLAIN) {
+        ZS_TRY(zs_m31_lde_commit_dev(h, d_cols, n_cols, log_n, log_blowup, d_lde, d_nodes, ZS_FLAG_PLAIN));
+        ZS_HIP(ctx, hipMemcpyAsync(d_chan, &commit_channel(), sizeof(B3Hasher), hipMemcpyHostToDevice, ctx->stream));
+        ZS_TRY(launch_channel_draw(ctx, d_chan, d_root, d_gamma));
+    } else {
+        // LDE, row leaves, then the node levels, whose root launch also draws gamma
+        ZS_TRY(zs_m31_lde_dev(h, d_cols, n_cols, log_n, log_blowup, d_lde));
+        ZS_HIP(ctx, hipMemcpyAsync(d_chan, &commit_channel(), sizeof(B3Hasher), hipMemcpyHostToDevice, ctx->stream));
+        ZS_TRY(launch_row_leaves(ctx, d_lde, N, n_cols, N, nodes));
+        ZS_TRY(launch_tree_levels(ctx, nodes, LT, 0, d_chan, d_gamma));
+    }
     ZS_TRY(launch_qm31_powers(ctx, d_gamma, d_powers, n_cols));
     ZS_TRY(launch_combine(ctx, d_lde, N, n_cols, N, d_powers, ctx->scratch[8].as<uint32_t>()));
     ZS_HIP(ctx, hipMemcpyAsync(trace_root, d_root, 32, hipMemcpyDeviceToHost, ctx->stream));

@@ -188,11 +188,17 @@ struct BlockTrees {
     uint32_t n;
     uint64_t child[MAX];
     uint32_t r[MAX], K[MAX], wg_end[MAX];
+    // set on the launch that computes the root of its one tree: the workgroup then
+    // absorbs the root into this device channel and draws a challenge into alpha
+    B3Hasher *chan;
+    uint32_t *alpha;
 };
 
 constexpr uint32_t BLOCK_NT = 256;
 
-inline size_t block_lds_words(uint32_t K) { return (size_t(6) << K) + (BLOCK_NT / 4) * 32; }
+ZS_HD size_t block_lds_words(uint32_t K) { return (size_t(6) << K) + (BLOCK_NT / 4) * 32; }
+// the channel state, its snapshot and the draw's staging after the tree words
+constexpr size_t kDrawLdsWords = 2 * sizeof(B3Hasher) / 4 + 24 + 16;
 
 __global__ __launch_bounds__(BLOCK_NT) void tree_block_kernel(uint32_t *base, const BlockTrees bt) {
     extern __shared__ uint32_t sm[];
@@ -244,6 +250,27 @@ __global__ __launch_bounds__(BLOCK_NT) void tree_block_kernel(uint32_t *base, co
         lvl += width * 8;
         width >>= 1;
     }
+    if (bt.chan) {
+        // src holds the root: absorb it and draw the next challenge
+        constexpr uint32_t HW = sizeof(B3Hasher) / 4;
+        B3Hasher *hs = reinterpret_cast<B3Hasher *>(sm + block_lds_words(K));
+        B3Hasher *snap = hs + 1;
+        uint32_t *qs = reinterpret_cast<uint32_t *>(snap + 1), *work = qs + 24;
+        const uint32_t *cw = reinterpret_cast<const uint32_t *>(bt.chan);
+        uint32_t *hw = reinterpret_cast<uint32_t *>(hs);
+        for (uint32_t k = tid; k < HW; k += BLOCK_NT) hw[k] = cw[k];
+        __syncthreads();
+        if (tid < 4) {
+            uint32_t d[8], q[4];
+            for (int w = 0; w < 8; w++) d[w] = src[w];
+            quad_channel_draw(*hs, *snap, qs, work, d, q, j, mo);
+            if (j == 0)
+                for (int k = 0; k < 4; k++) bt.alpha[k] = q[k];
+        }
+        __syncthreads();
+        uint32_t *ow = reinterpret_cast<uint32_t *>(bt.chan);
+        for (uint32_t k = tid; k < HW; k += BLOCK_NT) ow[k] = hw[k];
+    }
 }
 
 // levels with at least 2^17 children are hashed 3 at a time by one lane per
@@ -256,7 +283,9 @@ constexpr uint32_t kSubtreeMinLog = 17;
 // from 2^16 children down, workgroup subtrees of 8 levels, then one workgroup
 // per tree for the rest.
 int32_t launch_trees_multi(Ctx *ctx, uint32_t *base, const uint64_t *off_words, const uint32_t *log_leaves,
-                           uint32_t n_trees) {
+                           uint32_t n_trees, B3Hasher *draw_chan, uint32_t *draw_alpha) {
+    // a fused draw needs the root computed by one workgroup: one tree with at least one level
+    if (draw_chan && (n_trees != 1 || log_leaves[0] == 0)) return ZS_ERR_INPUT;
     std::vector<uint32_t> level(n_trees, 0);
     for (uint32_t start = 0; start < n_trees; start += MultiTrees::MAX) {
         const uint32_t cnt = std::min<uint32_t>(MultiTrees::MAX, n_trees - start);
@@ -298,8 +327,11 @@ int32_t launch_trees_multi(Ctx *ctx, uint32_t *base, const uint64_t *off_words, 
                 in.push_back(k);
             }
             if (bt.n == 0) break;
+            const bool last = draw_chan && bt.K[0] == bt.r[0];  // this launch computes the root
+            bt.chan = last ? draw_chan : nullptr;
+            bt.alpha = last ? draw_alpha : nullptr;
             ZS_LAUNCH(ctx, "tree_block_kernel", tree_block_kernel, dim3(wgs), dim3(BLOCK_NT),
-                      block_lds_words(kmax) * 4, ctx->stream, base, bt);
+                      (block_lds_words(kmax) + (last ? kDrawLdsWords : 0)) * 4, ctx->stream, base, bt);
             ZS_HIP(ctx, hipGetLastError());
             for (uint32_t i = 0; i < bt.n; i++) level[in[i]] += bt.K[i];
         }
@@ -309,10 +341,11 @@ int32_t launch_trees_multi(Ctx *ctx, uint32_t *base, const uint64_t *off_words, 
 
 // Build levels from_level+1 .. top over level from_level (already written): the
 // levels from there up are laid out like a tree of 2^(log_leaves - from_level) leaves.
-int32_t launch_tree_levels(Ctx *ctx, uint32_t *d_nodes, uint32_t log_leaves, uint32_t from_level) {
+int32_t launch_tree_levels(Ctx *ctx, uint32_t *d_nodes, uint32_t log_leaves, uint32_t from_level,
+                           B3Hasher *draw_chan, uint32_t *draw_alpha) {
     const uint64_t off = level_offset_words(log_leaves, from_level);
     const uint32_t lg = log_leaves - from_level;
-    return launch_trees_multi(ctx, d_nodes, &off, &lg, 1);
+    return launch_trees_multi(ctx, d_nodes, &off, &lg, 1, draw_chan, draw_alpha);
 }
 
 int32_t launch_row_commit(Ctx *ctx, const uint32_t *d_cols, uint64_t col_stride, uint32_t n_cols, uint32_t log_rows,
