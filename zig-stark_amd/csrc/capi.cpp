@@ -21,9 +21,11 @@ int32_t Ctx::ensure_scratch(int slot, size_t bytes) {
     DevBuf &b = scratch[slot];
     if (b.bytes >= bytes) return ZS_OK;
     if (b.ptr && !b.owned) {
-        b.ptr = nullptr;
-        b.bytes = 0;
-        b.owned = true;
+        // a lent buffer (a server request's share of its HBM budget) is never replaced
+        // by an allocation outside that budget
+        set_error("scratch slot " + std::to_string(slot) + ": lent buffer of " + std::to_string(b.bytes) +
+                  " bytes, " + std::to_string(bytes) + " needed");
+        return ZS_ERR_GENERIC;
     }
     if (b.ptr) {
         if (stream_ordered) {
