@@ -395,7 +395,7 @@ def main():
                     help="also run one 256 MiB device copy (for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: --gpus {args.gpus} is not supported: this build measures one GPU "
+        print(f"bench.py: --gpus {args.gpus} is not supported: the benchmark measures one GPU "
               f"(the commitment path is not sharded across devices); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
