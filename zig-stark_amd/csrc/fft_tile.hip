@@ -146,6 +146,82 @@ __device__ __forceinline__ void tile_run(uint32_t *tl, uint32_t lane, const uint
     if constexpr (GI + 1 < TC<C>::NG) tile_run<INV, C, GI + 1>(tl, lane, twl);
 }
 
+// The lowest register group of a forward tile (LO = 0, RL stages) with its
+// twiddles in registers instead of LDS.  Stage b < RL reads entries
+// (lane << (RB-1-b)) | (o << (RL-1-b)) | u of its heap level: 2^(RB-1-b)
+// consecutive words per lane, the same for every column a workgroup transforms,
+// so they are loaded once; the LDS set then holds only heap slots below 2^SLOG.
+__host__ __device__ constexpr int low_group_stages(int C) {
+    return C - ((C + C - 7) / (C - 6) - 1) * (C - 6);  // C - (NG - 1) RB
+}
+template <int C>
+struct LowTw {
+    static constexpr int RB = TC<C>::RB, NG = TC<C>::NG;
+    static constexpr int RL = low_group_stages(C);          // stages in the lowest group
+    static_assert(RL == C - (NG - 1) * RB, "lowest group");
+    static constexpr int N = (1 << RB) - (1 << (RB - RL));  // registers per lane
+    static constexpr int SLOG = C - RL;                     // LDS set: heap slots below 2^SLOG
+    static constexpr int off(int b) { return (1 << RB) - (1 << (RB - b)); }  // stage b's first register
+};
+
+template <bool INV, int C, int b>
+__device__ __forceinline__ void g_stage_rt(uint32_t (&v)[TC<C>::R], const uint32_t (&tr)[LowTw<C>::N]) {
+    constexpr int RB = TC<C>::RB, r = LowTw<C>::RL;
+#pragma unroll
+    for (int o = 0; o < (1 << (RB - r)); o++) {
+#pragma unroll
+        for (int u = 0; u < (1 << (r - 1 - b)); u++) {
+            const uint32_t w = tr[LowTw<C>::off(b) + ((o << (r - 1 - b)) | u)];
+            const int jb = (o << r) | (u << (b + 1));
+#pragma unroll
+            for (int l = 0; l < (1 << b); l++) {
+                const int j0 = jb | l, j1 = j0 | (1 << b);
+                const uint32_t a = v[j0];
+                if (INV) {
+                    v[j0] = m31_add(a, v[j1]);
+                    v[j1] = m31_mul_dbl(m31_sub(a, v[j1]), w);
+                } else {
+                    const uint32_t m = m31_mul_dbl(v[j1], w);
+                    v[j0] = m31_add(a, m);
+                    v[j1] = m31_sub(a, m);
+                }
+            }
+        }
+    }
+}
+template <bool INV, int C, int i>
+__device__ __forceinline__ void g_stages_rt(uint32_t (&v)[TC<C>::R], const uint32_t (&tr)[LowTw<C>::N]) {
+    constexpr int r = LowTw<C>::RL;
+    g_stage_rt<INV, C, INV ? i : r - 1 - i>(v, tr);
+    if constexpr (i + 1 < r) g_stages_rt<INV, C, i + 1>(v, tr);
+}
+// groups GI.. of a tile in LDS, the lowest one from register twiddles
+template <bool INV, int C, int GI>
+__device__ __forceinline__ void tile_run_rt(uint32_t *tl, uint32_t lane, const uint32_t *twl,
+                                            const uint32_t (&tr)[LowTw<C>::N]) {
+    using Gp = Grp<INV, C, GI>;
+    uint32_t v[TC<C>::R];
+    g_load<C, Gp::LO, Gp::r>(v, tl, lane);
+    if constexpr (Gp::GF + 1 < TC<C>::NG) g_stages<INV, C, Gp::LO, Gp::r, 0>(v, lane, twl);
+    else g_stages_rt<INV, C, 0>(v, tr);
+    g_store<C, Gp::LO, Gp::r>(v, tl, lane);
+    wave_lds_sync();
+    if constexpr (GI + 1 < TC<C>::NG) tile_run_rt<INV, C, GI + 1>(tl, lane, twl, tr);
+}
+// this lane's lowest-group twiddles (doubled) of the tile whose heap entries start at
+// heap(ls) + (kb >> (ls + 1)) in the table of log LT
+template <int C>
+__device__ __forceinline__ void load_low_twiddles(uint32_t (&tr)[LowTw<C>::N], const uint32_t *tw, uint32_t LT,
+                                                  uint32_t kb, uint32_t lane) {
+    constexpr int RB = TC<C>::RB;
+#pragma unroll
+    for (int b = 0; b < LowTw<C>::RL; b++) {
+        const uint32_t *src = tw + (1u << (LT - 1 - b)) + (kb >> (b + 1)) + (lane << (RB - 1 - b));
+#pragma unroll
+        for (int k = 0; k < (1 << (RB - 1 - b)); k++) tr[LowTw<C>::off(b) + k] = src[k] << 1;
+    }
+}
+
 // count of i >= 0 with first + i * step < end
 __device__ __forceinline__ uint32_t strided_count(uint32_t first, uint32_t step, uint32_t end) {
     return first < end ? (end - first + step - 1) / step : 0u;
@@ -191,9 +267,9 @@ __device__ __forceinline__ void stage_twiddles(uint32_t *twl, const uint32_t *tw
 // every set, so their stage, entry and LDS position are worked out once per m
 // and reused for all sets; the loads of up to 8 sets x 4 slots are in flight
 // together.  entry(set, ls, i) reads entry i of local stage ls of the set's table.
-template <int C, typename ENTRY>
+template <int C, int SLOG = C, typename ENTRY>
 __device__ __forceinline__ void stage_twiddle_sets(uint32_t *twl, uint32_t nsets, ENTRY entry) {
-    constexpr uint32_t T = 1u << C, KM = 4, SB = 8;
+    constexpr uint32_t T = 1u << SLOG, KM = 4, SB = 8;  // the sets keep heap slots below 2^SLOG
     const uint32_t nt = blockDim.x, tid = threadIdx.x;
     for (uint32_t m0 = 0; m0 * nt < T; m0 += KM) {
         uint32_t ls[KM], ent[KM], pos[KM];
@@ -218,7 +294,7 @@ __device__ __forceinline__ void stage_twiddle_sets(uint32_t *twl, uint32_t nsets
             for (uint32_t b = 0; b < SB; b++)
 #pragma unroll
                 for (uint32_t k = 0; k < KM; k++)
-                    if (s0 + b < nsets && ok[k]) twl[((s0 + b) << C) + pos[k]] = v[b][k] << 1;
+                    if (s0 + b < nsets && ok[k]) twl[((s0 + b) << SLOG) + pos[k]] = v[b][k] << 1;
         }
     }
 }
@@ -533,6 +609,7 @@ __global__ __launch_bounds__(tile_block_max<C>()) void fft_tile_kernel(FftPass p
 template <int C>
 __global__ __launch_bounds__(512, 2) void lde_eval_last_kernel(FftPass p) {
     constexpr uint32_t PER = (1u << C) >> 6;  // tile elements per lane
+    using LW = LowTw<C>;
     extern __shared__ uint32_t lds[];
     const uint32_t TS = p.TS, G = 1u << p.G_log, Q = 1u << p.Q_log;
     const uint32_t qblk = blockIdx.z << p.Q_log;
@@ -543,7 +620,7 @@ __global__ __launch_bounds__(512, 2) void lde_eval_last_kernel(FftPass p) {
     const uint64_t in_off = ((uint64_t)(qblk + (wave >> p.G_log)) << p.L) |
                             ((uint64_t)bitrev32(tau0 + (wave & (G - 1)), p.L - C) << C);
     uint32_t *tl = lds + wave * TS;
-    const uint32_t *twt = twl + (wave << C);
+    const uint32_t *twt = twl + (wave << LW::SLOG);
     // this thread's natural-order store run position
     const uint32_t run_log = p.G_log + p.Q_log;
     const uint32_t rp = tid & ((1u << run_log) - 1), rq = rp & (Q - 1), g = rp >> p.Q_log;
@@ -559,10 +636,15 @@ __global__ __launch_bounds__(512, 2) void lde_eval_last_kernel(FftPass p) {
 #pragma unroll
         for (uint32_t j = 0; j < PER; j++) pf[j] = src[j * 64];
     }
-    // tile set (q, g) covers k-space ((qblk + q) << L) | (bitrev(tau0 + g) << C)
-    stage_twiddle_sets<C>(twl, Q * G, [&](uint32_t set, uint32_t ls, uint32_t i) {
-        const uint32_t kb = ((qblk + (set >> p.G_log)) << p.L) | (bitrev32(tau0 + (set & (G - 1)), p.L - C) << C);
-        return p.tw[(1u << (p.LT - 1 - ls)) + (kb >> (ls + 1)) + i];
+    // tile set (q, g) covers k-space ((qblk + q) << L) | (bitrev(tau0 + g) << C); the lowest
+    // group's stages are this wave's registers, the others its set in LDS
+    const auto set_kb = [&](uint32_t set) {
+        return ((qblk + (set >> p.G_log)) << p.L) | (bitrev32(tau0 + (set & (G - 1)), p.L - C) << C);
+    };
+    uint32_t tr[LW::N];
+    load_low_twiddles<C>(tr, p.tw, p.LT, set_kb(wave), lane);
+    stage_twiddle_sets<C, LW::SLOG>(twl, Q * G, [&](uint32_t set, uint32_t ls, uint32_t i) {
+        return p.tw[(1u << (p.LT - 1 - ls)) + (set_kb(set) >> (ls + 1)) + i];
     });
     __syncthreads();  // twiddle sets staged
     for (uint32_t col = c0; col < c1; col++) {
@@ -579,7 +661,7 @@ __global__ __launch_bounds__(512, 2) void lde_eval_last_kernel(FftPass p) {
             for (uint32_t j = 0; j < PER; j++) pf[j] = src[j * 64];
         }
         wave_lds_sync();
-        tile_run<false, C, 0>(tl, lane, twt);
+        tile_run_rt<false, C, 0>(tl, lane, twt, tr);
         __syncthreads();  // every tile of the workgroup transformed
 #pragma unroll
         for (uint32_t i = 0; i < PER; i++) res[i] = rd[lidx(i << 6)];
@@ -601,6 +683,7 @@ __global__ __launch_bounds__(512, 2) void lde_eval_last_kernel(FftPass p) {
 template <int C>
 __global__ __launch_bounds__(512, 2) void lde_interp_first_kernel(FftPass p) {
     constexpr uint32_t PER = (1u << C) >> 6;
+    using LW = LowTw<C>;
     extern __shared__ uint32_t lds[];
     const uint32_t TS = p.TS, G = 1u << p.G_log;
     const uint32_t tau0 = xcd_block(blockIdx.x, gridDim.x) << p.G_log;
@@ -614,7 +697,7 @@ __global__ __launch_bounds__(512, 2) void lde_interp_first_kernel(FftPass p) {
     // store: this wave's tile, k-space tile bitrev(tau0 + wave), contiguous
     const uint64_t out_off = ((uint64_t)bitrev32(tau0 + wave, p.L - C) << C) + lane;
     uint32_t *tl = lds + wave * TS;
-    const uint32_t *twt = twl + (wave << C);
+    const uint32_t *twt = twl + (wave << LW::SLOG);
     const uint32_t c0 = blockIdx.y * p.ncb, c1 = min(c0 + p.ncb, p.n_cols);
     uint32_t pf[PER], res[PER];
     if (c0 < c1) {
@@ -622,8 +705,11 @@ __global__ __launch_bounds__(512, 2) void lde_interp_first_kernel(FftPass p) {
 #pragma unroll
         for (uint32_t j = 0; j < PER; j++) pf[j] = src[(uint64_t)bitrev32(j, C - 6) << sh];
     }
-    // tile set g covers k-space bitrev(tau0 + g) << C
-    stage_twiddle_sets<C>(twl, G, [&](uint32_t set, uint32_t ls, uint32_t i) {
+    // tile set g covers k-space bitrev(tau0 + g) << C; the lowest group's stages are this
+    // wave's registers, the others its set in LDS
+    uint32_t tr[LW::N];
+    load_low_twiddles<C>(tr, p.tw, p.LT, bitrev32(tau0 + wave, p.L - C) << C, lane);
+    stage_twiddle_sets<C, LW::SLOG>(twl, G, [&](uint32_t set, uint32_t ls, uint32_t i) {
         const uint32_t kb = bitrev32(tau0 + set, p.L - C) << C;
         return p.tw[(1u << (p.LT - 1 - ls)) + (kb >> (ls + 1)) + i];
     });
@@ -642,7 +728,7 @@ __global__ __launch_bounds__(512, 2) void lde_interp_first_kernel(FftPass p) {
             for (uint32_t j = 0; j < PER; j++) pf[j] = src[(uint64_t)bitrev32(j, C - 6) << sh];
         }
         __syncthreads();  // every tile loaded
-        tile_run<true, C, 0>(tl, lane, twt);
+        tile_run_rt<true, C, 0>(tl, lane, twt, tr);
 #pragma unroll
         for (uint32_t j = 0; j < PER; j++) res[j] = tl[lidx(lane + j * 64)];
         __syncthreads();  // every read done: the tiles may take the next column
@@ -745,7 +831,9 @@ int32_t run_tile_pass(Ctx *ctx, FftPass p, bool inverse, uint32_t n_cols) {
     uint32_t maxw = p.C >= 11 ? 8 : 16;
     dim3 block(64 * std::min<uint32_t>(ntile, maxw));
     if ((pipe_interp || pipe_eval) && ntile <= 8) {
-        const size_t lds = (size_t(ntile) * p.TS + (size_t(ntile) << p.C)) * 4;
+        // a set keeps only the stages above the lowest register group
+        const uint32_t set_log = p.C - low_group_stages((int)p.C);
+        const size_t lds = (size_t(ntile) * p.TS + (size_t(ntile) << set_log)) * 4;
         if (lds <= 160 * 1024) {
             const dim3 blk(64 * ntile);
 #define ZS_PIPE_CASE(CC)                                                                                        \
