@@ -20,11 +20,16 @@ namespace zs {
 // The compression a hasher runs: cv <- compress(cv, m, counter, block_len, flags).
 // B3Hasher's methods take one as a template argument; the default is the
 // single-lane b3_compress_cv (the device channel passes a quad-lane one).
+// lane() / lanes() / sync(): the threads that run the hasher together split its
+// byte and word copies (lane k takes every lanes()-th element), then sync().
 struct B3Single {
     ZS_HD void operator()(uint32_t cv[8], const uint32_t m[16], uint32_t counter_lo, uint32_t counter_hi,
                           uint32_t block_len, uint32_t flags) const {
         b3_compress_cv(cv, m, counter_lo, counter_hi, block_len, flags);
     }
+    ZS_HD uint32_t lane() const { return 0; }
+    ZS_HD uint32_t lanes() const { return 1; }
+    ZS_HD void sync() const {}
 };
 
 // parent node of BLAKE3's chunk tree through a given compression
@@ -80,6 +85,12 @@ struct B3Hasher {
         update(in, len, cf);
     }
     template <typename CF>
+    ZS_HD void zero_block(CF &cf) {
+        uint32_t *bw = reinterpret_cast<uint32_t *>(block);
+        for (uint32_t i = cf.lane(); i < 16; i += cf.lanes()) bw[i] = 0;
+        cf.sync();
+    }
+    template <typename CF>
     ZS_HD void update(const uint8_t *in, size_t len, CF &cf) {
         while (len > 0) {
             if (chunk_len() == 1024) {
@@ -93,7 +104,7 @@ struct B3Hasher {
                 b3_iv(cv);
                 block_len = 0;
                 blocks_compressed = 0;
-                for (int i = 0; i < 64; i++) block[i] = 0;
+                zero_block(cf);
             }
             if (block_len == 64) {
                 uint32_t m[16];
@@ -101,13 +112,14 @@ struct B3Hasher {
                 cf(cv, m, (uint32_t)chunk_counter, (uint32_t)(chunk_counter >> 32), 64, start_flag());
                 blocks_compressed++;
                 block_len = 0;
-                for (int i = 0; i < 64; i++) block[i] = 0;
+                zero_block(cf);
             }
             uint32_t want = 1024 - chunk_len();
             uint32_t room = 64 - block_len;
             size_t take = len < room ? len : room;
             if (take > want) take = want;
-            for (size_t i = 0; i < take; i++) block[block_len + i] = in[i];
+            for (size_t i = cf.lane(); i < take; i += cf.lanes()) block[block_len + i] = in[i];
+            cf.sync();
             block_len += (uint32_t)take;
             in += take;
             len -= take;
@@ -143,17 +155,21 @@ struct B3Hasher {
 };
 
 // copy of the parts of a hasher that are in use (the stack up to stack_len)
-ZS_HD void b3_copy_live(B3Hasher &dst, const B3Hasher &src) {
-    for (int i = 0; i < 8; i++) dst.cv[i] = src.cv[i];
-    dst.chunk_counter = src.chunk_counter;
+template <typename CF>
+ZS_HD void b3_copy_live(B3Hasher &dst, const B3Hasher &src, CF &cf) {
+    const uint32_t l = cf.lane(), n = cf.lanes();
+    for (uint32_t i = l; i < 8; i += n) dst.cv[i] = src.cv[i];
     const uint32_t *sb = reinterpret_cast<const uint32_t *>(src.block);
     uint32_t *db = reinterpret_cast<uint32_t *>(dst.block);
-    for (int i = 0; i < 16; i++) db[i] = sb[i];
+    for (uint32_t i = l; i < 16; i += n) db[i] = sb[i];
+    const uint32_t *ss = &src.stack[0][0];
+    uint32_t *ds = &dst.stack[0][0];
+    for (uint32_t i = l; i < 8 * src.stack_len; i += n) ds[i] = ss[i];
+    dst.chunk_counter = src.chunk_counter;
     dst.block_len = src.block_len;
     dst.blocks_compressed = src.blocks_compressed;
     dst.stack_len = src.stack_len;
-    for (uint32_t k = 0; k < src.stack_len; k++)
-        for (int i = 0; i < 8; i++) dst.stack[k][i] = src.stack[k][i];
+    cf.sync();
 }
 
 // One sampled block (channel.zig:49-69): the transcript snapshot extended by
@@ -162,7 +178,7 @@ ZS_HD void b3_copy_live(B3Hasher &dst, const B3Hasher &src) {
 // and then the block, at word 6; `snap` is working storage for the snapshot.
 template <typename CF>
 ZS_HD void channel_sample_block(B3Hasher &h, B3Hasher &snap, uint64_t counter, uint32_t *work, CF &cf) {
-    b3_copy_live(snap, h);
+    b3_copy_live(snap, h, cf);
     work[0] = 0x2d67697au;  // "zig-"
     work[1] = 0x72617473u;  // "star"
     work[2] = 0x61733a6bu;  // "k:sa"

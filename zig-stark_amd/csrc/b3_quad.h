@@ -102,13 +102,18 @@ __device__ __forceinline__ void quad_hash2(const uint32_t *xp, uint32_t *qm, con
     wave_lds_sync_m();  // every lane has read qm before the quad's next hash2 rewrites it
 }
 
-// A hasher's compression (B3Hasher's CF) run by one quad of lanes: every lane of
-// the quad holds the same chaining value and message; lane j computes column j
-// of the state and the quad trades the result through LDS.
+// A hasher's compression (B3Hasher's CF) run by the 16 quads of one whole wave:
+// every lane holds the same chaining value and message; lane j of each quad
+// computes column j of the state, and the quads trade the result through the
+// same LDS words (identical values).  The hasher's copies are split over the
+// 64 lanes.
 struct B3QuadCF {
-    uint32_t *qs;               // the quad's 24 LDS words: message, then chaining value
+    uint32_t *qs;               // 24 LDS words: message, then chaining value
     uint32_t j;                 // lane in the quad
     const uint32_t (&mo)[28];   // the lane's message-word table (kQuadMsg[j])
+    __device__ uint32_t lane() const { return __lane_id(); }
+    __device__ uint32_t lanes() const { return 64; }
+    __device__ void sync() const { wave_lds_sync_m(); }
     __device__ void operator()(uint32_t cv[8], const uint32_t m[16], uint32_t counter_lo, uint32_t counter_hi,
                                uint32_t block_len, uint32_t flags) {
         for (int i = 0; i < 16; i++) qs[i] = m[i];
@@ -127,9 +132,10 @@ struct B3QuadCF {
 };
 
 // Fiat-Shamir step on the device (fri.zig:205-210 with channel.zig:31-43,
-// 49-69, qm31.zig:95-101): the four lanes of one quad absorb the digest d into
-// the LDS-resident hasher h and draw a QM31 challenge into q (every lane gets
-// it).  qs: 24 LDS words, work: 16 LDS words, snap: LDS working storage.
+// 49-69, qm31.zig:95-101): the 64 lanes of one wave (all active) absorb the
+// digest d into the LDS-resident hasher h and draw a QM31 challenge into q
+// (every lane gets it).  qs: 24 LDS words, work: 16 LDS words, snap: LDS
+// working storage.
 __device__ __forceinline__ void quad_channel_draw(B3Hasher &h, B3Hasher &snap, uint32_t *qs, uint32_t *work,
                                                   const uint32_t d[8], uint32_t q[4], uint32_t j,
                                                   const uint32_t (&mo)[28]) {

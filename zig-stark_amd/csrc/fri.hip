@@ -161,9 +161,10 @@ namespace zs {
 
 // Fiat-Shamir on the device: absorb the 32-byte root at `digest` and draw a
 // QM31 challenge (fri.zig:205-210 with channel.zig:31-43, 49-69 and
-// qm31.zig:95-101), so a layer's fold never waits for the host.  The state is staged in LDS; the four lanes
-// of quad 0 run the hasher's control flow in lockstep (identical state, so
-// identical writes) and share each compression.
+// qm31.zig:95-101), so a layer's fold never waits for the host.  The state is
+// staged in LDS; the 64 lanes run the hasher's control flow in lockstep
+// (identical state, so identical writes), split its copies and run each
+// compression as 16 identical quads.
 __global__ __launch_bounds__(64) void channel_draw_kernel(B3Hasher *st, const uint32_t *digest, uint32_t *out) {
     __shared__ B3Hasher h, snap;
     __shared__ uint32_t qs[24], work[16];
@@ -173,13 +174,14 @@ __global__ __launch_bounds__(64) void channel_draw_kernel(B3Hasher *st, const ui
     const uint32_t *sw = reinterpret_cast<const uint32_t *>(st);
     for (uint32_t k = threadIdx.x; k < W; k += blockDim.x) hw[k] = sw[k];
     __syncthreads();
-    if (threadIdx.x < 4) {
+    {  // the whole wave
+        const uint32_t j = threadIdx.x & 3u;
         uint32_t mo[28];
 #pragma unroll
-        for (int k = 0; k < 28; k++) mo[k] = kQuadMsg[threadIdx.x][k];
+        for (int k = 0; k < 28; k++) mo[k] = kQuadMsg[j][k];
         uint32_t d[8], q[4];
         for (int i = 0; i < 8; i++) d[i] = digest[i];
-        quad_channel_draw(h, snap, qs, work, d, q, threadIdx.x, mo);
+        quad_channel_draw(h, snap, qs, work, d, q, j, mo);
         if (threadIdx.x == 0)
             for (int i = 0; i < 4; i++) out[i] = q[i];
     }
@@ -202,7 +204,7 @@ int32_t launch_channel_draw(Ctx *ctx, B3Hasher *d_state, const uint32_t *d_diges
 // launch and a fold launch, most of it launch latency.  Here one workgroup keeps
 // the layer values and the tree levels in LDS and, per layer, builds the tree
 // (quad-lane hash2 per node, each level also written to its place in HBM),
-// draws the challenge from the root on one quad (channel mode) or reads it
+// draws the challenge from the root on wave 0 (channel mode) or reads it
 // (given challenges: the trees are then built with the other layers'), folds,
 // and hashes the next layer's leaves (fri.zig:199-219).
 // ---------------------------------------------------------------------------
@@ -270,11 +272,11 @@ __global__ __launch_bounds__(FRI_TAIL_NT) void fri_tail_kernel(FriTail t) {
                 src ^= 1u;
             }
             // absorb the root (lv(src)) and draw the challenge (fri.zig:205-210)
-            if (tid < 4) {
+            if (tid < 64) {  // wave 0
                 uint32_t d[8], q[4];
                 for (int w = 0; w < 8; w++) d[w] = lv(src)[w];
                 quad_channel_draw(*h, *snap, qs, work, d, q, j, mo);
-                if (j == 0)
+                if (tid == 0)
                     for (int k = 0; k < 4; k++) {
                         al[k] = q[k];
                         t.alphas[4 * i + k] = q[k];

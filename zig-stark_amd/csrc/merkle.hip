@@ -260,11 +260,11 @@ __global__ __launch_bounds__(BLOCK_NT) void tree_block_kernel(uint32_t *base, co
         uint32_t *hw = reinterpret_cast<uint32_t *>(hs);
         for (uint32_t k = tid; k < HW; k += BLOCK_NT) hw[k] = cw[k];
         __syncthreads();
-        if (tid < 4) {
+        if (tid < 64) {  // wave 0
             uint32_t d[8], q[4];
             for (int w = 0; w < 8; w++) d[w] = src[w];
             quad_channel_draw(*hs, *snap, qs, work, d, q, j, mo);
-            if (j == 0)
+            if (tid == 0)
                 for (int k = 0; k < 4; k++) bt.alpha[k] = q[k];
         }
         __syncthreads();
