@@ -273,9 +273,10 @@ __global__ __launch_bounds__(BLOCK_NT) void tree_block_kernel(uint32_t *base, co
     }
 }
 
-// levels with at least 2^17 children are hashed 3 at a time by one lane per
-// 8-child subtree (throughput); below that, by workgroup subtrees (latency)
-constexpr uint32_t kSubtreeMinLog = 17;
+// levels with at least 2^18 children are hashed 3 at a time by one lane per
+// 8-child subtree (throughput); below that, by workgroup subtrees whose quads
+// share each hash (a lane's chain of 14 compressions would set the latency)
+constexpr uint32_t kSubtreeMinLog = 18;
 
 // Every node level of several trees whose level `level[k]` is written: tree k
 // has 2^log_leaves[k] leaves at word offset off_words[k] from `base`.  Wide
