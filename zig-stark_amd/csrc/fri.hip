@@ -259,13 +259,29 @@ __global__ __launch_bounds__(FRI_TAIL_NT) void fri_tail_kernel(FriTail t) {
                 uint32_t *ds = lv(src ^ 1u);
                 const uint64_t out = off + (uint64_t)P * 8;
                 P >>= 1;
-                for (uint32_t p = quad; p < P; p += FRI_TAIL_NT / 4) {
-                    uint32_t lo, hi;
-                    quad_hash2(cs + 16 * p, qm, mo, j, iv_lo, iv_hi, lo, hi);
-                    ds[8 * p + j] = lo;
-                    ds[8 * p + 4 + j] = hi;
-                    tg[out + 8 * p + j] = lo;
-                    tg[out + 8 * p + 4 + j] = hi;
+                if (P >= FRI_TAIL_NT / 4) {
+                    // one or more nodes per quad: one node per lane finishes sooner
+                    for (uint32_t p = tid; p < P; p += FRI_TAIL_NT) {
+                        uint32_t x[8], y[8], d[8];
+                        for (int w = 0; w < 8; w++) {
+                            x[w] = cs[16 * p + w];
+                            y[w] = cs[16 * p + 8 + w];
+                        }
+                        b3_hash2(x, y, d);
+                        for (int w = 0; w < 8; w++) {
+                            ds[8 * p + w] = d[w];
+                            tg[out + 8 * p + w] = d[w];
+                        }
+                    }
+                } else {
+                    for (uint32_t p = quad; p < P; p += FRI_TAIL_NT / 4) {
+                        uint32_t lo, hi;
+                        quad_hash2(cs + 16 * p, qm, mo, j, iv_lo, iv_hi, lo, hi);
+                        ds[8 * p + j] = lo;
+                        ds[8 * p + 4 + j] = hi;
+                        tg[out + 8 * p + j] = lo;
+                        tg[out + 8 * p + 4 + j] = hi;
+                    }
                 }
                 __syncthreads();
                 off = out;

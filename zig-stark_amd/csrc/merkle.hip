@@ -226,7 +226,9 @@ __global__ __launch_bounds__(BLOCK_NT) void tree_block_kernel(uint32_t *base, co
         const uint32_t P = 1u << (K - 1 - l);
         uint32_t *out = base + lvl + width * 8 + (uint64_t)s * P * 8;
         uint32_t *dst = bufs[l & 1];
-        if (P >= BLOCK_NT) {
+        if (P >= BLOCK_NT / 2) {
+            // two or more nodes per quad: one node per lane (a lane's compression has
+            // four independent G chains, a quad's one) finishes sooner
             for (uint32_t p = tid; p < P; p += BLOCK_NT) {
                 uint32_t x[8], y[8], d[8];
                 ld_digest(src + 16 * p, x);
