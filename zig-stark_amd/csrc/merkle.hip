@@ -168,7 +168,7 @@ struct MultiTrees {
     uint64_t child[MAX], lvl1[MAX], lane_end[MAX];
 };
 
-__global__ __launch_bounds__(256) void tree_subtree3_multi_kernel(uint32_t *base, const MultiTrees mt) {
+__global__ __launch_bounds__(256, 8) void tree_subtree3_multi_kernel(uint32_t *base, const MultiTrees mt) {
     uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= mt.lane_end[mt.n - 1]) return;
     uint32_t t = 0;
