@@ -79,11 +79,12 @@ __global__ __launch_bounds__(256) void fri_fold_kernel(const uint32_t *cur, uint
     qm31 y0 = ld_qm(cur + 4 * j);
     qm31 y1 = ld_qm(cur + 4 * (j + half));
     uint64_t e = j << layer;
-    qm31 xinv = qm_mul(inv_off, qm_mul(ld_qm(tables + 4 * (e & 1023)), ld_qm(tables + 4 * (1024 + (e >> 10)))));
-    qm31 alpha = ld_qm(alpha_p);
+    // (y0 + y1) / 2 + alpha (y0 - y1) x^-1 / 2 with x^-1 = inv_off * t_lo * t_hi: the factor
+    // alpha * inv_off / 2 is the same for every element (uniform, computed once per wave)
+    const qm31 a_off = qm_half(qm_mul(ld_qm(alpha_p), inv_off));
+    const qm31 t = qm_mul(ld_qm(tables + 4 * (e & 1023)), ld_qm(tables + 4 * (1024 + (e >> 10))));
     qm31 hs = qm_half(qm_add(y0, y1));
-    qm31 hd = qm_half(qm_mul(qm_sub(y0, y1), xinv));
-    qm31 v = qm_add(hs, qm_mul(alpha, hd));
+    qm31 v = qm_add(hs, qm_mul(qm_mul(a_off, t), qm_sub(y0, y1)));
     st_qm(next + 4 * j, v);
     if (next_leaves) {
         uint32_t w[4] = {v.a.c0, v.a.c1, v.b.c0, v.b.c1};
@@ -309,15 +310,13 @@ __global__ __launch_bounds__(FRI_TAIL_NT) void fri_tail_kernel(FriTail t) {
         uint32_t *vout = val(cur ^ 1u);
         uint32_t *gout = t.layers + t.lay_off[i + 1];
         uint32_t *gleaves = (i + 1 < t.L) ? t.trees + t.tree_off[i + 1] : nullptr;
-        const qm31 inv_off = t.inv_off[i];
+        const qm31 a_off = qm_half(qm_mul(alpha, t.inv_off[i]));  // alpha * inv_off / 2, as in fri_fold_kernel
         for (uint32_t k = tid; k < half; k += FRI_TAIL_NT) {
             const qm31 y0 = ld_qm(vin + 4 * k), y1 = ld_qm(vin + 4 * (k + half));
             const uint64_t e = (uint64_t)k << i;
-            const qm31 xinv =
-                qm_mul(inv_off, qm_mul(ld_qm(t.tables + 4 * (e & 1023)), ld_qm(t.tables + 4 * (1024 + (e >> 10)))));
+            const qm31 tw = qm_mul(ld_qm(t.tables + 4 * (e & 1023)), ld_qm(t.tables + 4 * (1024 + (e >> 10))));
             const qm31 hs = qm_half(qm_add(y0, y1));
-            const qm31 hd = qm_half(qm_mul(qm_sub(y0, y1), xinv));
-            const qm31 v = qm_add(hs, qm_mul(alpha, hd));
+            const qm31 v = qm_add(hs, qm_mul(qm_mul(a_off, tw), qm_sub(y0, y1)));
             st_qm(vout + 4 * k, v);
             st_qm(gout + 4 * k, v);
             if (gleaves) {
