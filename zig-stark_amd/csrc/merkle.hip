@@ -179,7 +179,7 @@ __global__ __launch_bounds__(256, 8) void tree_subtree3_multi_kernel(uint32_t *b
 
 
 // Workgroups that each build K levels of a subtree of 2^K children, keeping
-// the levels in LDS.  Levels with at least one node per thread hash one node per
+// the levels in LDS.  Levels with at least two nodes per quad hash one node per
 // lane; narrower levels use a quad per node.  Several trees can share a launch:
 // tree t starts at word offset child[t] (its level with 2^r[t] children) and owns
 // workgroups [wg_end[t-1], wg_end[t]), 2^(r[t]-K[t]) of them.
