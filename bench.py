@@ -439,10 +439,13 @@ def main():
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    # the timed region; every hot launch in it is bracketed by HIP events on the
-    # library's stream, which give the per-kernel durations below
-    ctx.profile(True)
     ms = timed(step, args.steps)
+    # the same K steps again with every hot launch bracketed by HIP events on the
+    # library's stream: the per-kernel durations below.  The event records keep
+    # consecutive kernels from overlapping, so this pass runs ~1.5 % longer and is
+    # reported beside the clean one, not as the value.
+    ctx.profile(True)
+    ms_events = timed(step, args.steps)
     kern = ctx.profile_read()
     ctx.profile(False)
     root_fused = d_nodes[-8:].cpu().numpy().view(np.uint32).tobytes()
@@ -493,6 +496,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms, 4),
+        "ms_per_step_with_kernel_events": round(ms_events, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

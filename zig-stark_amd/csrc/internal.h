@@ -30,13 +30,16 @@ namespace zs {
 
 // Launch a hot kernel; when the context is profiling, bracket it with events
 // on the launch stream so its duration can be read back (zs_hip_profile_read).
+// The events only time: they skip the system-scope fences (no cache flush
+// between the kernels they separate).
+#define ZS_PROF_EVENT(e) (hipEventCreateWithFlags(&(e), hipEventDisableSystemFence) == hipSuccess)
 #define ZS_LAUNCH(ctx, label, ...)                                                 \
     do {                                                                           \
         hipEvent_t _za = nullptr, _zb = nullptr;                                   \
-        if ((ctx)->profiling && hipEventCreate(&_za) == hipSuccess)                \
+        if ((ctx)->profiling && ZS_PROF_EVENT(_za))                                \
             (void)hipEventRecord(_za, (ctx)->stream);                              \
         hipLaunchKernelGGL(__VA_ARGS__);                                           \
-        if (_za && hipEventCreate(&_zb) == hipSuccess) {                           \
+        if (_za && ZS_PROF_EVENT(_zb)) {                                           \
             (void)hipEventRecord(_zb, (ctx)->stream);                              \
             (ctx)->prof.push_back({std::string(label), _za, _zb, (ctx)->lde_role}); \
         }                                                                          \
