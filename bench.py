@@ -442,7 +442,7 @@ def main():
     ms = timed(step, args.steps)
     # the same K steps again with every hot launch bracketed by HIP events on the
     # library's stream: the per-kernel durations below.  The event records keep
-    # consecutive kernels from overlapping, so this pass runs ~1.5 % longer and is
+    # consecutive kernels from overlapping, so this pass runs up to ~1.5 % longer and is
     # reported beside the clean one, not as the value.
     ctx.profile(True)
     ms_events = timed(step, args.steps)
