@@ -145,6 +145,23 @@ def test_fri_device_channel_graph_replays(zctx, torch_dev, dlog, beta, rem_log):
             assert ch.sample_bytes(32) == och.sample_bytes(32), (rep, flags)
 
 
+def test_fri_device_channel_2p20_matches_oracle(zctx, torch_dev):
+    """Channel challenges at 2^20: layer trees wide enough for the per-lane 3-level
+    subtrees, the workgroup subtrees (one node per lane, then quads) and the draw
+    in the launch that computes each root, then the one-workgroup tail."""
+    from conftest import to_dev
+    dlog, beta, rem_log = 20, 2, 5
+    cw = np.random.default_rng(2020).integers(0, P, (1 << dlog, 4), dtype=np.uint32)
+    d_cw = to_dev(cw.reshape(-1), torch_dev)
+    och = O.Channel(b"zig-stark:fri:2p20")
+    oroots, _, orem, _ = O.fri_commit(cw, beta, rem_log, channel=och, want_layers=False)
+    for flags in (0, Z.ZS_FLAG_NO_GRAPH):
+        ch = Z.Channel(b"zig-stark:fri:2p20")
+        roots, rem, _ = zctx.fri_commit_dev(d_cw.data_ptr(), dlog, beta, rem_log, channel=ch, flags=flags)
+        assert roots == oroots and np.array_equal(rem, orem), flags
+    assert ch.sample_bytes(32) == och.sample_bytes(32)
+
+
 @pytest.mark.parametrize("log_size,beta,rem_log,queries", [(8, 2, 5, 20), (12, 2, 3, 20), (6, 3, 4, 7)])
 def test_fri_prove_wire_bytes_match_oracle(zctx, torch_dev, log_size, beta, rem_log, queries):
     """proveCodeword's Fri.Proof in the reference's wire format, byte for byte."""
