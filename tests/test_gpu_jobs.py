@@ -23,7 +23,7 @@ def dev_job(torch_dev, cols, beta):
     return d_cols, d_lde, d_nodes
 
 
-@pytest.mark.parametrize("n_cols,lg", [(3, 4), (8, 6), (16, 9), (20, 11), (5, 12)])
+@pytest.mark.parametrize("n_cols,lg", [(3, 4), (8, 6), (16, 9), (20, 11), (5, 12), (8, 16)])
 def test_commit_job_matches_oracle(zctx, torch_dev, n_cols, lg):
     rng = np.random.default_rng(n_cols * 100 + lg)
     cols = rng.integers(0, P, (n_cols, 1 << lg), dtype=np.uint32)
