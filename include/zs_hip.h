@@ -200,6 +200,15 @@ int32_t zs_fri_prove_dev(zs_hip_ctx *ctx, const uint32_t *d_codeword, uint32_t l
                          uint8_t **proof_out, size_t *proof_len);
 int32_t zs_fri_read_layer(const zs_fri_result *res, uint32_t layer, uint32_t *out);
 void zs_fri_result_free(zs_fri_result *res);
+/* Fri.verify (src/m31/fri.zig:277-336) of a proof in the wire format above, on
+ * the host (no device needed): `chan` in the transcript state the prover's
+ * channel had before the commit phase; it is advanced as the reference's
+ * verify advances it (absorb roots, resample alphas, absorb the remainder,
+ * resample the query indices).  Returns ZS_OK with *out_ok 1 (accepted) or 0
+ * (a root count, query count, index, Merkle path, fold or remainder mismatch),
+ * or ZS_ERR_INPUT when the bytes do not deserialize (truncated, trailing bytes),
+ * as the capi's verify entry points return -2 (src/capi.zig:175-204). */
+int32_t zs_fri_verify(const uint8_t *proof, size_t proof_len, zs_channel *chan, uint8_t *out_ok);
 
 /* ---- one trace's full commitment, batches, and the serving loop --------------- */
 /* The commitment of one trace (the composition the reference leaves open; the

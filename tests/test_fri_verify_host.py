@@ -1,0 +1,71 @@
+"""The library's host verifier (zs_fri_verify, Fri.verify of src/m31/fri.zig:277-336)
+on proofs the oracle writes in the wire format: accepted as the oracle's verify
+accepts them, rejected on any tampered value, path, root, remainder or index,
+ZS_ERR_INPUT for bytes that do not deserialize.  CPU only: no device is used."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import zig_stark_amd as Z
+from test_oracle_commit import low_degree_codeword
+
+P = O.P
+
+
+def proof_bytes(seed, log_size, beta, rem_log, nq, label):
+    rng = np.random.default_rng(seed)
+    cw = low_degree_codeword(rng, log_size, beta)
+    proof = O.fri_prove(cw, beta, rem_log, nq, O.Channel(label))
+    return proof, O.fri_proof_bytes(proof, beta, rem_log)
+
+
+@pytest.mark.parametrize("log_size,beta,rem_log,nq", [(4, 3, 3, 8), (6, 2, 3, 5), (7, 1, 2, 4)])
+def test_host_verify_accepts_oracle_proofs(log_size, beta, rem_log, nq):
+    label = b"zig-stark:fri:host"
+    proof, b = proof_bytes(log_size * 10 + beta, log_size, beta, rem_log, nq, label)
+    assert O.fri_verify(proof, beta, rem_log, O.Channel(label))
+    ch, och = Z.Channel(label), O.Channel(label)
+    assert Z.fri_verify(b, ch)
+    O.fri_verify(proof, beta, rem_log, och)
+    assert ch.sample_bytes(32) == och.sample_bytes(32)  # the transcript advanced the same way
+    # another transcript draws other challenges and indices
+    assert not Z.fri_verify(b, Z.Channel(b"zig-stark:fri:other"))
+
+
+def test_host_verify_rejects_tampering():
+    label = b"zig-stark:fri:tamper"
+    log_size, beta, rem_log, nq = 5, 2, 3, 6
+    _, b = proof_bytes(11, log_size, beta, rem_log, nq, label)
+    dlog, L = log_size + beta, log_size + beta - rem_log
+    head = 1 + 1 + 8 + 1
+    roots_at = head + 8
+    rem_at = roots_at + 32 * L + 8
+    rc = 1 << (rem_log - beta)
+    q0 = rem_at + 16 * rc + 8  # first query: index, pair count, then pair 0
+    flips = {
+        "root": roots_at + 5,
+        "remainder": rem_at + 3,
+        "index": q0,
+        "value0": q0 + 16 + 2,
+        "value1": q0 + 16 + 16 + 1,
+        "path0": q0 + 16 + 32 + 8 + 7,
+    }
+    for what, at in flips.items():
+        t = bytearray(b)
+        t[at] ^= 0x01
+        assert not Z.fri_verify(bytes(t), Z.Channel(label)), what
+    assert Z.fri_verify(b, Z.Channel(label))
+
+
+def test_host_verify_rejects_malformed_bytes():
+    label = b"zig-stark:fri:wire"
+    _, b = proof_bytes(5, 4, 2, 3, 3, label)
+    for bad in (b[:-1], b + b"\0", b[:10], b""):
+        with pytest.raises(Z.ZsError) as e:
+            Z.fri_verify(bad, Z.Channel(label))
+        assert e.value.code == Z.ZS_ERR_INPUT
+    # a slice length larger than the bytes that follow
+    t = bytearray(b)
+    t[11:19] = (1 << 40).to_bytes(8, "little")
+    with pytest.raises(Z.ZsError):
+        Z.fri_verify(bytes(t), Z.Channel(label))

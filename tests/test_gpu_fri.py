@@ -62,6 +62,26 @@ def test_fri_low_degree_proof_verifies(zctx):
     bad = dict(proof, qvals=vals.copy())
     bad["qvals"][5] = (int(bad["qvals"][5]) + 1) % P
     assert not O.fri_verify(bad, 2, 3, O.Channel(b"zig-stark:fri:test"))
+    # the same proof in wire bytes, through the library's host verifier
+    assert Z.fri_verify(O.fri_proof_bytes(proof, 2, 3), Z.Channel(b"zig-stark:fri:test"))
+    assert not Z.fri_verify(O.fri_proof_bytes(bad, 2, 3), Z.Channel(b"zig-stark:fri:test"))
+
+
+def test_fri_prove_dev_low_degree_verifies_on_host(zctx, torch_dev):
+    """zs_fri_prove_dev's wire bytes for a low-degree codeword pass zs_fri_verify (and the
+    oracle's verify); one flipped value bit fails both."""
+    from conftest import to_dev
+    rng = np.random.default_rng(77)
+    log_size, beta, rem_log, nq = 9, 2, 4, 12
+    _, cw = low_degree_codeword_on_gpu(zctx, rng, log_size, beta)
+    d_cw = to_dev(cw.reshape(-1), torch_dev)
+    b = zctx.fri_prove_dev(d_cw.data_ptr(), log_size, beta, rem_log, nq, Z.Channel(b"zig-stark:fri:host"))
+    assert Z.fri_verify(b, Z.Channel(b"zig-stark:fri:host"))
+    L = log_size + beta - rem_log
+    q0 = 11 + 8 + 32 * L + 8 + 16 * (1 << (rem_log - beta)) + 8  # first query
+    t = bytearray(b)
+    t[q0 + 16 + 3] ^= 0x10  # pair 0, value0
+    assert not Z.fri_verify(bytes(t), Z.Channel(b"zig-stark:fri:host"))
 
 
 def test_fri_full_size_2p22(zctx):
@@ -174,5 +194,8 @@ def test_fri_prove_wire_bytes_match_oracle(zctx, torch_dev, log_size, beta, rem_
     got = zctx.fri_prove_dev(d_cw.data_ptr(), log_size, beta, rem_log, queries, Z.Channel(b"zig-stark:wire"))
     proof = O.fri_prove(cw, beta, rem_log, queries, O.Channel(b"zig-stark:wire"))
     assert got == O.fri_proof_bytes(proof, beta, rem_log)
+    # the host verifier decides as the oracle's verify (a random codeword is not low-degree)
+    assert Z.fri_verify(got, Z.Channel(b"zig-stark:wire")) == O.fri_verify(proof, beta, rem_log,
+                                                                           O.Channel(b"zig-stark:wire"))
     p = parse_fri_proof(got)
     assert len(p["queries"]) == queries and p["log_size"] == log_size

@@ -117,6 +117,7 @@ def _declare(L):
                                    ctypes.POINTER(sz)]
     L.zs_fib_stark_prove_dev.argtypes = [vp, vp, u32, u32, sz, u32, u32p, u32p, u32p, vp, HostAllocator,
                                          ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(sz)]
+    L.zs_fri_verify.argtypes = [ctypes.c_char_p, sz, vp, ctypes.POINTER(ctypes.c_uint8)]
     L.zs_free.argtypes = [HostAllocator, ctypes.c_void_p, sz]
     L.zs_free.restype = None
     L.zs_autotune_lde.argtypes = [vp, u32, u32, u32, u32p]
@@ -139,7 +140,8 @@ def _declare(L):
                  "zs_merkle_open", "zs_tree_read_level", "zs_fri_commit_dev", "zs_fri_commit", "zs_fri_query",
                  "zs_fri_read_layer", "zs_fill_m31_dev", "zs_hip_profile", "zs_hip_profile_read",
                  "zs_commit_job_dev", "zs_commit_batch_dev", "zs_server_submit", "zs_server_wait",
-                 "zs_server_memory", "zs_server_poll", "zs_qm31_trace_commit_dev", "zs_fib_trace", "zs_autotune_lde", "zs_fri_prove_dev", "zs_fib_stark_prove_dev"]:
+                 "zs_server_memory", "zs_server_poll", "zs_qm31_trace_commit_dev", "zs_fib_trace", "zs_autotune_lde", "zs_fri_prove_dev", "zs_fib_stark_prove_dev",
+                 "zs_fri_verify"]:
         getattr(L, name).restype = i32
 
 
@@ -291,6 +293,15 @@ class Response(ctypes.Structure):
         rc = 1 << (remainder_log - log_blowup)
         return (bytes(self.trace_root), [bytes(self.fri_roots[32 * i:32 * (i + 1)]) for i in range(L)],
                 np.array(self.remainder[:4 * rc], np.uint32).reshape(rc, 4))
+
+
+def fri_verify(proof, channel):
+    """Fri.verify (src/m31/fri.zig:277-336) of Fri.Proof wire bytes, on the host;
+    `channel` is advanced as the reference's verify advances it.  True or False;
+    ZsError(ZS_ERR_INPUT) when the bytes do not deserialize."""
+    ok = ctypes.c_uint8()
+    _check(lib().zs_fri_verify(bytes(proof), len(proof), channel.h, ctypes.byref(ok)))
+    return bool(ok.value)
 
 
 def fib_trace(a0, a1, log_n):
