@@ -209,6 +209,18 @@ void zs_fri_result_free(zs_fri_result *res);
  * or ZS_ERR_INPUT when the bytes do not deserialize (truncated, trailing bytes),
  * as the capi's verify entry points return -2 (src/capi.zig:175-204). */
 int32_t zs_fri_verify(const uint8_t *proof, size_t proof_len, zs_channel *chan, uint8_t *out_ok);
+/* GenericStark(FibSeededAir).verify (src/m31/stark.zig:660-780) of the
+ * GenericStark.Proof wire bytes zs_fib_stark_prove_dev writes, on the host:
+ * public inputs a0, a1, claimed (QM31 words) and `chan` in the prover's starting
+ * state, advanced as the reference's verify advances it (trace roots, alphas,
+ * betas, quotient root, z, DEEP values, gammas, then Fri.verify under the
+ * STARK's parameters, then every query's reveals: Merkle paths, the DEEP
+ * identity against the FRI layer-0 value, and Hc(x0) = Z_H(x0) Q(x0)).  One
+ * reveal per FRI query is required (the prover writes exactly that many).
+ * Returns ZS_OK with *out_ok 1 or 0, or ZS_ERR_INPUT for bytes that do not
+ * deserialize. */
+int32_t zs_fib_stark_verify(const uint8_t *proof, size_t proof_len, const uint32_t a0[4], const uint32_t a1[4],
+                            const uint32_t claimed[4], zs_channel *chan, uint8_t *out_ok);
 
 /* ---- one trace's full commitment, batches, and the serving loop --------------- */
 /* The commitment of one trace (the composition the reference leaves open; the

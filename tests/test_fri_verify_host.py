@@ -1,4 +1,5 @@
-"""The library's host verifier (zs_fri_verify, Fri.verify of src/m31/fri.zig:277-336)
+"""The library's host verifiers (zs_fri_verify: Fri.verify of src/m31/fri.zig:277-336;
+zs_fib_stark_verify: GenericStark(FibSeededAir).verify of src/m31/stark.zig:660-780)
 on proofs the oracle writes in the wire format: accepted as the oracle's verify
 accepts them, rejected on any tampered value, path, root, remainder or index,
 ZS_ERR_INPUT for bytes that do not deserialize.  CPU only: no device is used."""
@@ -69,3 +70,43 @@ def test_host_verify_rejects_malformed_bytes():
     t[11:19] = (1 << 40).to_bytes(8, "little")
     with pytest.raises(Z.ZsError):
         Z.fri_verify(bytes(t), Z.Channel(label))
+
+
+# ---- GenericStark(FibSeededAir).verify (stark.zig:660-780) -------------------------------
+def fib_case(tl, a1=5):
+    t = O.fib_trace(1, a1, 1 << tl)
+    return t, np.array([1, 0, 0, 0], np.uint32), np.array([a1, 0, 0, 0], np.uint32), t[0, -1]
+
+
+@pytest.mark.parametrize("tl,beta,nq,rl", [(4, 3, 12, 3), (5, 2, 8, 3), (6, 1, 5, 2)])
+def test_host_stark_verify_matches_oracle(tl, beta, nq, rl):
+    t, a0, a1, claimed = fib_case(tl)
+    proof = O.fib_stark_prove(t, beta, nq, rl, a0, a1, claimed, O.Channel(b"fib"))
+    ch, och = Z.Channel(b"fib"), O.Channel(b"fib")
+    assert Z.fib_stark_verify(proof, a0, a1, claimed, ch)
+    assert O.fib_stark_verify(proof, a0, a1, claimed, och)
+    assert ch.sample_bytes(32) == och.sample_bytes(32)
+    assert not Z.fib_stark_verify(proof, a0, a1, claimed, Z.Channel(b"other"))
+    bad = claimed.copy()
+    bad[0] = (int(bad[0]) + 1) % P
+    assert not Z.fib_stark_verify(proof, a0, a1, bad, Z.Channel(b"fib"))
+    for cut in (proof[:-1], proof + b"\0"):
+        with pytest.raises(Z.ZsError) as e:
+            Z.fib_stark_verify(cut, a0, a1, claimed, Z.Channel(b"fib"))
+        assert e.value.code == Z.ZS_ERR_INPUT
+
+
+def test_host_stark_verify_rejects_tampering_and_false_claims():
+    t, a0, a1, claimed = fib_case(4)
+    # stark.zig:1282-1298: a proof for a wrong claimed value does not verify
+    bad = claimed.copy()
+    bad[0] = (int(bad[0]) + 1) % P
+    wrong = O.fib_stark_prove(t, 3, 12, 3, a0, a1, bad, O.Channel(b"fib"))
+    assert not Z.fib_stark_verify(wrong, a0, a1, bad, Z.Channel(b"fib"))
+    proof = O.fib_stark_prove(t, 3, 12, 3, a0, a1, claimed, O.Channel(b"fib"))
+    # the quotient root, a DEEP evaluation, a revealed value, a path digest near the end
+    for off in (11 + 8 + 64 + 2 + 5, 11 + 8 + 64 + 2 + 32 + 8 + 3, len(proof) - 200, len(proof) - 7):
+        t2 = bytearray(proof)
+        t2[off] ^= 1
+        assert not Z.fib_stark_verify(bytes(t2), a0, a1, claimed, Z.Channel(b"fib")), off
+        assert not O.fib_stark_verify(bytes(t2), a0, a1, claimed, O.Channel(b"fib")), off

@@ -100,6 +100,7 @@ def test_fib_stark_proof_matches_oracle(zctx, torch_dev, tl, beta, nq, rl):
     assert got == want
     assert ch.sample_bytes(32) == och.sample_bytes(32)
     assert O.fib_stark_verify(got, a0v, a1v, claimed, O.Channel(b"zig-stark:fib"))
+    assert Z.fib_stark_verify(got, a0v, a1v, claimed, Z.Channel(b"zig-stark:fib"))
 
 
 def test_fib_stark_configuration_proof(zctx, torch_dev):
@@ -117,3 +118,6 @@ def test_fib_stark_configuration_proof(zctx, torch_dev):
     bad = np.array(claimed)
     bad[0] = (int(bad[0]) + 1) % P
     assert not O.fib_stark_verify(got, a0v, a1v, bad, O.Channel(b"zig-stark:fib"))
+    # the library's host verifier decides the same
+    assert Z.fib_stark_verify(got, a0v, a1v, claimed, Z.Channel(b"zig-stark:fib"))
+    assert not Z.fib_stark_verify(got, a0v, a1v, bad, Z.Channel(b"zig-stark:fib"))

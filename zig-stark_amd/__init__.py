@@ -118,6 +118,7 @@ def _declare(L):
     L.zs_fib_stark_prove_dev.argtypes = [vp, vp, u32, u32, sz, u32, u32p, u32p, u32p, vp, HostAllocator,
                                          ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(sz)]
     L.zs_fri_verify.argtypes = [ctypes.c_char_p, sz, vp, ctypes.POINTER(ctypes.c_uint8)]
+    L.zs_fib_stark_verify.argtypes = [ctypes.c_char_p, sz, u32p, u32p, u32p, vp, ctypes.POINTER(ctypes.c_uint8)]
     L.zs_free.argtypes = [HostAllocator, ctypes.c_void_p, sz]
     L.zs_free.restype = None
     L.zs_autotune_lde.argtypes = [vp, u32, u32, u32, u32p]
@@ -141,7 +142,7 @@ def _declare(L):
                  "zs_fri_read_layer", "zs_fill_m31_dev", "zs_hip_profile", "zs_hip_profile_read",
                  "zs_commit_job_dev", "zs_commit_batch_dev", "zs_server_submit", "zs_server_wait",
                  "zs_server_memory", "zs_server_poll", "zs_qm31_trace_commit_dev", "zs_fib_trace", "zs_autotune_lde", "zs_fri_prove_dev", "zs_fib_stark_prove_dev",
-                 "zs_fri_verify"]:
+                 "zs_fri_verify", "zs_fib_stark_verify"]:
         getattr(L, name).restype = i32
 
 
@@ -301,6 +302,17 @@ def fri_verify(proof, channel):
     ZsError(ZS_ERR_INPUT) when the bytes do not deserialize."""
     ok = ctypes.c_uint8()
     _check(lib().zs_fri_verify(bytes(proof), len(proof), channel.h, ctypes.byref(ok)))
+    return bool(ok.value)
+
+
+def fib_stark_verify(proof, a0, a1, claimed, channel):
+    """GenericStark(FibSeededAir).verify (src/m31/stark.zig:660-780) of the proof's
+    wire bytes, on the host.  True or False; ZsError(ZS_ERR_INPUT) for bytes that
+    do not deserialize."""
+    ok = ctypes.c_uint8()
+    q = lambda v: _p(_u32(v).reshape(4))
+    _check(lib().zs_fib_stark_verify(bytes(proof), len(proof), q(a0), q(a1), q(claimed), channel.h,
+                                     ctypes.byref(ok)))
     return bool(ok.value)
 
 
