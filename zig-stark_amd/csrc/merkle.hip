@@ -152,13 +152,6 @@ __device__ __forceinline__ void subtree_lane(const uint32_t *child, uint32_t *lv
     }
 }
 
-template <int NLEV>
-__global__ __launch_bounds__(256) void tree_subtree_kernel(const uint32_t *child, uint32_t *lvl1, uint64_t n_lanes) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_lanes) return;
-    subtree_lane<NLEV>(child, lvl1, n_lanes, i);
-}
-
 // Several independent trees in one launch (the layers of an FRI commitment):
 // tree t owns lanes [lane_end[t-1], lane_end[t]) and its levels at word offsets
 // child[t], lvl1[t] from `base`.
