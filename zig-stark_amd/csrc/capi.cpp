@@ -87,11 +87,6 @@ static int32_t dev_alloc(Ctx *ctx, DevBuf &b, size_t bytes) {
 }
 
 static bool is_pow2(size_t n) { return n && !(n & (n - 1)); }
-static uint32_t log2u(size_t n) {
-    uint32_t l = 0;
-    while ((size_t(1) << l) < n) l++;
-    return l;
-}
 
 extern "C" {
 
