@@ -216,7 +216,9 @@ int32_t zs_fri_verify(const uint8_t *proof, size_t proof_len, zs_channel *chan, 
  * betas, quotient root, z, DEEP values, gammas, then Fri.verify under the
  * STARK's parameters, then every query's reveals: Merkle paths, the DEEP
  * identity against the FRI layer-0 value, and Hc(x0) = Z_H(x0) Q(x0)).  One
- * reveal per FRI query is required (the prover writes exactly that many).
+ * reveal per FRI query is required (the prover writes exactly that many), and
+ * the FRI proof's own copy of the parameters must equal the STARK's (the
+ * reference reads but does not compare them; an honest prover writes the same).
  * Returns ZS_OK with *out_ok 1 or 0, or ZS_ERR_INPUT for bytes that do not
  * deserialize. */
 int32_t zs_fib_stark_verify(const uint8_t *proof, size_t proof_len, const uint32_t a0[4], const uint32_t a1[4],

@@ -110,3 +110,32 @@ def test_host_stark_verify_rejects_tampering_and_false_claims():
         t2[off] ^= 1
         assert not Z.fib_stark_verify(bytes(t2), a0, a1, claimed, Z.Channel(b"fib")), off
         assert not O.fib_stark_verify(bytes(t2), a0, a1, claimed, O.Channel(b"fib")), off
+
+
+def test_host_verifiers_survive_random_corruption():
+    """Random byte flips, truncations and extensions of valid proofs: every call
+    returns (accept, reject or ZS_ERR_INPUT) and the corrupted proofs never verify."""
+    rng = np.random.default_rng(99)
+    _, fri_b = proof_bytes(3, 5, 2, 3, 4, b"fuzz")
+    t, a0, a1, claimed = fib_case(4)
+    stark_b = O.fib_stark_prove(t, 2, 6, 3, a0, a1, claimed, O.Channel(b"fuzz"))
+    cases = [(fri_b, lambda b: Z.fri_verify(b, Z.Channel(b"fuzz"))),
+             (stark_b, lambda b: Z.fib_stark_verify(b, a0, a1, claimed, Z.Channel(b"fuzz")))]
+    for good, verify in cases:
+        assert verify(good)
+        for _ in range(150):
+            b = bytearray(good)
+            kind = rng.integers(0, 3)
+            if kind == 0:
+                for _ in range(int(rng.integers(1, 4))):
+                    b[int(rng.integers(0, len(b)))] ^= 1 << int(rng.integers(0, 8))
+            elif kind == 1:
+                b = b[: int(rng.integers(0, len(b)))]
+            else:
+                b += bytes(rng.integers(0, 256, int(rng.integers(1, 40)), dtype=np.uint8))
+            if bytes(b) == good:
+                continue
+            try:
+                assert not verify(bytes(b))
+            except Z.ZsError as e:
+                assert e.code == Z.ZS_ERR_INPUT

@@ -283,7 +283,13 @@ bool stark_verify(const StarkProof &sp, const qm31 a0, const qm31 a1, const qm31
         zs_channel_absorb(chan, reinterpret_cast<const uint8_t *>(wds), 16);
     }
     for (auto &g : gammas) g = sample();
-    // FRI under the STARK's parameters (friParams: the same four fields)
+    // FRI under the STARK's parameters (friParams: the same four fields).  The
+    // reference ignores the nested proof's own copy of them; an honest prover
+    // writes the same values, so a proof whose copy differs is rejected here.
+    const FriProof &fp = sp.fri;
+    if (fp.log_size != pp.log_size || fp.log_blowup != pp.log_blowup || fp.num_queries != pp.num_queries ||
+        fp.remainder_log != pp.remainder_log)
+        return false;
     if (!fri_verify(pp, sp.fri, chan)) return false;
     // one reveal per FRI query (the reference walks the reveals the proof holds;
     // the prover writes one per query)
