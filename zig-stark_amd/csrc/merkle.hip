@@ -272,12 +272,18 @@ __global__ __launch_bounds__(BLOCK_NT) void tree_block_kernel(uint32_t *base, co
 // 8-child subtree (throughput); below that, by workgroup subtrees whose quads
 // share each hash (a lane's chain of 14 compressions would set the latency)
 constexpr uint32_t kSubtreeMinLog = 18;
+// once a tree that wide makes the 3-level pass worth a launch, narrower trees of the
+// same launch join it down to 2^11 children: their lanes add little to a pass whose
+// time one lane's 7 compressions set, and it takes 3 levels (7/8 of the workgroups)
+// off the workgroup launch that follows
+constexpr uint32_t kSubtreeJoinLog = 11;
 
 // Every node level of several trees whose level `level[k]` is written: tree k
-// has 2^log_leaves[k] leaves at word offset off_words[k] from `base`.  Wide
-// levels go 3 at a time (one lane per 8-child subtree, all trees in one launch);
-// from 2^16 children down, workgroup subtrees of 8 levels, then one workgroup
-// per tree for the rest.
+// has 2^log_leaves[k] leaves at word offset off_words[k] from `base`.  While some
+// tree has a level of 2^18 children or more, every tree with at least 2^11
+// children goes 3 levels at a time (one lane per 8-child subtree, all trees in one
+// launch); then workgroup subtrees of 8 levels, then one workgroup per tree for
+// the rest.
 int32_t launch_trees_multi(Ctx *ctx, uint32_t *base, const uint64_t *off_words, const uint32_t *log_leaves,
                            uint32_t n_trees, B3Hasher *draw_chan, uint32_t *draw_alpha) {
     // a fused draw needs the root computed by one workgroup: one tree with at least one level
@@ -289,9 +295,12 @@ int32_t launch_trees_multi(Ctx *ctx, uint32_t *base, const uint64_t *off_words, 
             MultiTrees mt{};
             uint64_t lanes = 0;
             std::vector<uint32_t> in;
+            bool wide = false;
+            for (uint32_t k = start; k < start + cnt; k++) wide |= log_leaves[k] - level[k] >= kSubtreeMinLog;
+            if (!wide) break;
             for (uint32_t k = start; k < start + cnt; k++) {
                 uint32_t rem = log_leaves[k] - level[k];
-                if (rem < kSubtreeMinLog) continue;
+                if (rem < kSubtreeJoinLog) continue;
                 mt.child[mt.n] = off_words[k] + level_offset_words(log_leaves[k], level[k]);
                 mt.lvl1[mt.n] = off_words[k] + level_offset_words(log_leaves[k], level[k] + 1);
                 lanes += (uint64_t(1) << rem) >> 3;
