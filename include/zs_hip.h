@@ -251,7 +251,7 @@ typedef struct {
     uint32_t *remainder;     /* out, 4 * 2^(remainder_log - log_blowup) words */
     int32_t status;          /* out, ZS_OK or an error code */
 } zs_job;
-/* Commit many traces of uneven sizes.  n_workers > 1 runs them on that many
+/* Commit many traces of uneven sizes.  n_workers > 1 (at most 64) runs them on that many
  * worker contexts with their own streams, taking jobs from a shared queue so
  * their kernels overlap on the device; ZS_FLAG_PLAIN runs the reference-shaped
  * per-trace loop (one trace after another, per-column per-layer launches).

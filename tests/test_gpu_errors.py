@@ -43,3 +43,21 @@ def test_merkle_open_rejects_out_of_range_index(zctx):
     tree, root, _ = zctx.lde_commit(cols, 1)
     assert len(tree.open((1 << 5) - 1)) == 32 * 5
     assert _code(lambda: tree.open(1 << 5)) == Z.ZS_ERR_INPUT
+
+
+def test_log_sizes_that_wrap_are_rejected(zctx, torch_dev):
+    """log_n + log_blowup is checked without the 32-bit wrap: a huge log_blowup
+    whose sum with log_n wraps below 28 is still ZS_ERR_INPUT."""
+    import torch
+    out = torch.full((1 << 12,), 7, dtype=torch.int32, device=torch_dev)
+    q = out.data_ptr()
+    for lg, beta in [(20, (1 << 32) - 16), ((1 << 32) - 1, 2)]:
+        assert _code(lambda: zctx.lde_commit_dev(q, 1, lg, beta, q, q)) == Z.ZS_ERR_INPUT
+        assert _code(lambda: zctx.lde_dev(q, 1, lg, beta, q)) == Z.ZS_ERR_INPUT
+    zctx.synchronize()
+    assert bool((out == 7).all())
+
+
+def test_batch_rejects_too_many_workers(zctx):
+    assert _code(lambda: zctx.commit_batch_dev([], 2, 3, n_workers=65)) == Z.ZS_ERR_INPUT
+    assert zctx.commit_batch_dev([], 2, 3, n_workers=4) == []

@@ -250,7 +250,7 @@ int32_t zs_m31_circle_ifft_batch_dev(zs_hip_ctx *h, const uint32_t *d_evals, uin
 // ---- LDE + row commitment ------------------------------------------------------
 int32_t zs_m31_lde_commit_dev(zs_hip_ctx *h, const uint32_t *d_cols, uint32_t n_cols, uint32_t log_n,
                               uint32_t log_blowup, uint32_t *d_lde, uint8_t *d_nodes, uint32_t flags) {
-    if (!h || !d_cols || !d_lde || !d_nodes || n_cols == 0 || log_n < 1 || log_n + log_blowup > 28)
+    if (!h || !d_cols || !d_lde || !d_nodes || n_cols == 0 || log_n < 1 || log_sum_over(log_n, log_blowup, 28))
         return ZS_ERR_INPUT;
     Ctx *ctx = &h->c;
     uint32_t LT = log_n + log_blowup;
@@ -267,7 +267,7 @@ int32_t zs_m31_lde_commit_dev(zs_hip_ctx *h, const uint32_t *d_cols, uint32_t n_
 
 int32_t zs_m31_lde_dev(zs_hip_ctx *h, const uint32_t *d_cols, uint32_t n_cols, uint32_t log_n, uint32_t log_blowup,
                        uint32_t *d_lde) {
-    if (!h || !d_cols || !d_lde || n_cols == 0 || log_n < 1 || log_n + log_blowup > 28) return ZS_ERR_INPUT;
+    if (!h || !d_cols || !d_lde || n_cols == 0 || log_n < 1 || log_sum_over(log_n, log_blowup, 28)) return ZS_ERR_INPUT;
     Ctx *ctx = &h->c;
     {
         // the launch configuration chosen for this shape, if it was tuned
@@ -289,7 +289,7 @@ int32_t zs_m31_lde_dev(zs_hip_ctx *h, const uint32_t *d_cols, uint32_t n_cols, u
 // kind of LDE pass, the candidate under which that pass's kernels ran fastest: the
 // passes exchange data in one fixed layout, so their choices are independent.
 int32_t zs_autotune_lde(zs_hip_ctx *h, uint32_t n_cols, uint32_t log_n, uint32_t log_blowup, uint32_t out[7]) {
-    if (!h || n_cols == 0 || log_n < 1 || log_n + log_blowup > 28) return ZS_ERR_INPUT;
+    if (!h || n_cols == 0 || log_n < 1 || log_sum_over(log_n, log_blowup, 28)) return ZS_ERR_INPUT;
     Ctx *ctx = &h->c;
     ZS_HIP(ctx, hipSetDevice(ctx->device));
     ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -434,7 +434,7 @@ int32_t zs_m31_row_commit_dev(zs_hip_ctx *h, const uint32_t *d_cols, uint32_t n_
 
 int32_t zs_m31_lde_commit(zs_hip_ctx *h, const uint32_t *cols, uint32_t n_cols, uint32_t log_n, uint32_t log_blowup,
                           zs_tree **out_tree, uint8_t root[32], uint32_t *lde_out) {
-    if (!h || !cols || n_cols == 0 || log_n < 1 || log_n + log_blowup > 28) return ZS_ERR_INPUT;
+    if (!h || !cols || n_cols == 0 || log_n < 1 || log_sum_over(log_n, log_blowup, 28)) return ZS_ERR_INPUT;
     Ctx *ctx = &h->c;
     ZS_HIP(ctx, hipSetDevice(ctx->device));
     uint32_t LT = log_n + log_blowup;

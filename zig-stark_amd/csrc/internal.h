@@ -94,6 +94,8 @@ struct LdeTiles {
     TileConfig role[3];
 };
 LdeTiles default_lde_tiles();
+// a + b > max for log sizes, without the wrap of a 32-bit sum
+inline bool log_sum_over(uint32_t a, uint32_t b, uint32_t max) { return a > max || b > max || a + b > max; }
 // key of an LDE shape in the tuned table: columns rounded up to a power of two
 inline uint64_t lde_shape_key(uint32_t n_cols, uint32_t log_n, uint32_t log_blowup) {
     uint32_t cl = 0;

@@ -54,7 +54,8 @@ int32_t commit_job(zs_hip_ctx *h, const uint32_t *d_cols, uint32_t n_cols, uint3
                    uint8_t *fri_roots, uint32_t *remainder, uint32_t flags) {
     if (!h || !d_cols || !d_lde || !d_nodes || !trace_root || !fri_roots || !remainder) return ZS_ERR_INPUT;
     const uint32_t LT = log_n + log_blowup;
-    if (n_cols == 0 || n_cols > 4096 || log_n < 1 || LT > 28 || remainder_log < log_blowup || remainder_log >= LT)
+    if (n_cols == 0 || n_cols > 4096 || log_n < 1 || log_sum_over(log_n, log_blowup, 28) ||
+        remainder_log < log_blowup || remainder_log >= LT)
         return ZS_ERR_INPUT;
     Ctx *ctx = &h->c;
     const size_t N = size_t(1) << LT;
@@ -194,7 +195,8 @@ int32_t zs_commit_job_dev(zs_hip_ctx *ctx, const uint32_t *d_cols, uint32_t n_co
 
 int32_t zs_commit_batch_dev(zs_hip_ctx *h, zs_job *jobs, size_t n_jobs, uint32_t log_blowup, uint32_t remainder_log,
                             uint32_t n_workers, uint32_t flags) {
-    if (!h || (!jobs && n_jobs)) return ZS_ERR_INPUT;
+    if (!h || (!jobs && n_jobs) || n_workers > 64) return ZS_ERR_INPUT;
+    if (n_jobs == 0) return ZS_OK;
     Ctx *ctx = &h->c;
     auto run_one = [&](zs_hip_ctx *w, zs_job &j) {
         j.status = commit_job(w, j.d_cols, j.n_cols, j.log_n, log_blowup, remainder_log, j.d_lde, j.d_nodes,
@@ -354,7 +356,8 @@ zs_server *zs_server_new(int32_t device, uint32_t n_workers, uint64_t hbm_budget
 int32_t zs_server_submit(zs_server *s, const zs_request *req, uint64_t *ticket) {
     if (!s || !req || !ticket) return ZS_ERR_INPUT;
     const uint32_t LT = req->log_n + req->log_blowup;
-    if (req->n_cols == 0 || req->n_cols > 4096 || req->log_n < 1 || LT > 28 || req->remainder_log < req->log_blowup ||
+    if (req->n_cols == 0 || req->n_cols > 4096 || req->log_n < 1 || log_sum_over(req->log_n, req->log_blowup, 28) ||
+        req->remainder_log < req->log_blowup ||
         req->remainder_log >= LT || req->remainder_log - req->log_blowup > 6)
         return ZS_ERR_INPUT;
     auto q = std::make_unique<ServReq>();

@@ -137,7 +137,7 @@ int32_t zs_qm31_trace_commit_dev(zs_hip_ctx *h, const uint32_t *d_cols, uint32_t
                                  uint32_t log_blowup, uint32_t *d_codewords, uint8_t *d_nodes, uint8_t *roots_out,
                                  zs_channel *chan) {
     if (!h || !d_cols || !d_codewords || !d_nodes || !roots_out || n_cols == 0 || trace_log < 1 ||
-        trace_log + log_blowup > 24)
+        log_sum_over(trace_log, log_blowup, 24))
         return ZS_ERR_INPUT;
     Ctx *ctx = &h->c;
     ZS_HIP(ctx, hipSetDevice(ctx->device));

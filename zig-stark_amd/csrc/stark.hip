@@ -151,7 +151,7 @@ int32_t zs_fib_stark_prove_dev(zs_hip_ctx *h, const uint32_t *d_trace, uint32_t 
                                const uint32_t claimed[4], zs_channel *chan, zs_host_allocator host,
                                uint8_t **proof_out, size_t *proof_len) {
     if (!h || !d_trace || !a0 || !a1 || !claimed || !chan || !host.alloc || !proof_out || !proof_len ||
-        trace_log < 1 || log_blowup < 1 || trace_log + log_blowup > 24 || remainder_log < log_blowup ||
+        trace_log < 1 || log_blowup < 1 || log_sum_over(trace_log, log_blowup, 24) || remainder_log < log_blowup ||
         remainder_log >= trace_log + log_blowup || num_queries > 4096)
         return ZS_ERR_INPUT;
     Ctx *ctx = &h->c;
