@@ -205,7 +205,7 @@ int32_t zs_circle_coset_canonic_half(uint32_t log_size, zs_circle_coset *out) {
 static int32_t host_transform(zs_hip_ctx *h, bool inverse, const uint32_t *in, size_t n, const zs_circle_coset *hc,
                               uint32_t *out) {
     if (!h || !in || !out || !hc) return ZS_ERR_INPUT;
-    if (n < 2 || !is_pow2(n) || (size_t(1) << hc->log_size) != n / 2 || hc->log_size > 28) return ZS_ERR_INPUT;
+    if (n < 2 || !is_pow2(n) || hc->log_size > 28 || (size_t(1) << hc->log_size) != n / 2) return ZS_ERR_INPUT;
     Ctx *ctx = &h->c;
     ZS_HIP(ctx, hipSetDevice(ctx->device));
     DevBuf a, b;
