@@ -199,3 +199,28 @@ def test_fri_prove_wire_bytes_match_oracle(zctx, torch_dev, log_size, beta, rem_
                                                                            O.Channel(b"zig-stark:wire"))
     p = parse_fri_proof(got)
     assert len(p["queries"]) == queries and p["log_size"] == log_size
+
+
+def test_fri_2p22_layer_trees_match_per_level_trees(zctx, torch_dev):
+    """Every layer tree of a 2^22 seeded commit (built together, several trees per
+    launch) equals the same layer's tree built alone by one launch per level.
+    hashQM31(v) is BLAKE3 of v's 16 bytes, the row leaf of a 4-column row, so
+    the per-level row commitment over the layer's coordinates is an independent
+    builder of the same tree (src/m31/fri.zig:195-233, src/core/merkle/merkle.zig:17-40)."""
+    import torch
+    rng = np.random.default_rng(31)
+    dlog, beta, rem_log = 22, 2, 5
+    L = dlog - rem_log
+    cw = rng.integers(0, P, (1 << dlog, 4), dtype=np.uint32)
+    alphas = rng.integers(0, P, (L, 4), dtype=np.uint32)
+    roots, _, res = zctx.fri_commit(cw, beta, rem_log, alphas=alphas)
+    d_nodes = torch.empty(((2 << dlog) - 1) * 8, dtype=torch.int32, device=torch_dev)
+    for i in range(L):
+        vals = res.layer(i)
+        lg = dlog - i
+        d_cols = torch.from_numpy(np.ascontiguousarray(vals.T).view(np.int32)).to(torch_dev)
+        zctx.row_commit_dev(d_cols.data_ptr(), 4, lg, 1 << lg, d_nodes.data_ptr(), Z.ZS_FLAG_PLAIN)
+        zctx.synchronize()
+        top = ((2 << lg) - 1) * 8
+        root = d_nodes[top - 8:top].cpu().numpy().view(np.uint32).tobytes()
+        assert root == roots[i], i
