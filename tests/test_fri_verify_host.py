@@ -139,3 +139,22 @@ def test_host_verifiers_survive_random_corruption():
                 assert not verify(bytes(b))
             except Z.ZsError as e:
                 assert e.code == Z.ZS_ERR_INPUT
+
+
+def test_host_stark_verify_rejects_altered_nested_fri_parameters():
+    """The FRI proof inside a STARK proof repeats the STARK's parameters; a copy
+    that differs is rejected (the honest prover writes the same 11 bytes)."""
+    t, a0, a1, claimed = fib_case(4)
+    proof = O.fib_stark_prove(t, 2, 6, 3, a0, a1, claimed, O.Channel(b"nested"))
+    # params 11 B, trace roots 8 + 2 * 32, two absent optional root lists, the
+    # quotient root 32, DEEP evaluations 8 + 5 * 16: the FRI proof starts here
+    at = 11 + 8 + 64 + 2 + 32 + 8 + 5 * 16
+    assert proof[at:at + 11] == proof[:11]
+    assert Z.fib_stark_verify(proof, a0, a1, claimed, Z.Channel(b"nested"))
+    for off in (0, 1, 2, 10):  # log_size, log_blowup, num_queries, remainder_log
+        t2 = bytearray(proof)
+        t2[at + off] = (t2[at + off] + 1) & 0xFF
+        try:
+            assert not Z.fib_stark_verify(bytes(t2), a0, a1, claimed, Z.Channel(b"nested")), off
+        except Z.ZsError as e:
+            assert e.code == Z.ZS_ERR_INPUT
