@@ -12,6 +12,10 @@ for p in (ROOT, TESTS):
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) device and the built HIP library")
+    # autotune picks go to a file of this session, so no test reads picks an earlier run left
+    if "ZS_AUTOTUNE_CACHE" not in os.environ:
+        import tempfile
+        os.environ["ZS_AUTOTUNE_CACHE"] = os.path.join(tempfile.mkdtemp(prefix="zs_tests_"), "autotune.txt")
 
 
 def _load_pkg():

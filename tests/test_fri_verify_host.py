@@ -158,3 +158,91 @@ def test_host_stark_verify_rejects_altered_nested_fri_parameters():
             assert not Z.fib_stark_verify(bytes(t2), a0, a1, claimed, Z.Channel(b"nested")), off
         except Z.ZsError as e:
             assert e.code == Z.ZS_ERR_INPUT
+
+
+def forge_fri_proof(cw, beta, rem_log, nq, label, rem_points):
+    """A proveCodeword (fri.zig:157-273) written out in Python over oracle parts,
+    with one freedom a cheating prover has: the remainder interpolates the first
+    `rem_points` values of the last layer.  rem_points = 2^(rem_log - beta) is
+    the honest prover; rem_points = 2^rem_log interpolates the whole last layer,
+    so the final fold check holds for any codeword.  Returns the wire bytes."""
+    cw = np.asarray(cw, np.uint32).reshape(-1, 4)
+    n0 = cw.shape[0]
+    dlog = n0.bit_length() - 1
+    L = dlog - rem_log
+    ch = O.Channel(label)
+    layers, trees, roots = [cw], [], []
+    for i in range(L):
+        nodes = O.merkle_build(b"".join(O.hash_qm31(v) for v in layers[i]))
+        trees.append(nodes)
+        roots.append(O.tree_root(nodes))
+        ch.absorb(roots[-1])
+        layers.append(O.fri_fold(layers[i], dlog, i, ch.sample_qm31()))
+    xs = np.array([O.fri_domain_point(dlog, L, j) for j in range(rem_points)])
+    rem = O.poly_interpolate(xs, layers[L][:rem_points])
+    for c in rem:
+        ch.absorb(np.asarray(c, np.uint32).tobytes())
+    out = bytearray()
+    out += bytes([dlog - beta, beta]) + nq.to_bytes(8, "little") + bytes([rem_log])
+    out += L.to_bytes(8, "little") + b"".join(roots)
+    out += len(rem).to_bytes(8, "little") + np.asarray(rem, np.uint32).tobytes()
+    out += nq.to_bytes(8, "little")
+    for _ in range(nq):
+        p = ch.sample_index(n0)
+        out += p.to_bytes(8, "little") + L.to_bytes(8, "little")
+        for i in range(L):
+            ni = n0 >> i
+            pn = p % (ni // 2)
+            out += layers[i][pn].tobytes() + layers[i][pn + ni // 2].tobytes()
+            for idx in (pn, pn + ni // 2):
+                out += (dlog - i).to_bytes(8, "little") + O.merkle_open(trees[i], ni, idx)
+            p = pn
+    return bytes(out)
+
+
+def test_host_verify_rejects_full_length_remainder():
+    """A remainder longer than 2^(remainder_log - log_blowup) coefficients can
+    interpolate any last layer, so a codeword of full degree would pass every
+    fold and remainder check.  The forger is first pinned to the oracle's honest
+    prover (same bytes), then given a random codeword and a remainder through
+    all 2^remainder_log points of the last layer: rejected."""
+    label = b"zig-stark:fri:cheat"
+    log_size, beta, rem_log, nq = 4, 2, 3, 5
+    rng = np.random.default_rng(7)
+    honest_cw = low_degree_codeword(rng, log_size, beta)
+    _, want = proof_bytes(7, log_size, beta, rem_log, nq, label)
+    honest = forge_fri_proof(honest_cw, beta, rem_log, nq, label, 1 << (rem_log - beta))
+    assert honest == want
+    assert Z.fri_verify(honest, Z.Channel(label))
+    junk = rng.integers(0, P, ((1 << (log_size + beta)), 4), dtype=np.uint32)
+    cheat = forge_fri_proof(junk, beta, rem_log, nq, label, 1 << rem_log)
+    assert not Z.fri_verify(cheat, Z.Channel(label))
+    # one coefficient more than the honest length is rejected as well
+    longer = forge_fri_proof(honest_cw, beta, rem_log, nq, label, (1 << (rem_log - beta)) + 1)
+    assert not Z.fri_verify(longer, Z.Channel(label))
+
+
+def test_host_verify_rejects_parameters_the_prover_refuses():
+    """remainder_log = log_size + log_blowup (no fold round: the reference's
+    verify checks nothing per query) and remainder_log < log_blowup (the
+    reference prover asserts against it, fri.zig:221-227) never verify."""
+    label = b"zig-stark:fri:params"
+    rng = np.random.default_rng(8)
+    log_size, beta, nq = 3, 2, 4
+    dlog = log_size + beta
+    for rem_log in (dlog, beta - 1):
+        ch = O.Channel(label)
+        k = 1 << max(rem_log - beta, 0)
+        rem = rng.integers(0, P, (k, 4), dtype=np.uint32)
+        L = max(dlog - rem_log, 0)
+        forged = bytearray(bytes([log_size, beta]) + nq.to_bytes(8, "little") + bytes([rem_log]))
+        forged += L.to_bytes(8, "little")
+        for i in range(L):  # roots of whatever layers: never looked at before the parameters
+            forged += bytes(32)
+        forged += k.to_bytes(8, "little") + rem.tobytes()
+        forged += nq.to_bytes(8, "little")
+        for c in rem:
+            ch.absorb(c.tobytes())
+        for _ in range(nq):
+            forged += ch.sample_index(1 << dlog).to_bytes(8, "little") + (0).to_bytes(8, "little")
+        assert not Z.fri_verify(bytes(forged), Z.Channel(label)), rem_log

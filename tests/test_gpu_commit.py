@@ -148,3 +148,43 @@ def test_autotuned_lde_is_bit_exact(zctx, torch_dev, n_cols, lg, beta):
     for c in (0, n_cols - 1):
         assert np.array_equal(lde[c], O.lde(cols[c], beta))
     assert from_dev(d_nodes)[-8:].view(np.uint32).tobytes() == O.tree_root(O.row_commit(lde))
+
+
+_TUNE_SCRIPT = r"""
+import json, sys, time
+sys.path.insert(0, sys.argv[1])
+import __graft_entry__
+Z = __graft_entry__.load_package()
+ctx = Z.Context(0)
+t0 = time.perf_counter()
+r = ctx.autotune_lde(8, 12, 2)
+print(json.dumps({"s": time.perf_counter() - t0, "picks": [list(r["interp"]), list(r["mid"]), list(r["eval"])]}))
+ctx.close()
+"""
+
+
+def test_autotune_picks_reused_by_a_second_process(tmp_path):
+    """The first process tunes a shape and appends its picks to the autotune file;
+    a second process on the same device reads them instead of tuning again."""
+    import json
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    cache = tmp_path / "autotune.txt"
+    env = dict(os.environ, ZS_AUTOTUNE_CACHE=str(cache))
+    env.pop("ZS_AUTOTUNE_REFRESH", None)
+
+    def run():
+        r = subprocess.run([sys.executable, "-c", _TUNE_SCRIPT, root], env=env, capture_output=True, text=True,
+                           timeout=300)
+        assert r.returncode == 0, r.stderr[-2000:]
+        return json.loads(r.stdout.strip().splitlines()[-1])
+
+    first = run()
+    lines = cache.read_text().splitlines()
+    assert len(lines) == 1
+    second = run()
+    assert cache.read_text().splitlines() == lines  # nothing tuned, nothing appended
+    assert second["picks"] == first["picks"]
+    assert second["s"] < first["s"]

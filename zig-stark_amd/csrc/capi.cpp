@@ -5,6 +5,12 @@
 // transforms with the CPU transform's signature, and error codes in the
 // zig-capi.h convention.  Unlike the reference there is no CPU fallback: a
 // missing device or a HIP failure is returned as ZS_ERR_DEVICE.
+#include <fcntl.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -44,6 +50,23 @@ int32_t Ctx::ensure_scratch(int slot, size_t bytes) {
         return e == hipErrorOutOfMemory ? ZS_ERR_OOM : ZS_ERR_DEVICE;
     }
     b.bytes = bytes;
+    return ZS_OK;
+}
+
+int32_t Ctx::ensure_pinned(size_t bytes) {
+    if (pinned_bytes >= bytes) return ZS_OK;
+    if (pinned) {
+        ZS_HIP(this, hipStreamSynchronize(stream));
+        (void)hipHostFree(pinned);
+        pinned = nullptr;
+        pinned_bytes = 0;
+    }
+    if (hipHostMalloc(&pinned, bytes, hipHostMallocDefault) != hipSuccess) {
+        pinned = nullptr;
+        set_error("pinned staging allocation failed");
+        return ZS_ERR_OOM;
+    }
+    pinned_bytes = bytes;
     return ZS_OK;
 }
 
@@ -120,6 +143,7 @@ void zs_hip_destroy(zs_hip_ctx *ctx) {
     if (ctx->c.capture_stream) (void)hipStreamDestroy(ctx->c.capture_stream);
     ctx->c.release_scratch();
     (void)hipStreamSynchronize(ctx->c.stream);
+    if (ctx->c.pinned) (void)hipHostFree(ctx->c.pinned);
     if (ctx->c.own_stream) (void)hipStreamDestroy(ctx->c.stream);
     delete ctx;
 }
@@ -285,13 +309,112 @@ int32_t zs_m31_lde_dev(zs_hip_ctx *h, const uint32_t *d_cols, uint32_t n_cols, u
                       big_words ? ctx->scratch[2].as<uint32_t>() : nullptr);
 }
 
+}  // extern "C"
+
+// ---- persisted autotune picks --------------------------------------------------------
+// One text line per tuned shape: "<device> <shape key> <6 tile values> <us>", where
+// <device> names the architecture, the compute-unit count and this library's
+// version.  zs_autotune_lde reads the file before tuning a shape and appends a
+// line after tuning one, so a later process skips the tuning.  Path:
+// $ZS_AUTOTUNE_CACHE, else $XDG_CACHE_HOME/zig-stark-hip/autotune.txt, else
+// $HOME/.cache/zig-stark-hip/autotune.txt; ZS_AUTOTUNE_CACHE=off disables the
+// file and ZS_AUTOTUNE_REFRESH=1 re-tunes (the last line of a key wins).
+namespace {
+std::string autotune_cache_path() {
+    const char *e = getenv("ZS_AUTOTUNE_CACHE");
+    if (e && *e) return std::string(e) == "off" ? std::string() : std::string(e);
+    const char *x = getenv("XDG_CACHE_HOME");
+    const char *home = getenv("HOME");
+    if (x && *x) return std::string(x) + "/zig-stark-hip/autotune.txt";
+    if (home && *home) return std::string(home) + "/.cache/zig-stark-hip/autotune.txt";
+    return std::string();
+}
+
+std::string device_key(int device) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return std::string();
+    std::string arch = prop.gcnArchName;
+    for (char &c : arch)
+        if (c == ' ' || c == ':') c = '_';
+    uint64_t vh = 1469598103934665603ull;  // FNV-1a of the version string
+    for (const char *v = zs_hip_version(); *v; v++) vh = (vh ^ (uint8_t)*v) * 1099511628211ull;
+    char buf[160];
+    snprintf(buf, sizeof buf, "%s-cu%d-v%016llx", arch.c_str(), prop.multiProcessorCount, (unsigned long long)vh);
+    return buf;
+}
+
+bool autotune_cache_lookup(int device, uint64_t key, LdeTiles *t, uint32_t *us) {
+    const std::string path = autotune_cache_path(), dev = device_key(device);
+    if (path.empty() || dev.empty() || (getenv("ZS_AUTOTUNE_REFRESH") && atoi(getenv("ZS_AUTOTUNE_REFRESH"))))
+        return false;
+    FILE *f = fopen(path.c_str(), "r");
+    if (!f) return false;
+    char line[512];
+    bool found = false;
+    while (fgets(line, sizeof line, f)) {
+        char d[200];
+        unsigned long long k;
+        unsigned v[7];
+        if (sscanf(line, "%199s %llx %u %u %u %u %u %u %u", d, &k, &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6]) != 9)
+            continue;
+        if (dev != d || k != key) continue;
+        bool ok = true;
+        for (int r = 0; r < 3; r++) ok = ok && v[2 * r] >= 11 && v[2 * r] <= 14 && v[2 * r + 1] >= 1 && v[2 * r + 1] <= 8;
+        if (!ok) continue;  // not a configuration the tuner can pick: ignored
+        for (int r = 0; r < 3; r++) t->role[r] = {v[2 * r], v[2 * r + 1]};
+        *us = v[6];
+        found = true;  // keep reading: the last line of a key wins
+    }
+    fclose(f);
+    return found;
+}
+
+void autotune_cache_store(int device, uint64_t key, const LdeTiles &t, uint32_t us) {
+    const std::string path = autotune_cache_path(), dev = device_key(device);
+    if (path.empty() || dev.empty()) return;
+    for (size_t p = path.find('/', 1); p != std::string::npos; p = path.find('/', p + 1))
+        (void)mkdir(path.substr(0, p).c_str(), 0755);
+    char line[512];
+    const int n = snprintf(line, sizeof line, "%s %llx %u %u %u %u %u %u %u\n", dev.c_str(), (unsigned long long)key,
+                           t.role[0].elems_log, t.role[0].cols_per_wg, t.role[1].elems_log, t.role[1].cols_per_wg,
+                           t.role[2].elems_log, t.role[2].cols_per_wg, us);
+    const int fd = open(path.c_str(), O_WRONLY | O_CREAT | O_APPEND, 0644);
+    if (fd < 0) return;
+    if (n > 0 && n < (int)sizeof line) (void)!write(fd, line, (size_t)n);  // one append per line
+    close(fd);
+}
+}  // namespace
+
+extern "C" {
+
 // Times each candidate configuration with per-kernel events and keeps, for each
 // kind of LDE pass, the candidate under which that pass's kernels ran fastest: the
 // passes exchange data in one fixed layout, so their choices are independent.
+// A shape tuned by an earlier process on the same device is read from the
+// autotune file instead.
 int32_t zs_autotune_lde(zs_hip_ctx *h, uint32_t n_cols, uint32_t log_n, uint32_t log_blowup, uint32_t out[7]) {
     if (!h || n_cols == 0 || log_n < 1 || log_sum_over(log_n, log_blowup, 28)) return ZS_ERR_INPUT;
     Ctx *ctx = &h->c;
     ZS_HIP(ctx, hipSetDevice(ctx->device));
+    {
+        LdeTiles cached;
+        uint32_t us = 0;
+        const uint64_t key = lde_shape_key(n_cols, log_n, log_blowup);
+        if (autotune_cache_lookup(ctx->device, key, &cached, &us)) {
+            {
+                std::lock_guard<std::mutex> lk(ctx->caches->mu);
+                ctx->caches->tuned[key] = cached;
+            }
+            if (out) {
+                for (int k = 0; k < 3; k++) {
+                    out[2 * k] = cached.role[k].elems_log;
+                    out[2 * k + 1] = cached.role[k].cols_per_wg;
+                }
+                out[6] = us;
+            }
+            return ZS_OK;
+        }
+    }
     ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     const uint32_t LT = log_n + log_blowup;
     const size_t n = size_t(1) << log_n, N = size_t(1) << LT;
@@ -404,6 +527,7 @@ int32_t zs_autotune_lde(zs_hip_ctx *h, uint32_t n_cols, uint32_t log_n, uint32_t
         if (rc == ZS_OK) ctx->caches->tuned[key] = best;
         else ctx->caches->tuned.erase(key);
     }
+    if (rc == ZS_OK) autotune_cache_store(ctx->device, key, best, (uint32_t)(total_ms * 1000.0f));
     if (rc == ZS_OK && out) {
         for (int k = 0; k < 3; k++) {
             out[2 * k] = best.role[k].elems_log;
@@ -506,8 +630,10 @@ int32_t zs_tree_read_level(const zs_tree *t, uint32_t level, uint8_t *out) {
 void zs_tree_free(zs_tree *t) { delete t; }
 
 // ---- FRI ---------------------------------------------------------------------------
+}  // extern "C"
+
 // univariate.zig:79-139 (O(n^2) Lagrange; the interpolant is unique)
-static void interpolate_host(const std::vector<qm31> &xs, const std::vector<qm31> &ys, std::vector<qm31> &coeffs) {
+void zs::interpolate_host(const std::vector<qm31> &xs, const std::vector<qm31> &ys, std::vector<qm31> &coeffs) {
     size_t n = xs.size();
     const qm31 zero{{0, 0}, {0, 0}}, one{{1, 0}, {0, 0}};
     std::vector<qm31> p(n + 1, zero), acc(n, zero), tmp(n, zero);
@@ -530,14 +656,17 @@ static void interpolate_host(const std::vector<qm31> &xs, const std::vector<qm31
     coeffs = acc;
 }
 
-}  // extern "C"
-
 int32_t zs::fri_commit_impl(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t dlog, uint32_t log_blowup,
                             uint32_t remainder_log, const uint32_t *alphas, B3Hasher *d_chan, uint32_t flags,
-                            uint8_t *roots_out, uint32_t *remainder_out, zs_fri_result **out_result) {
-    if (!h || !d_codeword || !roots_out || !remainder_out) return ZS_ERR_INPUT;
+                            uint8_t *roots_out, uint32_t *remainder_out, zs_fri_result **out_result,
+                            uint8_t *stage) {
+    if (!h || !d_codeword || (!stage && (!roots_out || !remainder_out)) || (stage && out_result))
+        return ZS_ERR_INPUT;
     if (dlog > 28 || remainder_log > dlog || remainder_log < log_blowup || dlog - remainder_log < 1)
         return ZS_ERR_INPUT;
+    // the remainder is interpolated on the host in O(rc^2): at most kMaxRemainderLog
+    // doublings above the blowup (64 coefficients)
+    if (remainder_log - log_blowup > kMaxRemainderLog) return ZS_ERR_INPUT;
     if (!alphas && !d_chan) return ZS_ERR_INPUT;
     Ctx *ctx = &h->c;
     ZS_HIP(ctx, hipSetDevice(ctx->device));
@@ -582,12 +711,16 @@ int32_t zs::fri_commit_impl(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t 
     auto layer = [&](uint32_t i) -> uint32_t * {
         return i == 0 ? const_cast<uint32_t *>(lay0) : layers + r->layer_off[i];
     };
-    // slot 6: the challenges (16 B per layer), then the layer roots gathered for one copy (32 B per layer)
-    ZS_TRY(ctx->ensure_scratch(6, size_t(L) * 48));
+    // slot 6: the challenges (16 B per layer), then the layer roots gathered for one copy (32 B
+    // per layer) and right after them the remainder coefficients (16 B each)
+    const uint32_t rc = 1u << (remainder_log - log_blowup);
+    ZS_TRY(ctx->ensure_scratch(6, size_t(L) * 48 + size_t(rc) * 16));
     alpha_dev = ctx->scratch[6].as<uint32_t>();
     uint32_t *roots_dev = alpha_dev + 4 * size_t(L);
-    const uint32_t *fri_tab;
+    uint32_t *rem_dev = roots_dev + 8 * size_t(L);
+    const uint32_t *fri_tab, *rem_mat;
     ZS_TRY(get_fri_tables(ctx, dlog, &fri_tab));  // cached before any capture
+    ZS_TRY(get_remainder_matrix(ctx, L, remainder_log, rc, &rem_mat));
     if (alphas)
         ZS_HIP(ctx, hipMemcpyAsync(alpha_dev, alphas, size_t(L) * 16, hipMemcpyHostToDevice, ctx->stream));
     // Leaves of layer 0, then per layer: node levels, (challenge), fold + next
@@ -641,12 +774,14 @@ int32_t zs::fri_commit_impl(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t 
         }
         return i_tail < L ? tail() : ZS_OK;
     };
-    // every layer's root into roots_dev, so the host reads them with one copy
+    // every layer's root into roots_dev and the remainder right after them (fri.zig:228-233:
+    // the interpolant of the first rc points of layer L), so the host reads both with one copy
     auto enqueue_all = [&]() -> int32_t {
         ZS_TRY(enqueue());
         std::vector<uint64_t> root_off(L);
         for (uint32_t i = 0; i < L; i++) root_off[i] = r->tree_off[i] + tree_words(dlog - i) - 8;
-        return launch_gather_roots(ctx, trees, root_off.data(), L, roots_dev);
+        ZS_TRY(launch_gather_roots(ctx, trees, root_off.data(), L, roots_dev));
+        return launch_fri_remainder(ctx, layer(L), rem_mat, rc, rem_dev);
     };
     const bool graph = !out_result && !ctx->profiling && !(flags & ZS_FLAG_NO_GRAPH);
     if (!graph) {
@@ -654,7 +789,8 @@ int32_t zs::fri_commit_impl(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t 
     } else {
         std::array<uint64_t, 6> key = {reinterpret_cast<uint64_t>(lay0), reinterpret_cast<uint64_t>(layers),
                                        reinterpret_cast<uint64_t>(trees), reinterpret_cast<uint64_t>(alpha_dev),
-                                       reinterpret_cast<uint64_t>(d_chan), (uint64_t(dlog) << 32) | L};
+                                       reinterpret_cast<uint64_t>(d_chan),
+                                       (uint64_t(dlog) << 32) | (uint64_t(L) << 16) | rc};
         auto it = ctx->fri_graphs.find(key);
         if (it == ctx->fri_graphs.end()) {
             if (ctx->fri_graphs.size() >= 32) {
@@ -687,22 +823,14 @@ int32_t zs::fri_commit_impl(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t 
         }
         ZS_HIP(ctx, hipGraphLaunch(it->second, ctx->stream));
     }
-    ZS_HIP(ctx, hipMemcpyAsync(roots_out, roots_dev, size_t(L) * 32, hipMemcpyDeviceToHost, ctx->stream));
-    // remainder: interpolate the first 2^(remainder_log - log_blowup) points of layer L (fri.zig:228-233)
-    size_t rc = size_t(1) << (remainder_log - log_blowup);
-    std::vector<uint32_t> last(rc * 4);
-    ZS_HIP(ctx, hipMemcpyAsync(last.data(), layer(L), rc * 16, hipMemcpyDeviceToHost, ctx->stream));
-    ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    std::vector<qm31> xs(rc), ys(rc), co;
-    qm31 w = qm31_root_of_unity(dlog - L);
-    qm31 acc = fri_offset_pow(L);
-    for (size_t j = 0; j < rc; j++) {
-        xs[j] = acc;
-        acc = qm_mul(acc, w);
-        ys[j] = qm_from_words(&last[4 * j]);
+    if (stage) {  // the caller synchronises later and reads the roots, then the remainder, from stage
+        ZS_HIP(ctx, hipMemcpyAsync(stage, roots_dev, size_t(L) * 32 + size_t(rc) * 16, hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        return ZS_OK;
     }
-    interpolate_host(xs, ys, co);
-    for (size_t k = 0; k < rc; k++) qm_to_words(co[k], remainder_out + 4 * k);
+    ZS_HIP(ctx, hipMemcpyAsync(roots_out, roots_dev, size_t(L) * 32, hipMemcpyDeviceToHost, ctx->stream));
+    ZS_HIP(ctx, hipMemcpyAsync(remainder_out, rem_dev, size_t(rc) * 16, hipMemcpyDeviceToHost, ctx->stream));
+    ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (out_result) *out_result = guard.release();
     return ZS_OK;
 }

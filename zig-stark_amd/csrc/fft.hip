@@ -75,17 +75,31 @@ cpoint coset_at_host(const zs_circle_coset &c, uint64_t i) {
     return cp_add({c.offset_x, c.offset_y}, cp_mul({c.step_x, c.step_y}, i));
 }
 
+// Twiddle tables are shared by the contexts of a batch or server, so the cache lock
+// is held only for lookups and inserts: a missing table is built on the caller's
+// stream outside the lock (other workers' lookups proceed meanwhile), and if
+// another thread inserted the same table first, that one is kept and this one
+// freed.  Tables of arbitrary (non-canonic) cosets are only reachable through the
+// synchronous host-buffer transforms, which leave no kernel in flight when they
+// return; at most kMaxOtherCosets of them are kept, the oldest evicted first.
+constexpr size_t kMaxOtherCosets = 8;
+
 int32_t get_twiddles(Ctx *ctx, const zs_circle_coset &hc, const TwiddleTable **out) {
     auto key = std::make_tuple(hc.offset_x, hc.offset_y, hc.step_x, hc.step_y, hc.log_size);
     DeviceCaches &dc = *ctx->caches;
-    std::lock_guard<std::mutex> lk(dc.mu);
-    auto it = dc.twiddles.find(key);
-    if (it != dc.twiddles.end()) {
-        *out = it->second.get();
-        return ZS_OK;
+    {
+        std::lock_guard<std::mutex> lk(dc.mu);
+        auto it = dc.twiddles.find(key);
+        if (it != dc.twiddles.end()) {
+            *out = it->second.get();
+            return ZS_OK;
+        }
     }
     uint32_t L = hc.log_size + 1;
     if (L < 3 || L > 29) return ZS_ERR_INPUT;
+    zs_circle_coset canon;
+    const bool canonic = zs_circle_coset_canonic_half(L, &canon) == ZS_OK && canon.offset_x == hc.offset_x &&
+                         canon.offset_y == hc.offset_y && canon.step_x == hc.step_x && canon.step_y == hc.step_y;
     auto t = std::make_unique<TwiddleTable>();
     t->log_n = L;
     size_t bytes = (size_t(1) << L) * 4;
@@ -106,6 +120,19 @@ int32_t get_twiddles(Ctx *ctx, const zs_circle_coset &hc, const TwiddleTable **o
                        t->inv.as<uint32_t>(), L, cp);
     ZS_HIP(ctx, hipGetLastError());
     ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));  // complete before other streams read it
+    std::lock_guard<std::mutex> lk(dc.mu);
+    auto it = dc.twiddles.find(key);
+    if (it != dc.twiddles.end()) {  // another thread built it meanwhile; ours is freed here
+        *out = it->second.get();
+        return ZS_OK;
+    }
+    if (!canonic) {
+        dc.other_cosets.push_back(key);
+        while (dc.other_cosets.size() > kMaxOtherCosets) {
+            dc.twiddles.erase(dc.other_cosets.front());
+            dc.other_cosets.pop_front();
+        }
+    }
     *out = t.get();
     dc.twiddles[key] = std::move(t);
     return ZS_OK;

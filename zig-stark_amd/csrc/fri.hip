@@ -31,13 +31,16 @@ qm31 qm31_root_of_unity(uint32_t log_size) {  // qm31.zig:80-89
 
 qm31 fri_offset_pow(uint32_t layer) { return qm_pow(qm_from_words(FRI_OFFSET_W), uint64_t(1) << layer); }
 
+// built outside the cache lock (see get_twiddles), inserted unless another thread won
 int32_t get_fri_tables(Ctx *ctx, uint32_t dlog, const uint32_t **out) {
     DeviceCaches &dc = *ctx->caches;
-    std::lock_guard<std::mutex> lk(dc.mu);
-    auto it = dc.fri_tables.find(dlog);
-    if (it != dc.fri_tables.end()) {
-        *out = it->second->as<uint32_t>();
-        return ZS_OK;
+    {
+        std::lock_guard<std::mutex> lk(dc.mu);
+        auto it = dc.fri_tables.find(dlog);
+        if (it != dc.fri_tables.end()) {
+            *out = it->second->as<uint32_t>();
+            return ZS_OK;
+        }
     }
     size_t n_hi = dlog > 11 ? (size_t(1) << (dlog - 11)) : 1;
     std::vector<uint32_t> host((1024 + n_hi) * 4);
@@ -58,6 +61,12 @@ int32_t get_fri_tables(Ctx *ctx, uint32_t dlog, const uint32_t **out) {
     buf->bytes = host.size() * 4;
     ZS_HIP(ctx, hipMemcpyAsync(buf->ptr, host.data(), host.size() * 4, hipMemcpyHostToDevice, ctx->stream));
     ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::lock_guard<std::mutex> lk(dc.mu);
+    auto it = dc.fri_tables.find(dlog);
+    if (it != dc.fri_tables.end()) {
+        *out = it->second->as<uint32_t>();
+        return ZS_OK;
+    }
     *out = buf->as<uint32_t>();
     dc.fri_tables[dlog] = std::move(buf);
     return ZS_OK;
@@ -344,6 +353,70 @@ int32_t launch_fri_tail(Ctx *ctx, const FriTail &t) {
     const size_t lds = (size_t(20) << FRI_TAIL_LOG) * 4 + size_t(32) * (FRI_TAIL_NT / 4) * 4 + 2 * sizeof(B3Hasher) +
                        (24 + 16 + 4) * 4;
     ZS_LAUNCH(ctx, "fri_tail_kernel", fri_tail_kernel, dim3(1), dim3(FRI_TAIL_NT), lds, ctx->stream, t);
+    ZS_HIP(ctx, hipGetLastError());
+    return ZS_OK;
+}
+
+// FRI remainder (fri.zig:228-233): the coefficients of the unique interpolant
+// (univariate.zig:79-139) of the first rc values of layer L on the points
+// x_j = FRI_OFFSET^(2^L) * w^j, w of order 2^remainder_log.  The points depend
+// only on (L, remainder_log, rc), so the interpolation is a fixed linear map:
+// coeffs = M * ys with M the inverse Vandermonde matrix, built once per shape on
+// the host (column j = the interpolant of the j-th unit vector) and cached on
+// the device.  Thread k computes coefficient k.
+__global__ __launch_bounds__(64) void fri_remainder_kernel(const uint32_t *last, const uint32_t *mat, uint32_t rc,
+                                                           uint32_t *out) {
+    const uint32_t k = threadIdx.x;
+    if (k >= rc) return;
+    qm31 acc = {{0, 0}, {0, 0}};
+    for (uint32_t j = 0; j < rc; j++) acc = qm_add(acc, qm_mul(ld_qm(mat + 4 * (k * rc + j)), ld_qm(last + 4 * j)));
+    st_qm(out + 4 * k, acc);
+}
+
+int32_t get_remainder_matrix(Ctx *ctx, uint32_t L, uint32_t remainder_log, uint32_t rc, const uint32_t **out) {
+    DeviceCaches &dc = *ctx->caches;
+    const uint64_t key = (uint64_t(L) << 32) | (uint64_t(remainder_log) << 16) | rc;
+    {
+        std::lock_guard<std::mutex> lk(dc.mu);
+        auto it = dc.remainder_mats.find(key);
+        if (it != dc.remainder_mats.end()) {
+            *out = it->second->as<uint32_t>();
+            return ZS_OK;
+        }
+    }
+    std::vector<qm31> xs(rc), ys(rc), co;
+    const qm31 w = qm31_root_of_unity(remainder_log), zero = {{0, 0}, {0, 0}}, one = {{1, 0}, {0, 0}};
+    qm31 acc = fri_offset_pow(L);
+    for (uint32_t j = 0; j < rc; j++) {
+        xs[j] = acc;
+        acc = qm_mul(acc, w);
+    }
+    std::vector<uint32_t> host(size_t(rc) * rc * 4);
+    for (uint32_t j = 0; j < rc; j++) {
+        for (uint32_t i = 0; i < rc; i++) ys[i] = i == j ? one : zero;
+        interpolate_host(xs, ys, co);
+        for (uint32_t k = 0; k < rc; k++) qm_to_words(co[k], &host[4 * (size_t(k) * rc + j)]);
+    }
+    auto buf = std::make_unique<DevBuf>();
+    ZS_HIP(ctx, hipMalloc(&buf->ptr, host.size() * 4));
+    buf->bytes = host.size() * 4;
+    ZS_HIP(ctx, hipMemcpyAsync(buf->ptr, host.data(), host.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::lock_guard<std::mutex> lk(dc.mu);
+    auto it = dc.remainder_mats.find(key);
+    if (it != dc.remainder_mats.end()) {
+        *out = it->second->as<uint32_t>();
+        return ZS_OK;
+    }
+    *out = buf->as<uint32_t>();
+    dc.remainder_mats[key] = std::move(buf);
+    return ZS_OK;
+}
+
+int32_t launch_fri_remainder(Ctx *ctx, const uint32_t *d_last, const uint32_t *d_mat, uint32_t rc, uint32_t *d_out) {
+    if (rc == 0 || rc > 64) return ZS_ERR_INPUT;
+    ZS_LAUNCH(ctx, "fri_remainder_kernel", fri_remainder_kernel, dim3(1), dim3(64), 0, ctx->stream, d_last, d_mat, rc,
+              d_out);
     ZS_HIP(ctx, hipGetLastError());
     return ZS_OK;
 }

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <array>
+#include <deque>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -107,11 +108,15 @@ inline uint64_t lde_shape_key(uint32_t n_cols, uint32_t log_n, uint32_t log_blow
 // batch workers, or a server's workers): built once, read by every stream.
 struct DeviceCaches {
     std::mutex mu;
-    std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t, uint32_t>, std::unique_ptr<TwiddleTable>> twiddles;
+    using CosetKey = std::tuple<uint32_t, uint32_t, uint32_t, uint32_t, uint32_t>;
+    std::map<CosetKey, std::unique_ptr<TwiddleTable>> twiddles;
+    std::deque<CosetKey> other_cosets;  // non-canonic entries of `twiddles`, oldest first
     // FRI inverse-domain tables keyed by dlog: [lo table 2^10 | hi table] QM31 words
     std::map<uint32_t, std::unique_ptr<DevBuf>> fri_tables;
     // QM31 power tables of the trace-commitment DFTs, keyed by (kind << 32) | log length
     std::map<uint64_t, std::unique_ptr<DevBuf>> qtables;
+    // FRI remainder interpolation matrices, keyed by (L << 32) | (remainder_log << 16) | coefficients
+    std::map<uint64_t, std::unique_ptr<DevBuf>> remainder_mats;
     // launch configurations chosen by zs_autotune_lde, keyed by lde_shape_key
     std::map<uint64_t, LdeTiles> tuned;
 };
@@ -132,11 +137,16 @@ struct Ctx {
     // grow-only scratch arenas
     DevBuf scratch[10];  // 0 transforms, 1-2 LDE, 3 plain path, 4-6 FRI, 7-9 commit jobs
     // FRI commit phases with challenges given up front, captured once per buffer set and
-    // replayed (key: codeword, layers, trees, challenges, dlog, layer count)
+    // replayed (key: codeword, layers, trees, challenges, channel, dlog / layer count / remainder length)
     hipStream_t capture_stream = nullptr;
     std::map<std::array<uint64_t, 6>, hipGraphExec_t> fri_graphs;
 
+    // grow-only pinned host buffer (staged results of batched jobs)
+    void *pinned = nullptr;
+    size_t pinned_bytes = 0;
+
     void set_error(const std::string &s) { last_error = s; }
+    int32_t ensure_pinned(size_t bytes);
     int32_t ensure_scratch(int slot, size_t bytes);
     void release_scratch();
     // use [ptr, ptr + bytes) as scratch slot `slot` until release_scratch (not freed here)
@@ -224,14 +234,26 @@ struct FriTail {
 };
 int32_t launch_fri_tail(Ctx *ctx, const FriTail &t);
 int32_t launch_qm31_powers(Ctx *ctx, const uint32_t *d_x, uint32_t *d_out, uint32_t n);
+// UnivariateQM31.interpolate (univariate.zig:79-139) on the host, O(n^2) Lagrange
+void interpolate_host(const std::vector<qm31> &xs, const std::vector<qm31> &ys, std::vector<qm31> &coeffs);
+// the FRI remainder as a device map: coefficients = M * (first rc values of layer L)
+int32_t get_remainder_matrix(Ctx *ctx, uint32_t L, uint32_t remainder_log, uint32_t rc, const uint32_t **out);
+int32_t launch_fri_remainder(Ctx *ctx, const uint32_t *d_last, const uint32_t *d_mat, uint32_t rc, uint32_t *d_out);
 // FRI commit phase on a device codeword; challenges from `alphas` (host) or drawn
-// on the device from the channel state at d_chan (left advanced past the last root)
+// on the device from the channel state at d_chan (left advanced past the last root).
+// With `stage` (host memory, pinned for a true asynchronous copy) the L roots and
+// then the remainder words are copied there on the stream and the call returns
+// without synchronising (roots_out and remainder_out are then unused); otherwise
+// they are in roots_out / remainder_out when it returns.
 int32_t fri_commit_impl(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t dlog, uint32_t log_blowup,
                         uint32_t remainder_log, const uint32_t *alphas, B3Hasher *d_chan, uint32_t flags,
-                        uint8_t *roots_out, uint32_t *remainder_out, zs_fri_result **out_result);
+                        uint8_t *roots_out, uint32_t *remainder_out, zs_fri_result **out_result,
+                        uint8_t *stage = nullptr);
 int32_t launch_combine(Ctx *ctx, const uint32_t *d_lde, uint64_t cs, uint32_t n_cols, uint64_t n,
                        const uint32_t *d_powers, uint32_t *d_out);
 
+// FRI remainders have at most 2^kMaxRemainderLog coefficients (remainder_log - log_blowup <= 6)
+constexpr uint32_t kMaxRemainderLog = 6;
 // host-side constants
 qm31 fri_offset_pow(uint32_t layer);  // FRI_OFFSET^(2^layer)
 qm31 qm31_root_of_unity(uint32_t log_size);

@@ -49,13 +49,18 @@ const B3Hasher &commit_channel() {
 // trace root and draws gamma; g = sum_c gamma^c lde_c; FRI on g with challenges
 // drawn from the same channel.  Scratch: 7 channel state, 8 the codeword g,
 // 9 gamma and its powers; FRI uses 4-6 and the LDE 1-2.
+// With `stage` (pinned host memory of kStageBytes) the call returns without
+// synchronising: the trace root, then the FRI roots, then the remainder words
+// are copied there on the stream, and trace_root / fri_roots / remainder are
+// unused; the caller synchronises the stream before reading them.
 int32_t commit_job(zs_hip_ctx *h, const uint32_t *d_cols, uint32_t n_cols, uint32_t log_n, uint32_t log_blowup,
                    uint32_t remainder_log, uint32_t *d_lde, uint8_t *d_nodes, uint8_t trace_root[32],
-                   uint8_t *fri_roots, uint32_t *remainder, uint32_t flags) {
-    if (!h || !d_cols || !d_lde || !d_nodes || !trace_root || !fri_roots || !remainder) return ZS_ERR_INPUT;
+                   uint8_t *fri_roots, uint32_t *remainder, uint32_t flags, uint8_t *stage = nullptr) {
+    if (!h || !d_cols || !d_lde || !d_nodes || (!stage && (!trace_root || !fri_roots || !remainder)))
+        return ZS_ERR_INPUT;
     const uint32_t LT = log_n + log_blowup;
     if (n_cols == 0 || n_cols > 4096 || log_n < 1 || log_sum_over(log_n, log_blowup, 28) ||
-        remainder_log < log_blowup || remainder_log >= LT)
+        remainder_log < log_blowup || remainder_log >= LT || remainder_log - log_blowup > kMaxRemainderLog)
         return ZS_ERR_INPUT;
     Ctx *ctx = &h->c;
     const size_t N = size_t(1) << LT;
@@ -79,11 +84,15 @@ int32_t commit_job(zs_hip_ctx *h, const uint32_t *d_cols, uint32_t n_cols, uint3
     }
     ZS_TRY(launch_qm31_powers(ctx, d_gamma, d_powers, n_cols));
     ZS_TRY(launch_combine(ctx, d_lde, N, n_cols, N, d_powers, ctx->scratch[8].as<uint32_t>()));
-    ZS_HIP(ctx, hipMemcpyAsync(trace_root, d_root, 32, hipMemcpyDeviceToHost, ctx->stream));
-    // the FRI call ends with a synchronisation that also covers the trace root copy
+    ZS_HIP(ctx, hipMemcpyAsync(stage ? stage : trace_root, d_root, 32, hipMemcpyDeviceToHost, ctx->stream));
+    // without a stage the FRI call ends with a synchronisation that also covers the trace root copy
     return fri_commit_impl(h, ctx->scratch[8].as<uint32_t>(), LT, log_blowup, remainder_log, nullptr, d_chan,
-                           flags & ZS_FLAG_NO_GRAPH, fri_roots, remainder, nullptr);
+                           flags & ZS_FLAG_NO_GRAPH, fri_roots, remainder, nullptr, stage ? stage + 32 : nullptr);
 }
+
+// staged results of one job: trace root, at most 28 FRI roots, at most 2^kMaxRemainderLog
+// remainder coefficients
+constexpr size_t kStageBytes = 32 + 28 * 32 + (size_t(16) << kMaxRemainderLog);
 
 // Device memory of one commit job: its input, LDE and row tree, and every scratch
 // buffer commit_job uses (slot numbers of Ctx::scratch), each 256-byte aligned.
@@ -108,7 +117,8 @@ JobLayout job_layout(uint32_t n_cols, uint32_t log_n, uint32_t log_blowup, uint3
     for (uint32_t i = 0; i < L; i++) ft += tree_words(LT - i) * 4;
     j.len[JobLayout::FRI_LAYERS] = fl;
     j.len[JobLayout::FRI_TREES] = ft;
-    j.len[JobLayout::ALPHAS] = uint64_t(L) * 48;  // challenges, then the gathered layer roots
+    // challenges, then the gathered layer roots and the remainder coefficients (fri_commit_impl's slot 6)
+    j.len[JobLayout::ALPHAS] = uint64_t(L) * 48 + (uint64_t(16) << (remainder_log - log_blowup));
     j.len[JobLayout::CHAN] = sizeof(B3Hasher);
     j.len[JobLayout::COMB] = 16 * N;
     j.len[JobLayout::POWERS] = 16 * (c + 1);
@@ -198,12 +208,12 @@ int32_t zs_commit_batch_dev(zs_hip_ctx *h, zs_job *jobs, size_t n_jobs, uint32_t
     if (!h || (!jobs && n_jobs) || n_workers > 64) return ZS_ERR_INPUT;
     if (n_jobs == 0) return ZS_OK;
     Ctx *ctx = &h->c;
-    auto run_one = [&](zs_hip_ctx *w, zs_job &j) {
-        j.status = commit_job(w, j.d_cols, j.n_cols, j.log_n, log_blowup, remainder_log, j.d_lde, j.d_nodes,
-                              j.trace_root, j.fri_roots, j.remainder, flags);
-    };
     if (n_workers <= 1 || (flags & ZS_FLAG_PLAIN)) {
-        for (size_t i = 0; i < n_jobs; i++) run_one(h, jobs[i]);
+        for (size_t i = 0; i < n_jobs; i++) {
+            zs_job &j = jobs[i];
+            j.status = commit_job(h, j.d_cols, j.n_cols, j.log_n, log_blowup, remainder_log, j.d_lde, j.d_nodes,
+                                  j.trace_root, j.fri_roots, j.remainder, flags);
+        }
     } else {
         ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));  // inputs written on the caller's stream are complete
         auto &ws = ctx->workers;
@@ -212,17 +222,66 @@ int32_t zs_commit_batch_dev(zs_hip_ctx *h, zs_job *jobs, size_t n_jobs, uint32_t
             if (!w) return ZS_ERR_DEVICE;
             ws.push_back(w);
         }
+        // largest traces first, so that no large trace starts last and sets the tail
+        std::vector<size_t> order(n_jobs);
+        for (size_t i = 0; i < n_jobs; i++) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+            const uint64_t ea = uint64_t(jobs[a].n_cols) << std::min(jobs[a].log_n, 40u);
+            const uint64_t eb = uint64_t(jobs[b].n_cols) << std::min(jobs[b].log_n, 40u);
+            return ea > eb;
+        });
+        // Every job's results land in pinned staging.  A worker keeps at most kInFlight jobs
+        // queued on its stream: it takes the next job once the oldest of them has finished on
+        // the device, so the traces are spread over the streams by completion (a worker that
+        // drew large traces takes fewer), while the host prepares one job's launches during
+        // another's kernels instead of waiting for each result.
+        constexpr size_t kInFlight = 2;
+        ZS_TRY(ctx->ensure_pinned(n_jobs * kStageBytes));
+        uint8_t *stage = static_cast<uint8_t *>(ctx->pinned);
         std::atomic<size_t> next{0};
+        std::vector<uint32_t> ran_on(n_jobs, 0);
+        std::vector<uint8_t> stream_failed(n_workers, 0);
         std::vector<std::thread> th;
         for (uint32_t t = 0; t < n_workers; t++) {
             th.emplace_back([&, t] {
                 zs_hip_ctx *w = static_cast<zs_hip_ctx *>(ws[t]);
                 (void)hipSetDevice(ctx->device);
-                for (size_t i; (i = next.fetch_add(1)) < n_jobs;) run_one(w, jobs[i]);
-                (void)hipStreamSynchronize(w->c.stream);
+                hipEvent_t ev[kInFlight] = {};
+                bool ev_ok = true;
+                for (auto &e : ev) ev_ok = ev_ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+                size_t queued = 0;  // jobs recorded on ev[], oldest at queued % kInFlight
+                for (size_t k;;) {
+                    if (ev_ok && queued >= kInFlight && hipEventSynchronize(ev[queued % kInFlight]) != hipSuccess)
+                        ev_ok = false;
+                    if ((k = next.fetch_add(1)) >= n_jobs) break;
+                    const size_t i = order[k];
+                    zs_job &j = jobs[i];
+                    ran_on[i] = t;
+                    j.status = commit_job(w, j.d_cols, j.n_cols, j.log_n, log_blowup, remainder_log, j.d_lde,
+                                          j.d_nodes, nullptr, nullptr, nullptr, flags, stage + i * kStageBytes);
+                    if (!ev_ok || hipEventRecord(ev[queued % kInFlight], w->c.stream) != hipSuccess) {
+                        ev_ok = false;  // no completion events: one job at a time
+                        (void)hipStreamSynchronize(w->c.stream);
+                    }
+                    queued++;
+                }
+                stream_failed[t] = hipStreamSynchronize(w->c.stream) != hipSuccess;
+                for (auto &e : ev)
+                    if (e) (void)hipEventDestroy(e);
             });
         }
         for (auto &x : th) x.join();
+        for (size_t i = 0; i < n_jobs; i++) {
+            zs_job &j = jobs[i];
+            if (stream_failed[ran_on[i]]) j.status = ZS_ERR_DEVICE;  // no result of that stream is complete
+            if (j.status != ZS_OK) continue;
+            const uint32_t L = j.log_n + log_blowup - remainder_log;
+            const size_t rc = size_t(1) << (remainder_log - log_blowup);
+            const uint8_t *st = stage + i * kStageBytes;
+            std::memcpy(j.trace_root, st, 32);
+            if (j.fri_roots) std::memcpy(j.fri_roots, st + 32, size_t(L) * 32);
+            if (j.remainder) std::memcpy(j.remainder, st + 32 + size_t(L) * 32, rc * 16);
+        }
     }
     for (size_t i = 0; i < n_jobs; i++)
         if (jobs[i].status != ZS_OK) return jobs[i].status;
@@ -241,8 +300,16 @@ struct ServReq {
     zs_response resp;
     JobLayout lay;
     uint64_t bytes, off = 0;
+    uint32_t bypassed = 0;  // later requests admitted while this one did not fit
     bool done = false;
+    bool claimed = false;  // a wait or poll has taken this ticket
 };
+
+// Admission order: arrival order, except that a request may pass earlier ones
+// whose memory is not free yet; each earlier request can be passed at most
+// kMaxBypass times, after which nothing behind it is admitted until it fits, so
+// a large request cannot starve under a stream of small ones.
+constexpr uint32_t kMaxBypass = 8;
 
 struct zs_server {
     int device = 0;
@@ -257,8 +324,18 @@ struct zs_server {
     std::deque<ServReq *> queue;
     std::map<uint64_t, std::unique_ptr<ServReq>> reqs;
     uint64_t next_id = 1;
+    uint32_t waiters = 0;  // threads inside zs_server_wait
     bool stop = false;
 };
+
+// the queued request to admit next (its position in the queue), or queue.end()
+static std::deque<ServReq *>::iterator admissible(zs_server *s) {
+    for (auto it = s->queue.begin(); it != s->queue.end(); ++it) {
+        if (s->arena.fits((*it)->bytes)) return it;
+        if ((*it)->bypassed >= kMaxBypass) break;  // this one goes next; nothing passes it
+    }
+    return s->queue.end();
+}
 
 static void serve_one(zs_server *s, zs_hip_ctx *w, ServReq *q) {
     Ctx *ctx = &w->c;
@@ -294,21 +371,14 @@ static void serve_loop(zs_server *s, zs_hip_ctx *w) {
         {
             std::unique_lock<std::mutex> lk(s->mu);
             s->cv_work.wait(lk, [&] {
-                if (s->stop && s->queue.empty()) return true;
-                for (ServReq *x : s->queue)
-                    if (s->arena.fits(x->bytes)) return true;
-                return false;
+                return (s->stop && s->queue.empty()) || admissible(s) != s->queue.end();
             });
             if (s->stop && s->queue.empty()) return;
-            // the first queued request whose memory is free (earlier large ones wait)
-            for (auto it = s->queue.begin(); it != s->queue.end(); ++it) {
-                if (s->arena.alloc((*it)->bytes, &(*it)->off)) {
-                    q = *it;
-                    s->queue.erase(it);
-                    break;
-                }
-            }
-            if (!q) continue;
+            auto it = admissible(s);
+            if (it == s->queue.end() || !s->arena.alloc((*it)->bytes, &(*it)->off)) continue;
+            for (auto e = s->queue.begin(); e != it; ++e) (*e)->bypassed++;
+            q = *it;
+            s->queue.erase(it);
             s->in_use += q->bytes;
             s->peak = std::max(s->peak, s->in_use);
             q->resp.t_admit = now_s();
@@ -358,7 +428,7 @@ int32_t zs_server_submit(zs_server *s, const zs_request *req, uint64_t *ticket) 
     const uint32_t LT = req->log_n + req->log_blowup;
     if (req->n_cols == 0 || req->n_cols > 4096 || req->log_n < 1 || log_sum_over(req->log_n, req->log_blowup, 28) ||
         req->remainder_log < req->log_blowup ||
-        req->remainder_log >= LT || req->remainder_log - req->log_blowup > 6)
+        req->remainder_log >= LT || req->remainder_log - req->log_blowup > kMaxRemainderLog)
         return ZS_ERR_INPUT;
     auto q = std::make_unique<ServReq>();
     q->r = *req;
@@ -380,15 +450,21 @@ int32_t zs_server_submit(zs_server *s, const zs_request *req, uint64_t *ticket) 
     return ZS_OK;
 }
 
+// A ticket is answered once: the first wait or poll claims it, and any later (or
+// concurrent) wait or poll of the same ticket returns ZS_ERR_INPUT.
 int32_t zs_server_wait(zs_server *s, uint64_t ticket, zs_response *out) {
     if (!s || !out) return ZS_ERR_INPUT;
     std::unique_lock<std::mutex> lk(s->mu);
     auto it = s->reqs.find(ticket);
-    if (it == s->reqs.end()) return ZS_ERR_INPUT;
+    if (it == s->reqs.end() || it->second->claimed) return ZS_ERR_INPUT;
     ServReq *q = it->second.get();
+    q->claimed = true;
+    s->waiters++;
     s->cv_done.wait(lk, [&] { return q->done; });
     *out = q->resp;
-    s->reqs.erase(it);
+    s->reqs.erase(ticket);
+    s->waiters--;
+    if (s->stop) s->cv_done.notify_all();  // zs_server_free waits for the last waiter to leave
     return ZS_OK;
 }
 
@@ -396,7 +472,7 @@ int32_t zs_server_poll(zs_server *s, uint64_t ticket, zs_response *out, int32_t 
     if (!s || !out || !ready) return ZS_ERR_INPUT;
     std::lock_guard<std::mutex> lk(s->mu);
     auto it = s->reqs.find(ticket);
-    if (it == s->reqs.end()) return ZS_ERR_INPUT;
+    if (it == s->reqs.end() || it->second->claimed) return ZS_ERR_INPUT;
     *ready = it->second->done ? 1 : 0;
     if (*ready) {
         *out = it->second->resp;
@@ -413,6 +489,9 @@ int32_t zs_server_memory(zs_server *s, uint64_t *in_use, uint64_t *peak) {
     return ZS_OK;
 }
 
+// Every queued request is served before the workers stop; threads blocked in
+// zs_server_wait are answered and have left before the server is deleted.  No
+// call on the server may start once zs_server_free has been called.
 void zs_server_free(zs_server *s) {
     if (!s) return;
     {
@@ -421,6 +500,11 @@ void zs_server_free(zs_server *s) {
     }
     s->cv_work.notify_all();
     for (auto &t : s->threads) t.join();
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->cv_done.notify_all();
+        s->cv_done.wait(lk, [&] { return s->waiters == 0; });
+    }
     for (zs_hip_ctx *w : s->workers) zs_hip_destroy(w);
     delete s;
 }

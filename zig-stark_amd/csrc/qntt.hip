@@ -81,14 +81,17 @@ __global__ __launch_bounds__(256) void qntt_stage_kernel(uint32_t *data, uint64_
 }
 
 // device table of 2^len_log QM31 values first * step^j
+// built outside the cache lock (see get_twiddles), inserted unless another thread won
 int32_t qm31_table(Ctx *ctx, uint32_t kind, uint32_t log_len, qm31 first, qm31 step, const uint32_t **out) {
     DeviceCaches &dc = *ctx->caches;
-    std::lock_guard<std::mutex> lk(dc.mu);
     const uint64_t key = (uint64_t(kind) << 32) | log_len;
-    auto it = dc.qtables.find(key);
-    if (it != dc.qtables.end()) {
-        *out = it->second->as<uint32_t>();
-        return ZS_OK;
+    {
+        std::lock_guard<std::mutex> lk(dc.mu);
+        auto it = dc.qtables.find(key);
+        if (it != dc.qtables.end()) {
+            *out = it->second->as<uint32_t>();
+            return ZS_OK;
+        }
     }
     const size_t len = size_t(1) << log_len;
     std::vector<uint32_t> host(len * 4);
@@ -102,6 +105,12 @@ int32_t qm31_table(Ctx *ctx, uint32_t kind, uint32_t log_len, qm31 first, qm31 s
     buf->bytes = host.size() * 4;
     ZS_HIP(ctx, hipMemcpyAsync(buf->ptr, host.data(), host.size() * 4, hipMemcpyHostToDevice, ctx->stream));
     ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::lock_guard<std::mutex> lk(dc.mu);
+    auto it = dc.qtables.find(key);
+    if (it != dc.qtables.end()) {
+        *out = it->second->as<uint32_t>();
+        return ZS_OK;
+    }
     *out = buf->as<uint32_t>();
     dc.qtables[key] = std::move(buf);
     return ZS_OK;

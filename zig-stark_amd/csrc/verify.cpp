@@ -104,11 +104,20 @@ bool parse(const uint8_t *bytes, size_t n, FriProof &pf) {
     return read_fri_proof(r, pf) && r.left == 0;  // deserialize rejects trailing bytes (serialization.zig:44-49)
 }
 
-// the FriParams a verification runs under (the reference passes them beside the proof)
+// the FriParams a verification runs under (the reference passes them beside the proof).
+// Beyond the reference's verify, which only needs numFoldRounds not to underflow,
+// the parameters must be ones its prover accepts (fri.zig:221-227 asserts
+// remainder_log >= log_blowup) and fold at least once: with remainder_log == dlog
+// or a remainder as long as the last layer, every query would pass without any
+// degree bound.
 struct FriParams {
     uint32_t log_size, log_blowup, remainder_log;
     uint64_t num_queries;
-    bool ok() const { return log_size + log_blowup <= 32 && remainder_log <= log_size + log_blowup; }
+    bool ok() const {
+        return log_size + log_blowup <= 32 && remainder_log >= log_blowup && remainder_log < log_size + log_blowup;
+    }
+    // coefficients the prover writes: the interpolant of 2^(remainder_log - log_blowup) points
+    uint64_t remainder_len() const { return uint64_t(1) << (remainder_log - log_blowup); }
 };
 
 // MerkleTree.verify (src/core/merkle/merkle.zig:72-89)
@@ -157,7 +166,10 @@ bool fri_verify(const FriParams &fp, const FriProof &pf, zs_channel *chan) {
     const uint32_t dlog = fp.log_size + fp.log_blowup;
     const uint32_t L = dlog - fp.remainder_log;  // numFoldRounds
     const uint64_t n0 = uint64_t(1) << dlog;
-    if (pf.roots.size() != L) return false;
+    if (!fp.ok() || pf.roots.size() != L) return false;
+    // a longer remainder could interpolate any last layer (no degree bound); a shorter one
+    // is not what the prover writes
+    if (pf.remainder.size() != fp.remainder_len()) return false;
     if (pf.queries.size() != fp.num_queries) return false;
     // replay the transcript: absorb roots, resample alphas, absorb the remainder
     std::vector<qm31> alphas(L);
