@@ -117,6 +117,18 @@ int32_t zs_m31_lde_commit_dev(zs_hip_ctx *ctx, const uint32_t *d_cols, uint32_t 
 /* The LDE alone (no commitment); same layouts as zs_m31_lde_commit_dev. */
 int32_t zs_m31_lde_dev(zs_hip_ctx *ctx, const uint32_t *d_cols, uint32_t n_cols, uint32_t log_n, uint32_t log_blowup,
                        uint32_t *d_lde);
+/* No counterpart in the reference: zs_m31_lde_commit_dev for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments (no flags), layouts and
+ * results; it calls no hipMalloc/hipFree.  The LDE's intermediate buffers and
+ * twiddle tables live in one fixed device array of 256 MiB per GPU, reserved for
+ * the life of the process and set up by zs_hip_init.  Limits: log_n + log_blowup
+ * <= 22, and n_cols * 2^log_n words of coefficients plus, when log_n > 11, n_cols *
+ * 2^(log_n + log_blowup) words of intermediate values within 192 MiB (8 columns of
+ * 2^20 at blowup 4 fit); larger shapes return ZS_ERR_INPUT.  Every call on a GPU
+ * uses the same array: calls that overlap on one GPU (other streams or threads) race. */
+int32_t zs_m31_lde_commit_dev_noalloc(zs_hip_ctx *ctx, const uint32_t *d_cols, uint32_t n_cols, uint32_t log_n,
+                                      uint32_t log_blowup, uint32_t *d_lde, uint8_t *d_nodes);
 /* Launch-configuration autotuner (no reference counterpart; the reference has
  * no GPU LDE).  Runs zs_m31_lde_dev for one shape under each candidate
  * configuration of the register-group passes (2^11..2^14 elements per

@@ -85,6 +85,7 @@ def _declare(L):
     L.zs_m31_circle_fft_batch_dev.argtypes = [vp, vp, vp, u32, u32, sz]
     L.zs_m31_circle_ifft_batch_dev.argtypes = [vp, vp, vp, u32, u32, sz]
     L.zs_m31_lde_commit_dev.argtypes = [vp, vp, u32, u32, u32, vp, vp, u32]
+    L.zs_m31_lde_commit_dev_noalloc.argtypes = [vp, vp, u32, u32, u32, vp, vp]
     L.zs_m31_lde_commit.argtypes = [vp, u32p, u32, u32, u32, ctypes.POINTER(vp), ctypes.c_char_p, u32p]
     L.zs_m31_row_commit_dev.argtypes = [vp, vp, u32, u32, sz, vp, u32]
     L.zs_m31_lde_dev.argtypes = [vp, vp, u32, u32, u32, vp]
@@ -136,7 +137,8 @@ def _declare(L):
                                       ctypes.POINTER(u32), u32, ctypes.POINTER(u32)]
     for name in ["zs_hip_set_stream", "zs_hip_synchronize", "zs_circle_coset_canonic_half", "zs_m31_circle_fft",
                  "zs_m31_circle_ifft", "zs_m31_circle_fft_batch_dev", "zs_m31_circle_ifft_batch_dev",
-                 "zs_m31_lde_commit_dev", "zs_m31_lde_commit", "zs_m31_row_commit_dev", "zs_m31_lde_dev",
+                 "zs_m31_lde_commit_dev", "zs_m31_lde_commit_dev_noalloc", "zs_m31_lde_commit",
+                 "zs_m31_row_commit_dev", "zs_m31_lde_dev",
                  "zs_m31_row_leaves_dev", "zs_tree_root",
                  "zs_merkle_open", "zs_tree_read_level", "zs_fri_commit_dev", "zs_fri_commit", "zs_fri_query",
                  "zs_fri_read_layer", "zs_fill_m31_dev", "zs_hip_profile", "zs_hip_profile_read",
@@ -474,6 +476,11 @@ class Context:
 
     def lde_commit_dev(self, d_cols, n_cols, log_n, log_blowup, d_lde, d_nodes, flags=0):
         self.check(lib().zs_m31_lde_commit_dev(self.h, d_cols, n_cols, log_n, log_blowup, d_lde, d_nodes, flags))
+
+    def lde_commit_dev_noalloc(self, d_cols, n_cols, log_n, log_blowup, d_lde, d_nodes):
+        """zs_m31_lde_commit_dev_noalloc: the same commitment with no device allocation
+        (fixed per-GPU array; shapes beyond it raise ZsError(ZS_ERR_INPUT))."""
+        self.check(lib().zs_m31_lde_commit_dev_noalloc(self.h, d_cols, n_cols, log_n, log_blowup, d_lde, d_nodes))
 
     def lde_dev(self, d_cols, n_cols, log_n, log_blowup, d_lde):
         self.check(lib().zs_m31_lde_dev(self.h, d_cols, n_cols, log_n, log_blowup, d_lde))

@@ -122,6 +122,7 @@ int32_t zs_hip_init(int32_t device, uint32_t flags, zs_hip_ctx **out_ctx) {
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count <= 0 || device < 0 || device >= count) return ZS_ERR_DEVICE;
     if (hipSetDevice(device) != hipSuccess) return ZS_ERR_DEVICE;
+    noalloc_setup(device);  // once per device: the allocation-free entry point's fixed array
     auto *c = new (std::nothrow) zs_hip_ctx();
     if (!c) return ZS_ERR_OOM;
     c->c.device = device;

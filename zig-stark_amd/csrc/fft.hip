@@ -71,6 +71,21 @@ __global__ void twiddle_gen_kernel(uint32_t *fwd, uint32_t *inv, uint32_t L, Cos
     inv[e] = m31_inv(v);
 }
 
+hipError_t launch_twiddle_gen(hipStream_t stream, const zs_circle_coset &hc, uint32_t *fwd, uint32_t *inv) {
+    const uint32_t L = hc.log_size + 1;
+    CosetPowers cp;
+    cpoint o{hc.offset_x, hc.offset_y}, st{hc.step_x, hc.step_y};
+    for (int j = 0; j < 32; j++) {
+        cp.off[j] = o;
+        cp.step[j] = st;
+        o = cp_double(o);
+        st = cp_double(st);
+    }
+    const uint32_t n = 1u << L;
+    hipLaunchKernelGGL(twiddle_gen_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, fwd, inv, L, cp);
+    return hipGetLastError();
+}
+
 cpoint coset_at_host(const zs_circle_coset &c, uint64_t i) {
     return cp_add({c.offset_x, c.offset_y}, cp_mul({c.step_x, c.step_y}, i));
 }
@@ -107,18 +122,7 @@ int32_t get_twiddles(Ctx *ctx, const zs_circle_coset &hc, const TwiddleTable **o
     t->fwd.bytes = bytes;
     ZS_HIP(ctx, hipMalloc(&t->inv.ptr, bytes));
     t->inv.bytes = bytes;
-    CosetPowers cp;
-    cpoint o{hc.offset_x, hc.offset_y}, st{hc.step_x, hc.step_y};
-    for (int j = 0; j < 32; j++) {
-        cp.off[j] = o;
-        cp.step[j] = st;
-        o = cp_double(o);
-        st = cp_double(st);
-    }
-    uint32_t n = 1u << L;
-    hipLaunchKernelGGL(twiddle_gen_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, t->fwd.as<uint32_t>(),
-                       t->inv.as<uint32_t>(), L, cp);
-    ZS_HIP(ctx, hipGetLastError());
+    ZS_HIP(ctx, launch_twiddle_gen(ctx->stream, hc, t->fwd.as<uint32_t>(), t->inv.as<uint32_t>()));
     ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));  // complete before other streams read it
     std::lock_guard<std::mutex> lk(dc.mu);
     auto it = dc.twiddles.find(key);
@@ -444,20 +448,36 @@ __global__ void pad_ab_kernel(const uint32_t *in, uint32_t *out, uint32_t n_cols
 // LDE with the register-group kernels: interpolation passes below the top one,
 // the fused top pass (interpolation top + 1/n + every q's evaluation top), then
 // the remaining evaluation passes, the last writing natural order.
-static int32_t launch_lde_tiled(Ctx *ctx, const uint32_t *d_in, uint64_t in_stride, uint32_t n_cols, uint32_t log_n,
-                                uint32_t log_blowup, uint32_t *d_out, uint64_t out_stride, uint32_t *d_coef,
-                                uint32_t *d_big, const zs_circle_coset &hc_small, const zs_circle_coset &hc_big,
-                                const std::vector<PassRange> &pi, const std::vector<PassRange> &pf) {
+// the LDE's two twiddle tables: given by the caller, or from the context's cache
+static int32_t lde_tables(Ctx *ctx, const LdeTables *given, const zs_circle_coset &hc_small,
+                          const zs_circle_coset &hc_big, const uint32_t **inv_small, const uint32_t **fwd_big) {
+    if (given) {
+        *inv_small = given->inv_small;
+        *fwd_big = given->fwd_big;
+        return (*inv_small && *fwd_big) ? ZS_OK : ZS_ERR_INPUT;
+    }
     const TwiddleTable *ti, *tf;
     ZS_TRY(get_twiddles(ctx, hc_small, &ti));
     ZS_TRY(get_twiddles(ctx, hc_big, &tf));
+    *inv_small = ti->inv.as<uint32_t>();
+    *fwd_big = tf->fwd.as<uint32_t>();
+    return ZS_OK;
+}
+
+static int32_t launch_lde_tiled(Ctx *ctx, const uint32_t *d_in, uint64_t in_stride, uint32_t n_cols, uint32_t log_n,
+                                uint32_t log_blowup, uint32_t *d_out, uint64_t out_stride, uint32_t *d_coef,
+                                uint32_t *d_big, const zs_circle_coset &hc_small, const zs_circle_coset &hc_big,
+                                const std::vector<PassRange> &pi, const std::vector<PassRange> &pf,
+                                const LdeTables *given) {
+    const uint32_t *tw_inv, *tw_fwd;
+    ZS_TRY(lde_tables(ctx, given, hc_small, hc_big, &tw_inv, &tw_fwd));
     const uint32_t LT = log_n + log_blowup;
     const size_t n = size_t(1) << log_n, N = size_t(1) << LT;
     const size_t np = pf.size();
     if (np == 1) {
         ctx->tile = ctx->lde_tiles.role[LDE_MID];
         ctx->lde_role = LDE_MID;
-        return run_lde_fused(ctx, d_in, in_stride, d_out, out_stride, ti->inv.as<uint32_t>(), tf->fwd.as<uint32_t>(),
+        return run_lde_fused(ctx, d_in, in_stride, d_out, out_stride, tw_inv, tw_fwd,
                              log_n, log_blowup, 0, log_n, n_cols);
     }
     ctx->tile = ctx->lde_tiles.role[LDE_INTERP];
@@ -469,7 +489,7 @@ static int32_t launch_lde_tiled(Ctx *ctx, const uint32_t *d_in, uint64_t in_stri
         p.in_cs = first ? in_stride : n;
         p.out = d_coef;
         p.out_cs = n;
-        p.tw = ti->inv.as<uint32_t>();
+        p.tw = tw_inv;
         p.L = log_n; p.LT = log_n; p.beta = 0;
         p.s_lo = pi[i].s_lo; p.C = pi[i].C; p.Q_log = 0;
         p.in_mode = first ? IN_NATURAL : IN_KSPACE;
@@ -480,7 +500,7 @@ static int32_t launch_lde_tiled(Ctx *ctx, const uint32_t *d_in, uint64_t in_stri
     }
     ctx->tile = ctx->lde_tiles.role[LDE_MID];
     ctx->lde_role = LDE_MID;
-    ZS_TRY(run_lde_fused(ctx, d_coef, n, d_big, N, ti->inv.as<uint32_t>(), tf->fwd.as<uint32_t>(), log_n,
+    ZS_TRY(run_lde_fused(ctx, d_coef, n, d_big, N, tw_inv, tw_fwd, log_n,
                          log_blowup, pf[0].s_lo, pf[0].C, n_cols));
     ctx->tile = ctx->lde_tiles.role[LDE_EVAL];
     ctx->lde_role = LDE_EVAL;
@@ -491,7 +511,7 @@ static int32_t launch_lde_tiled(Ctx *ctx, const uint32_t *d_in, uint64_t in_stri
         p.in_cs = N;
         p.out = last ? d_out : d_big;
         p.out_cs = last ? out_stride : N;
-        p.tw = tf->fwd.as<uint32_t>();
+        p.tw = tw_fwd;
         p.L = log_n; p.LT = LT; p.beta = log_blowup;
         p.s_lo = pf[i].s_lo; p.C = pf[i].C;
         p.Q_log = last ? std::min<uint32_t>(log_blowup, ctx->tile.elems_log > p.C ? ctx->tile.elems_log - p.C : 0) : 0;
@@ -506,7 +526,7 @@ static int32_t launch_lde_tiled(Ctx *ctx, const uint32_t *d_in, uint64_t in_stri
 
 static int32_t launch_lde_impl(Ctx *ctx, const uint32_t *d_in, uint64_t in_stride, uint32_t n_cols, uint32_t log_n,
                                uint32_t log_blowup, uint32_t *d_out, uint64_t out_stride, uint32_t *d_coef,
-                               uint32_t *d_big) {
+                               uint32_t *d_big, const LdeTables *given) {
     uint32_t LT = log_n + log_blowup;
     if (log_n < 1 || LT > 28) return ZS_ERR_INPUT;
     if (n_cols == 0) return ZS_OK;
@@ -529,7 +549,7 @@ static int32_t launch_lde_impl(Ctx *ctx, const uint32_t *d_in, uint64_t in_strid
         bool tiled = log_n >= 8 && lde_fused_ok(pf[0].C, log_blowup, pf.size() == 1);
         for (auto &r : pf) tiled = tiled && tile_pass_ok(r.C);
         if (tiled) return launch_lde_tiled(ctx, d_in, in_stride, n_cols, log_n, log_blowup, d_out, out_stride,
-                                           d_coef, d_big, hc_small, hc_big, pi, pf);
+                                           d_coef, d_big, hc_small, hc_big, pi, pf, given);
     }
     // 1) interpolate: natural evaluations -> interleaved coefficients scaled by 1/n (k-space)
     ctx->tile = ctx->lde_tiles.role[LDE_INTERP];
@@ -537,8 +557,14 @@ static int32_t launch_lde_impl(Ctx *ctx, const uint32_t *d_in, uint64_t in_strid
     if (log_n < 3) {
         ZS_TRY(run_small(ctx, true, hc_small, d_in, in_stride, d_coef, n, n_cols, true));
     } else {
-        const TwiddleTable *ti;
-        ZS_TRY(get_twiddles(ctx, hc_small, &ti));
+        const uint32_t *tw_inv;
+        if (given) {
+            if (!(tw_inv = given->inv_small)) return ZS_ERR_INPUT;
+        } else {
+            const TwiddleTable *ti;
+            ZS_TRY(get_twiddles(ctx, hc_small, &ti));
+            tw_inv = ti->inv.as<uint32_t>();
+        }
         auto plan = plan_passes(log_n, true, PASS_CMAX);
         for (size_t i = 0; i < plan.size(); i++) {
             FftPass p{};
@@ -547,7 +573,7 @@ static int32_t launch_lde_impl(Ctx *ctx, const uint32_t *d_in, uint64_t in_strid
             p.in_cs = first ? in_stride : n;
             p.out = d_coef;
             p.out_cs = n;
-            p.tw = ti->inv.as<uint32_t>();
+            p.tw = tw_inv;
             p.L = log_n; p.LT = log_n; p.beta = 0;
             p.s_lo = plan[i].s_lo; p.C = plan[i].C; p.Q_log = 0;
             p.in_mode = first ? IN_NATURAL : IN_KSPACE;
@@ -560,8 +586,14 @@ static int32_t launch_lde_impl(Ctx *ctx, const uint32_t *d_in, uint64_t in_strid
     // 2) evaluate on the 2^beta quarter transforms of the big canonic domain
     ctx->tile = ctx->lde_tiles.role[LDE_EVAL];
     ctx->lde_role = LDE_EVAL;
-    const TwiddleTable *tf;
-    ZS_TRY(get_twiddles(ctx, hc_big, &tf));
+    const uint32_t *tw_fwd;
+    if (given) {
+        if (!(tw_fwd = given->fwd_big)) return ZS_ERR_INPUT;
+    } else {
+        const TwiddleTable *tf;
+        ZS_TRY(get_twiddles(ctx, hc_big, &tf));
+        tw_fwd = tf->fwd.as<uint32_t>();
+    }
     auto plan = plan_passes(log_n, false, PASS_CMAX);
     size_t N = size_t(1) << LT;
     for (size_t i = 0; i < plan.size(); i++) {
@@ -571,7 +603,7 @@ static int32_t launch_lde_impl(Ctx *ctx, const uint32_t *d_in, uint64_t in_strid
         p.in_cs = first ? n : N;
         p.out = last ? d_out : d_big;
         p.out_cs = last ? out_stride : N;
-        p.tw = tf->fwd.as<uint32_t>();
+        p.tw = tw_fwd;
         p.L = log_n; p.LT = LT; p.beta = log_blowup;
         p.s_lo = plan[i].s_lo; p.C = plan[i].C;
         // the last pass keeps 2^Q_log of the 2^beta replicas in one workgroup so its natural-order
@@ -623,10 +655,11 @@ __global__ void plain_deinterleave_kernel(const uint32_t *c, uint32_t *ab, uint3
 // the LDE's passes take their launch shapes from ctx->lde_tiles; the context's
 // transform configuration is left as it was
 int32_t launch_lde(Ctx *ctx, const uint32_t *d_in, uint64_t in_stride, uint32_t n_cols, uint32_t log_n,
-                   uint32_t log_blowup, uint32_t *d_out, uint64_t out_stride, uint32_t *d_coef, uint32_t *d_big) {
+                   uint32_t log_blowup, uint32_t *d_out, uint64_t out_stride, uint32_t *d_coef, uint32_t *d_big,
+                   const LdeTables *tables) {
     const TileConfig saved = ctx->tile;
     const int32_t rc = launch_lde_impl(ctx, d_in, in_stride, n_cols, log_n, log_blowup, d_out, out_stride, d_coef,
-                                       d_big);
+                                       d_big, tables);
     ctx->tile = saved;
     ctx->lde_role = -1;
     return rc;

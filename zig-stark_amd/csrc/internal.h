@@ -153,6 +153,10 @@ struct Ctx {
     void lend_scratch(int slot, void *ptr, size_t bytes);
 };
 
+// ---- the allocation-free LDE's fixed device array (noalloc.hip) ----
+// once per device, with the device current: the array's address and its twiddle tables
+void noalloc_setup(int device);
+
 // ---- twiddles (fft.hip) ----
 int32_t get_twiddles(Ctx *ctx, const zs_circle_coset &hc, const TwiddleTable **out);
 int32_t get_canonic_twiddles(Ctx *ctx, uint32_t log_n, const TwiddleTable **out);
@@ -165,11 +169,20 @@ int32_t launch_fft(Ctx *ctx, const zs_circle_coset &hc, const uint32_t *d_in, ui
 // Inverse circle FFT: natural-order evaluations -> A-then-B coefficients.
 int32_t launch_ifft(Ctx *ctx, const zs_circle_coset &hc, const uint32_t *d_in, uint64_t in_stride, uint32_t *d_out,
                     uint64_t out_stride, uint32_t n_cols);
+// Twiddle tables of an LDE given by the caller (heap layout, see TwiddleTable): the inverse
+// table of canonicHalf(log_n) and the forward table of canonicHalf(log_n + log_blowup).
+struct LdeTables {
+    const uint32_t *inv_small, *fwd_big;
+};
+// twiddle_gen_kernel of the half-coset into fwd / inv (2^(log_size+1) words each) on `stream`
+hipError_t launch_twiddle_gen(hipStream_t stream, const zs_circle_coset &hc, uint32_t *fwd, uint32_t *inv);
 // Blowup LDE on canonic cosets: columns of 2^log_n values -> 2^(log_n+log_blowup) values.
 // d_coef: scratch of n_cols * 2^log_n words; d_big: scratch of n_cols * 2^(log_n+log_blowup) words
-// (may be nullptr when the transform fits one pass).
+// (may be nullptr when the transform fits one pass).  tables: null to use the context's
+// cached tables (built on first use), or the caller's (no device allocation at all).
 int32_t launch_lde(Ctx *ctx, const uint32_t *d_in, uint64_t in_stride, uint32_t n_cols, uint32_t log_n,
-                   uint32_t log_blowup, uint32_t *d_out, uint64_t out_stride, uint32_t *d_coef, uint32_t *d_big);
+                   uint32_t log_blowup, uint32_t *d_out, uint64_t out_stride, uint32_t *d_coef, uint32_t *d_big,
+                   const LdeTables *tables = nullptr);
 size_t lde_scratch_words(uint32_t n_cols, uint32_t log_n, uint32_t log_blowup, size_t *coef_words);
 // Reference-shaped per-column, per-layer path (one launch per butterfly layer).
 int32_t launch_lde_plain(Ctx *ctx, const uint32_t *d_in, uint64_t in_stride, uint32_t n_cols, uint32_t log_n,
