@@ -11,7 +11,7 @@ import subprocess
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+LIB_PATH = os.environ.get("ZS_ORACLE_LIBRARY") or os.path.join(ROOT, "oracle", "liboracle.so")
 P = 2147483647
 
 _lib = None

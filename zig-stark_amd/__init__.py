@@ -18,7 +18,9 @@ import os
 import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG, "libzs_hip.so")
+# ZS_HIP_LIBRARY selects another build of the same library (the sanitizer build of
+# tests/run_sanitized.sh); by default the in-tree libzs_hip.so
+LIB_PATH = os.environ.get("ZS_HIP_LIBRARY") or os.path.join(PKG, "libzs_hip.so")
 HEADER = os.path.join(os.path.dirname(PKG), "include", "zs_hip.h")
 P = 2147483647
 

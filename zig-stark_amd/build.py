@@ -23,9 +23,20 @@ FLAGS = [
 ]
 
 
-def _compile(src):
-    obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC, *FLAGS, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
+# Host-side AddressSanitizer + UndefinedBehaviorSanitizer build (device code is
+# never instrumented: each -fsanitize= applies to the host pass only).  It goes to
+# _build_san/libzs_hip_san.so and is loaded through ZS_HIP_LIBRARY with the clang
+# ASan runtime preloaded (tests/run_sanitized.sh).
+SAN_FLAGS = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+             "-Xarch_host", "-fno-sanitize-recover=undefined", "-Xarch_host", "-fno-omit-frame-pointer",
+             "-shared-libasan", "-g"]
+SAN_OBJ = os.path.join(PKG, "_build_san")
+SAN_OUT = os.path.join(SAN_OBJ, "libzs_hip_san.so")
+
+
+def _compile(src, sanitize=False):
+    obj = os.path.join(SAN_OBJ if sanitize else OBJ, os.path.splitext(src)[0] + ".o")
+    cmd = [HIPCC, *FLAGS, *(SAN_FLAGS if sanitize else []), "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src}:\n{r.stdout}\n{r.stderr}")
@@ -33,23 +44,26 @@ def _compile(src):
     return obj, warn
 
 
-def build(verbose=False):
-    os.makedirs(OBJ, exist_ok=True)
+def build(verbose=False, sanitize=False):
+    os.makedirs(SAN_OBJ if sanitize else OBJ, exist_ok=True)
     with ThreadPoolExecutor(max_workers=min(len(SOURCES), 8)) as ex:
-        results = list(ex.map(_compile, SOURCES))
+        results = list(ex.map(lambda s: _compile(s, sanitize), SOURCES))
     objs = [o for o, _ in results]
     if verbose:
         for _, w in results:
             for line in w:
                 print(line, file=sys.stderr)
-    tmp = OUT + ".tmp"
+    out = SAN_OUT if sanitize else OUT
+    tmp = out + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", *objs, "-o", tmp]
+    if sanitize:
+        cmd[1:1] = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined", "-shared-libasan"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
-    os.replace(tmp, OUT)
-    return OUT
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    print(build(verbose=True, sanitize="--sanitize" in sys.argv[1:]))

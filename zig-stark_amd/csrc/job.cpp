@@ -21,14 +21,11 @@
 #include <vector>
 
 #include "internal.h"
+#include "serve_core.h"
 
 using namespace zs;
 
 namespace {
-
-double now_s() {
-    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
-}
 
 constexpr char kCommitSep[] = "zig-stark:hip:commit";
 
@@ -145,53 +142,6 @@ zs_hip_ctx *new_worker(int device, const std::shared_ptr<DeviceCaches> &caches) 
     return w;
 }
 
-// First-fit allocator over one device allocation (offsets, 256-byte granules);
-// free blocks are kept coalesced.
-class Arena {
-  public:
-    void reset(uint64_t size) {
-        free_.clear();
-        if (size) free_[0] = size;
-    }
-    bool alloc(uint64_t bytes, uint64_t *off) {
-        bytes = (bytes + 255) & ~uint64_t(255);
-        for (auto it = free_.begin(); it != free_.end(); ++it) {
-            if (it->second < bytes) continue;
-            *off = it->first;
-            uint64_t rest = it->second - bytes, at = it->first + bytes;
-            free_.erase(it);
-            if (rest) free_[at] = rest;
-            return true;
-        }
-        return false;
-    }
-    bool fits(uint64_t bytes) const {
-        bytes = (bytes + 255) & ~uint64_t(255);
-        for (auto &f : free_)
-            if (f.second >= bytes) return true;
-        return false;
-    }
-    void release(uint64_t off, uint64_t bytes) {
-        bytes = (bytes + 255) & ~uint64_t(255);
-        auto it = free_.emplace(off, bytes).first;
-        auto nx = std::next(it);
-        if (nx != free_.end() && it->first + it->second == nx->first) {
-            it->second += nx->second;
-            free_.erase(nx);
-        }
-        if (it != free_.begin()) {
-            auto pv = std::prev(it);
-            if (pv->first + pv->second == it->first) {
-                pv->second += it->second;
-                free_.erase(it);
-            }
-        }
-    }
-
-  private:
-    std::map<uint64_t, uint64_t> free_;
-};
-
 }  // namespace
 
 extern "C" {
@@ -291,57 +241,30 @@ int32_t zs_commit_batch_dev(zs_hip_ctx *h, zs_job *jobs, size_t n_jobs, uint32_t
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
-// serving loop
+// serving loop: the queue, admission, arena, tickets and shutdown are the
+// HIP-free ServeCore (serve_core.h); a request's device work is serve_one
 // ---------------------------------------------------------------------------
-struct ServReq {
-    uint64_t id;
+struct ServData {
     zs_request r;
     std::vector<uint32_t> host;
     zs_response resp;
     JobLayout lay;
-    uint64_t bytes, off = 0;
-    uint32_t bypassed = 0;  // later requests admitted while this one did not fit
-    bool done = false;
-    bool claimed = false;  // a wait or poll has taken this ticket
 };
-
-// Admission order: arrival order, except that a request may pass earlier ones
-// whose memory is not free yet; each earlier request can be passed at most
-// kMaxBypass times, after which nothing behind it is admitted until it fits, so
-// a large request cannot starve under a stream of small ones.
-constexpr uint32_t kMaxBypass = 8;
+using ServCore = ServeCore<ServData>;
 
 struct zs_server {
     int device = 0;
-    uint64_t budget = 0;
-    uint64_t in_use = 0, peak = 0;
     DevBuf pool;  // the budget, allocated once; requests are carved from it
-    Arena arena;
     std::vector<zs_hip_ctx *> workers;
-    std::vector<std::thread> threads;
-    std::mutex mu;
-    std::condition_variable cv_work, cv_done;
-    std::deque<ServReq *> queue;
-    std::map<uint64_t, std::unique_ptr<ServReq>> reqs;
-    uint64_t next_id = 1;
-    uint32_t waiters = 0;  // threads inside zs_server_wait
-    bool stop = false;
+    std::unique_ptr<ServCore> core;
 };
 
-// the queued request to admit next (its position in the queue), or queue.end()
-static std::deque<ServReq *>::iterator admissible(zs_server *s) {
-    for (auto it = s->queue.begin(); it != s->queue.end(); ++it) {
-        if (s->arena.fits((*it)->bytes)) return it;
-        if ((*it)->bypassed >= kMaxBypass) break;  // this one goes next; nothing passes it
-    }
-    return s->queue.end();
-}
-
-static void serve_one(zs_server *s, zs_hip_ctx *w, ServReq *q) {
+static void serve_one(zs_server *s, zs_hip_ctx *w, ServCore::Item &it) {
     Ctx *ctx = &w->c;
-    const zs_request &r = q->r;
-    const JobLayout &j = q->lay;
-    char *base = s->pool.as<char>() + q->off;
+    ServData &q = it.data;
+    const zs_request &r = q.r;
+    const JobLayout &j = q.lay;
+    char *base = s->pool.as<char>() + it.off;
     auto part = [&](int k) { return reinterpret_cast<void *>(base + j.off[k]); };
     for (int k = 0; k < JobLayout::NPARTS; k++)
         if (kSlotOf[k] >= 0) ctx->lend_scratch(kSlotOf[k], part(k), j.len[k]);
@@ -349,7 +272,7 @@ static void serve_one(zs_server *s, zs_hip_ctx *w, ServReq *q) {
     const size_t elems = size_t(r.n_cols) << r.log_n;
     int32_t rc;
     if (r.host_cols)
-        rc = hipMemcpyAsync(d_cols, q->host.data(), elems * 4, hipMemcpyHostToDevice, ctx->stream) == hipSuccess
+        rc = hipMemcpyAsync(d_cols, q.host.data(), elems * 4, hipMemcpyHostToDevice, ctx->stream) == hipSuccess
                  ? ZS_OK
                  : ZS_ERR_DEVICE;
     else
@@ -357,43 +280,19 @@ static void serve_one(zs_server *s, zs_hip_ctx *w, ServReq *q) {
     if (!rc)
         rc = commit_job(w, d_cols, r.n_cols, r.log_n, r.log_blowup, r.remainder_log,
                         static_cast<uint32_t *>(part(JobLayout::LDE)), static_cast<uint8_t *>(part(JobLayout::TREE)),
-                        q->resp.trace_root, q->resp.fri_roots, q->resp.remainder, ZS_FLAG_NO_GRAPH);
+                        q.resp.trace_root, q.resp.fri_roots, q.resp.remainder, ZS_FLAG_NO_GRAPH);
     (void)hipStreamSynchronize(ctx->stream);
     ctx->release_scratch();  // (each request's buffers differ, so its FRI is not captured as a graph)
-    q->resp.status = rc;
-    q->resp.fri_layers = r.log_n + r.log_blowup - r.remainder_log;
+    q.resp.status = rc;
+    q.resp.fri_layers = r.log_n + r.log_blowup - r.remainder_log;
 }
 
-static void serve_loop(zs_server *s, zs_hip_ctx *w) {
-    (void)hipSetDevice(s->device);
-    for (;;) {
-        ServReq *q = nullptr;
-        {
-            std::unique_lock<std::mutex> lk(s->mu);
-            s->cv_work.wait(lk, [&] {
-                return (s->stop && s->queue.empty()) || admissible(s) != s->queue.end();
-            });
-            if (s->stop && s->queue.empty()) return;
-            auto it = admissible(s);
-            if (it == s->queue.end() || !s->arena.alloc((*it)->bytes, &(*it)->off)) continue;
-            for (auto e = s->queue.begin(); e != it; ++e) (*e)->bypassed++;
-            q = *it;
-            s->queue.erase(it);
-            s->in_use += q->bytes;
-            s->peak = std::max(s->peak, s->in_use);
-            q->resp.t_admit = now_s();
-        }
-        serve_one(s, w, q);
-        {
-            std::lock_guard<std::mutex> lk(s->mu);
-            q->resp.t_done = now_s();
-            q->done = true;
-            s->in_use -= q->bytes;
-            s->arena.release(q->off, q->bytes);
-        }
-        s->cv_done.notify_all();
-        s->cv_work.notify_all();
-    }
+static void take_response(const ServCore::Item &it, zs_response *out) {
+    *out = it.data.resp;
+    out->t_submit = it.t_submit;
+    out->t_admit = it.t_admit;
+    out->t_done = it.t_done;
+    out->device_bytes = it.bytes;
 }
 
 extern "C" {
@@ -403,23 +302,26 @@ zs_server *zs_server_new(int32_t device, uint32_t n_workers, uint64_t hbm_budget
     if (hipSetDevice(device) != hipSuccess) return nullptr;
     auto *s = new zs_server();
     s->device = device;
-    s->budget = (hbm_budget_bytes + 255) & ~uint64_t(255);
-    if (hipMalloc(&s->pool.ptr, s->budget) != hipSuccess) {
+    const uint64_t budget = Arena::round(hbm_budget_bytes);
+    if (hipMalloc(&s->pool.ptr, budget) != hipSuccess) {
         s->pool.ptr = nullptr;
         delete s;
         return nullptr;
     }
-    s->pool.bytes = s->budget;
-    s->arena.reset(s->budget);
+    s->pool.bytes = budget;
     for (uint32_t i = 0; i < n_workers; i++) {
         zs_hip_ctx *w = new_worker(device, s->workers.empty() ? nullptr : s->workers[0]->c.caches);
         if (!w) {
-            zs_server_free(s);
+            for (zs_hip_ctx *x : s->workers) zs_hip_destroy(x);
+            delete s;
             return nullptr;
         }
         s->workers.push_back(w);
     }
-    for (zs_hip_ctx *w : s->workers) s->threads.emplace_back(serve_loop, s, w);
+    s->core = std::make_unique<ServCore>(budget, [s](size_t wi, ServCore::Item &it) {
+        serve_one(s, s->workers[wi], it);
+    });
+    s->core->start(n_workers, [device](size_t) { (void)hipSetDevice(device); });
     return s;
 }
 
@@ -430,62 +332,39 @@ int32_t zs_server_submit(zs_server *s, const zs_request *req, uint64_t *ticket) 
         req->remainder_log < req->log_blowup ||
         req->remainder_log >= LT || req->remainder_log - req->log_blowup > kMaxRemainderLog)
         return ZS_ERR_INPUT;
-    auto q = std::make_unique<ServReq>();
-    q->r = *req;
-    std::memset(&q->resp, 0, sizeof q->resp);
-    q->resp.t_submit = now_s();
-    if (req->host_cols) q->host.assign(req->host_cols, req->host_cols + (size_t(req->n_cols) << req->log_n));
-    q->lay = job_layout(req->n_cols, req->log_n, req->log_blowup, req->remainder_log);
-    q->bytes = q->lay.total;
-    q->resp.device_bytes = q->bytes;
-    if (q->bytes > s->budget) return ZS_ERR_OOM;
-    {
-        std::lock_guard<std::mutex> lk(s->mu);
-        q->id = s->next_id++;
-        *ticket = q->id;
-        s->queue.push_back(q.get());
-        s->reqs[q->id] = std::move(q);
-    }
-    s->cv_work.notify_all();
-    return ZS_OK;
+    auto it = std::make_unique<ServCore::Item>();
+    ServData &q = it->data;
+    q.r = *req;
+    std::memset(&q.resp, 0, sizeof q.resp);
+    if (req->host_cols) q.host.assign(req->host_cols, req->host_cols + (size_t(req->n_cols) << req->log_n));
+    q.lay = job_layout(req->n_cols, req->log_n, req->log_blowup, req->remainder_log);
+    it->bytes = q.lay.total;
+    return s->core->submit(std::move(it), ticket) ? ZS_OK : ZS_ERR_OOM;
 }
 
 // A ticket is answered once: the first wait or poll claims it, and any later (or
 // concurrent) wait or poll of the same ticket returns ZS_ERR_INPUT.
 int32_t zs_server_wait(zs_server *s, uint64_t ticket, zs_response *out) {
     if (!s || !out) return ZS_ERR_INPUT;
-    std::unique_lock<std::mutex> lk(s->mu);
-    auto it = s->reqs.find(ticket);
-    if (it == s->reqs.end() || it->second->claimed) return ZS_ERR_INPUT;
-    ServReq *q = it->second.get();
-    q->claimed = true;
-    s->waiters++;
-    s->cv_done.wait(lk, [&] { return q->done; });
-    *out = q->resp;
-    s->reqs.erase(ticket);
-    s->waiters--;
-    if (s->stop) s->cv_done.notify_all();  // zs_server_free waits for the last waiter to leave
+    std::unique_ptr<ServCore::Item> it;
+    if (!s->core->wait(ticket, &it)) return ZS_ERR_INPUT;
+    take_response(*it, out);
     return ZS_OK;
 }
 
 int32_t zs_server_poll(zs_server *s, uint64_t ticket, zs_response *out, int32_t *ready) {
     if (!s || !out || !ready) return ZS_ERR_INPUT;
-    std::lock_guard<std::mutex> lk(s->mu);
-    auto it = s->reqs.find(ticket);
-    if (it == s->reqs.end() || it->second->claimed) return ZS_ERR_INPUT;
-    *ready = it->second->done ? 1 : 0;
-    if (*ready) {
-        *out = it->second->resp;
-        s->reqs.erase(it);
-    }
+    std::unique_ptr<ServCore::Item> it;
+    bool r = false;
+    if (!s->core->poll(ticket, &it, &r)) return ZS_ERR_INPUT;
+    *ready = r ? 1 : 0;
+    if (r) take_response(*it, out);
     return ZS_OK;
 }
 
 int32_t zs_server_memory(zs_server *s, uint64_t *in_use, uint64_t *peak) {
     if (!s) return ZS_ERR_INPUT;
-    std::lock_guard<std::mutex> lk(s->mu);
-    if (in_use) *in_use = s->in_use;
-    if (peak) *peak = s->peak;
+    s->core->memory(in_use, peak);
     return ZS_OK;
 }
 
@@ -494,17 +373,8 @@ int32_t zs_server_memory(zs_server *s, uint64_t *in_use, uint64_t *peak) {
 // call on the server may start once zs_server_free has been called.
 void zs_server_free(zs_server *s) {
     if (!s) return;
-    {
-        std::lock_guard<std::mutex> lk(s->mu);
-        s->stop = true;
-    }
-    s->cv_work.notify_all();
-    for (auto &t : s->threads) t.join();
-    {
-        std::unique_lock<std::mutex> lk(s->mu);
-        s->cv_done.notify_all();
-        s->cv_done.wait(lk, [&] { return s->waiters == 0; });
-    }
+    s->core->stop();
+    s->core.reset();
     for (zs_hip_ctx *w : s->workers) zs_hip_destroy(w);
     delete s;
 }
