@@ -340,6 +340,63 @@ int32_t zs_fib_stark_prove_dev(zs_hip_ctx *ctx, const uint32_t *d_trace, uint32_
                                const uint32_t claimed[4], zs_channel *chan, zs_host_allocator host,
                                uint8_t **proof_out, size_t *proof_len);
 
+/* ---- GenericStark(Air) with preprocessed columns and LogUp lookups ---------------- */
+/* An AIR as data (the reference's comptime Air type, src/m31/stark.zig:19-69):
+ *  - transition constraint k (Air.evalTransition) is the sum of the terms whose
+ *    `constraint` is k, a term being coeff * (product of its factors), a factor
+ *    being current[column], next[column] or the evaluation point x;
+ *  - boundary assertions (Air.boundaryAssertions for the public inputs):
+ *    (column, step, value);
+ *  - LogUp metadata per relation r (lookup_selector_columns, ..._multiplicity_columns,
+ *    lookup_key_columns, lookup_table_columns): relation r's key (trace) columns are
+ *    key_columns[key_start[r] .. key_start[r+1]), its table (preprocessed) columns
+ *    table_columns[table_start[r] .. table_start[r+1]);
+ *  - Air.maxConstraintDegree(n) = degree_mul * (n - 1) + degree_add, or, with both
+ *    0, the largest term degree (n - 1 per column factor, 1 per x factor).
+ * Limits of the device prover: 32 committed columns (trace + accumulators +
+ * preprocessed), 64 terms, 16 boundary assertions, 4 relations, 8 key columns
+ * per relation. */
+#define ZS_AIR_CUR 0u
+#define ZS_AIR_NEXT 1u
+#define ZS_AIR_X 2u
+typedef struct { uint32_t kind, column; } zs_air_factor;
+typedef struct {
+    uint32_t constraint;
+    uint32_t coeff[4];   /* QM31 */
+    uint32_t n_factors;  /* 0..4 */
+    zs_air_factor factor[4];
+} zs_air_term;
+typedef struct { uint32_t column, reserved; uint64_t step; uint32_t value[4]; } zs_boundary;
+typedef struct {
+    uint32_t num_columns, num_preprocessed, num_transition_constraints, num_terms;
+    const zs_air_term *terms;
+    uint32_t degree_mul, degree_add;
+    uint32_t num_boundary, reserved;
+    const zs_boundary *boundary;
+    uint32_t num_lookup_columns, num_lookup_relations;
+    const uint32_t *selector_column, *multiplicity_column;
+    const uint32_t *key_start, *key_columns, *table_start, *table_columns;
+} zs_air;
+/* GenericStark(Air).proveWithPreprocessed (src/m31/stark.zig:311-657) on device
+ * columns (d_pre: num_preprocessed, d_trace: num_columns QM31 columns of
+ * 2^trace_log values, column-major; d_pre may be NULL without preprocessed
+ * columns): preprocessed, trace and accumulator commitments, the quotient (every
+ * synthesized LogUp, selector-binarity and boundary term) and the DEEP codeword
+ * with batch inversion, FRI under friParams (stark.zig:197-222) and the query
+ * reveals on the device; the transcript and the O(n) evaluations at z on the host.
+ * The proof is returned as GenericStark.Proof wire bytes allocated with `host`.
+ * ZS_ERR_INPUT for parameters friParams rejects (remainder_log below the FRI
+ * blowup or not below trace_log + log_blowup) and for AIRs beyond the limits. */
+int32_t zs_stark_prove_dev(zs_hip_ctx *ctx, const zs_air *air, const uint32_t *d_pre, const uint32_t *d_trace,
+                           uint32_t trace_log, uint32_t log_blowup, size_t num_queries, uint32_t remainder_log,
+                           zs_channel *chan, zs_host_allocator host, uint8_t **proof_out, size_t *proof_len);
+/* GenericStark(Air).verify (stark.zig:660-780) of the wire bytes on the host, with
+ * the checks of zs_fib_stark_verify (one reveal per FRI query, the nested FRI
+ * parameters equal to friParams).  ZS_OK with *out_ok 1 or 0; ZS_ERR_INPUT for
+ * bytes that do not deserialize or an AIR beyond the limits. */
+int32_t zs_stark_verify(const uint8_t *proof, size_t proof_len, const zs_air *air, zs_channel *chan,
+                        uint8_t *out_ok);
+
 /* ---- synthetic input ---------------------------------------------------------- */
 /* Seeded uniform M31 values: element i is the top 31 bits of the (i+1)-th
  * SplitMix64 output from `seed` (2^31 - 1 rejected). */

@@ -1518,3 +1518,439 @@ int zo_fib_stark_verify(const uint8_t *proof, size_t len, const uint32_t a0[4], 
     free(frem); free(fvals); free(fpaths); free(fidx);
     return ok;
 }
+
+/* ======================================================================== */
+/* GenericStark(Air): src/m31/stark.zig:311-657 (proveWithPreprocessed) and */
+/* :660-780 (verify), with preprocessed columns and the synthesized LogUp    */
+/* machinery, for an AIR given as data (zo_air, zs_oracle.h).                */
+/* ======================================================================== */
+typedef struct {
+    const zo_air *air;
+    uint32_t m, n_pre, n_rel, n_lc, T, B, m_total, n_trans_total, total_bound, n_deep, reveal_len;
+} stark_dims;
+static stark_dims stark_dims_of(const zo_air *a) {
+    stark_dims d;
+    d.air = a;
+    d.m = a->num_columns; d.n_pre = a->num_preprocessed; d.n_rel = a->num_lookup_relations;
+    d.n_lc = a->num_lookup_columns; d.T = a->num_transition_constraints; d.B = a->num_boundary;
+    d.m_total = d.m + d.n_rel;                 /* stark.zig:184-188 */
+    d.n_trans_total = d.T + 2 * d.n_rel;
+    d.total_bound = d.B + d.n_rel;
+    d.n_deep = 2 * d.m_total + 1;
+    d.reveal_len = 2 * d.m_total + 1 + d.n_pre;
+    return d;
+}
+/* the AIR's maxConstraintDegree(n), then GenericStark's (stark.zig:212-216) */
+static uint64_t air_degree(const zo_air *a, uint64_t n) {
+    if (a->degree_mul || a->degree_add) return (uint64_t)a->degree_mul * (n - 1) + a->degree_add;
+    uint64_t best = 0;
+    for (uint32_t t = 0; t < a->num_terms; t++) {
+        uint64_t dg = 0;
+        for (uint32_t f = 0; f < a->terms[t].n_factors; f++) dg += a->terms[t].factor[f].kind == ZO_AIR_X ? 1 : n - 1;
+        if (dg > best) best = dg;
+    }
+    return best;
+}
+static uint64_t max_constraint_degree(const zo_air *a, uint64_t n) {
+    uint64_t air = air_degree(a, n), lookup = a->num_lookup_relations ? 3 * (n - 1) : 0;
+    return air > lookup ? air : lookup;
+}
+/* compositionLogSize (stark.zig:197-206) and friParams (:208-222) */
+int zo_stark_fri_params(const zo_air *air, uint32_t tl, uint32_t lb, uint32_t *fri_log_size, uint32_t *fri_lb) {
+    const uint64_t n = (uint64_t)1 << tl, dc = max_constraint_degree(air, n);
+    const uint64_t dg = (n - 1) > (dc > n ? dc - n : 0) ? (n - 1) : dc - n;
+    uint32_t log = 0;
+    for (uint64_t bound = 1; bound <= dg; bound <<= 1) log++;
+    if (log > tl + lb) return -2;
+    *fri_log_size = log;
+    *fri_lb = tl + lb - log;
+    return 0;
+}
+/* combinedKey (stark.zig:225-229): sum_j alpha_j * row[cols[j]] */
+static zo_qm31 combined_key(const uint32_t *cols, uint32_t k0, uint32_t k1, const zo_qm31 *alpha_cols,
+                            const zo_qm31 *row) {
+    zo_qm31 k = Q_ZERO;
+    for (uint32_t j = k0; j < k1; j++) k = zo_qm31_add(k, zo_qm31_mul(alpha_cols[j - k0], row[cols[j]]));
+    return k;
+}
+/* rowContribution (stark.zig:260-275) */
+static zo_qm31 row_contribution(const zo_air *a, uint32_t r, const zo_qm31 *ac, const zo_qm31 *ar,
+                                const zo_qm31 *row, const zo_qm31 *row_pre) {
+    const zo_qm31 sel = row[a->selector_column[r]], mult = row[a->multiplicity_column[r]];
+    const zo_qm31 key = combined_key(a->key_columns, a->key_start[r], a->key_start[r + 1], ac, row);
+    const zo_qm31 tkey = combined_key(a->table_columns, a->table_start[r], a->table_start[r + 1], ac, row_pre);
+    const zo_qm31 lt = zo_qm31_mul(sel, zo_qm31_mul(mult, zo_qm31_inv(zo_qm31_sub(ar[r], key))));
+    const zo_qm31 tt = zo_qm31_mul(zo_qm31_sub(Q_ONE, sel), zo_qm31_inv(zo_qm31_sub(ar[r], tkey)));
+    return zo_qm31_sub(lt, tt);
+}
+/* lookupConstraintAt (stark.zig:231-253) */
+static zo_qm31 lookup_constraint(const zo_air *a, uint32_t r, const zo_qm31 *ac, const zo_qm31 *ar,
+                                 const zo_qm31 *cur, const zo_qm31 *row_pre, zo_qm31 acc_cur, zo_qm31 acc_next) {
+    const zo_qm31 sel = cur[a->selector_column[r]], mult = cur[a->multiplicity_column[r]];
+    const zo_qm31 key = combined_key(a->key_columns, a->key_start[r], a->key_start[r + 1], ac, cur);
+    const zo_qm31 tkey = combined_key(a->table_columns, a->table_start[r], a->table_start[r + 1], ac, row_pre);
+    const zo_qm31 dacc = zo_qm31_sub(acc_next, acc_cur);
+    const zo_qm31 lb = zo_qm31_sub(zo_qm31_mul(dacc, zo_qm31_sub(ar[r], key)), mult);
+    const zo_qm31 tb = zo_qm31_add(zo_qm31_mul(dacc, zo_qm31_sub(ar[r], tkey)), Q_ONE);
+    return zo_qm31_add(zo_qm31_mul(sel, lb), zo_qm31_mul(zo_qm31_sub(Q_ONE, sel), tb));
+}
+/* Air.evalTransition for the AIR's terms */
+static void eval_transition(const zo_air *a, zo_qm31 x, const zo_qm31 *cur, const zo_qm31 *nxt, zo_qm31 *res) {
+    for (uint32_t k = 0; k < a->num_transition_constraints; k++) res[k] = Q_ZERO;
+    for (uint32_t t = 0; t < a->num_terms; t++) {
+        const zo_air_term *tm = &a->terms[t];
+        zo_qm31 p = ld4(tm->coeff);
+        for (uint32_t f = 0; f < tm->n_factors; f++) {
+            const zo_air_factor fc = tm->factor[f];
+            p = zo_qm31_mul(p, fc.kind == ZO_AIR_X ? x : fc.kind == ZO_AIR_NEXT ? nxt[fc.column] : cur[fc.column]);
+        }
+        res[tm->constraint] = zo_qm31_add(res[tm->constraint], p);
+    }
+}
+typedef struct {
+    const zo_qm31 *alphas, *betas, *alpha_cols, *alpha_rels, *pk, *bval;
+    const uint32_t *bcol;
+    zo_qm31 last;
+} stark_weights;
+/* Hc(x) of stark.zig:504-530 (also :753-778 and :819-845): cur / nxt hold the
+ * m_total trace and accumulator values at x and w x, row_pre the preprocessed ones */
+static zo_qm31 composition(const stark_dims *d, const stark_weights *wt, zo_qm31 x, const zo_qm31 *cur,
+                           const zo_qm31 *nxt, const zo_qm31 *row_pre, zo_qm31 zh, zo_qm31 *res) {
+    const zo_air *a = d->air;
+    eval_transition(a, x, cur, nxt, res);
+    zo_qm31 h = Q_ZERO, xl = zo_qm31_sub(x, wt->last);
+    for (uint32_t k = 0; k < d->T; k++) h = zo_qm31_add(h, zo_qm31_mul(wt->alphas[k], zo_qm31_mul(res[k], xl)));
+    for (uint32_t r = 0; r < d->n_rel; r++) {
+        const zo_qm31 c = lookup_constraint(a, r, wt->alpha_cols, wt->alpha_rels, cur, row_pre, cur[d->m + r],
+                                            nxt[d->m + r]);
+        h = zo_qm31_add(h, zo_qm31_mul(wt->alphas[d->T + r], c));
+    }
+    for (uint32_t r = 0; r < d->n_rel; r++) {
+        const zo_qm31 sel = cur[a->selector_column[r]];
+        h = zo_qm31_add(h, zo_qm31_mul(wt->alphas[d->T + d->n_rel + r], zo_qm31_mul(sel, zo_qm31_sub(Q_ONE, sel))));
+    }
+    for (uint32_t k = 0; k < d->total_bound; k++) {
+        const zo_qm31 term = zo_qm31_mul(zo_qm31_mul(zo_qm31_sub(cur[wt->bcol[k]], wt->bval[k]), zh),
+                                         zo_qm31_inv(zo_qm31_sub(x, wt->pk[k])));
+        h = zo_qm31_add(h, zo_qm31_mul(wt->betas[k], term));
+    }
+    return h;
+}
+/* stark.zig:430-449 */
+void zo_stark_accumulators(const zo_air *a, uint32_t tl, const uint32_t *pre, const uint32_t *trace,
+                           const uint32_t *alpha_cols, const uint32_t *alpha_rels, uint32_t *out) {
+    const size_t n = (size_t)1 << tl;
+    const uint32_t m = a->num_columns, np = a->num_preprocessed, nr = a->num_lookup_relations;
+    zo_qm31 *row = malloc(sizeof(zo_qm31) * (m + 1)), *rp = malloc(sizeof(zo_qm31) * (np + 1));
+    zo_qm31 *ac = malloc(sizeof(zo_qm31) * (a->num_lookup_columns + 1)), *ar = malloc(sizeof(zo_qm31) * (nr + 1));
+    for (uint32_t j = 0; j < a->num_lookup_columns; j++) ac[j] = ld4(alpha_cols + 4 * j);
+    for (uint32_t r = 0; r < nr; r++) ar[r] = ld4(alpha_rels + 4 * r);
+    for (uint32_t r = 0; r < nr; r++) st4(out + 4 * n * r, Q_ZERO);
+    for (size_t i = 1; i < n; i++) {
+        for (uint32_t j = 0; j < m; j++) row[j] = ld4(trace + 4 * (n * j + i - 1));
+        for (uint32_t j = 0; j < np; j++) rp[j] = ld4(pre + 4 * (n * j + i - 1));
+        for (uint32_t r = 0; r < nr; r++)
+            st4(out + 4 * (n * r + i),
+                zo_qm31_add(ld4(out + 4 * (n * r + i - 1)), row_contribution(a, r, ac, ar, row, rp)));
+    }
+    free(row); free(rp); free(ac); free(ar);
+}
+
+/* interpolate columns over H, evaluate on D, build hashQM31 trees; roots absorbed when chan is set */
+static void commit_columns(const uint32_t *cols, uint32_t ncols, size_t n, size_t N, const uint32_t *h,
+                           const zo_qm31 *dp, uint32_t *coeffs, uint32_t *codewords, uint8_t *trees, uint8_t *roots,
+                           uint8_t *leaves, zo_channel *chan) {
+    for (uint32_t j = 0; j < ncols; j++) {
+        zo_poly_interpolate(h, cols + 4 * n * j, n, coeffs + 4 * n * j);
+        for (size_t i = 0; i < N; i++) {
+            zo_poly_eval(coeffs + 4 * n * j, n, dp[i].v, codewords + 4 * (N * j + i));
+            zo_hash_qm31(codewords + 4 * (N * j + i), leaves + 32 * i);
+        }
+        uint8_t *t = trees + (2 * N - 1) * 32 * j;
+        zo_merkle_build(leaves, N, t);
+        memcpy(roots + 32 * j, t + (2 * N - 2) * 32, 32);
+        if (chan) zo_channel_absorb(chan, roots + 32 * j, 32);
+    }
+}
+
+uint8_t *zo_stark_prove(const zo_air *air, uint32_t tl, uint32_t lb, size_t nq, uint32_t rl, const uint32_t *pre,
+                        const uint32_t *trace, zo_channel *chan, size_t *out_len) {
+    const stark_dims D = stark_dims_of(air);
+    const uint32_t dlog = tl + lb;
+    uint32_t fls, flb;
+    if (!chan || tl < 1 || zo_stark_fri_params(air, tl, lb, &fls, &flb) || rl >= dlog || rl < flb) return NULL;
+    const size_t n = (size_t)1 << tl, N = (size_t)1 << dlog, shift = (size_t)1 << lb;
+    const zo_qm31 w = zo_qm31_root_of_unity(tl), w_ev = zo_qm31_root_of_unity(dlog);
+    const size_t tsz = (2 * N - 1) * 32;
+    uint32_t *h = malloc(n * 16), *co = malloc((size_t)(D.m_total + 1) * n * 16), *pco = malloc((D.n_pre + 1) * n * 16);
+    uint32_t *cw = malloc((size_t)(D.m_total + 1) * N * 16), *pcw = malloc((D.n_pre + 1) * N * 16);
+    uint32_t *qc = malloc(N * 16), *g = malloc(N * 16), *acc = malloc((D.n_rel + 1) * n * 16);
+    zo_qm31 *dp = malloc(N * sizeof(zo_qm31));
+    uint8_t *leaves = malloc(N * 32), *trees = malloc((D.m_total + 1) * tsz), *ptrees = malloc((D.n_pre + 1) * tsz);
+    uint8_t *roots = malloc(32 * (D.m_total + 1)), *proots = malloc(32 * (D.n_pre + 1)), qroot[32];
+    zo_qm31 *ac = malloc(sizeof(zo_qm31) * (D.n_lc + 1)), *ar = malloc(sizeof(zo_qm31) * (D.n_rel + 1));
+    zo_qm31 *al = malloc(sizeof(zo_qm31) * (D.n_trans_total + 1)), *be = malloc(sizeof(zo_qm31) * (D.total_bound + 1));
+    zo_qm31 *pk = malloc(sizeof(zo_qm31) * (D.total_bound + 1)), *bv = malloc(sizeof(zo_qm31) * (D.total_bound + 1));
+    uint32_t *bc = malloc(sizeof(uint32_t) * (D.total_bound + 1));
+    zo_qm31 *cur = malloc(sizeof(zo_qm31) * (D.m_total + 1)), *nxt = malloc(sizeof(zo_qm31) * (D.m_total + 1));
+    zo_qm31 *rp = malloc(sizeof(zo_qm31) * (D.n_pre + 1)), *res = malloc(sizeof(zo_qm31) * (D.T + 1));
+    zo_qm31 *de = malloc(sizeof(zo_qm31) * D.n_deep), *ga = malloc(sizeof(zo_qm31) * D.n_deep);
+    /* H and D points (stark.zig:343-372) */
+    zo_qm31 x = Q_ONE;
+    for (size_t i = 0; i < n; i++) { st4(h + 4 * i, x); x = zo_qm31_mul(x, w); }
+    dp[0] = ld4(ZO_FRI_OFFSET);
+    for (size_t i = 1; i < N; i++) dp[i] = zo_qm31_mul(dp[i - 1], w_ev);
+    /* preprocessed commitment, absorbed first; then the trace (stark.zig:374-414) */
+    commit_columns(pre, D.n_pre, n, N, h, dp, pco, pcw, ptrees, proots, leaves, chan);
+    commit_columns(trace, D.m, n, N, h, dp, co, cw, trees, roots, leaves, chan);
+    /* lookup challenges, accumulators on H, their commitment (stark.zig:416-468) */
+    for (uint32_t j = 0; j < D.n_lc; j++) ac[j] = sample_q(chan);
+    for (uint32_t r = 0; r < D.n_rel; r++) ar[r] = sample_q(chan);
+    if (D.n_rel) {
+        uint32_t *acw = malloc(4 * D.n_lc * 4 + 16), *arw = malloc(4 * D.n_rel * 4 + 16);
+        for (uint32_t j = 0; j < D.n_lc; j++) st4(acw + 4 * j, ac[j]);
+        for (uint32_t r = 0; r < D.n_rel; r++) st4(arw + 4 * r, ar[r]);
+        zo_stark_accumulators(air, tl, pre, trace, acw, arw, acc);
+        free(acw); free(arw);
+        commit_columns(acc, D.n_rel, n, N, h, dp, co + 4 * n * D.m, cw + 4 * N * D.m, trees + tsz * D.m,
+                       roots + 32 * D.m, leaves, chan);
+    }
+    /* transition and boundary weights (stark.zig:470-483) */
+    for (uint32_t k = 0; k < D.n_trans_total; k++) al[k] = sample_q(chan);
+    for (uint32_t k = 0; k < D.B; k++) { bc[k] = air->boundary[k].column; bv[k] = ld4(air->boundary[k].value);
+                                         pk[k] = zo_qm31_pow(w, air->boundary[k].step); }
+    for (uint32_t r = 0; r < D.n_rel; r++) { bc[D.B + r] = D.m + r; bv[D.B + r] = Q_ZERO; pk[D.B + r] = Q_ONE; }
+    for (uint32_t k = 0; k < D.total_bound; k++) be[k] = sample_q(chan);
+    stark_weights wt = {al, be, ac, ar, pk, bv, bc, ld4(h + 4 * (n - 1))};
+    /* quotient on D (stark.zig:485-531) and its commitment */
+    for (size_t i = 0; i < N; i++) {
+        const zo_qm31 zh = zo_qm31_sub(zo_qm31_pow(dp[i], n), Q_ONE);
+        const size_t ni = (i + shift) % N;
+        for (uint32_t j = 0; j < D.m_total; j++) { cur[j] = ld4(cw + 4 * (N * j + i)); nxt[j] = ld4(cw + 4 * (N * j + ni)); }
+        for (uint32_t j = 0; j < D.n_pre; j++) rp[j] = ld4(pcw + 4 * (N * j + i));
+        st4(qc + 4 * i, zo_qm31_mul(composition(&D, &wt, dp[i], cur, nxt, rp, zh, res), zo_qm31_inv(zh)));
+        zo_hash_qm31(qc + 4 * i, leaves + 32 * i);
+    }
+    uint8_t *qtree = malloc(tsz);
+    zo_merkle_build(leaves, N, qtree);
+    memcpy(qroot, qtree + (2 * N - 2) * 32, 32);
+    zo_channel_absorb(chan, qroot, 32);
+    /* z, DEEP evaluations, Q(z) (computeQuotientAt, stark.zig:793-846), DEEP weights */
+    const zo_qm31 z = sample_q(chan), wz = zo_qm31_mul(z, w);
+    for (uint32_t j = 0; j < D.m_total; j++) {
+        zo_poly_eval(co + 4 * n * j, n, z.v, de[j].v);
+        zo_poly_eval(co + 4 * n * j, n, wz.v, de[D.m_total + j].v);
+    }
+    {
+        for (uint32_t j = 0; j < D.n_pre; j++) zo_poly_eval(pco + 4 * n * j, n, z.v, rp[j].v);
+        const zo_qm31 zh = zo_qm31_sub(zo_qm31_pow(z, n), Q_ONE);
+        de[2 * D.m_total] = zo_qm31_mul(composition(&D, &wt, z, de, de + D.m_total, rp, zh, res), zo_qm31_inv(zh));
+    }
+    for (uint32_t k = 0; k < D.n_deep; k++) absorb_qm31(chan, de[k]);
+    for (uint32_t k = 0; k < D.n_deep; k++) ga[k] = sample_q(chan);
+    /* DEEP combined codeword (stark.zig:568-580) */
+    for (size_t i = 0; i < N; i++) {
+        const zo_qm31 inv_dz = zo_qm31_inv(zo_qm31_sub(dp[i], z));
+        const size_t ni = (i + shift) % N;
+        zo_qm31 gv = Q_ZERO;
+        for (uint32_t j = 0; j < D.m_total; j++) {
+            gv = zo_qm31_add(gv, zo_qm31_mul(ga[j], zo_qm31_mul(zo_qm31_sub(ld4(cw + 4 * (N * j + i)), de[j]), inv_dz)));
+            gv = zo_qm31_add(gv, zo_qm31_mul(ga[D.m_total + j], zo_qm31_mul(zo_qm31_sub(ld4(cw + 4 * (N * j + ni)),
+                                                                                      de[D.m_total + j]), inv_dz)));
+        }
+        gv = zo_qm31_add(gv, zo_qm31_mul(ga[2 * D.m_total],
+                                         zo_qm31_mul(zo_qm31_sub(ld4(qc + 4 * i), de[2 * D.m_total]), inv_dz)));
+        st4(g + 4 * i, gv);
+    }
+    /* FRI on g under friParams (stark.zig:582-583) */
+    const uint32_t L = dlog - rl;
+    const size_t rc = (size_t)1 << (rl - flb), per_q = zo_fri_query_path_bytes(dlog, rl);
+    uint8_t *froots = malloc(32 * L + 1), *fpaths = malloc(per_q * nq + 1);
+    uint32_t *frem = malloc(16 * rc), *fvals = malloc(16 * 2 * L * nq + 16);
+    size_t *fidx = malloc(sizeof(size_t) * (nq + 1));
+    int rcode = zo_fri_prove(g, dlog, flb, rl, nq, chan, froots, frem, fidx, fvals, fpaths);
+    uint8_t *out = NULL;
+    if (rcode == 0) {
+        for (int pass = 0; pass < 2; pass++) { /* GenericStark.Proof (stark.zig:107-123) */
+            wire_buf wb = {pass ? out : NULL, 0};
+            w_stark_params(&wb, tl, lb, nq, rl);
+            w_uint(&wb, D.m, 8);
+            w_bytes(&wb, roots, 32 * (size_t)D.m);
+            w_uint(&wb, D.n_pre ? 1 : 0, 1); /* ?[]Digest */
+            if (D.n_pre) { w_uint(&wb, D.n_pre, 8); w_bytes(&wb, proots, 32 * (size_t)D.n_pre); }
+            w_uint(&wb, D.n_rel ? 1 : 0, 1);
+            if (D.n_rel) { w_uint(&wb, D.n_rel, 8); w_bytes(&wb, roots + 32 * D.m, 32 * (size_t)D.n_rel); }
+            w_bytes(&wb, qroot, 32);
+            w_uint(&wb, D.n_deep, 8);
+            for (uint32_t k = 0; k < D.n_deep; k++) w_qm31(&wb, de[k].v);
+            wb.n += zo_fri_proof_serialize(fls, flb, rl, nq, froots, frem, fidx, fvals, fpaths,
+                                           pass ? out + wb.n : NULL);
+            w_uint(&wb, nq, 8);
+            uint8_t path[32 * 32];
+            for (size_t q = 0; q < nq; q++) {
+                const size_t p0 = fidx[q], pn = (p0 + shift) % N;
+                w_uint(&wb, p0, 8);
+                w_uint(&wb, D.reveal_len, 8);
+                /* values: columns at p0, at pn, Q at p0, preprocessed at p0 (stark.zig:593-625) */
+                for (int half = 0; half < 2; half++)
+                    for (uint32_t j = 0; j < D.m_total; j++) w_qm31(&wb, cw + 4 * (N * j + (half ? pn : p0)));
+                w_qm31(&wb, qc + 4 * p0);
+                for (uint32_t k = 0; k < D.n_pre; k++) w_qm31(&wb, pcw + 4 * (N * k + p0));
+                w_uint(&wb, D.reveal_len, 8);
+                for (int half = 0; half < 2; half++)
+                    for (uint32_t j = 0; j < D.m_total; j++) {
+                        zo_merkle_open(trees + tsz * j, N, half ? pn : p0, path);
+                        w_uint(&wb, dlog, 8);
+                        w_bytes(&wb, path, (size_t)dlog * 32);
+                    }
+                zo_merkle_open(qtree, N, p0, path);
+                w_uint(&wb, dlog, 8);
+                w_bytes(&wb, path, (size_t)dlog * 32);
+                for (uint32_t k = 0; k < D.n_pre; k++) {
+                    zo_merkle_open(ptrees + tsz * k, N, p0, path);
+                    w_uint(&wb, dlog, 8);
+                    w_bytes(&wb, path, (size_t)dlog * 32);
+                }
+            }
+            if (!pass) { out = malloc(wb.n); *out_len = wb.n; }
+        }
+    }
+    free(h); free(co); free(pco); free(cw); free(pcw); free(qc); free(g); free(acc); free(dp); free(leaves);
+    free(trees); free(ptrees); free(roots); free(proots); free(ac); free(ar); free(al); free(be); free(pk); free(bv);
+    free(bc); free(cur); free(nxt); free(rp); free(res); free(de); free(ga); free(qtree);
+    free(froots); free(fpaths); free(frem); free(fvals); free(fidx);
+    return out;
+}
+
+/* read a ?[]Digest of exactly `want` digests when `want` > 0, absent otherwise */
+static const uint8_t *r_opt_digests(wire_rd *r, uint32_t want) {
+    const uint64_t present = r_uint(r, 1);
+    if (present > 1 || (present != (want > 0))) { r->bad = 1; return NULL; }
+    if (!present) return NULL;
+    if (r_uint(r, 8) != want) { r->bad = 1; return NULL; }
+    return r_take(r, 32 * (size_t)want);
+}
+
+int zo_stark_verify(const zo_air *air, const uint8_t *proof, size_t len, zo_channel *chan) {
+    const stark_dims D = stark_dims_of(air);
+    wire_rd r = {proof, len, 0, 0};
+    const uint32_t tl = (uint32_t)r_uint(&r, 1), lb = (uint32_t)r_uint(&r, 1);
+    const size_t nq = (size_t)r_uint(&r, 8);
+    const uint32_t rl = (uint32_t)r_uint(&r, 1);
+    uint32_t fls, flb;
+    if (r.bad || tl < 1 || tl + lb > 24 || nq > 4096 || zo_stark_fri_params(air, tl, lb, &fls, &flb) ||
+        rl >= tl + lb || rl < flb)
+        return 0;
+    const size_t n = (size_t)1 << tl, N = (size_t)1 << (tl + lb), shift = (size_t)1 << lb;
+    const uint32_t dlog = tl + lb, L = dlog - rl;
+    const size_t rc = (size_t)1 << (rl - flb);
+    if (r_uint(&r, 8) != D.m) return 0;
+    const uint8_t *troots = r_take(&r, 32 * (size_t)D.m);
+    const uint8_t *proots = r_opt_digests(&r, D.n_pre), *aroots = r_opt_digests(&r, D.n_rel);
+    const uint8_t *qroot = r_take(&r, 32);
+    if (r.bad || r_uint(&r, 8) != D.n_deep) return 0;
+    zo_qm31 *de = malloc(sizeof(zo_qm31) * D.n_deep), *ga = malloc(sizeof(zo_qm31) * D.n_deep);
+    for (uint32_t k = 0; k < D.n_deep; k++) de[k] = r_qm31(&r);
+    /* nested Fri.Proof; its parameters must be friParams(params) */
+    if (r_uint(&r, 1) != fls || r_uint(&r, 1) != flb || r_uint(&r, 8) != nq || r_uint(&r, 1) != rl ||
+        r_uint(&r, 8) != L) { free(de); free(ga); return 0; }
+    const uint8_t *froots = r_take(&r, 32 * (size_t)L);
+    if (r_uint(&r, 8) != rc) { free(de); free(ga); return 0; }
+    uint32_t *frem = malloc(16 * rc), *fvals = malloc(16 * 2 * (size_t)L * nq + 16);
+    const size_t per_q = zo_fri_query_path_bytes(dlog, rl);
+    uint8_t *fpaths = malloc(per_q * nq + 1);
+    size_t *fidx = malloc(sizeof(size_t) * (nq + 1));
+    for (size_t k = 0; k < rc; k++) st4(frem + 4 * k, r_qm31(&r));
+    if (r_uint(&r, 8) != nq) r.bad = 1;
+    for (size_t q = 0; q < nq && !r.bad; q++) {
+        fidx[q] = (size_t)r_uint(&r, 8);
+        if (r_uint(&r, 8) != L) { r.bad = 1; break; }
+        uint8_t *pp = fpaths + q * per_q;
+        for (uint32_t i = 0; i < L && !r.bad; i++) {
+            st4(fvals + 4 * ((q * L + i) * 2), r_qm31(&r));
+            st4(fvals + 4 * ((q * L + i) * 2 + 1), r_qm31(&r));
+            for (int s = 0; s < 2 && !r.bad; s++) {
+                if (r_uint(&r, 8) != dlog - i) { r.bad = 1; break; }
+                const uint8_t *p = r_take(&r, 32 * (size_t)(dlog - i));
+                if (p) memcpy(pp, p, 32 * (size_t)(dlog - i));
+                pp += 32 * (size_t)(dlog - i);
+            }
+        }
+    }
+    int ok = !r.bad;
+    zo_qm31 *ac = malloc(sizeof(zo_qm31) * (D.n_lc + 1)), *ar = malloc(sizeof(zo_qm31) * (D.n_rel + 1));
+    zo_qm31 *al = malloc(sizeof(zo_qm31) * (D.n_trans_total + 1)), *be = malloc(sizeof(zo_qm31) * (D.total_bound + 1));
+    zo_qm31 *pk = malloc(sizeof(zo_qm31) * (D.total_bound + 1)), *bv = malloc(sizeof(zo_qm31) * (D.total_bound + 1));
+    uint32_t *bc = malloc(sizeof(uint32_t) * (D.total_bound + 1));
+    zo_qm31 *v = malloc(sizeof(zo_qm31) * D.reveal_len), *res = malloc(sizeof(zo_qm31) * (D.T + 1));
+    if (ok) {
+        const zo_qm31 w = zo_qm31_root_of_unity(tl), w_ev = zo_qm31_root_of_unity(dlog);
+        /* replay the transcript (stark.zig:676-705) */
+        for (uint32_t k = 0; k < D.n_pre; k++) zo_channel_absorb(chan, proots + 32 * k, 32);
+        for (uint32_t j = 0; j < D.m; j++) zo_channel_absorb(chan, troots + 32 * j, 32);
+        for (uint32_t j = 0; j < D.n_lc; j++) ac[j] = sample_q(chan);
+        for (uint32_t rr = 0; rr < D.n_rel; rr++) ar[rr] = sample_q(chan);
+        for (uint32_t rr = 0; rr < D.n_rel; rr++) zo_channel_absorb(chan, aroots + 32 * rr, 32);
+        for (uint32_t k = 0; k < D.n_trans_total; k++) al[k] = sample_q(chan);
+        for (uint32_t k = 0; k < D.B; k++) { bc[k] = air->boundary[k].column; bv[k] = ld4(air->boundary[k].value);
+                                             pk[k] = zo_qm31_pow(w, air->boundary[k].step); }
+        for (uint32_t rr = 0; rr < D.n_rel; rr++) { bc[D.B + rr] = D.m + rr; bv[D.B + rr] = Q_ZERO; pk[D.B + rr] = Q_ONE; }
+        for (uint32_t k = 0; k < D.total_bound; k++) be[k] = sample_q(chan);
+        zo_channel_absorb(chan, qroot, 32);
+        const zo_qm31 z = sample_q(chan);
+        for (uint32_t k = 0; k < D.n_deep; k++) absorb_qm31(chan, de[k]);
+        for (uint32_t k = 0; k < D.n_deep; k++) ga[k] = sample_q(chan);
+        ok = zo_fri_verify(dlog, flb, rl, nq, chan, froots, frem, fidx, fvals, fpaths);
+        stark_weights wt = {al, be, ac, ar, pk, bv, bc, zo_qm31_pow(w, n - 1)};
+        if (ok && r_uint(&r, 8) != nq) ok = 0;
+        for (size_t q = 0; q < nq && ok; q++) {
+            const size_t p0 = (size_t)r_uint(&r, 8);
+            if (p0 != fidx[q] || r_uint(&r, 8) != D.reveal_len) { ok = 0; break; }
+            for (uint32_t k = 0; k < D.reveal_len; k++) v[k] = r_qm31(&r);
+            if (r_uint(&r, 8) != D.reveal_len) { ok = 0; break; }
+            const size_t pn = (p0 + shift) % N;
+            /* Merkle checks (stark.zig:716-731) */
+            for (uint32_t k = 0; k < D.reveal_len && ok; k++) {
+                if (r_uint(&r, 8) != dlog) { ok = 0; break; }
+                const uint8_t *path = r_take(&r, 32 * (size_t)dlog), *root;
+                size_t at;
+                if (k < 2 * D.m_total) {
+                    const uint32_t j = k % D.m_total;
+                    root = j < D.m ? troots + 32 * j : aroots + 32 * (j - D.m);
+                    at = k < D.m_total ? p0 : pn;
+                } else if (k == 2 * D.m_total) {
+                    root = qroot; at = p0;
+                } else {
+                    root = proots + 32 * (k - 2 * D.m_total - 1); at = p0;
+                }
+                uint8_t leaf[32];
+                zo_hash_qm31(v[k].v, leaf);
+                if (!path || !zo_merkle_verify(root, at, leaf, path, dlog)) ok = 0;
+            }
+            if (!ok) break;
+            const zo_qm31 x0 = zo_qm31_mul(ld4(ZO_FRI_OFFSET), zo_qm31_pow(w_ev, p0));
+            /* DEEP identity against the FRI layer-0 leaf (stark.zig:733-751) */
+            const zo_qm31 inv_dz = zo_qm31_inv(zo_qm31_sub(x0, z));
+            zo_qm31 gv = Q_ZERO;
+            for (uint32_t j = 0; j < D.m_total; j++) {
+                gv = zo_qm31_add(gv, zo_qm31_mul(ga[j], zo_qm31_mul(zo_qm31_sub(v[j], de[j]), inv_dz)));
+                gv = zo_qm31_add(gv, zo_qm31_mul(ga[D.m_total + j],
+                                                 zo_qm31_mul(zo_qm31_sub(v[D.m_total + j], de[D.m_total + j]), inv_dz)));
+            }
+            gv = zo_qm31_add(gv, zo_qm31_mul(ga[2 * D.m_total],
+                                             zo_qm31_mul(zo_qm31_sub(v[2 * D.m_total], de[2 * D.m_total]), inv_dz)));
+            const size_t p1 = p0 % (N / 2);
+            const zo_qm31 leafv = ld4(fvals + 4 * ((q * L) * 2 + (p0 == p1 ? 0 : 1)));
+            if (memcmp(gv.v, leafv.v, 16) != 0) { ok = 0; break; }
+            /* Hc(x0) == Z_H(x0) * Q(x0) (stark.zig:753-779) */
+            const zo_qm31 zh = zo_qm31_sub(zo_qm31_pow(x0, n), Q_ONE);
+            const zo_qm31 hv = composition(&D, &wt, x0, v, v + D.m_total, v + 2 * D.m_total + 1, zh, res);
+            const zo_qm31 rhs = zo_qm31_mul(zh, v[2 * D.m_total]);
+            if (memcmp(hv.v, rhs.v, 16) != 0) ok = 0;
+        }
+        if (ok && (r.bad || r.pos != r.n)) ok = 0; /* truncated or trailing bytes */
+    }
+    free(de); free(ga); free(frem); free(fvals); free(fpaths); free(fidx);
+    free(ac); free(ar); free(al); free(be); free(pk); free(bv); free(bc); free(v); free(res);
+    return ok;
+}

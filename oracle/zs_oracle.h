@@ -179,6 +179,53 @@ int zo_fib_stark_verify(const uint8_t *proof, size_t len, const uint32_t a0[4], 
                         const uint32_t claimed[4], zo_channel *chan);
 void zo_free(void *p);
 
+/* ---- GenericStark(Air).proveWithPreprocessed / verify (src/m31/stark.zig:311-780) ----
+ * The AIR as data.  Transition constraint k at a point is the sum of its terms,
+ * a term being coeff * (product of its factors): current[column], next[column]
+ * or the evaluation point x (evalTransition, stark.zig:26-35).  Boundary
+ * assertions (column, step, value) are what Air.boundaryAssertions returns for
+ * the public inputs.  The LogUp metadata per relation: the selector and
+ * multiplicity trace columns and the key (trace) and table (preprocessed)
+ * column lists, relation r's at key_columns[key_start[r] .. key_start[r+1]).
+ * maxConstraintDegree(n) = degree_mul * (n - 1) + degree_add, or, with both 0,
+ * the largest term degree (n - 1 per column factor, 1 per x factor). */
+#define ZO_AIR_CUR 0u
+#define ZO_AIR_NEXT 1u
+#define ZO_AIR_X 2u
+typedef struct { uint32_t kind, column; } zo_air_factor;
+typedef struct {
+    uint32_t constraint;
+    uint32_t coeff[4];
+    uint32_t n_factors; /* 0..4 */
+    zo_air_factor factor[4];
+} zo_air_term;
+typedef struct { uint32_t column, reserved; uint64_t step; uint32_t value[4]; } zo_boundary;
+typedef struct {
+    uint32_t num_columns, num_preprocessed, num_transition_constraints, num_terms;
+    const zo_air_term *terms;
+    uint32_t degree_mul, degree_add;
+    uint32_t num_boundary, reserved;
+    const zo_boundary *boundary;
+    uint32_t num_lookup_columns, num_lookup_relations;
+    const uint32_t *selector_column, *multiplicity_column;
+    const uint32_t *key_start, *key_columns, *table_start, *table_columns;
+} zo_air;
+/* pre: num_preprocessed QM31 columns, trace: num_columns QM31 columns, each of
+ * 2^trace_log values (column-major).  Returns malloc'd GenericStark.Proof wire
+ * bytes (free with zo_free) or NULL when the parameters are not ones the
+ * reference prover accepts (friParams' assertions). */
+uint8_t *zo_stark_prove(const zo_air *air, uint32_t trace_log, uint32_t log_blowup, size_t num_queries,
+                        uint32_t remainder_log, const uint32_t *pre, const uint32_t *trace, zo_channel *chan,
+                        size_t *out_len);
+int zo_stark_verify(const zo_air *air, const uint8_t *proof, size_t len, zo_channel *chan);
+/* FRI parameters of the DEEP polynomial (friParams, stark.zig:208-222): log_size and FRI log_blowup */
+int zo_stark_fri_params(const zo_air *air, uint32_t trace_log, uint32_t log_blowup, uint32_t *fri_log_size,
+                        uint32_t *fri_log_blowup);
+/* the accumulator columns on H (stark.zig:430-449) for the given lookup challenges:
+ * out = num_lookup_relations QM31 columns of 2^trace_log values */
+void zo_stark_accumulators(const zo_air *air, uint32_t trace_log, const uint32_t *pre, const uint32_t *trace,
+                           const uint32_t *alpha_cols, const uint32_t *alpha_rels, uint32_t *out);
+
 /* ---- seeded synthetic inputs ------------------------------------------------ */
 /* SplitMix64 stream; M31 = 31 random bits with 0x7FFFFFFF rejected */
 void zo_fill_m31(uint64_t seed, uint32_t *out, size_t n);

@@ -109,6 +109,12 @@ def _declare(L):
     L.zo_free.argtypes = [ctypes.c_void_p]
     L.zo_wire_golden_example.restype = sz
     L.zo_wire_golden_example.argtypes = [ctypes.c_char_p]
+    L.zo_stark_prove.restype = ctypes.c_void_p
+    L.zo_stark_prove.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, sz, ctypes.c_uint32, u32p, u32p,
+                                 ctypes.c_void_p, ctypes.POINTER(sz)]
+    L.zo_stark_verify.argtypes = [ctypes.c_void_p, u8p, sz, ctypes.c_void_p]
+    L.zo_stark_fri_params.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, u32p, u32p]
+    L.zo_stark_accumulators.argtypes = [ctypes.c_void_p, ctypes.c_uint32, u32p, u32p, u32p, u32p, u32p]
     L.zo_fri_proof_serialize.restype = sz
     L.zo_fri_proof_serialize.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, sz, u8p, u32p,
                                          ctypes.POINTER(sz), u32p, u8p, ctypes.c_char_p]
@@ -532,3 +538,46 @@ def fib_stark_prove(trace, log_blowup, num_queries, remainder_log, a0, a1, claim
 def fib_stark_verify(proof, a0, a1, claimed, channel):
     q = lambda v: _p(u32(v).reshape(4))
     return bool(lib().zo_fib_stark_verify(bytes(proof), len(proof), q(a0), q(a1), q(claimed), channel.h))
+
+
+# ---- GenericStark(Air) for an AIR given as data (tests/airs.py) -------------------
+def stark_prove(air, pre, trace, log_blowup, num_queries, remainder_log, channel):
+    """proveWithPreprocessed (stark.zig:311-657) -> GenericStark.Proof wire bytes, or None
+    when friParams rejects the parameters.  pre / trace: (cols, n, 4) QM31 arrays."""
+    trace = u32(trace)
+    tl = trace.shape[1].bit_length() - 1
+    pre = u32(pre) if pre is not None and len(pre) else np.zeros(4, np.uint32)
+    st = air.struct()
+    n = sz()
+    ptr = lib().zo_stark_prove(ctypes.byref(st), tl, log_blowup, num_queries, remainder_log, _p(pre.reshape(-1)),
+                               _p(trace.reshape(-1)), channel.h, ctypes.byref(n))
+    if not ptr:
+        return None
+    try:
+        return ctypes.string_at(ptr, n.value)
+    finally:
+        lib().zo_free(ptr)
+
+
+def stark_verify(air, proof, channel):
+    st = air.struct()
+    return bool(lib().zo_stark_verify(ctypes.byref(st), bytes(proof), len(proof), channel.h))
+
+
+def stark_fri_params(air, trace_log, log_blowup):
+    a, b = ctypes.c_uint32(), ctypes.c_uint32()
+    st = air.struct()
+    rc = lib().zo_stark_fri_params(ctypes.byref(st), trace_log, log_blowup, ctypes.byref(a), ctypes.byref(b))
+    return (a.value, b.value) if rc == 0 else None
+
+
+def stark_accumulators(air, pre, trace, alpha_cols, alpha_rels):
+    trace = u32(trace)
+    n = trace.shape[1]
+    tl = n.bit_length() - 1
+    pre = u32(pre) if pre is not None and len(pre) else np.zeros(4, np.uint32)
+    st = air.struct()
+    out = np.zeros((max(1, len(air.lookups)), n, 4), np.uint32)
+    lib().zo_stark_accumulators(ctypes.byref(st), tl, _p(pre.reshape(-1)), _p(trace.reshape(-1)),
+                                _p(u32(alpha_cols).reshape(-1)), _p(u32(alpha_rels).reshape(-1)), _p(out))
+    return out[:len(air.lookups)]

@@ -88,6 +88,9 @@ def _declare(L):
     L.zs_m31_circle_ifft_batch_dev.argtypes = [vp, vp, vp, u32, u32, sz]
     L.zs_m31_lde_commit_dev.argtypes = [vp, vp, u32, u32, u32, vp, vp, u32]
     L.zs_m31_lde_commit_dev_noalloc.argtypes = [vp, vp, u32, u32, u32, vp, vp]
+    L.zs_stark_prove_dev.argtypes = [vp, vp, vp, vp, u32, u32, sz, u32, vp, HostAllocator, ctypes.POINTER(vp),
+                                     ctypes.POINTER(sz)]
+    L.zs_stark_verify.argtypes = [ctypes.c_char_p, sz, vp, vp, ctypes.POINTER(ctypes.c_uint8)]
     L.zs_m31_lde_commit.argtypes = [vp, u32p, u32, u32, u32, ctypes.POINTER(vp), ctypes.c_char_p, u32p]
     L.zs_m31_row_commit_dev.argtypes = [vp, vp, u32, u32, sz, vp, u32]
     L.zs_m31_lde_dev.argtypes = [vp, vp, u32, u32, u32, vp]
@@ -146,7 +149,7 @@ def _declare(L):
                  "zs_fri_read_layer", "zs_fill_m31_dev", "zs_hip_profile", "zs_hip_profile_read",
                  "zs_commit_job_dev", "zs_commit_batch_dev", "zs_server_submit", "zs_server_wait",
                  "zs_server_memory", "zs_server_poll", "zs_qm31_trace_commit_dev", "zs_fib_trace", "zs_autotune_lde", "zs_fri_prove_dev", "zs_fib_stark_prove_dev",
-                 "zs_fri_verify", "zs_fib_stark_verify"]:
+                 "zs_fri_verify", "zs_fib_stark_verify", "zs_stark_prove_dev", "zs_stark_verify"]:
         getattr(L, name).restype = i32
 
 
@@ -298,6 +301,15 @@ class Response(ctypes.Structure):
         rc = 1 << (remainder_log - log_blowup)
         return (bytes(self.trace_root), [bytes(self.fri_roots[32 * i:32 * (i + 1)]) for i in range(L)],
                 np.array(self.remainder[:4 * rc], np.uint32).reshape(rc, 4))
+
+
+def stark_verify(proof, air, channel):
+    """GenericStark(Air).verify (src/m31/stark.zig:660-780) of GenericStark.Proof wire
+    bytes on the host; `air` is a zs_air ctypes structure.  True or False;
+    ZsError(ZS_ERR_INPUT) for bytes that do not deserialize."""
+    ok = ctypes.c_uint8()
+    _check(lib().zs_stark_verify(bytes(proof), len(proof), ctypes.byref(air), channel.h, ctypes.byref(ok)))
+    return bool(ok.value)
 
 
 def fri_verify(proof, channel):
@@ -548,6 +560,18 @@ class Context:
         ptr, n = ctypes.c_void_p(), sz()
         self.check(lib().zs_fri_prove_dev(self.h, d_codeword, log_size, log_blowup, remainder_log, num_queries,
                                           channel.h, HOST, ctypes.byref(ptr), ctypes.byref(n)))
+        try:
+            return ctypes.string_at(ptr.value, n.value)
+        finally:
+            lib().zs_free(HOST, ptr, n)
+
+    def stark_prove_dev(self, air, d_pre, d_trace, trace_log, log_blowup, num_queries, remainder_log, channel):
+        """GenericStark(Air).proveWithPreprocessed on device columns (`air`: a zs_air ctypes
+        structure; d_pre: device pointer or None); returns the wire bytes."""
+        ptr, n = ctypes.c_void_p(), sz()
+        self.check(lib().zs_stark_prove_dev(self.h, ctypes.byref(air), d_pre, d_trace, trace_log, log_blowup,
+                                            num_queries, remainder_log, channel.h, HOST, ctypes.byref(ptr),
+                                            ctypes.byref(n)))
         try:
             return ctypes.string_at(ptr.value, n.value)
         finally:

@@ -119,6 +119,8 @@ struct DeviceCaches {
     std::map<uint64_t, std::unique_ptr<DevBuf>> remainder_mats;
     // launch configurations chosen by zs_autotune_lde, keyed by lde_shape_key
     std::map<uint64_t, LdeTiles> tuned;
+    // a commit job's starting channel state (Channel.init("zig-stark:hip:commit")) on the device
+    std::unique_ptr<DevBuf> commit_channel;
 };
 
 struct Ctx {

@@ -46,10 +46,39 @@ const B3Hasher &commit_channel() {
 // trace root and draws gamma; g = sum_c gamma^c lde_c; FRI on g with challenges
 // drawn from the same channel.  Scratch: 7 channel state, 8 the codeword g,
 // 9 gamma and its powers; FRI uses 4-6 and the LDE 1-2.
-// With `stage` (pinned host memory of kStageBytes) the call returns without
-// synchronising: the trace root, then the FRI roots, then the remainder words
-// are copied there on the stream, and trace_root / fri_roots / remainder are
-// unused; the caller synchronises the stream before reading them.
+// the starting channel state on the device, built once per cache (outside its lock, like
+// the twiddle tables) so each job starts its channel with a device-to-device copy
+int32_t device_commit_channel(Ctx *ctx, const B3Hasher **out) {
+    DeviceCaches &dc = *ctx->caches;
+    {
+        std::lock_guard<std::mutex> lk(dc.mu);
+        if (dc.commit_channel) {
+            *out = dc.commit_channel->as<B3Hasher>();
+            return ZS_OK;
+        }
+    }
+    auto buf = std::make_unique<DevBuf>();
+    ZS_HIP(ctx, hipMalloc(&buf->ptr, sizeof(B3Hasher)));
+    buf->bytes = sizeof(B3Hasher);
+    ZS_HIP(ctx, hipMemcpyAsync(buf->ptr, &commit_channel(), sizeof(B3Hasher), hipMemcpyHostToDevice, ctx->stream));
+    ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::lock_guard<std::mutex> lk(dc.mu);
+    if (!dc.commit_channel) dc.commit_channel = std::move(buf);
+    *out = dc.commit_channel->as<B3Hasher>();
+    return ZS_OK;
+}
+
+// staged results of one job: trace root, at most 28 FRI roots, at most 2^kMaxRemainderLog
+// remainder coefficients
+constexpr size_t kStageBytes = 32 + 28 * 32 + (size_t(16) << kMaxRemainderLog);
+
+// The results go to host memory through a pinned stage (the trace root, then the
+// FRI roots, then the remainder words), so no copy into pageable memory blocks the
+// host while the chain is being queued.  With `stage` (pinned, kStageBytes) given,
+// the call returns without synchronising and trace_root / fri_roots / remainder
+// are unused: the caller synchronises the stream before reading the stage.
+// Without it the context's own pinned buffer is the stage, and the call returns
+// with the results in trace_root / fri_roots / remainder.
 int32_t commit_job(zs_hip_ctx *h, const uint32_t *d_cols, uint32_t n_cols, uint32_t log_n, uint32_t log_blowup,
                    uint32_t remainder_log, uint32_t *d_lde, uint8_t *d_nodes, uint8_t trace_root[32],
                    uint8_t *fri_roots, uint32_t *remainder, uint32_t flags, uint8_t *stage = nullptr) {
@@ -67,29 +96,39 @@ int32_t commit_job(zs_hip_ctx *h, const uint32_t *d_cols, uint32_t n_cols, uint3
     ZS_TRY(ctx->ensure_scratch(8, N * 16));
     ZS_TRY(ctx->ensure_scratch(9, (size_t(n_cols) + 1) * 16));
     B3Hasher *d_chan = ctx->scratch[7].as<B3Hasher>();
+    const B3Hasher *d_init;
+    ZS_TRY(device_commit_channel(ctx, &d_init));
     uint32_t *d_gamma = ctx->scratch[9].as<uint32_t>(), *d_powers = d_gamma + 4;
     if (flags & ZS_FLAG_PLAIN) {
         ZS_TRY(zs_m31_lde_commit_dev(h, d_cols, n_cols, log_n, log_blowup, d_lde, d_nodes, ZS_FLAG_PLAIN));
-        ZS_HIP(ctx, hipMemcpyAsync(d_chan, &commit_channel(), sizeof(B3Hasher), hipMemcpyHostToDevice, ctx->stream));
+        ZS_HIP(ctx, hipMemcpyAsync(d_chan, d_init, sizeof(B3Hasher), hipMemcpyDeviceToDevice, ctx->stream));
         ZS_TRY(launch_channel_draw(ctx, d_chan, d_root, d_gamma));
     } else {
         // LDE, row leaves, then the node levels, whose root launch also draws gamma
         ZS_TRY(zs_m31_lde_dev(h, d_cols, n_cols, log_n, log_blowup, d_lde));
-        ZS_HIP(ctx, hipMemcpyAsync(d_chan, &commit_channel(), sizeof(B3Hasher), hipMemcpyHostToDevice, ctx->stream));
+        ZS_HIP(ctx, hipMemcpyAsync(d_chan, d_init, sizeof(B3Hasher), hipMemcpyDeviceToDevice, ctx->stream));
         ZS_TRY(launch_row_leaves(ctx, d_lde, N, n_cols, N, nodes));
         ZS_TRY(launch_tree_levels(ctx, nodes, LT, 0, d_chan, d_gamma));
     }
     ZS_TRY(launch_qm31_powers(ctx, d_gamma, d_powers, n_cols));
     ZS_TRY(launch_combine(ctx, d_lde, N, n_cols, N, d_powers, ctx->scratch[8].as<uint32_t>()));
-    ZS_HIP(ctx, hipMemcpyAsync(stage ? stage : trace_root, d_root, 32, hipMemcpyDeviceToHost, ctx->stream));
-    // without a stage the FRI call ends with a synchronisation that also covers the trace root copy
-    return fri_commit_impl(h, ctx->scratch[8].as<uint32_t>(), LT, log_blowup, remainder_log, nullptr, d_chan,
-                           flags & ZS_FLAG_NO_GRAPH, fri_roots, remainder, nullptr, stage ? stage + 32 : nullptr);
+    const bool own_stage = stage == nullptr;
+    if (own_stage) {
+        ZS_TRY(ctx->ensure_pinned(kStageBytes));
+        stage = static_cast<uint8_t *>(ctx->pinned);
+    }
+    ZS_HIP(ctx, hipMemcpyAsync(stage, d_root, 32, hipMemcpyDeviceToHost, ctx->stream));
+    ZS_TRY(fri_commit_impl(h, ctx->scratch[8].as<uint32_t>(), LT, log_blowup, remainder_log, nullptr, d_chan,
+                           flags & ZS_FLAG_NO_GRAPH, nullptr, nullptr, nullptr, stage + 32));
+    if (!own_stage) return ZS_OK;
+    ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const uint32_t L = LT - remainder_log;
+    std::memcpy(trace_root, stage, 32);
+    std::memcpy(fri_roots, stage + 32, size_t(L) * 32);
+    std::memcpy(remainder, stage + 32 + size_t(L) * 32, (size_t(16) << (remainder_log - log_blowup)));
+    return ZS_OK;
 }
 
-// staged results of one job: trace root, at most 28 FRI roots, at most 2^kMaxRemainderLog
-// remainder coefficients
-constexpr size_t kStageBytes = 32 + 28 * 32 + (size_t(16) << kMaxRemainderLog);
 
 // Device memory of one commit job: its input, LDE and row tree, and every scratch
 // buffer commit_job uses (slot numbers of Ctx::scratch), each 256-byte aligned.

@@ -5,8 +5,10 @@
 // layers hash paths and folds), so it stays on the CPU; it shares the field,
 // BLAKE3 and channel code with the device path.
 #include <cstring>
+#include <memory>
 #include <vector>
 
+#include "air.h"
 #include "internal.h"
 
 namespace {
@@ -208,13 +210,7 @@ bool fri_verify(const FriParams &fp, const FriProof &pf, zs_channel *chan) {
     return true;
 }
 
-// ---- GenericStark(FibSeededAir) (stark.zig:107-130 Proof, :660-780 verify) ----
-// FibAir: 2 trace columns, transitions next[0] - cur[1], next[1] - cur[0] - cur[1]
-// (stark.zig:886-891); the seeded boundary (0, 0) = a0, (1, 0) = a1,
-// (0, n - 1) = claimed.  No preprocessed columns, no lookups: 2 * 2 + 1 DEEP
-// values and reveals per query (current row, next row, quotient).
-constexpr uint32_t FIB_M = 2, FIB_NDEEP = 2 * FIB_M + 1;
-
+// ---- GenericStark.Proof (stark.zig:107-130) ----
 struct Reveal {
     uint64_t index;
     std::vector<qm31> values;
@@ -261,32 +257,35 @@ bool parse_stark(const uint8_t *bytes, size_t n, StarkProof &sp) {
     return r.ok && r.left == 0;
 }
 
-qm31 qm_of(const uint32_t w[4]) {
-    uint32_t x[4];
-    for (int k = 0; k < 4; k++) x[k] = m31_from_u32(w[k]);
-    return qm_from_words(x);
-}
-
-bool stark_verify(const StarkProof &sp, const qm31 a0, const qm31 a1, const qm31 claimed, zs_channel *chan) {
+// GenericStark(Air).verify (stark.zig:660-780) for an AIR given as data
+bool stark_verify_air(const StarkProof &sp, const AirDesc &d, zs_channel *chan) {
     const FriParams &pp = sp.params;
     const uint32_t tl = pp.log_size, dlog = pp.log_size + pp.log_blowup;
+    uint32_t fls, flb;
+    if (tl < 1 || dlog > 24 || !air_fri_params(d, tl, pp.log_blowup, &fls, &flb)) return false;
+    const FriParams fp{fls, flb, pp.remainder_log, pp.num_queries};  // friParams(params)
+    if (!fp.ok()) return false;
     const uint64_t n = uint64_t(1) << tl, N = uint64_t(1) << dlog, shift = uint64_t(1) << pp.log_blowup;
-    if (sp.trace_roots.size() != FIB_M || sp.deep_evals.size() != FIB_NDEEP) return false;
-    const qm31 w = qm31_root_of_unity(tl), w_ev = qm31_root_of_unity(dlog);
-    const qm31 one = {{1, 0}, {0, 0}}, zero = {{0, 0}, {0, 0}};
+    const uint32_t mt = d.m_total;
+    if (sp.trace_roots.size() != d.m || sp.deep_evals.size() != d.n_deep) return false;
+    if (sp.pre_roots.size() != d.n_pre || sp.acc_roots.size() != d.n_rel) return false;
+    const qm31 w = qm31_root_of_unity(tl), w_ev = qm31_root_of_unity(dlog), one = {{1, 0}, {0, 0}};
     auto sample = [&]() {
         uint32_t a[4];
         zs_channel_sample_qm31(chan, a);
         return qm_from_words(a);
     };
     // the transcript in the order the prover wrote it (stark.zig:676-705)
-    for (const Digest &d : sp.trace_roots) zs_channel_absorb(chan, d.b, 32);
-    qm31 alphas[2], betas[3], gammas[FIB_NDEEP];
-    for (auto &a : alphas) a = sample();
-    const uint32_t b_col[3] = {0, 1, 0};
-    const uint64_t b_step[3] = {0, 0, n - 1};
-    const qm31 b_val[3] = {a0, a1, claimed};
-    for (auto &b : betas) b = sample();
+    AirWeights wt{};
+    for (const Digest &dg : sp.pre_roots) zs_channel_absorb(chan, dg.b, 32);
+    for (const Digest &dg : sp.trace_roots) zs_channel_absorb(chan, dg.b, 32);
+    for (uint32_t j = 0; j < d.n_lc; j++) wt.alpha_cols[j] = sample();
+    for (uint32_t r = 0; r < d.n_rel; r++) wt.alpha_rels[r] = sample();
+    for (const Digest &dg : sp.acc_roots) zs_channel_absorb(chan, dg.b, 32);
+    for (uint32_t k = 0; k < d.n_trans_total; k++) wt.alpha[k] = sample();
+    for (uint32_t k = 0; k < d.total_bound; k++) wt.pk[k] = qm_pow(w, d.bstep[k]);
+    for (uint32_t k = 0; k < d.total_bound; k++) wt.beta[k] = sample();
+    wt.last = qm_pow(w, n - 1);
     zs_channel_absorb(chan, sp.quotient_root.b, 32);
     const qm31 z = sample();
     for (const qm31 &v : sp.deep_evals) {
@@ -294,60 +293,71 @@ bool stark_verify(const StarkProof &sp, const qm31 a0, const qm31 a1, const qm31
         qm_to_words(v, wds);
         zs_channel_absorb(chan, reinterpret_cast<const uint8_t *>(wds), 16);
     }
-    for (auto &g : gammas) g = sample();
-    // FRI under the STARK's parameters (friParams: the same four fields).  The
-    // reference ignores the nested proof's own copy of them; an honest prover
-    // writes the same values, so a proof whose copy differs is rejected here.
-    const FriProof &fp = sp.fri;
-    if (fp.log_size != pp.log_size || fp.log_blowup != pp.log_blowup || fp.num_queries != pp.num_queries ||
-        fp.remainder_log != pp.remainder_log)
+    std::vector<qm31> gam(d.n_deep);
+    for (auto &g : gam) g = sample();
+    // FRI under friParams; the nested proof's own copy of them must be the same
+    const FriProof &fr = sp.fri;
+    if (fr.log_size != fls || fr.log_blowup != flb || fr.num_queries != pp.num_queries ||
+        fr.remainder_log != pp.remainder_log)
         return false;
-    if (!fri_verify(pp, sp.fri, chan)) return false;
-    // one reveal per FRI query (the reference walks the reveals the proof holds;
-    // the prover writes one per query)
-    if (sp.queries.size() != sp.fri.queries.size()) return false;
-    const qm31 last = qm_pow(w, n - 1);
-    qm31 pk[3];
-    for (int k = 0; k < 3; k++) pk[k] = qm_pow(w, b_step[k]);
+    if (!fri_verify(fp, fr, chan)) return false;
+    if (sp.queries.size() != fr.queries.size()) return false;
+    const uint32_t R = d.reveal_len;
     for (size_t qi = 0; qi < sp.queries.size(); qi++) {
         const Reveal &qv = sp.queries[qi];
-        if (qv.index != sp.fri.queries[qi].index) return false;
-        if (qv.values.size() != FIB_NDEEP || qv.paths.size() != FIB_NDEEP) return false;
+        if (qv.index != fr.queries[qi].index) return false;
+        if (qv.values.size() != R || qv.paths.size() != R) return false;
         const uint64_t p0 = qv.index, pn = (p0 + shift) % N;
         const qm31 *v = qv.values.data();
-        uint32_t leaf[8];
-        for (uint32_t j = 0; j < FIB_M; j++) {
-            hash_qm31(v[j], leaf);
-            if (!merkle_verify(sp.trace_roots[j], p0, leaf, qv.paths[j])) return false;
-            hash_qm31(v[FIB_M + j], leaf);
-            if (!merkle_verify(sp.trace_roots[j], pn, leaf, qv.paths[FIB_M + j])) return false;
+        // Merkle checks (stark.zig:716-731)
+        for (uint32_t k = 0; k < R; k++) {
+            const Digest *root;
+            uint64_t at = p0;
+            if (k < 2 * mt) {
+                const uint32_t j = k % mt;
+                root = j < d.m ? &sp.trace_roots[j] : &sp.acc_roots[j - d.m];
+                if (k >= mt) at = pn;
+            } else if (k == 2 * mt) {
+                root = &sp.quotient_root;
+            } else {
+                root = &sp.pre_roots[k - 2 * mt - 1];
+            }
+            uint32_t leaf[8];
+            hash_qm31(v[k], leaf);
+            if (!merkle_verify(*root, at, leaf, qv.paths[k])) return false;
         }
-        hash_qm31(v[2 * FIB_M], leaf);
-        if (!merkle_verify(sp.quotient_root, p0, leaf, qv.paths[2 * FIB_M])) return false;
         const qm31 x0 = qm_mul(fri_offset_pow(0), qm_pow(w_ev, p0));
-        // DEEP identity: g(x0) equals the FRI layer-0 value at p0
+        // DEEP identity: g(x0) equals the FRI layer-0 value at p0 (stark.zig:733-751)
         const qm31 inv_dz = qm_inv(qm_sub(x0, z));
-        qm31 g = zero;
-        for (uint32_t k = 0; k < FIB_NDEEP; k++)
-            g = qm_add(g, qm_mul(gammas[k], qm_mul(qm_sub(v[k], sp.deep_evals[k]), inv_dz)));
-        if (sp.fri.queries[qi].pairs.empty()) return false;
-        const auto &pair0 = sp.fri.queries[qi].pairs[0];
-        if (!qm_eq(g, p0 == p0 % (N / 2) ? pair0.v0 : pair0.v1)) return false;
-        // constraints: Hc(x0) == Z_H(x0) Q(x0)
-        const qm31 zh = qm_sub(qm_pow(x0, n), one);
-        const qm31 res[2] = {qm_sub(v[2], v[1]), qm_sub(qm_sub(v[3], v[0]), v[1])};
-        qm31 h = zero;
-        for (int k = 0; k < 2; k++) h = qm_add(h, qm_mul(alphas[k], qm_mul(res[k], qm_sub(x0, last))));
-        for (int k = 0; k < 3; k++) {
-            const qm31 term = qm_mul(qm_mul(qm_sub(v[b_col[k]], b_val[k]), zh), qm_inv(qm_sub(x0, pk[k])));
-            h = qm_add(h, qm_mul(betas[k], term));
+        qm31 g = {{0, 0}, {0, 0}};
+        for (uint32_t j = 0; j < mt; j++) {
+            g = qm_add(g, qm_mul(gam[j], qm_mul(qm_sub(v[j], sp.deep_evals[j]), inv_dz)));
+            g = qm_add(g, qm_mul(gam[mt + j], qm_mul(qm_sub(v[mt + j], sp.deep_evals[mt + j]), inv_dz)));
         }
-        if (!qm_eq(h, qm_mul(zh, v[2 * FIB_M]))) return false;
+        g = qm_add(g, qm_mul(gam[2 * mt], qm_mul(qm_sub(v[2 * mt], sp.deep_evals[2 * mt]), inv_dz)));
+        if (fr.queries[qi].pairs.empty()) return false;
+        const auto &pair0 = fr.queries[qi].pairs[0];
+        if (!qm_eq(g, p0 == p0 % (N / 2) ? pair0.v0 : pair0.v1)) return false;
+        // constraints: Hc(x0) == Z_H(x0) Q(x0) (stark.zig:753-779)
+        const qm31 zh = qm_sub(qm_pow(x0, n), one);
+        const qm31 hv = air_composition_host(d, wt, x0, v, v + mt, v + 2 * mt + 1, zh);
+        if (!qm_eq(hv, qm_mul(zh, v[2 * mt]))) return false;
     }
     return true;
 }
 
 }  // namespace
+
+extern "C" int32_t zs_stark_verify(const uint8_t *proof, size_t proof_len, const zs_air *air, zs_channel *chan,
+                                   uint8_t *out_ok) {
+    if (!chan || !out_ok || !air || (!proof && proof_len)) return ZS_ERR_INPUT;
+    auto d = std::make_unique<AirDesc>();
+    ZS_TRY(air_from_abi(air, d.get()));
+    StarkProof sp;
+    if (!parse_stark(proof, proof_len, sp)) return ZS_ERR_INPUT;
+    *out_ok = stark_verify_air(sp, *d, chan) ? 1 : 0;
+    return ZS_OK;
+}
 
 extern "C" int32_t zs_fri_verify(const uint8_t *proof, size_t proof_len, zs_channel *chan, uint8_t *out_ok) {
     if (!chan || !out_ok || (!proof && proof_len)) return ZS_ERR_INPUT;
@@ -365,9 +375,41 @@ extern "C" int32_t zs_fib_stark_verify(const uint8_t *proof, size_t proof_len, c
     if (!chan || !out_ok || !a0 || !a1 || !claimed || (!proof && proof_len)) return ZS_ERR_INPUT;
     StarkProof sp;
     if (!parse_stark(proof, proof_len, sp)) return ZS_ERR_INPUT;
-    *out_ok = sp.params.ok() && sp.params.log_size >= 1 &&
-                      stark_verify(sp, qm_of(a0), qm_of(a1), qm_of(claimed), chan)
-                  ? 1
-                  : 0;
+    // FibAir (stark.zig:877-904) as data with the seeded boundary; the trace length
+    // comes from the proof's parameters
+    const uint32_t tl = sp.params.log_size;
+    if (!sp.params.ok() || tl < 1 || tl > 24) {
+        *out_ok = 0;
+        return ZS_OK;
+    }
+    zs_air_term terms[5] = {};
+    const uint32_t con[5] = {0, 0, 1, 1, 1}, kind[5] = {ZS_AIR_NEXT, ZS_AIR_CUR, ZS_AIR_NEXT, ZS_AIR_CUR, ZS_AIR_CUR};
+    const uint32_t col[5] = {0, 1, 1, 0, 1}, coef[5] = {1, P - 1, 1, P - 1, P - 1};
+    for (int t = 0; t < 5; t++) {
+        terms[t].constraint = con[t];
+        terms[t].coeff[0] = coef[t];
+        terms[t].n_factors = 1;
+        terms[t].factor[0] = {kind[t], col[t]};
+    }
+    zs_boundary bnd[3] = {};
+    const uint32_t *vals[3] = {a0, a1, claimed};
+    const uint32_t bcol[3] = {0, 1, 0};
+    const uint64_t bstep[3] = {0, 0, (uint64_t(1) << tl) - 1};
+    for (int k = 0; k < 3; k++) {
+        bnd[k].column = bcol[k];
+        bnd[k].step = bstep[k];
+        std::memcpy(bnd[k].value, vals[k], 16);
+    }
+    zs_air air{};
+    air.num_columns = 2;
+    air.num_transition_constraints = 2;
+    air.num_terms = 5;
+    air.terms = terms;
+    air.degree_mul = 1;
+    air.num_boundary = 3;
+    air.boundary = bnd;
+    auto d = std::make_unique<AirDesc>();
+    ZS_TRY(air_from_abi(&air, d.get()));
+    *out_ok = stark_verify_air(sp, *d, chan) ? 1 : 0;
     return ZS_OK;
 }
