@@ -224,6 +224,46 @@ def stark_leg(Z, ctx, torch, dev, with_cpu):
     return out
 
 
+def logup_leg(Z, ctx, torch, dev, with_cpu):
+    """GenericStark with a LogUp lookup: the reference's range-check AIR (a value
+    column looked up in a preprocessed table; stark.zig:957-1040) at 2^10, 2^16 and
+    2^20 rows, blowup 4, 20 queries, remainder 2^3.  Device proof time; every proof
+    verified on the host; at 2^10 the oracle's prover (the reference's O(n^2)
+    interpolation) timed on one host thread and its bytes compared."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import airs
+    air = airs.range_check_air()
+    st = air.struct()
+    out = {"workload": "range-check AIR (3 trace columns, 1 preprocessed table column, one LogUp relation), "
+                       "blowup 4, 20 queries, remainder 2^3", "sizes": {}}
+    for tl in (10, 16, 20):
+        trace, table = airs.range_check_trace(1 << tl)
+        d_t = torch.from_numpy(trace.reshape(-1).view("int32")).to(dev)
+        d_p = torch.from_numpy(table.reshape(-1).view("int32")).to(dev)
+
+        def prove():
+            return ctx.stark_prove_dev(st, d_p.data_ptr(), d_t.data_ptr(), tl, 2, 20, 3, Z.Channel(b"zig-stark:lookup"))
+
+        proof = prove()
+        reps = 5 if tl < 20 else 2
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            prove()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        rec = {"prove_ms": round(ms, 3), "proof_bytes": len(proof),
+               "verifies_on_host": Z.stark_verify(proof, st, Z.Channel(b"zig-stark:lookup"))}
+        if with_cpu and tl == 10:
+            import oracle_lib as O
+            t0 = time.perf_counter()
+            want = O.stark_prove(air, table, trace, 2, 20, 3, O.Channel(b"zig-stark:lookup"))
+            cpu_s = time.perf_counter() - t0
+            rec.update({"cpu_oracle_prove_s": round(cpu_s, 3), "proof_bit_exact_vs_cpu": want == proof,
+                        "speedup_vs_cpu": round(cpu_s * 1e3 / ms, 1),
+                        "cpu_kind": "port (stark.zig prover restated in C, O(n^2) interpolation, 1 thread)"})
+        out["sizes"][f"2^{tl}"] = rec
+    return out
+
+
 def fill_seed_value(seed):
     """First value of the seeded uniform M31 stream (the library's zs_fill_m31_dev)."""
     import torch
@@ -568,6 +608,7 @@ def main():
     full = None if args.no_fri else full_commit_leg(Z, ctx, d_cols, n_cols, lg, beta, d_lde, d_nodes,
                                                     args.plain_steps)
     stark = stark_leg(Z, ctx, torch, dev, not args.no_cpu)
+    logup = logup_leg(Z, ctx, torch, dev, not args.no_cpu)
     batch = (batch_leg(Z, torch, dev, stream, args.batch_traces, not args.no_autotune)
              if args.batch_traces > 0 else None)
     serve = serve_leg(Z, args.serve_requests) if args.serve_requests > 0 else None
@@ -617,6 +658,7 @@ def main():
         "fri": fri,
         "full_commit": full,
         "stark": stark,
+        "logup_stark": logup,
         "batch": batch,
         "serve": serve,
         "root": root_fused.hex(),

@@ -88,3 +88,21 @@ def test_prover_rejects_parameters_and_airs(zctx, torch_dev):
         gpu_prove(zctx, torch_dev, airs.Air(3, lookups=[(2, 1, [0], [0])], num_lookup_columns=1), trace, None, 3, 4,
                   3, LABEL)
     assert e.value.code == Z.ZS_ERR_INPUT
+
+
+@pytest.mark.parametrize("name,tl", [("range_check", 13), ("and_table", 14), ("range_check", 16)])
+def test_multi_tile_traces_verify(zctx, torch_dev, name, tl):
+    """Above 2^12 rows the accumulator scan and the evaluations at z run over
+    several tiles (2048 and 4096 rows each); the oracle's O(n^2) interpolation is
+    too slow to compare bytes here, so the host verifier (itself pinned on the
+    oracle's proofs) decides: an honest trace's proof verifies, and one looked-up
+    value missing from the table in the last tile gives a proof that does not."""
+    mk_air, mk_trace = airs.LOOKUP_AIRS[name]
+    air = mk_air()
+    trace, table = mk_trace(1 << tl)
+    got, _ = gpu_prove(zctx, torch_dev, air, trace, table, 2, 20, 3, LABEL)
+    assert Z.stark_verify(got, air.struct(), Z.Channel(LABEL))
+    bad = trace.copy()
+    bad[0, (1 << (tl - 1)) - 3, 0] = 9  # a selected row of the last tile of the lookup half
+    got, _ = gpu_prove(zctx, torch_dev, air, bad, table, 2, 20, 3, LABEL)
+    assert not Z.stark_verify(got, air.struct(), Z.Channel(LABEL))

@@ -2,8 +2,9 @@
 // gfx950 (src/m31/stark.zig:311-657): preprocessed columns, the synthesized LogUp
 // accumulators (stark.zig:416-468), the quotient with every transition,
 // lookup, selector-binarity and boundary term (:485-531) and the DEEP codeword
-// (:568-580) on the device; the transcript, the O(n) evaluations at z and the
-// proof bytes on the host.
+// (:568-580) on the device, and so are the evaluations of every committed
+// column at z and w z (from interpolation coefficients kept on the device); the
+// transcript, the composition at z and the proof bytes on the host.
 //
 // Inversions: the reference inverts per element (alpha - key, alpha - tkey per row
 // and relation; x - p_k per boundary assertion and x - z per point of D).  Here
@@ -201,31 +202,121 @@ __global__ __launch_bounds__(256) void logup_contrib_kernel(const AirDesc *__res
 }
 
 // acc_r[0] = 0, acc_r[i] = acc_r[i-1] + contribution_r(i-1) (stark.zig:436-449): an
-// exclusive prefix sum, one workgroup per relation
-constexpr uint32_t kScanThreads = 1024;
-__global__ __launch_bounds__(kScanThreads) void acc_scan_kernel(const uint32_t *contrib, uint32_t n, uint32_t *acc) {
-    __shared__ uint32_t sums[4 * kScanThreads];
-    const uint32_t r = blockIdx.x, t = threadIdx.x;
-    const uint32_t per = (n + kScanThreads - 1) / kScanThreads;
-    const uint32_t b0 = min(n, t * per), b1 = min(n, b0 + per);
+// exclusive prefix sum per relation in three launches -- tile sums, an exclusive
+// scan of the tile sums, each tile rescanned from its offset.  Field addition is
+// exact, so any grouping gives the reference's values.
+constexpr uint32_t kScanThreads = 256, kScanPer = 8, kScanTile = kScanThreads * kScanPer;
+constexpr uint32_t kScanSumsThreads = 1024;
+
+// the sum of tile b of relation r: thread t adds elements t, t + 256, ... of the tile
+__global__ __launch_bounds__(kScanThreads) void acc_tile_sums_kernel(const uint32_t *contrib, uint32_t n,
+                                                                     uint32_t nb, uint32_t *sums) {
+    __shared__ uint32_t part[4 * kScanThreads];
+    const uint32_t r = blockIdx.y, b = blockIdx.x, t = threadIdx.x;
     const uint32_t *c = contrib + 4 * (uint64_t)r * n;
     qm31 s = {{0, 0}, {0, 0}};
-    for (uint32_t i = b0; i < b1; i++) s = qm_add(s, ldq(c + 4 * i));
-    stq(sums + 4 * t, s);
+    for (uint32_t k = 0; k < kScanPer; k++) {
+        const uint64_t i = (uint64_t)b * kScanTile + k * kScanThreads + t;
+        if (i < n) s = qm_add(s, ldq(c + 4 * i));
+    }
+    stq(part + 4 * t, s);
     __syncthreads();
-    for (uint32_t off = 1; off < kScanThreads; off <<= 1) {  // inclusive scan of the block sums
-        qm31 v = ldq(sums + 4 * t);
-        if (t >= off) v = qm_add(v, ldq(sums + 4 * (t - off)));
-        __syncthreads();
-        stq(sums + 4 * t, v);
+    for (uint32_t h = kScanThreads / 2; h > 0; h >>= 1) {
+        if (t < h) stq(part + 4 * t, qm_add(ldq(part + 4 * t), ldq(part + 4 * (t + h))));
         __syncthreads();
     }
-    qm31 run = t ? ldq(sums + 4 * (t - 1)) : qm31{{0, 0}, {0, 0}};
-    uint32_t *a = acc + 4 * (uint64_t)r * n;
+    if (t == 0) stq(sums + 4 * ((uint64_t)r * nb + b), ldq(part));
+}
+
+// sums[r][b] <- the sum of tiles 0 .. b-1 of relation r, one workgroup per relation
+__global__ __launch_bounds__(kScanSumsThreads) void acc_scan_sums_kernel(uint32_t *sums, uint32_t nb) {
+    __shared__ uint32_t run[4 * kScanSumsThreads];
+    const uint32_t r = blockIdx.x, t = threadIdx.x;
+    const uint32_t per = (nb + kScanSumsThreads - 1) / kScanSumsThreads;
+    const uint32_t b0 = min(nb, t * per), b1 = min(nb, b0 + per);
+    uint32_t *s = sums + 4 * (uint64_t)r * nb;
+    qm31 v = {{0, 0}, {0, 0}};
+    for (uint32_t i = b0; i < b1; i++) v = qm_add(v, ldq(s + 4 * i));
+    stq(run + 4 * t, v);
+    __syncthreads();
+    for (uint32_t off = 1; off < kScanSumsThreads; off <<= 1) {  // inclusive scan of the thread sums
+        qm31 u = ldq(run + 4 * t);
+        if (t >= off) u = qm_add(u, ldq(run + 4 * (t - off)));
+        __syncthreads();
+        stq(run + 4 * t, u);
+        __syncthreads();
+    }
+    qm31 acc = t ? ldq(run + 4 * (t - 1)) : qm31{{0, 0}, {0, 0}};
     for (uint32_t i = b0; i < b1; i++) {
-        stq(a + 4 * i, run);
-        run = qm_add(run, ldq(c + 4 * i));
+        const qm31 x = ldq(s + 4 * i);
+        stq(s + 4 * i, acc);
+        acc = qm_add(acc, x);
     }
+}
+
+// tile b of relation r: thread t holds elements [8t, 8t + 8) of the tile; their
+// exclusive prefix within the tile plus the tile's offset
+__global__ __launch_bounds__(kScanThreads) void acc_apply_kernel(const uint32_t *contrib, uint32_t n,
+                                                                 const uint32_t *offs, uint32_t nb, uint32_t *acc) {
+    __shared__ uint32_t part[4 * kScanThreads];
+    const uint32_t r = blockIdx.y, b = blockIdx.x, t = threadIdx.x;
+    const uint64_t i0 = (uint64_t)b * kScanTile + (uint64_t)t * kScanPer;
+    const uint32_t *c = contrib + 4 * (uint64_t)r * n;
+    qm31 v[kScanPer];
+    qm31 s = {{0, 0}, {0, 0}};
+#pragma unroll
+    for (uint32_t k = 0; k < kScanPer; k++) {
+        v[k] = i0 + k < n ? ldq(c + 4 * (i0 + k)) : qm31{{0, 0}, {0, 0}};
+        s = qm_add(s, v[k]);
+    }
+    stq(part + 4 * t, s);
+    __syncthreads();
+    for (uint32_t off = 1; off < kScanThreads; off <<= 1) {
+        qm31 u = ldq(part + 4 * t);
+        if (t >= off) u = qm_add(u, ldq(part + 4 * (t - off)));
+        __syncthreads();
+        stq(part + 4 * t, u);
+        __syncthreads();
+    }
+    qm31 run = ldq(offs + 4 * ((uint64_t)r * nb + b));
+    if (t) run = qm_add(run, ldq(part + 4 * (t - 1)));
+    uint32_t *a = acc + 4 * (uint64_t)r * n;
+#pragma unroll
+    for (uint32_t k = 0; k < kScanPer; k++) {
+        if (i0 + k < n) stq(a + 4 * (i0 + k), run);
+        run = qm_add(run, v[k]);
+    }
+}
+
+// UnivariateQM31.eval (sum_i c_i x^i) of column cols[e] at point pts[e], for every
+// pair e = blockIdx.y: thread t of tile b adds c_i x^i for i = tile start + t + 256 k;
+// one partial per tile, summed on the host (field addition: exact in any order)
+constexpr uint32_t kEvalThreads = 256, kEvalPer = 16, kEvalTile = kEvalThreads * kEvalPer;
+struct EvalJob {
+    uint32_t col[2 * AIR_MAX_COLS + 1], pt[2 * AIR_MAX_COLS + 1];
+    qm31 x[2], x_step[2];  // the points and their 256th powers
+};
+__global__ __launch_bounds__(kEvalThreads) void poly_eval_kernel(const uint32_t *co, uint32_t n, const EvalJob job,
+                                                                 uint32_t nb, uint32_t *partial) {
+    __shared__ uint32_t part[4 * kEvalThreads];
+    const uint32_t e = blockIdx.y, b = blockIdx.x, t = threadIdx.x;
+    const uint32_t *c = co + 4 * (uint64_t)job.col[e] * n;
+    const uint64_t i0 = (uint64_t)b * kEvalTile + t;
+    const qm31 step = job.x_step[job.pt[e]];
+    qm31 p = qm_pow(job.x[job.pt[e]], i0), s = {{0, 0}, {0, 0}};
+    for (uint32_t k = 0; k < kEvalPer; k++) {
+        const uint64_t i = i0 + (uint64_t)k * kEvalThreads;
+        if (i >= n) break;
+        s = qm_add(s, qm_mul(ldq(c + 4 * i), p));
+        p = qm_mul(p, step);
+    }
+    stq(part + 4 * t, s);
+    __syncthreads();
+    for (uint32_t h = kEvalThreads / 2; h > 0; h >>= 1) {
+        if (t < h) stq(part + 4 * t, qm_add(ldq(part + 4 * t), ldq(part + 4 * (t + h))));
+        __syncthreads();
+    }
+    if (t == 0) stq(partial + 4 * ((uint64_t)e * nb + b), ldq(part));
 }
 
 struct AirDomain {
@@ -330,12 +421,6 @@ void absorb(zs_channel *ch, qm31 x) {
         for (int s = 0; s < 4; s++) b[4 * k + s] = (uint8_t)(w[k] >> (8 * s));
     zs_channel_absorb(ch, b, 16);
 }
-// UnivariateQM31.eval by Horner over coefficients co[off .. off + n)
-qm31 horner(const std::vector<qm31> &co, size_t off, size_t n, qm31 x) {
-    qm31 r = {{0, 0}, {0, 0}};
-    for (size_t i = n; i > 0; i--) r = qm_add(qm_mul(r, x), co[off + i - 1]);
-    return r;
-}
 struct Wire {
     std::vector<uint8_t> b;
     void bytes(const void *d, size_t n) {
@@ -380,36 +465,31 @@ extern "C" int32_t zs_stark_prove_dev(zs_hip_ctx *h, const zs_air *air, const ui
     const qm31 w = qm31_root_of_unity(trace_log), w_ev = qm31_root_of_unity(dlog);
     const uint32_t mt = d.m_total, ncw = mt + 1 + d.n_pre;  // codewords: trace | acc | Q | pre
     const size_t tw = tree_words(dlog);
-    DevBuf cwb, trees, g, zhb, contrib, acc, dconst, wconst, deep;
+    DevBuf cwb, trees, g, zhb, contrib, acc, dconst, wconst, deep, coef, sums;
     ZS_HIP(ctx, hipMalloc(&cwb.ptr, ncw * N * 16));
+    ZS_HIP(ctx, hipMalloc(&coef.ptr, size_t(mt + d.n_pre) * n * 16));
     ZS_HIP(ctx, hipMalloc(&trees.ptr, ncw * tw * 4));
     ZS_HIP(ctx, hipMalloc(&g.ptr, N * 16));
     ZS_HIP(ctx, hipMalloc(&zhb.ptr, shift * 16));
     ZS_HIP(ctx, hipMalloc(&dconst.ptr, sizeof(AirDesc)));
     ZS_HIP(ctx, hipMalloc(&wconst.ptr, sizeof(AirWeights)));
     ZS_HIP(ctx, hipMalloc(&deep.ptr, sizeof(DeepConst)));
+    const uint32_t scan_nb = (uint32_t)((n + kScanTile - 1) / kScanTile);
     if (d.n_rel) {
         ZS_HIP(ctx, hipMalloc(&contrib.ptr, d.n_rel * n * 16));
         ZS_HIP(ctx, hipMalloc(&acc.ptr, d.n_rel * n * 16));
+        ZS_HIP(ctx, hipMalloc(&sums.ptr, size_t(d.n_rel) * scan_nb * 16));
     }
     uint32_t *cw = cwb.as<uint32_t>(), *tr = trees.as<uint32_t>();
     uint32_t *q_cw = cw + 4 * N * mt, *pre_cw = cw + 4 * N * (mt + 1);
     uint32_t *q_tree = tr + tw * mt, *pre_tree = tr + tw * (mt + 1);
     ZS_HIP(ctx, hipMemcpyAsync(dconst.ptr, &d, sizeof(AirDesc), hipMemcpyHostToDevice, ctx->stream));
-    // interpolation coefficients of every committed column (for the evaluations at z)
-    std::vector<qm31> co(size_t(mt) * n), pco(size_t(d.n_pre) * n);
-    const uint32_t n_inv = m31_inv(m31_from_u32((uint32_t)n));
-    auto take_coeffs = [&](std::vector<qm31> &dst, size_t first_col, uint32_t cols) -> int32_t {
-        // n * (the coefficients), left in scratch 0 by the trace commitment
-        std::vector<uint32_t> raw(size_t(cols) * n * 4);
-        ZS_HIP(ctx, hipMemcpyAsync(raw.data(), ctx->scratch[0].ptr, raw.size() * 4, hipMemcpyDeviceToHost,
-                                   ctx->stream));
-        ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        for (size_t i = 0; i < size_t(cols) * n; i++) {
-            uint32_t v[4];
-            for (int k = 0; k < 4; k++) v[k] = m31_mul(raw[4 * i + k], n_inv);
-            dst[first_col * n + i] = qm_from_words(v);
-        }
+    // n * (the interpolation coefficients) of every committed column, kept on the
+    // device for the evaluations at z: [trace | acc | pre], column c at 4 n c.  The
+    // trace commitment leaves them in scratch 0; copied out before the next one.
+    auto take_coeffs = [&](size_t first_col, uint32_t cols) -> int32_t {
+        ZS_HIP(ctx, hipMemcpyAsync(coef.as<uint32_t>() + 4 * first_col * n, ctx->scratch[0].ptr,
+                                   size_t(cols) * n * 16, hipMemcpyDeviceToDevice, ctx->stream));
         return ZS_OK;
     };
     // 1. preprocessed commitment, absorbed first, then the trace (stark.zig:374-414)
@@ -417,11 +497,11 @@ extern "C" int32_t zs_stark_prove_dev(zs_hip_ctx *h, const zs_air *air, const ui
     if (d.n_pre) {
         ZS_TRY(zs_qm31_trace_commit_dev(h, d_pre, d.n_pre, trace_log, log_blowup, pre_cw,
                                         reinterpret_cast<uint8_t *>(pre_tree), proots.data(), chan));
-        ZS_TRY(take_coeffs(pco, 0, d.n_pre));
+        ZS_TRY(take_coeffs(mt, d.n_pre));
     }
     ZS_TRY(zs_qm31_trace_commit_dev(h, d_trace, d.m, trace_log, log_blowup, cw, reinterpret_cast<uint8_t *>(tr),
                                     roots.data(), chan));
-    ZS_TRY(take_coeffs(co, 0, d.m));
+    ZS_TRY(take_coeffs(0, d.m));
     // 2. lookup challenges, accumulators on H and their commitment (stark.zig:416-468)
     auto wt = std::make_unique<AirWeights>();
     for (uint32_t j = 0; j < d.n_lc; j++) wt->alpha_cols[j] = sample(chan);
@@ -433,12 +513,19 @@ extern "C" int32_t zs_stark_prove_dev(zs_hip_ctx *h, const zs_air *air, const ui
                   ctx->stream, dconst.as<AirDesc>(), wconst.as<AirWeights>(), d_trace, d_pre, (uint32_t)n,
                   contrib.as<uint32_t>());
         ZS_HIP(ctx, hipGetLastError());
-        ZS_LAUNCH(ctx, "acc_scan_kernel", acc_scan_kernel, dim3(d.n_rel), dim3(kScanThreads), 0, ctx->stream,
-                  contrib.as<uint32_t>(), (uint32_t)n, acc.as<uint32_t>());
+        ZS_LAUNCH(ctx, "acc_tile_sums_kernel", acc_tile_sums_kernel, dim3(scan_nb, d.n_rel), dim3(kScanThreads), 0,
+                  ctx->stream, contrib.as<uint32_t>(), (uint32_t)n, scan_nb, sums.as<uint32_t>());
+        ZS_HIP(ctx, hipGetLastError());
+        ZS_LAUNCH(ctx, "acc_scan_sums_kernel", acc_scan_sums_kernel, dim3(d.n_rel), dim3(kScanSumsThreads), 0,
+                  ctx->stream, sums.as<uint32_t>(), scan_nb);
+        ZS_HIP(ctx, hipGetLastError());
+        ZS_LAUNCH(ctx, "acc_apply_kernel", acc_apply_kernel, dim3(scan_nb, d.n_rel), dim3(kScanThreads), 0,
+                  ctx->stream, contrib.as<uint32_t>(), (uint32_t)n, sums.as<uint32_t>(), scan_nb,
+                  acc.as<uint32_t>());
         ZS_HIP(ctx, hipGetLastError());
         ZS_TRY(zs_qm31_trace_commit_dev(h, acc.as<uint32_t>(), d.n_rel, trace_log, log_blowup, cw + 4 * N * d.m,
                                         reinterpret_cast<uint8_t *>(tr + tw * d.m), roots.data() + 32 * d.m, chan));
-        ZS_TRY(take_coeffs(co, d.m, d.n_rel));
+        ZS_TRY(take_coeffs(d.m, d.n_rel));
     }
     // 3. transition weights, boundary assertions, boundary weights (stark.zig:470-483)
     for (uint32_t k = 0; k < d.n_trans_total; k++) wt->alpha[k] = sample(chan);
@@ -487,13 +574,37 @@ extern "C" int32_t zs_stark_prove_dev(zs_hip_ctx *h, const zs_air *air, const ui
     auto dc = std::make_unique<DeepConst>();
     dc->z = sample(chan);
     const qm31 wz = qm_mul(dc->z, w);
-    for (uint32_t j = 0; j < mt; j++) {
-        dc->de[j] = horner(co, size_t(j) * n, n, dc->z);
-        dc->de[mt + j] = horner(co, size_t(j) * n, n, wz);
+    std::vector<qm31> pz(d.n_pre + 1);
+    {
+        // every trace and accumulator column at z and w z, the preprocessed ones at z
+        const uint32_t ne = 2 * mt + d.n_pre, eval_nb = (uint32_t)((n + kEvalTile - 1) / kEvalTile);
+        EvalJob job{};
+        for (uint32_t j = 0; j < mt; j++) {
+            job.col[j] = job.col[mt + j] = j;
+            job.pt[mt + j] = 1;
+        }
+        for (uint32_t j = 0; j < d.n_pre; j++) job.col[2 * mt + j] = mt + j;
+        job.x[0] = dc->z;
+        job.x[1] = wz;
+        for (int k = 0; k < 2; k++) job.x_step[k] = qm_pow(job.x[k], kEvalThreads);
+        DevBuf dpart;
+        std::vector<uint32_t> part(size_t(ne) * eval_nb * 4);
+        ZS_HIP(ctx, hipMalloc(&dpart.ptr, part.size() * 4));
+        ZS_LAUNCH(ctx, "poly_eval_kernel", poly_eval_kernel, dim3(eval_nb, ne), dim3(kEvalThreads), 0, ctx->stream,
+                  coef.as<uint32_t>(), (uint32_t)n, job, eval_nb, dpart.as<uint32_t>());
+        ZS_HIP(ctx, hipGetLastError());
+        ZS_HIP(ctx, hipMemcpyAsync(part.data(), dpart.ptr, part.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
+        ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        const qm31 n_inv = {{m31_inv(m31_from_u32((uint32_t)n)), 0}, {0, 0}};
+        for (uint32_t e = 0; e < ne; e++) {
+            qm31 v = {{0, 0}, {0, 0}};
+            for (uint32_t b = 0; b < eval_nb; b++) v = qm_add(v, qm_from_words(&part[4 * (size_t(e) * eval_nb + b)]));
+            v = qm_mul(v, n_inv);
+            if (e < 2 * mt) dc->de[e] = v;
+            else pz[e - 2 * mt] = v;
+        }
     }
     {
-        std::vector<qm31> pz(d.n_pre + 1);
-        for (uint32_t j = 0; j < d.n_pre; j++) pz[j] = horner(pco, size_t(j) * n, n, dc->z);
         const qm31 zh = qm_sub(qm_pow(dc->z, n), one);
         dc->de[2 * mt] = qm_mul(air_composition_host(d, *wt, dc->z, dc->de, dc->de + mt, pz.data(), zh), qm_inv(zh));
     }
