@@ -44,6 +44,7 @@ extern "C" {
 /* flags for zs_m31_lde_commit_dev / zs_fri_commit_dev */
 #define ZS_FLAG_PLAIN 1u /* reference-shaped per-column, per-layer launches (comparison path) */
 #define ZS_FLAG_NO_GRAPH 2u /* FRI with given challenges: launch each kernel instead of replaying a graph */
+#define ZS_FLAG_PERSISTENT 4u /* batch: LDE, row leaves and node levels of all traces in one persistent launch */
 
 typedef struct zs_hip_ctx zs_hip_ctx;
 typedef struct zs_tree zs_tree;
@@ -267,7 +268,12 @@ typedef struct {
  * worker contexts with their own streams, taking jobs from a shared queue so
  * their kernels overlap on the device; ZS_FLAG_PLAIN runs the reference-shaped
  * per-trace loop (one trace after another, per-column per-layer launches).
- * Returns ZS_OK when every job's status is ZS_OK. */
+ * ZS_FLAG_PERSISTENT first runs the LDE, row leaves and node levels of every trace
+ * with log_n >= 3 and at most 256 columns (log_blowup <= 8, log_n + log_blowup <= 26) in one persistent
+ * launch (several when their scratch -- n_cols * 2^(log_n+log_blowup) words per trace --
+ * exceeds the free device memory), then each trace's gamma draw, combination and FRI
+ * on the workers; other traces take the worker path.  Results are the same bytes on
+ * every path.  Returns ZS_OK when every job's status is ZS_OK. */
 int32_t zs_commit_batch_dev(zs_hip_ctx *ctx, zs_job *jobs, size_t n_jobs, uint32_t log_blowup,
                             uint32_t remainder_log, uint32_t n_workers, uint32_t flags);
 

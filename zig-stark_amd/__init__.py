@@ -27,6 +27,7 @@ P = 2147483647
 ZS_OK, ZS_ERR_GENERIC, ZS_ERR_INPUT, ZS_ERR_OOM, ZS_ERR_PROTOCOL, ZS_ERR_DEVICE = 0, -1, -2, -3, -4, -5
 ZS_FLAG_PLAIN = 1
 ZS_FLAG_NO_GRAPH = 2
+ZS_FLAG_PERSISTENT = 4
 
 _lib = None
 

@@ -15,7 +15,7 @@ OUT = os.path.join(PKG, "libzs_hip.so")
 OBJ = os.path.join(PKG, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["fft.hip", "fft_tile.hip", "merkle.hip", "fri.hip", "qntt.hip", "stark.hip", "capi.cpp", "channel.cpp", "job.cpp", "verify.cpp", "noalloc.hip", "air.hip"]
+SOURCES = ["fft.hip", "fft_tile.hip", "merkle.hip", "fri.hip", "qntt.hip", "stark.hip", "capi.cpp", "channel.cpp", "job.cpp", "verify.cpp", "noalloc.hip", "air.hip", "persist.hip"]
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -59,6 +59,8 @@ inline bool tile_pass_ok(uint32_t C) { return C >= 8 && C <= 11; }
 // transforms' stages [L-1, s_lo]; in: natural (s_lo == 0) or k-space coefficients,
 // out: natural LDE (s_lo == 0) or k-space big array
 bool lde_fused_ok(uint32_t C, uint32_t beta, bool single);
+// LDS words from one register-group tile to the next when `il` tiles share a copy instruction
+uint32_t tile_stride(uint32_t C, uint32_t il);
 int32_t run_lde_fused(Ctx *ctx, const uint32_t *d_in, uint64_t in_cs, uint32_t *d_out, uint64_t out_cs,
                       const uint32_t *tw_inv, const uint32_t *tw_fwd, uint32_t L, uint32_t beta, uint32_t s_lo,
                       uint32_t C, uint32_t n_cols);

@@ -137,7 +137,7 @@ struct Ctx {
     LdeTiles lde_tiles = default_lde_tiles();  // used by the next LDE, per kind of pass
     int lde_role = -1;                         // the kind of LDE pass being launched (profiling)
     // grow-only scratch arenas
-    DevBuf scratch[10];  // 0 transforms, 1-2 LDE, 3 plain path, 4-6 FRI, 7-9 commit jobs
+    DevBuf scratch[12];  // 0 transforms, 1-2 LDE, 3 plain path, 4-6 FRI, 7-9 commit jobs, 10-11 persistent batch
     // FRI commit phases with challenges given up front, captured once per buffer set and
     // replayed (key: codeword, layers, trees, challenges, channel, dlog / layer count / remainder length)
     hipStream_t capture_stream = nullptr;
@@ -198,6 +198,14 @@ int32_t launch_row_commit(Ctx *ctx, const uint32_t *d_cols, uint64_t col_stride,
                           uint32_t *d_nodes);
 int32_t launch_row_commit_plain(Ctx *ctx, const uint32_t *d_cols, uint64_t col_stride, uint32_t n_cols,
                                 uint32_t log_rows, uint32_t *d_nodes);
+// ---- the persistent batch launch (persist.hip) ----
+// shapes it takes; device bytes of its pool per trace
+bool persistent_supported(uint32_t n_cols, uint32_t log_n, uint32_t log_blowup);
+size_t persistent_pool_bytes(uint32_t n_cols, uint32_t log_n, uint32_t log_blowup);
+// LDE, row leaves and every node level of jobs[idx[..]] in one persistent launch on
+// ctx->stream (synchronised before returning); FRI is left to the caller
+int32_t persistent_lde_trees(Ctx *ctx, zs_job *jobs, const std::vector<size_t> &idx, uint32_t log_blowup);
+
 int32_t launch_row_leaves(Ctx *ctx, const uint32_t *d_cols, uint64_t col_stride, uint32_t n_cols, uint64_t n,
                           uint32_t *d_nodes);
 // node levels above an already written level 0 of 2^log_leaves digests; with

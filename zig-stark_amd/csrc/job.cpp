@@ -79,15 +79,22 @@ constexpr size_t kStageBytes = 32 + 28 * 32 + (size_t(16) << kMaxRemainderLog);
 // are unused: the caller synchronises the stream before reading the stage.
 // Without it the context's own pinned buffer is the stage, and the call returns
 // with the results in trace_root / fri_roots / remainder.
+// With tree_done the LDE and row tree are already in d_lde / d_nodes (the persistent
+// batch launch wrote them): the job starts at the channel's draw of gamma.
+bool job_shape_ok(uint32_t n_cols, uint32_t log_n, uint32_t log_blowup, uint32_t remainder_log) {
+    const uint32_t LT = log_n + log_blowup;
+    return !(n_cols == 0 || n_cols > 4096 || log_n < 1 || log_sum_over(log_n, log_blowup, 28) ||
+             remainder_log < log_blowup || remainder_log >= LT || remainder_log - log_blowup > kMaxRemainderLog);
+}
+
 int32_t commit_job(zs_hip_ctx *h, const uint32_t *d_cols, uint32_t n_cols, uint32_t log_n, uint32_t log_blowup,
                    uint32_t remainder_log, uint32_t *d_lde, uint8_t *d_nodes, uint8_t trace_root[32],
-                   uint8_t *fri_roots, uint32_t *remainder, uint32_t flags, uint8_t *stage = nullptr) {
+                   uint8_t *fri_roots, uint32_t *remainder, uint32_t flags, uint8_t *stage = nullptr,
+                   bool tree_done = false) {
     if (!h || !d_cols || !d_lde || !d_nodes || (!stage && (!trace_root || !fri_roots || !remainder)))
         return ZS_ERR_INPUT;
     const uint32_t LT = log_n + log_blowup;
-    if (n_cols == 0 || n_cols > 4096 || log_n < 1 || log_sum_over(log_n, log_blowup, 28) ||
-        remainder_log < log_blowup || remainder_log >= LT || remainder_log - log_blowup > kMaxRemainderLog)
-        return ZS_ERR_INPUT;
+    if (!job_shape_ok(n_cols, log_n, log_blowup, remainder_log)) return ZS_ERR_INPUT;
     Ctx *ctx = &h->c;
     const size_t N = size_t(1) << LT;
     uint32_t *nodes = reinterpret_cast<uint32_t *>(d_nodes);
@@ -99,7 +106,10 @@ int32_t commit_job(zs_hip_ctx *h, const uint32_t *d_cols, uint32_t n_cols, uint3
     const B3Hasher *d_init;
     ZS_TRY(device_commit_channel(ctx, &d_init));
     uint32_t *d_gamma = ctx->scratch[9].as<uint32_t>(), *d_powers = d_gamma + 4;
-    if (flags & ZS_FLAG_PLAIN) {
+    if (tree_done) {
+        ZS_HIP(ctx, hipMemcpyAsync(d_chan, d_init, sizeof(B3Hasher), hipMemcpyDeviceToDevice, ctx->stream));
+        ZS_TRY(launch_channel_draw(ctx, d_chan, d_root, d_gamma));
+    } else if (flags & ZS_FLAG_PLAIN) {
         ZS_TRY(zs_m31_lde_commit_dev(h, d_cols, n_cols, log_n, log_blowup, d_lde, d_nodes, ZS_FLAG_PLAIN));
         ZS_HIP(ctx, hipMemcpyAsync(d_chan, d_init, sizeof(B3Hasher), hipMemcpyDeviceToDevice, ctx->stream));
         ZS_TRY(launch_channel_draw(ctx, d_chan, d_root, d_gamma));
@@ -181,6 +191,40 @@ zs_hip_ctx *new_worker(int device, const std::shared_ptr<DeviceCaches> &caches) 
     return w;
 }
 
+// The persistent launch's share of a batch: the jobs it takes, in groups whose pool
+// fits the device memory free now (largest traces first).  Jobs outside every group
+// (other shapes, or a trace whose pool alone does not fit) are left to commit_job.
+int32_t persistent_groups(Ctx *ctx, const zs_job *jobs, size_t n_jobs, uint32_t log_blowup, uint32_t remainder_log,
+                          std::vector<std::vector<size_t>> *groups) {
+    size_t free_b = 0, total_b = 0;
+    ZS_HIP(ctx, hipMemGetInfo(&free_b, &total_b));
+    const size_t margin = size_t(1) << 30;
+    const size_t budget = free_b + ctx->scratch[10].bytes > margin ? free_b + ctx->scratch[10].bytes - margin : 0;
+    std::vector<size_t> take;
+    for (size_t i = 0; i < n_jobs; i++) {
+        const zs_job &j = jobs[i];
+        if (job_shape_ok(j.n_cols, j.log_n, log_blowup, remainder_log) &&
+            persistent_supported(j.n_cols, j.log_n, log_blowup) && j.d_cols && j.d_lde && j.d_nodes &&
+            persistent_pool_bytes(j.n_cols, j.log_n, log_blowup) <= budget)
+            take.push_back(i);
+    }
+    std::stable_sort(take.begin(), take.end(), [&](size_t a, size_t b) {
+        return persistent_pool_bytes(jobs[a].n_cols, jobs[a].log_n, log_blowup) >
+               persistent_pool_bytes(jobs[b].n_cols, jobs[b].log_n, log_blowup);
+    });
+    size_t used = 0;
+    for (size_t i : take) {
+        const size_t b = persistent_pool_bytes(jobs[i].n_cols, jobs[i].log_n, log_blowup);
+        if (groups->empty() || used + b > budget) {
+            groups->emplace_back();
+            used = 0;
+        }
+        groups->back().push_back(i);
+        used += b;
+    }
+    return ZS_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -197,12 +241,31 @@ int32_t zs_commit_batch_dev(zs_hip_ctx *h, zs_job *jobs, size_t n_jobs, uint32_t
     if (!h || (!jobs && n_jobs) || n_workers > 64) return ZS_ERR_INPUT;
     if (n_jobs == 0) return ZS_OK;
     Ctx *ctx = &h->c;
-    if (n_workers <= 1 || (flags & ZS_FLAG_PLAIN)) {
-        for (size_t i = 0; i < n_jobs; i++) {
-            zs_job &j = jobs[i];
-            j.status = commit_job(h, j.d_cols, j.n_cols, j.log_n, log_blowup, remainder_log, j.d_lde, j.d_nodes,
-                                  j.trace_root, j.fri_roots, j.remainder, flags);
+    // ZS_FLAG_PERSISTENT: LDE, row leaves and node levels of the batch in one persistent
+    // launch per memory group; then every job's gamma draw, combination and FRI below
+    std::vector<uint8_t> tree_done(n_jobs, 0);
+    if ((flags & ZS_FLAG_PERSISTENT) && !(flags & ZS_FLAG_PLAIN)) {
+        ZS_HIP(ctx, hipSetDevice(ctx->device));
+        ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));  // inputs written on the caller's stream are complete
+        std::vector<std::vector<size_t>> groups;
+        ZS_TRY(persistent_groups(ctx, jobs, n_jobs, log_blowup, remainder_log, &groups));
+        for (const auto &g : groups) {
+            const int32_t rc = persistent_lde_trees(ctx, jobs, g, log_blowup);
+            for (size_t i : g) {
+                if (rc == ZS_OK) tree_done[i] = 1;
+                else jobs[i].status = rc;
+            }
+            if (rc != ZS_OK) return rc;
         }
+    }
+    auto run_one = [&](zs_hip_ctx *w, size_t i, uint8_t *stage) {
+        zs_job &j = jobs[i];
+        return commit_job(w, j.d_cols, j.n_cols, j.log_n, log_blowup, remainder_log, j.d_lde, j.d_nodes,
+                          stage ? nullptr : j.trace_root, stage ? nullptr : j.fri_roots,
+                          stage ? nullptr : j.remainder, flags, stage, tree_done[i] != 0);
+    };
+    if (n_workers <= 1 || (flags & ZS_FLAG_PLAIN)) {
+        for (size_t i = 0; i < n_jobs; i++) jobs[i].status = run_one(h, i, nullptr);
     } else {
         ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));  // inputs written on the caller's stream are complete
         auto &ws = ctx->workers;
@@ -244,10 +307,8 @@ int32_t zs_commit_batch_dev(zs_hip_ctx *h, zs_job *jobs, size_t n_jobs, uint32_t
                         ev_ok = false;
                     if ((k = next.fetch_add(1)) >= n_jobs) break;
                     const size_t i = order[k];
-                    zs_job &j = jobs[i];
                     ran_on[i] = t;
-                    j.status = commit_job(w, j.d_cols, j.n_cols, j.log_n, log_blowup, remainder_log, j.d_lde,
-                                          j.d_nodes, nullptr, nullptr, nullptr, flags, stage + i * kStageBytes);
+                    jobs[i].status = run_one(w, i, stage + i * kStageBytes);
                     if (!ev_ok || hipEventRecord(ev[queued % kInFlight], w->c.stream) != hipSuccess) {
                         ev_ok = false;  // no completion events: one job at a time
                         (void)hipStreamSynchronize(w->c.stream);

@@ -14,75 +14,21 @@
 // subtree and hashes all 7 of its nodes, so no lane idles while the level
 // narrows; from 2^16 nodes down, workgroups keep 8-level subtrees in LDS and
 // hash the narrow levels with four lanes per compression.
-#include "b3_quad.h"
-#include "internal.h"
+#include "merkle_dev.h"
 
 namespace zs {
-
-// BLAKE3 of one chunk (<= 256 columns = 1024 bytes) of a row.
-__device__ __forceinline__ void row_chunk_cv(const uint32_t *cols, uint64_t cs, uint64_t row, uint32_t c0,
-                                             uint32_t ncols, uint32_t chunk_idx, bool root, uint32_t cv[8]) {
-    b3_iv(cv);
-    uint32_t len = ncols * 4;
-    uint32_t nblocks = (len + 63) / 64;
-    for (uint32_t b = 0; b < nblocks; b++) {
-        uint32_t m[16];
-#pragma unroll
-        for (int w = 0; w < 16; w++) {
-            uint32_t c = b * 16 + w;
-            m[w] = c < ncols ? cols[(uint64_t)(c0 + c) * cs + row] : 0u;
-        }
-        uint32_t blen = (len - b * 64) < 64 ? (len - b * 64) : 64;
-        uint32_t flags = (b == 0 ? B3_CHUNK_START : 0) | (b == nblocks - 1 ? (B3_CHUNK_END | (root ? B3_ROOT : 0)) : 0);
-        b3_compress_cv(cv, m, chunk_idx, 0, blen, flags);
-    }
-}
 
 // rows of <= 256 columns: one BLAKE3 chunk per leaf
 __global__ __launch_bounds__(256) void row_leaf_kernel(const uint32_t *cols, uint64_t cs, uint32_t n_cols,
                                                        uint64_t n_rows, uint32_t *nodes) {
-    uint64_t row = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= n_rows) return;
-    uint32_t cv[8];
-    row_chunk_cv(cols, cs, row, 0, n_cols, 0, true, cv);
-    uint4 *o = reinterpret_cast<uint4 *>(nodes + row * 8);
-    o[0] = make_uint4(cv[0], cv[1], cv[2], cv[3]);
-    o[1] = make_uint4(cv[4], cv[5], cv[6], cv[7]);
+    const uint64_t row = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row < n_rows) row_leaf_body(cols, cs, n_cols, row, nodes);
 }
-
-// rows of more than 256 columns: BLAKE3 chunk tree with a chaining-value stack
+// rows of more than 256 columns
 __global__ __launch_bounds__(256) void row_leaf_multichunk_kernel(const uint32_t *cols, uint64_t cs, uint32_t n_cols,
                                                                   uint64_t n_rows, uint32_t *nodes) {
-    uint64_t row = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= n_rows) return;
-    uint32_t cv[8];
-    uint32_t n_chunks = (n_cols + 255) / 256;
-    uint32_t stack[16][8];
-    uint32_t sp = 0;
-    for (uint32_t c = 0; c + 1 < n_chunks; c++) {
-        row_chunk_cv(cols, cs, row, c * 256, 256, c, false, cv);
-        uint32_t total = c + 1;
-        while ((total & 1u) == 0) {
-            sp--;
-            uint32_t t[8];
-            b3_parent(stack[sp], cv, false, t);
-            for (int w = 0; w < 8; w++) cv[w] = t[w];
-            total >>= 1;
-        }
-        for (int w = 0; w < 8; w++) stack[sp][w] = cv[w];
-        sp++;
-    }
-    uint32_t last = n_chunks - 1;
-    row_chunk_cv(cols, cs, row, last * 256, n_cols - last * 256, last, false, cv);
-    while (sp > 0) {
-        sp--;
-        uint32_t t[8];
-        b3_parent(stack[sp], cv, sp == 0, t);
-        for (int w = 0; w < 8; w++) cv[w] = t[w];
-    }
-    uint4 *o = reinterpret_cast<uint4 *>(nodes + row * 8);
-    o[0] = make_uint4(cv[0], cv[1], cv[2], cv[3]);
-    o[1] = make_uint4(cv[4], cv[5], cv[6], cv[7]);
+    const uint64_t row = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row < n_rows) row_leaf_multichunk_body(cols, cs, n_cols, row, nodes);
 }
 
 int32_t launch_row_leaves(Ctx *ctx, const uint32_t *d_cols, uint64_t col_stride, uint32_t n_cols,
@@ -111,56 +57,6 @@ __global__ __launch_bounds__(256) void qm31_leaf_kernel(const uint32_t *vals, ui
 }
 
 
-__device__ __forceinline__ void ld_digest(const uint32_t *p, uint32_t d[8]) {
-    const uint4 *q = reinterpret_cast<const uint4 *>(p);
-    uint4 a = q[0], b = q[1];
-    d[0] = a.x; d[1] = a.y; d[2] = a.z; d[3] = a.w; d[4] = b.x; d[5] = b.y; d[6] = b.z; d[7] = b.w;
-}
-__device__ __forceinline__ void st_digest(uint32_t *p, const uint32_t d[8]) {
-    uint4 *q = reinterpret_cast<uint4 *>(p);
-    q[0] = make_uint4(d[0], d[1], d[2], d[3]);
-    q[1] = make_uint4(d[4], d[5], d[6], d[7]);
-}
-
-// NLEV (1..3) levels per launch with no idle lanes: lane i owns the subtree
-// over children 2^NLEV * i .. 2^NLEV * (i + 1) - 1 and writes all its nodes.
-template <int NLEV>
-__device__ __forceinline__ void subtree_lane(const uint32_t *child, uint32_t *lvl1, uint64_t n_lanes, uint64_t i) {
-    constexpr int W = 1 << NLEV;
-    uint32_t *lvl2 = lvl1 + n_lanes * (W / 2) * 8;
-    uint32_t *lvl3 = lvl2 + n_lanes * (W / 4) * 8;
-    uint32_t a[8], b[8], n1[W / 2][8];
-#pragma unroll
-    for (int k = 0; k < W / 2; k++) {
-        ld_digest(child + (i * W + 2 * k) * 8, a);
-        ld_digest(child + (i * W + 2 * k + 1) * 8, b);
-        b3_hash2(a, b, n1[k]);
-        st_digest(lvl1 + (i * (W / 2) + k) * 8, n1[k]);
-    }
-    if constexpr (NLEV >= 2) {
-        uint32_t n2[W / 4][8];
-#pragma unroll
-        for (int k = 0; k < W / 4; k++) {
-            b3_hash2(n1[2 * k], n1[2 * k + 1], n2[k]);
-            st_digest(lvl2 + (i * (W / 4) + k) * 8, n2[k]);
-        }
-        if constexpr (NLEV >= 3) {
-            uint32_t n3[8];
-            b3_hash2(n2[0], n2[1], n3);
-            st_digest(lvl3 + i * 8, n3);
-        }
-    }
-}
-
-// Several independent trees in one launch (the layers of an FRI commitment):
-// tree t owns lanes [lane_end[t-1], lane_end[t]) and its levels at word offsets
-// child[t], lvl1[t] from `base`.
-struct MultiTrees {
-    static constexpr int MAX = 32;
-    uint32_t n;
-    uint64_t child[MAX], lvl1[MAX], lane_end[MAX];
-};
-
 __global__ __launch_bounds__(256, 8) void tree_subtree3_multi_kernel(uint32_t *base, const MultiTrees mt) {
     uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= mt.lane_end[mt.n - 1]) return;
@@ -171,101 +67,8 @@ __global__ __launch_bounds__(256, 8) void tree_subtree3_multi_kernel(uint32_t *b
 }
 
 
-// Workgroups that each build K levels of a subtree of 2^K children, keeping
-// the levels in LDS.  Levels with at least two nodes per quad hash one node per
-// lane; narrower levels use a quad per node.  Several trees can share a launch:
-// tree t starts at word offset child[t] (its level with 2^r[t] children) and owns
-// workgroups [wg_end[t-1], wg_end[t]), 2^(r[t]-K[t]) of them.
-struct BlockTrees {
-    static constexpr int MAX = 32;
-    uint32_t n;
-    uint64_t child[MAX];
-    uint32_t r[MAX], K[MAX], wg_end[MAX];
-    // set on the launch that computes the root of its one tree: the workgroup then
-    // absorbs the root into this device channel and draws a challenge into alpha
-    B3Hasher *chan;
-    uint32_t *alpha;
-};
-
-constexpr uint32_t BLOCK_NT = 256;
-
-ZS_HD size_t block_lds_words(uint32_t K) { return (size_t(6) << K) + (BLOCK_NT / 4) * 32; }
-// the channel state, its snapshot and the draw's staging after the tree words
-constexpr size_t kDrawLdsWords = 2 * sizeof(B3Hasher) / 4 + 24 + 16;
-
 __global__ __launch_bounds__(BLOCK_NT) void tree_block_kernel(uint32_t *base, const BlockTrees bt) {
-    extern __shared__ uint32_t sm[];
-    const uint32_t b = blockIdx.x;
-    uint32_t t = 0;
-    while (b >= bt.wg_end[t]) t++;
-    const uint32_t s = b - (t ? bt.wg_end[t - 1] : 0u);
-    const uint32_t K = bt.K[t], r = bt.r[t];
-    const uint32_t tid = threadIdx.x, j = tid & 3u, quad = tid >> 2;
-    constexpr uint32_t NQ = BLOCK_NT / 4;
-    uint32_t *bufs[2] = {sm, sm + (4u << K)};
-    uint32_t *qm = sm + (6u << K) + quad * 32;
-    const uint32_t iv_lo = kIV[j], iv_hi = kIV[4 + j];
-    uint32_t mo[28];
-#pragma unroll
-    for (int k = 0; k < 28; k++) mo[k] = kQuadMsg[j][k];
-    qm[20 + j] = 0u;
-    qm[24 + j] = 0u;
-    qm[28 + j] = 0u;
-    uint64_t lvl = bt.child[t];  // word offset of the current child level
-    uint64_t width = uint64_t(1) << r;
-    const uint32_t *src = base + lvl + (uint64_t)s * (8u << K);
-    __syncthreads();
-    for (uint32_t l = 0; l < K; l++) {
-        const uint32_t P = 1u << (K - 1 - l);
-        uint32_t *out = base + lvl + width * 8 + (uint64_t)s * P * 8;
-        uint32_t *dst = bufs[l & 1];
-        if (P >= BLOCK_NT / 2) {
-            // two or more nodes per quad: one node per lane (a lane's compression has
-            // four independent G chains, a quad's one) finishes sooner
-            for (uint32_t p = tid; p < P; p += BLOCK_NT) {
-                uint32_t x[8], y[8], d[8];
-                ld_digest(src + 16 * p, x);
-                ld_digest(src + 16 * p + 8, y);
-                b3_hash2(x, y, d);
-                st_digest(out + 8 * p, d);
-                st_digest(dst + 8 * p, d);
-            }
-        } else {
-            for (uint32_t p = quad; p < P; p += NQ) {
-                uint32_t lo, hi;
-                quad_hash2(src + 16 * p, qm, mo, j, iv_lo, iv_hi, lo, hi);
-                out[8 * p + j] = lo;
-                out[8 * p + 4 + j] = hi;
-                dst[8 * p + j] = lo;
-                dst[8 * p + 4 + j] = hi;
-            }
-        }
-        __syncthreads();
-        src = dst;
-        lvl += width * 8;
-        width >>= 1;
-    }
-    if (bt.chan) {
-        // src holds the root: absorb it and draw the next challenge
-        constexpr uint32_t HW = sizeof(B3Hasher) / 4;
-        B3Hasher *hs = reinterpret_cast<B3Hasher *>(sm + block_lds_words(K));
-        B3Hasher *snap = hs + 1;
-        uint32_t *qs = reinterpret_cast<uint32_t *>(snap + 1), *work = qs + 24;
-        const uint32_t *cw = reinterpret_cast<const uint32_t *>(bt.chan);
-        uint32_t *hw = reinterpret_cast<uint32_t *>(hs);
-        for (uint32_t k = tid; k < HW; k += BLOCK_NT) hw[k] = cw[k];
-        __syncthreads();
-        if (tid < 64) {  // wave 0
-            uint32_t d[8], q[4];
-            for (int w = 0; w < 8; w++) d[w] = src[w];
-            quad_channel_draw(*hs, *snap, qs, work, d, q, j, mo);
-            if (tid == 0)
-                for (int k = 0; k < 4; k++) bt.alpha[k] = q[k];
-        }
-        __syncthreads();
-        uint32_t *ow = reinterpret_cast<uint32_t *>(bt.chan);
-        for (uint32_t k = tid; k < HW; k += BLOCK_NT) ow[k] = hw[k];
-    }
+    tree_block_body<BLOCK_NT>(base, bt, blockIdx.x, threadIdx.x);
 }
 
 // levels with at least 2^18 children are hashed 3 at a time by one lane per
