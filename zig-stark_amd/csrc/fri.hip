@@ -11,8 +11,7 @@
 // with no inversion.  Field results are canonical, so the output equals the
 // reference's value for value.  The kernel also writes the hashQM31 leaf of
 // each new value, fusing the next layer's leaf pass into the fold.
-#include "b3_quad.h"
-#include "internal.h"
+#include "fri_dev.h"
 
 namespace zs {
 
@@ -85,24 +84,11 @@ __global__ __launch_bounds__(256) void fri_fold_kernel(const uint32_t *cur, uint
                                                        uint32_t *next, uint32_t *next_leaves) {
     uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= half) return;
-    qm31 y0 = ld_qm(cur + 4 * j);
-    qm31 y1 = ld_qm(cur + 4 * (j + half));
-    uint64_t e = j << layer;
-    // (y0 + y1) / 2 + alpha (y0 - y1) x^-1 / 2 with x^-1 = inv_off * t_lo * t_hi: the factor
-    // alpha * inv_off / 2 is the same for every element (uniform, computed once per wave)
-    const qm31 a_off = qm_half(qm_mul(ld_qm(alpha_p), inv_off));
-    const qm31 t = qm_mul(ld_qm(tables + 4 * (e & 1023)), ld_qm(tables + 4 * (1024 + (e >> 10))));
-    qm31 hs = qm_half(qm_add(y0, y1));
-    qm31 v = qm_add(hs, qm_mul(qm_mul(a_off, t), qm_sub(y0, y1)));
-    st_qm(next + 4 * j, v);
-    if (next_leaves) {
-        uint32_t w[4] = {v.a.c0, v.a.c1, v.b.c0, v.b.c1};
-        uint32_t d[8];
-        b3_hash_qm31(w, d);
-        uint4 *o = reinterpret_cast<uint4 *>(next_leaves + j * 8);
-        o[0] = make_uint4(d[0], d[1], d[2], d[3]);
-        o[1] = make_uint4(d[4], d[5], d[6], d[7]);
-    }
+    // the factor alpha * inv_off / 2 is the same for every element (uniform, computed once per wave)
+    const qm31 a_off = qm_half(qm_mul(ldg_qm(alpha_p), inv_off));
+    const qm31 v = fold_value(cur, half, j, layer, tables, a_off);
+    stg_qm(next + 4 * j, v);
+    if (next_leaves) store_qm31_leaf(next_leaves + j * 8, v);
 }
 
 int32_t launch_fri_fold(Ctx *ctx, const uint32_t *d_cur, uint64_t n, uint32_t dlog, uint32_t layer,
@@ -132,26 +118,7 @@ __global__ __launch_bounds__(256) void combine_kernel(const uint32_t *lde, uint6
     __syncthreads();
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    uint32_t a[4] = {0, 0, 0, 0};
-    uint32_t c = 0;
-    for (; c + 4 <= n_cols; c += 4) {
-        uint32_t v[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) v[k] = lde[(uint64_t)(c + k) * cs + i];
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            uint64_t acc = 0;
-#pragma unroll
-            for (int k = 0; k < 4; k++) acc += (uint64_t)pw[4 * (c + k) + q] * v[k];
-            a[q] = m31_add(a[q], m31_reduce64(acc));
-        }
-    }
-    for (; c < n_cols; c++) {
-        uint32_t v = lde[(uint64_t)c * cs + i];
-#pragma unroll
-        for (int q = 0; q < 4; q++) a[q] = m31_add(a[q], m31_mul(pw[4 * c + q], v));
-    }
-    *reinterpret_cast<uint4 *>(out + 4 * i) = make_uint4(a[0], a[1], a[2], a[3]);
+    stg4(out + 4 * i, combine_row(lde, cs, n_cols, i, pw));
 }
 
 int32_t launch_combine(Ctx *ctx, const uint32_t *d_lde, uint64_t cs, uint32_t n_cols, uint64_t n,
@@ -178,26 +145,7 @@ namespace zs {
 __global__ __launch_bounds__(64) void channel_draw_kernel(B3Hasher *st, const uint32_t *digest, uint32_t *out) {
     __shared__ B3Hasher h, snap;
     __shared__ uint32_t qs[24], work[16];
-    constexpr uint32_t W = sizeof(B3Hasher) / 4;
-    static_assert(sizeof(B3Hasher) % 4 == 0, "hasher state is word-sized");
-    uint32_t *hw = reinterpret_cast<uint32_t *>(&h);
-    const uint32_t *sw = reinterpret_cast<const uint32_t *>(st);
-    for (uint32_t k = threadIdx.x; k < W; k += blockDim.x) hw[k] = sw[k];
-    __syncthreads();
-    {  // the whole wave
-        const uint32_t j = threadIdx.x & 3u;
-        uint32_t mo[28];
-#pragma unroll
-        for (int k = 0; k < 28; k++) mo[k] = kQuadMsg[j][k];
-        uint32_t d[8], q[4];
-        for (int i = 0; i < 8; i++) d[i] = digest[i];
-        quad_channel_draw(h, snap, qs, work, d, q, j, mo);
-        if (threadIdx.x == 0)
-            for (int i = 0; i < 4; i++) out[i] = q[i];
-    }
-    __syncthreads();
-    uint32_t *dw = reinterpret_cast<uint32_t *>(st);
-    for (uint32_t k = threadIdx.x; k < W; k += blockDim.x) dw[k] = hw[k];
+    channel_draw_body(st, nullptr, digest, out, h, snap, qs, work, threadIdx.x, 64);
 }
 
 int32_t launch_channel_draw(Ctx *ctx, B3Hasher *d_state, const uint32_t *d_digest, uint32_t *d_out) {

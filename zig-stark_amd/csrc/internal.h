@@ -52,6 +52,23 @@ namespace zs {
         if (_rc != ZS_OK) return _rc; \
     } while (0)
 
+// Loads and stores of device (global) memory.  Device bodies whose pointers come from
+// descriptors in memory rather than from kernel arguments (the tasks of the persistent
+// batch kernel) would otherwise get flat accesses, which also count against the LDS
+// counter; in a kernel these are plain accesses.
+#define ZS_GLOBAL __attribute__((address_space(1)))
+__device__ __forceinline__ uint32_t ldg(const uint32_t *p) { return *(const ZS_GLOBAL uint32_t *)p; }
+__device__ __forceinline__ void stg(uint32_t *p, uint32_t v) { *(ZS_GLOBAL uint32_t *)p = v; }
+typedef uint32_t zs_u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ uint4 ldg4(const uint32_t *p) {
+    const zs_u32x4 v = *(const ZS_GLOBAL zs_u32x4 *)p;
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void stg4(uint32_t *p, uint4 v) {
+    const zs_u32x4 w = {v.x, v.y, v.z, v.w};
+    *(ZS_GLOBAL zs_u32x4 *)p = w;
+}
+
 // A device allocation that frees itself.
 struct DevBuf {
     void *ptr = nullptr;
@@ -201,10 +218,13 @@ int32_t launch_row_commit_plain(Ctx *ctx, const uint32_t *d_cols, uint64_t col_s
 // ---- the persistent batch launch (persist.hip) ----
 // shapes it takes; device bytes of its pool per trace
 bool persistent_supported(uint32_t n_cols, uint32_t log_n, uint32_t log_blowup);
-size_t persistent_pool_bytes(uint32_t n_cols, uint32_t log_n, uint32_t log_blowup);
-// LDE, row leaves and every node level of jobs[idx[..]] in one persistent launch on
-// ctx->stream (synchronised before returning); FRI is left to the caller
-int32_t persistent_lde_trees(Ctx *ctx, zs_job *jobs, const std::vector<size_t> &idx, uint32_t log_blowup);
+size_t persistent_pool_bytes(uint32_t n_cols, uint32_t log_n, uint32_t log_blowup, uint32_t remainder_log);
+// the whole commitment of jobs[idx[..]] (LDE, row tree, gamma, combination, FRI) in one
+// persistent launch on ctx->stream; synchronised, results and statuses set on return
+int32_t persistent_commit(Ctx *ctx, zs_job *jobs, const std::vector<size_t> &idx, uint32_t log_blowup,
+                          uint32_t remainder_log);
+// the channel every commit job starts from, Channel.init("zig-stark:hip:commit"), on the device
+int32_t device_commit_channel(Ctx *ctx, const B3Hasher **out);
 
 int32_t launch_row_leaves(Ctx *ctx, const uint32_t *d_cols, uint64_t col_stride, uint32_t n_cols, uint64_t n,
                           uint32_t *d_nodes);

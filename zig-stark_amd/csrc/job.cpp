@@ -41,37 +41,23 @@ const B3Hasher &commit_channel() {
     return st;
 }
 
-// One trace's commitment, queued on the context's stream with no host round
-// trip until the end: LDE and row tree; the channel (on the device) absorbs the
-// trace root and draws gamma; g = sum_c gamma^c lde_c; FRI on g with challenges
-// drawn from the same channel.  Scratch: 7 channel state, 8 the codeword g,
-// 9 gamma and its powers; FRI uses 4-6 and the LDE 1-2.
-// the starting channel state on the device, built once per cache (outside its lock, like
-// the twiddle tables) so each job starts its channel with a device-to-device copy
-int32_t device_commit_channel(Ctx *ctx, const B3Hasher **out) {
-    DeviceCaches &dc = *ctx->caches;
-    {
-        std::lock_guard<std::mutex> lk(dc.mu);
-        if (dc.commit_channel) {
-            *out = dc.commit_channel->as<B3Hasher>();
-            return ZS_OK;
-        }
-    }
-    auto buf = std::make_unique<DevBuf>();
-    ZS_HIP(ctx, hipMalloc(&buf->ptr, sizeof(B3Hasher)));
-    buf->bytes = sizeof(B3Hasher);
-    ZS_HIP(ctx, hipMemcpyAsync(buf->ptr, &commit_channel(), sizeof(B3Hasher), hipMemcpyHostToDevice, ctx->stream));
-    ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    std::lock_guard<std::mutex> lk(dc.mu);
-    if (!dc.commit_channel) dc.commit_channel = std::move(buf);
-    *out = dc.commit_channel->as<B3Hasher>();
-    return ZS_OK;
+// the shapes commit_job takes
+bool job_shape_ok(uint32_t n_cols, uint32_t log_n, uint32_t log_blowup, uint32_t remainder_log) {
+    const uint32_t LT = log_n + log_blowup;
+    return !(n_cols == 0 || n_cols > 4096 || log_n < 1 || log_sum_over(log_n, log_blowup, 28) ||
+             remainder_log < log_blowup || remainder_log >= LT || remainder_log - log_blowup > kMaxRemainderLog);
 }
 
 // staged results of one job: trace root, at most 28 FRI roots, at most 2^kMaxRemainderLog
 // remainder coefficients
 constexpr size_t kStageBytes = 32 + 28 * 32 + (size_t(16) << kMaxRemainderLog);
 
+// One trace's commitment, queued on the context's stream with no host round
+// trip until the end: LDE and row tree; the channel (on the device) absorbs the
+// trace root and draws gamma; g = sum_c gamma^c lde_c; FRI on g with challenges
+// drawn from the same channel.  Scratch: 7 channel state, 8 the codeword g,
+// 9 gamma and its powers; FRI uses 4-6 and the LDE 1-2.
+//
 // The results go to host memory through a pinned stage (the trace root, then the
 // FRI roots, then the remainder words), so no copy into pageable memory blocks the
 // host while the chain is being queued.  With `stage` (pinned, kStageBytes) given,
@@ -79,18 +65,9 @@ constexpr size_t kStageBytes = 32 + 28 * 32 + (size_t(16) << kMaxRemainderLog);
 // are unused: the caller synchronises the stream before reading the stage.
 // Without it the context's own pinned buffer is the stage, and the call returns
 // with the results in trace_root / fri_roots / remainder.
-// With tree_done the LDE and row tree are already in d_lde / d_nodes (the persistent
-// batch launch wrote them): the job starts at the channel's draw of gamma.
-bool job_shape_ok(uint32_t n_cols, uint32_t log_n, uint32_t log_blowup, uint32_t remainder_log) {
-    const uint32_t LT = log_n + log_blowup;
-    return !(n_cols == 0 || n_cols > 4096 || log_n < 1 || log_sum_over(log_n, log_blowup, 28) ||
-             remainder_log < log_blowup || remainder_log >= LT || remainder_log - log_blowup > kMaxRemainderLog);
-}
-
 int32_t commit_job(zs_hip_ctx *h, const uint32_t *d_cols, uint32_t n_cols, uint32_t log_n, uint32_t log_blowup,
                    uint32_t remainder_log, uint32_t *d_lde, uint8_t *d_nodes, uint8_t trace_root[32],
-                   uint8_t *fri_roots, uint32_t *remainder, uint32_t flags, uint8_t *stage = nullptr,
-                   bool tree_done = false) {
+                   uint8_t *fri_roots, uint32_t *remainder, uint32_t flags, uint8_t *stage = nullptr) {
     if (!h || !d_cols || !d_lde || !d_nodes || (!stage && (!trace_root || !fri_roots || !remainder)))
         return ZS_ERR_INPUT;
     const uint32_t LT = log_n + log_blowup;
@@ -106,10 +83,7 @@ int32_t commit_job(zs_hip_ctx *h, const uint32_t *d_cols, uint32_t n_cols, uint3
     const B3Hasher *d_init;
     ZS_TRY(device_commit_channel(ctx, &d_init));
     uint32_t *d_gamma = ctx->scratch[9].as<uint32_t>(), *d_powers = d_gamma + 4;
-    if (tree_done) {
-        ZS_HIP(ctx, hipMemcpyAsync(d_chan, d_init, sizeof(B3Hasher), hipMemcpyDeviceToDevice, ctx->stream));
-        ZS_TRY(launch_channel_draw(ctx, d_chan, d_root, d_gamma));
-    } else if (flags & ZS_FLAG_PLAIN) {
+    if (flags & ZS_FLAG_PLAIN) {
         ZS_TRY(zs_m31_lde_commit_dev(h, d_cols, n_cols, log_n, log_blowup, d_lde, d_nodes, ZS_FLAG_PLAIN));
         ZS_HIP(ctx, hipMemcpyAsync(d_chan, d_init, sizeof(B3Hasher), hipMemcpyDeviceToDevice, ctx->stream));
         ZS_TRY(launch_channel_draw(ctx, d_chan, d_root, d_gamma));
@@ -205,16 +179,16 @@ int32_t persistent_groups(Ctx *ctx, const zs_job *jobs, size_t n_jobs, uint32_t 
         const zs_job &j = jobs[i];
         if (job_shape_ok(j.n_cols, j.log_n, log_blowup, remainder_log) &&
             persistent_supported(j.n_cols, j.log_n, log_blowup) && j.d_cols && j.d_lde && j.d_nodes &&
-            persistent_pool_bytes(j.n_cols, j.log_n, log_blowup) <= budget)
+            persistent_pool_bytes(j.n_cols, j.log_n, log_blowup, remainder_log) <= budget)
             take.push_back(i);
     }
     std::stable_sort(take.begin(), take.end(), [&](size_t a, size_t b) {
-        return persistent_pool_bytes(jobs[a].n_cols, jobs[a].log_n, log_blowup) >
-               persistent_pool_bytes(jobs[b].n_cols, jobs[b].log_n, log_blowup);
+        return persistent_pool_bytes(jobs[a].n_cols, jobs[a].log_n, log_blowup, remainder_log) >
+               persistent_pool_bytes(jobs[b].n_cols, jobs[b].log_n, log_blowup, remainder_log);
     });
     size_t used = 0;
     for (size_t i : take) {
-        const size_t b = persistent_pool_bytes(jobs[i].n_cols, jobs[i].log_n, log_blowup);
+        const size_t b = persistent_pool_bytes(jobs[i].n_cols, jobs[i].log_n, log_blowup, remainder_log);
         if (groups->empty() || used + b > budget) {
             groups->emplace_back();
             used = 0;
@@ -226,6 +200,28 @@ int32_t persistent_groups(Ctx *ctx, const zs_job *jobs, size_t n_jobs, uint32_t 
 }
 
 }  // namespace
+
+// the starting channel state on the device, built once per cache (outside its lock, like
+// the twiddle tables) so each job starts its channel with a device-to-device copy
+int32_t zs::device_commit_channel(Ctx *ctx, const B3Hasher **out) {
+    DeviceCaches &dc = *ctx->caches;
+    {
+        std::lock_guard<std::mutex> lk(dc.mu);
+        if (dc.commit_channel) {
+            *out = dc.commit_channel->as<B3Hasher>();
+            return ZS_OK;
+        }
+    }
+    auto buf = std::make_unique<DevBuf>();
+    ZS_HIP(ctx, hipMalloc(&buf->ptr, sizeof(B3Hasher)));
+    buf->bytes = sizeof(B3Hasher);
+    ZS_HIP(ctx, hipMemcpyAsync(buf->ptr, &commit_channel(), sizeof(B3Hasher), hipMemcpyHostToDevice, ctx->stream));
+    ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::lock_guard<std::mutex> lk(dc.mu);
+    if (!dc.commit_channel) dc.commit_channel = std::move(buf);
+    *out = dc.commit_channel->as<B3Hasher>();
+    return ZS_OK;
+}
 
 extern "C" {
 
@@ -241,18 +237,18 @@ int32_t zs_commit_batch_dev(zs_hip_ctx *h, zs_job *jobs, size_t n_jobs, uint32_t
     if (!h || (!jobs && n_jobs) || n_workers > 64) return ZS_ERR_INPUT;
     if (n_jobs == 0) return ZS_OK;
     Ctx *ctx = &h->c;
-    // ZS_FLAG_PERSISTENT: LDE, row leaves and node levels of the batch in one persistent
-    // launch per memory group; then every job's gamma draw, combination and FRI below
-    std::vector<uint8_t> tree_done(n_jobs, 0);
+    // ZS_FLAG_PERSISTENT: the whole commitment of the jobs it takes in one persistent
+    // launch per memory group; the others below, on the workers
+    std::vector<uint8_t> done(n_jobs, 0);
     if ((flags & ZS_FLAG_PERSISTENT) && !(flags & ZS_FLAG_PLAIN)) {
         ZS_HIP(ctx, hipSetDevice(ctx->device));
         ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));  // inputs written on the caller's stream are complete
         std::vector<std::vector<size_t>> groups;
         ZS_TRY(persistent_groups(ctx, jobs, n_jobs, log_blowup, remainder_log, &groups));
         for (const auto &g : groups) {
-            const int32_t rc = persistent_lde_trees(ctx, jobs, g, log_blowup);
+            const int32_t rc = persistent_commit(ctx, jobs, g, log_blowup, remainder_log);
             for (size_t i : g) {
-                if (rc == ZS_OK) tree_done[i] = 1;
+                if (rc == ZS_OK) done[i] = 1;
                 else jobs[i].status = rc;
             }
             if (rc != ZS_OK) return rc;
@@ -262,10 +258,14 @@ int32_t zs_commit_batch_dev(zs_hip_ctx *h, zs_job *jobs, size_t n_jobs, uint32_t
         zs_job &j = jobs[i];
         return commit_job(w, j.d_cols, j.n_cols, j.log_n, log_blowup, remainder_log, j.d_lde, j.d_nodes,
                           stage ? nullptr : j.trace_root, stage ? nullptr : j.fri_roots,
-                          stage ? nullptr : j.remainder, flags, stage, tree_done[i] != 0);
+                          stage ? nullptr : j.remainder, flags, stage);
     };
-    if (n_workers <= 1 || (flags & ZS_FLAG_PLAIN)) {
-        for (size_t i = 0; i < n_jobs; i++) jobs[i].status = run_one(h, i, nullptr);
+    std::vector<size_t> rest;
+    for (size_t i = 0; i < n_jobs; i++)
+        if (!done[i]) rest.push_back(i);
+    if (rest.empty()) {
+    } else if (n_workers <= 1 || (flags & ZS_FLAG_PLAIN)) {
+        for (size_t i : rest) jobs[i].status = run_one(h, i, nullptr);
     } else {
         ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));  // inputs written on the caller's stream are complete
         auto &ws = ctx->workers;
@@ -275,8 +275,7 @@ int32_t zs_commit_batch_dev(zs_hip_ctx *h, zs_job *jobs, size_t n_jobs, uint32_t
             ws.push_back(w);
         }
         // largest traces first, so that no large trace starts last and sets the tail
-        std::vector<size_t> order(n_jobs);
-        for (size_t i = 0; i < n_jobs; i++) order[i] = i;
+        std::vector<size_t> order = rest;
         std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
             const uint64_t ea = uint64_t(jobs[a].n_cols) << std::min(jobs[a].log_n, 40u);
             const uint64_t eb = uint64_t(jobs[b].n_cols) << std::min(jobs[b].log_n, 40u);
@@ -305,7 +304,7 @@ int32_t zs_commit_batch_dev(zs_hip_ctx *h, zs_job *jobs, size_t n_jobs, uint32_t
                 for (size_t k;;) {
                     if (ev_ok && queued >= kInFlight && hipEventSynchronize(ev[queued % kInFlight]) != hipSuccess)
                         ev_ok = false;
-                    if ((k = next.fetch_add(1)) >= n_jobs) break;
+                    if ((k = next.fetch_add(1)) >= order.size()) break;
                     const size_t i = order[k];
                     ran_on[i] = t;
                     jobs[i].status = run_one(w, i, stage + i * kStageBytes);
@@ -321,7 +320,7 @@ int32_t zs_commit_batch_dev(zs_hip_ctx *h, zs_job *jobs, size_t n_jobs, uint32_t
             });
         }
         for (auto &x : th) x.join();
-        for (size_t i = 0; i < n_jobs; i++) {
+        for (size_t i : rest) {
             zs_job &j = jobs[i];
             if (stream_failed[ran_on[i]]) j.status = ZS_ERR_DEVICE;  // no result of that stream is complete
             if (j.status != ZS_OK) continue;

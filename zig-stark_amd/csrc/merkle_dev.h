@@ -6,6 +6,27 @@
 
 namespace zs {
 
+// a digest in global memory
+__device__ __forceinline__ void ld_digest_g(const uint32_t *p, uint32_t d[8]) {
+    const uint4 a = ldg4(p), b = ldg4(p + 4);
+    d[0] = a.x; d[1] = a.y; d[2] = a.z; d[3] = a.w; d[4] = b.x; d[5] = b.y; d[6] = b.z; d[7] = b.w;
+}
+__device__ __forceinline__ void st_digest_g(uint32_t *p, const uint32_t d[8]) {
+    stg4(p, make_uint4(d[0], d[1], d[2], d[3]));
+    stg4(p + 4, make_uint4(d[4], d[5], d[6], d[7]));
+}
+// a digest in global memory or LDS
+__device__ __forceinline__ void ld_digest(const uint32_t *p, uint32_t d[8]) {
+    const uint4 *q = reinterpret_cast<const uint4 *>(p);
+    uint4 a = q[0], b = q[1];
+    d[0] = a.x; d[1] = a.y; d[2] = a.z; d[3] = a.w; d[4] = b.x; d[5] = b.y; d[6] = b.z; d[7] = b.w;
+}
+__device__ __forceinline__ void st_digest(uint32_t *p, const uint32_t d[8]) {
+    uint4 *q = reinterpret_cast<uint4 *>(p);
+    q[0] = make_uint4(d[0], d[1], d[2], d[3]);
+    q[1] = make_uint4(d[4], d[5], d[6], d[7]);
+}
+
 // BLAKE3 of one chunk (<= 256 columns = 1024 bytes) of a row.
 __device__ __forceinline__ void row_chunk_cv(const uint32_t *cols, uint64_t cs, uint64_t row, uint32_t c0,
                                              uint32_t ncols, uint32_t chunk_idx, bool root, uint32_t cv[8]) {
@@ -17,7 +38,7 @@ __device__ __forceinline__ void row_chunk_cv(const uint32_t *cols, uint64_t cs, 
 #pragma unroll
         for (int w = 0; w < 16; w++) {
             uint32_t c = b * 16 + w;
-            m[w] = c < ncols ? cols[(uint64_t)(c0 + c) * cs + row] : 0u;
+            m[w] = c < ncols ? ldg(cols + (uint64_t)(c0 + c) * cs + row) : 0u;
         }
         uint32_t blen = (len - b * 64) < 64 ? (len - b * 64) : 64;
         uint32_t flags = (b == 0 ? B3_CHUNK_START : 0) | (b == nblocks - 1 ? (B3_CHUNK_END | (root ? B3_ROOT : 0)) : 0);
@@ -30,9 +51,7 @@ __device__ __forceinline__ void row_leaf_body(const uint32_t *cols, uint64_t cs,
                                               uint32_t *nodes) {
     uint32_t cv[8];
     row_chunk_cv(cols, cs, row, 0, n_cols, 0, true, cv);
-    uint4 *o = reinterpret_cast<uint4 *>(nodes + row * 8);
-    o[0] = make_uint4(cv[0], cv[1], cv[2], cv[3]);
-    o[1] = make_uint4(cv[4], cv[5], cv[6], cv[7]);
+    st_digest_g(nodes + row * 8, cv);
 }
 
 // rows of more than 256 columns: BLAKE3 chunk tree with a chaining-value stack
@@ -68,17 +87,6 @@ __device__ __forceinline__ void row_leaf_multichunk_body(const uint32_t *cols, u
     o[1] = make_uint4(cv[4], cv[5], cv[6], cv[7]);
 }
 
-__device__ __forceinline__ void ld_digest(const uint32_t *p, uint32_t d[8]) {
-    const uint4 *q = reinterpret_cast<const uint4 *>(p);
-    uint4 a = q[0], b = q[1];
-    d[0] = a.x; d[1] = a.y; d[2] = a.z; d[3] = a.w; d[4] = b.x; d[5] = b.y; d[6] = b.z; d[7] = b.w;
-}
-__device__ __forceinline__ void st_digest(uint32_t *p, const uint32_t d[8]) {
-    uint4 *q = reinterpret_cast<uint4 *>(p);
-    q[0] = make_uint4(d[0], d[1], d[2], d[3]);
-    q[1] = make_uint4(d[4], d[5], d[6], d[7]);
-}
-
 // NLEV (1..3) levels per launch with no idle lanes: lane i owns the subtree
 // over children 2^NLEV * i .. 2^NLEV * (i + 1) - 1 and writes all its nodes.
 template <int NLEV>
@@ -89,22 +97,22 @@ __device__ __forceinline__ void subtree_lane(const uint32_t *child, uint32_t *lv
     uint32_t a[8], b[8], n1[W / 2][8];
 #pragma unroll
     for (int k = 0; k < W / 2; k++) {
-        ld_digest(child + (i * W + 2 * k) * 8, a);
-        ld_digest(child + (i * W + 2 * k + 1) * 8, b);
+        ld_digest_g(child + (i * W + 2 * k) * 8, a);
+        ld_digest_g(child + (i * W + 2 * k + 1) * 8, b);
         b3_hash2(a, b, n1[k]);
-        st_digest(lvl1 + (i * (W / 2) + k) * 8, n1[k]);
+        st_digest_g(lvl1 + (i * (W / 2) + k) * 8, n1[k]);
     }
     if constexpr (NLEV >= 2) {
         uint32_t n2[W / 4][8];
 #pragma unroll
         for (int k = 0; k < W / 4; k++) {
             b3_hash2(n1[2 * k], n1[2 * k + 1], n2[k]);
-            st_digest(lvl2 + (i * (W / 4) + k) * 8, n2[k]);
+            st_digest_g(lvl2 + (i * (W / 4) + k) * 8, n2[k]);
         }
         if constexpr (NLEV >= 3) {
             uint32_t n3[8];
             b3_hash2(n2[0], n2[1], n3);
-            st_digest(lvl3 + i * 8, n3);
+            st_digest_g(lvl3 + i * 8, n3);
         }
     }
 }
@@ -186,15 +194,15 @@ __device__ __forceinline__ void tree_block_body(uint32_t *base, const BT &bt, ui
                 ld_digest(src + 16 * p, x);
                 ld_digest(src + 16 * p + 8, y);
                 b3_hash2(x, y, d);
-                st_digest(out + 8 * p, d);
+                st_digest_g(out + 8 * p, d);
                 st_digest(dst + 8 * p, d);
             }
         } else {
             for (uint32_t p = quad; p < P; p += NQ) {
                 uint32_t lo, hi;
                 quad_hash2(src + 16 * p, qm, mo, j, iv_lo, iv_hi, lo, hi);
-                out[8 * p + j] = lo;
-                out[8 * p + 4 + j] = hi;
+                stg(out + 8 * p + j, lo);
+                stg(out + 8 * p + 4 + j, hi);
                 dst[8 * p + j] = lo;
                 dst[8 * p + 4 + j] = hi;
             }

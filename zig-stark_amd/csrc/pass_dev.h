@@ -57,7 +57,7 @@ __device__ __forceinline__ void fft_pass_body(const FftPass &p, const VB vb, uin
             case IN_NATURAL: addr = bitrev32(k, p.L); break;
             default: addr = (k & 1u) ? (half_n + (k >> 1)) : (k >> 1); break;
         }
-        uint32_t v = in_col[addr];
+        uint32_t v = ldg(in_col + addr);
         if (p.in_mode == IN_SHARED) {
             for (uint32_t qq = 0; qq < Q; qq++) lds[(qq * G + g) * TP + t] = v;
         } else {
@@ -80,7 +80,7 @@ __device__ __forceinline__ void fft_pass_body(const FftPass &p, const VB vb, uin
             uint32_t t0 = ((j >> ls) << (ls + 1)) | (j & ((1u << ls) - 1));
             uint32_t t1 = t0 | (1u << ls);
             uint32_t K = ((qblk + ql) << p.L) | fixed(g) | (t0 << p.s_lo);
-            uint32_t w = tws[K >> (s + 1)];
+            uint32_t w = ldg(tws + (K >> (s + 1)));
             uint32_t *base = lds + tile * TP;
             uint32_t v0 = base[t0], v1 = base[t1];
             if (INV) {
@@ -126,7 +126,7 @@ __device__ __forceinline__ void fft_pass_body(const FftPass &p, const VB vb, uin
             case OUT_NATURAL: addr = ((uint64_t)bitrev32(k, p.L) << p.beta) | bitrev32(q, p.beta); break;
             default: addr = (k & 1u) ? (half_n + (k >> 1)) : (k >> 1); break;
         }
-        out_col[addr] = v;
+        stg(out_col + addr, v);
     }
 }
 

@@ -23,6 +23,7 @@
 #include <cstring>
 #include <vector>
 
+#include "fri_dev.h"
 #include "merkle_dev.h"
 #include "pass_dev.h"
 #include "tile_dev.h"
@@ -39,10 +40,21 @@ enum PKind : uint32_t {
     PK_LEAF,              // row leaves (<= 256 columns: one BLAKE3 chunk per row)
     PK_SUBTREE3,          // three node levels, one lane per 8-child subtree
     PK_BLOCK,             // up to eight node levels, one workgroup per subtree
+    PK_DRAW,              // absorb a root into the trace's channel, draw a challenge (one workgroup)
+    PK_POWERS,            // gamma^c for every column (one workgroup)
+    PK_COMBINE,           // g = sum_c gamma^c lde_c and g's leaves (FRI layer 0)
+    PK_FOLD,              // one FRI fold and the next layer's leaves
+    PK_REMAINDER,         // the remainder coefficients; the roots gathered for the host
 };
 
 constexpr uint32_t kPThreads = 512;
-constexpr uint32_t kMaxPhases = 64;
+constexpr uint32_t kMaxPhases = 192;
+// Workgroups are dealt to the 8 XCDs round-robin by index, and each XCD has its own L2.
+// A phase's task list is cut into 8 contiguous ranges, one per XCD: neighbouring tile
+// groups, whose 32-byte runs share cache lines, then meet in one L2 (as xcd_block
+// arranges for the single-trace kernels).  A workgroup whose XCD's range is used up
+// takes tasks from the other ranges in turn.
+constexpr uint32_t kXcds = 8;
 
 struct LeafArgs {
     const uint32_t *cols;
@@ -59,6 +71,35 @@ struct BlockArgs {
     uint32_t *base;
     OneTree bt;
 };
+struct DrawArgs {
+    B3Hasher *chan;
+    const B3Hasher *init;  // or null: continue the channel's transcript
+    const uint32_t *digest;
+    uint32_t *out;
+};
+struct PowArgs {
+    const uint32_t *x;
+    uint32_t *out;
+    uint32_t n;
+};
+struct CombArgs {
+    const uint32_t *lde, *powers;
+    uint64_t cs, n;
+    uint32_t *g, *leaves;
+    uint32_t n_cols;
+};
+struct FoldArgs {
+    const uint32_t *cur, *tables, *alpha;
+    uint32_t *next, *next_leaves;  // next_leaves: null for the last layer
+    uint64_t half;
+    qm31 inv_off;  // (FRI_OFFSET^(2^layer))^-1
+    uint32_t layer;
+};
+struct RemArgs {
+    const uint32_t *last, *mat, *trees, *row_root;
+    uint32_t *out;  // trace root, the L layer roots, the rc coefficients
+    uint32_t rc, L, dlog;
+};
 
 struct PStage {
     uint32_t kind, C, gx, gy;  // virtual grid of a pass stage: x fastest, then y, then z
@@ -69,13 +110,18 @@ struct PStage {
         LeafArgs leaf;
         SubArgs sub;
         BlockArgs blk;
+        DrawArgs draw;
+        PowArgs pow;
+        CombArgs comb;
+        FoldArgs fold;
+        RemArgs rem;
     } u;
 };
 
 struct PCtl {
     uint32_t arrive;  // workgroups that reached a barrier, summed over the barriers
     uint32_t abort;   // a barrier wait ran past its bound: the results are incomplete
-    unsigned long long claim[kMaxPhases];  // next task of each phase
+    unsigned long long claim[kMaxPhases][kXcds];  // next task of each phase's per-XCD range
 };
 
 struct PPlan {
@@ -122,7 +168,8 @@ __device__ bool grid_barrier(const PPlan &pl, uint32_t k, uint32_t *s_ok) {
 // A task's descriptor with every word made wave-uniform (scalar registers): the task
 // functions below are called, not inlined -- one register allocation per body, as in
 // the single-trace kernels, where inlining them all into one kernel spilled -- and a
-// call passes its arguments in vector registers.
+// call passes its arguments in vector registers.  The bodies reach HBM through
+// ldg / stg, since pointers read from a descriptor are not known to be global.
 template <class T>
 __device__ __forceinline__ T uniform_copy(const T &src) {
     static_assert(sizeof(T) % 4 == 0, "whole words");
@@ -145,22 +192,25 @@ __device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
            __builtin_amdgcn_readfirstlane((uint32_t)v);
 }
 
+__device__ __forceinline__ FftPass pass_desc(const PStage *sp) { return uniform_copy(sp->u.p); }
+
 #define ZS_TASK __device__ __attribute__((noinline))
 template <int C>
 ZS_TASK void task_interp_first(const PStage *sp, uint64_t u) {
-    lde_interp_first_body<C>(uniform_copy(sp->u.p), task_vb(*sp, uniform_u64(u)));
+    lde_interp_first_body<C>(pass_desc(sp), task_vb(*sp, uniform_u64(u)));
 }
 template <int C>
 ZS_TASK void task_mid(const PStage *sp, uint64_t u) {
-    lde_mid_body<C>(uniform_copy(sp->u.f), task_vb(*sp, uniform_u64(u)));
+    const LdeFused f = uniform_copy(sp->u.f);
+    lde_mid_body<C>(f, task_vb(*sp, uniform_u64(u)));
 }
 template <int C>
 ZS_TASK void task_eval_last(const PStage *sp, uint64_t u) {
-    lde_eval_last_body<C>(uniform_copy(sp->u.p), task_vb(*sp, uniform_u64(u)));
+    lde_eval_last_body<C>(pass_desc(sp), task_vb(*sp, uniform_u64(u)));
 }
 template <bool INV>
 ZS_TASK void task_pass(const PStage *sp, uint64_t u) {
-    fft_pass_body<INV>(uniform_copy(sp->u.p), task_vb(*sp, uniform_u64(u)), threadIdx.x, kPThreads);
+    fft_pass_body<INV>(pass_desc(sp), task_vb(*sp, uniform_u64(u)), threadIdx.x, kPThreads);
 }
 ZS_TASK void task_leaves(const PStage *sp, uint64_t u) {
     const LeafArgs a = uniform_copy(sp->u.leaf);
@@ -175,6 +225,65 @@ ZS_TASK void task_subtree3(const PStage *sp, uint64_t u) {
 ZS_TASK void task_block(const PStage *sp, uint64_t u) {
     const BlockArgs a = uniform_copy(sp->u.blk);
     tree_block_body<kPThreads>(a.base, a.bt, (uint32_t)uniform_u64(u), threadIdx.x);
+}
+ZS_TASK void task_draw(const PStage *sp, uint64_t) {
+    extern __shared__ uint32_t lds[];
+    const DrawArgs a = uniform_copy(sp->u.draw);
+    constexpr uint32_t HW = (sizeof(B3Hasher) / 4 + 1) & ~1u;  // hasher states 8-byte aligned
+    B3Hasher *h = reinterpret_cast<B3Hasher *>(lds), *snap = reinterpret_cast<B3Hasher *>(lds + HW);
+    channel_draw_body(a.chan, a.init, a.digest, a.out, *h, *snap, lds + 2 * HW, lds + 2 * HW + 24, threadIdx.x,
+                      kPThreads);
+}
+ZS_TASK void task_powers(const PStage *sp, uint64_t) {
+    const PowArgs a = uniform_copy(sp->u.pow);
+    const uint32_t c = threadIdx.x;
+    if (c >= a.n) return;
+    qm31 b = ldg_qm(a.x), r = {{1, 0}, {0, 0}};
+    for (uint32_t e = c; e; e >>= 1) {
+        if (e & 1u) r = qm_mul(r, b);
+        b = qm_mul(b, b);
+    }
+    stg_qm(a.out + 4 * c, r);
+}
+ZS_TASK void task_combine(const PStage *sp, uint64_t u) {
+    extern __shared__ uint32_t lds[];
+    const CombArgs a = uniform_copy(sp->u.comb);
+    for (uint32_t k = threadIdx.x; k < 4 * a.n_cols; k += kPThreads) lds[k] = ldg(a.powers + k);
+    __syncthreads();
+    const uint64_t i = uniform_u64(u) * kPThreads + threadIdx.x;
+    if (i >= a.n) return;
+    const uint4 v = combine_row(a.lde, a.cs, a.n_cols, i, lds);
+    stg4(a.g + 4 * i, v);
+    store_qm31_leaf(a.leaves + 8 * i, qm31{{v.x, v.y}, {v.z, v.w}});
+}
+ZS_TASK void task_fold(const PStage *sp, uint64_t u) {
+    const FoldArgs a = uniform_copy(sp->u.fold);
+    const uint64_t j = uniform_u64(u) * kPThreads + threadIdx.x;
+    if (j >= a.half) return;
+    const qm31 a_off = qm_half(qm_mul(ldg_qm(a.alpha), a.inv_off));
+    const qm31 v = fold_value(a.cur, a.half, j, a.layer, a.tables, a_off);
+    stg_qm(a.next + 4 * j, v);
+    if (a.next_leaves) store_qm31_leaf(a.next_leaves + 8 * j, v);
+}
+// threads [0, 8): the trace root; [8, 8 + 8L): layer i's root word w; then the coefficients
+// (fri.zig:228-233: the interpolant of layer L's first rc values, a fixed matrix)
+ZS_TASK void task_remainder(const PStage *sp, uint64_t) {
+    const RemArgs a = uniform_copy(sp->u.rem);
+    const uint32_t t = threadIdx.x;
+    if (t < 8) {
+        stg(a.out + t, ldg(a.row_root + t));
+    } else if (t < 8 + 8 * a.L) {
+        const uint32_t i = (t - 8) >> 3, w = (t - 8) & 7u;
+        uint64_t off = 0;  // words before tree i
+        for (uint32_t k = 0; k < i; k++) off += ((uint64_t(2) << (a.dlog - k)) - 1) * 8;
+        off += ((uint64_t(2) << (a.dlog - i)) - 1) * 8 - 8;  // its root
+        stg(a.out + t, ldg(a.trees + off + w));
+    } else if (t < 8 + 8 * a.L + a.rc) {
+        const uint32_t k = t - 8 - 8 * a.L;
+        qm31 acc = {{0, 0}, {0, 0}};
+        for (uint32_t j = 0; j < a.rc; j++) acc = qm_add(acc, qm_mul(ldg_qm(a.mat + 4 * (k * a.rc + j)), ldg_qm(a.last + 4 * j)));
+        stg_qm(a.out + 8 + 8 * a.L + 4 * k, acc);
+    }
 }
 #undef ZS_TASK
 
@@ -200,30 +309,58 @@ __device__ __forceinline__ void run_task(const PStage *sp, uint64_t u) {
         case PK_PASS_FWD: task_pass<false>(sp, u); break;
         case PK_LEAF: task_leaves(sp, u); break;
         case PK_SUBTREE3: task_subtree3(sp, u); break;
-        default: task_block(sp, u); break;  // PK_BLOCK
+        case PK_BLOCK: task_block(sp, u); break;
+        case PK_DRAW: task_draw(sp, u); break;
+        case PK_POWERS: task_powers(sp, u); break;
+        case PK_COMBINE: task_combine(sp, u); break;
+        case PK_FOLD: task_fold(sp, u); break;
+        default: task_remainder(sp, u); break;  // PK_REMAINDER
     }
 }
 
 __global__ __launch_bounds__(kPThreads, 4) void persistent_batch_kernel(const PPlan pl) {
-    __shared__ uint32_t s_go, s_stage, s_ok;
+    __shared__ uint32_t s_go, s_stage, s_ok, s_src, s_cur;
     __shared__ unsigned long long s_local;
-    const uint32_t tid = threadIdx.x;
+    const uint32_t tid = threadIdx.x, xcd = blockIdx.x % kXcds;
     for (uint32_t ph = 0; ph < pl.n_phases; ph++) {
         const uint32_t f0 = pl.phase_first[ph], f1 = pl.phase_first[ph + 1];
         const uint64_t total = pl.stages[f1 - 1].end;
+        if (tid == 0) {
+            s_src = 0;     // ranges tried so far: own XCD's first
+            s_cur = ~0u;   // the stage of this workgroup's last task from range s_src (none yet)
+        }
         for (;;) {
             if (tid == 0) {
-                const unsigned long long t = atomicAdd(&pl.ctl->claim[ph], 1ull);
-                uint32_t go = t < total && !ld_relaxed(&pl.ctl->abort);
-                if (go) {
-                    uint32_t lo = f0, hi = f1 - 1;  // the first stage whose end exceeds t
-                    while (lo < hi) {
-                        const uint32_t mid = (lo + hi) / 2;
-                        if (pl.stages[mid].end > t) hi = mid;
-                        else lo = mid + 1;
+                uint32_t go = 0;
+                unsigned long long t = 0;
+                while (!go && s_src < kXcds && !ld_relaxed(&pl.ctl->abort)) {
+                    const uint32_t r = (xcd + s_src) % kXcds;
+                    const unsigned long long lo = total * r / kXcds, hi = total * (r + 1) / kXcds;
+                    if (lo < hi) t = lo + atomicAdd(&pl.ctl->claim[ph][r], 1ull);
+                    if (lo < hi && t < hi) {
+                        go = 1;
+                    } else {
+                        s_src++;
+                        s_cur = ~0u;
                     }
-                    s_stage = lo;
-                    s_local = t - (lo > f0 ? pl.stages[lo - 1].end : 0);
+                }
+                if (go) {
+                    // the stage of task t: tasks from one range come in increasing order, so
+                    // after the first (a binary search) the cursor moves forward
+                    uint32_t st = s_cur;
+                    if (st == ~0u) {
+                        uint32_t lo = f0, hi = f1 - 1;  // the first stage whose end exceeds t
+                        while (lo < hi) {
+                            const uint32_t mid = (lo + hi) / 2;
+                            if (pl.stages[mid].end > t) hi = mid;
+                            else lo = mid + 1;
+                        }
+                        st = lo;
+                    }
+                    while (pl.stages[st].end <= t) st++;
+                    s_cur = st;
+                    s_stage = st;
+                    s_local = t - (st > f0 ? pl.stages[st - 1].end : 0);
                 }
                 s_go = go;
             }
@@ -263,10 +400,24 @@ uint64_t stage_tasks(const PStage &s) {
             return (s.u.leaf.n_rows + kPThreads - 1) / kPThreads;
         case PK_SUBTREE3:
             return (s.u.sub.lanes + kPThreads - 1) / kPThreads;
-        default:
+        case PK_BLOCK:
             return s.u.blk.bt.wg_end[0];
+        case PK_COMBINE:
+            return (s.u.comb.n + kPThreads - 1) / kPThreads;
+        case PK_FOLD:
+            return (s.u.fold.half + kPThreads - 1) / kPThreads;
+        default:  // PK_DRAW, PK_POWERS, PK_REMAINDER: one workgroup
+            return 1;
     }
 }
+
+// columns one pass task runs with one set of staged twiddles (the pass bodies overlap a
+// column's loads with the previous column's butterflies, so more columns per task also
+// amortise the claim and the call)
+#ifndef ZS_TASK_COLS
+#define ZS_TASK_COLS 8
+#endif
+constexpr uint32_t kTaskCols = ZS_TASK_COLS;
 
 uint32_t ncb_for(uint32_t n_cols, uint32_t cols_per_wg) { return std::max<uint32_t>(1, std::min(cols_per_wg, n_cols)); }
 
@@ -287,9 +438,6 @@ PStage generic_pass(const FftPass &p0, bool inverse, uint32_t n_cols, size_t *ld
     return s;
 }
 
-// LDE stages of one trace: in (c x n, stride n) -> out (c x N, stride N).  coef holds c x n
-// words, big c x N words; they do not overlap each other, and coef may lie in out only when
-// the forward transform has more than one pass.
 // the pipelined kernels' shapes (two passes per direction of 8..10 stages), with the fused
 // middle's tiles and 1 + 2^beta twiddle sets small enough for two workgroups per CU
 bool pipelined_lde(uint32_t L, uint32_t beta) {
@@ -298,11 +446,13 @@ bool pipelined_lde(uint32_t L, uint32_t beta) {
     return (size_t(8) * tile_stride(C, 8) + (((size_t(1) << beta) + 1) << C)) * 4 <= 80 * 1024;
 }
 
+// LDE stages of one trace: in (c x n, stride n) -> out (c x N, stride N).  coef holds c x n
+// words, big c x N words; they do not overlap each other, and coef may lie in out only when
+// the forward transform has more than one pass.
 void lde_stages(const zs_job &j, uint32_t beta, const uint32_t *tw_inv, const uint32_t *tw_fwd, uint32_t *coef,
                 uint32_t *big, std::vector<PStage> &out, size_t *lds) {
     const uint32_t L = j.log_n, LT = L + beta, c = j.n_cols;
     const uint64_t n = uint64_t(1) << L, N = uint64_t(1) << LT;
-    const LdeTiles tiles = default_lde_tiles();
     const auto pi = plan_passes(L, true, PASS_CMAX), pf = plan_passes(L, false, PASS_CMAX);
     if (pipelined_lde(L, beta)) {
         // interpolation's first pass: 8 tiles (waves) per workgroup
@@ -313,7 +463,7 @@ void lde_stages(const zs_job &j, uint32_t beta, const uint32_t *tw_inv, const ui
             p.in = j.d_cols; p.in_cs = n; p.out = coef; p.out_cs = n; p.tw = tw_inv;
             p.L = L; p.LT = L; p.beta = 0; p.s_lo = pi[0].s_lo; p.C = pi[0].C; p.G_log = 3; p.Q_log = 0;
             p.in_mode = IN_NATURAL; p.out_mode = OUT_KSPACE; p.natural_tiles = 1; p.scale = 0;
-            p.n_cols = c; p.ncb = ncb_for(c, tiles.role[LDE_INTERP].cols_per_wg);
+            p.n_cols = c; p.ncb = ncb_for(c, kTaskCols);
             p.TS = tile_stride(p.C, 8);
             s.C = p.C;
             s.gx = (1u << (L - p.C)) >> 3;
@@ -331,7 +481,7 @@ void lde_stages(const zs_job &j, uint32_t beta, const uint32_t *tw_inv, const ui
             p.in = coef; p.in_cs = n; p.out = big; p.out_cs = N;
             p.L = L; p.LT = L; p.beta = beta; p.s_lo = pf[0].s_lo; p.C = pf[0].C; p.scale = 0;
             p.G_log = 3; p.Q_log = 0; p.in_mode = IN_KSPACE; p.out_mode = OUT_KSPACE; p.natural_tiles = 0;
-            p.n_cols = c; p.ncb = ncb_for(c, tiles.role[LDE_MID].cols_per_wg);
+            p.n_cols = c; p.ncb = ncb_for(c, kTaskCols);
             p.TS = tile_stride(p.C, 8);
             f.tw_inv = tw_inv; f.tw_fwd = tw_fwd; f.inv_n = m31_inv(uint32_t(1) << L);
             s.C = p.C;
@@ -349,7 +499,7 @@ void lde_stages(const zs_job &j, uint32_t beta, const uint32_t *tw_inv, const ui
             p.L = L; p.LT = LT; p.beta = beta; p.s_lo = pf[1].s_lo; p.C = pf[1].C;
             p.Q_log = std::min<uint32_t>(beta, 3); p.G_log = 3 - p.Q_log;
             p.in_mode = IN_KSPACE; p.out_mode = OUT_NATURAL; p.natural_tiles = 1; p.scale = 0;
-            p.n_cols = c; p.ncb = ncb_for(c, tiles.role[LDE_EVAL].cols_per_wg);
+            p.n_cols = c; p.ncb = ncb_for(c, kTaskCols);
             p.TS = tile_stride(p.C, 8);
             s.C = p.C;
             s.gx = (1u << (L - p.C)) >> p.G_log;
@@ -383,17 +533,10 @@ void lde_stages(const zs_job &j, uint32_t beta, const uint32_t *tw_inv, const ui
     }
 }
 
-// the row leaves and node levels of one trace (merkle.hip launch_trees_multi per tree:
-// 3-level lane subtrees while 2^18 children or more remain, then 8-level workgroup subtrees)
-void tree_stages(const zs_job &j, uint32_t beta, std::vector<PStage> &out, size_t *lds) {
-    const uint32_t LT = j.log_n + beta;
-    uint32_t *nodes = reinterpret_cast<uint32_t *>(j.d_nodes);
-    {
-        PStage s{};
-        s.kind = PK_LEAF;
-        s.u.leaf = LeafArgs{j.d_lde, uint64_t(1) << LT, uint64_t(1) << LT, j.n_cols, nodes};
-        out.push_back(s);
-    }
+// the node levels of a tree of 2^LT leaves at `nodes` (merkle.hip launch_trees_multi for
+// one tree: 3-level lane subtrees while 2^18 children or more remain, then 8-level
+// workgroup subtrees)
+void level_stages(uint32_t *nodes, uint32_t LT, std::vector<PStage> &out, size_t *lds) {
     uint32_t level = 0;
     while (LT - level >= 18) {
         PStage s{};
@@ -422,6 +565,97 @@ void tree_stages(const zs_job &j, uint32_t beta, std::vector<PStage> &out, size_
     }
 }
 
+// Device memory of a trace's FRI commitment (job.cpp commit_job's scratch), laid out in
+// the trace's pool region once its LDE no longer needs it: the channel, gamma and its
+// powers, layers 0..L (layer 0 = g), the L layer trees, the L challenges.
+struct FriLayout {
+    uint64_t chan, gamma, powers, layer[29], tree[28], alphas, total;
+};
+FriLayout fri_layout(uint32_t n_cols, uint32_t dlog, uint32_t L) {
+    FriLayout f{};
+    uint64_t o = 0;
+    auto take = [&](uint64_t bytes) {
+        const uint64_t at = o;
+        o += (bytes + 255) & ~uint64_t(255);
+        return at;
+    };
+    f.chan = take(sizeof(B3Hasher));
+    f.gamma = take(16);
+    f.powers = take(16 * uint64_t(n_cols));
+    for (uint32_t i = 0; i <= L; i++) f.layer[i] = take((uint64_t(16) << dlog) >> i);
+    // the trees back to back (the remainder task finds the roots by their sizes)
+    uint64_t tw = 0;
+    for (uint32_t i = 0; i < L; i++) tw += tree_words(dlog - i) * 4;
+    f.tree[0] = take(tw);
+    for (uint32_t i = 1; i < L; i++) f.tree[i] = f.tree[i - 1] + tree_words(dlog - i + 1) * 4;
+    f.alphas = take(16 * uint64_t(L));
+    f.total = o;
+    return f;
+}
+uint32_t result_words(uint32_t L, uint32_t rc) { return 8 + 8 * L + 4 * rc; }
+
+struct FriShared {
+    const B3Hasher *init;  // Channel.init("zig-stark:hip:commit") on the device
+    const uint32_t *tables, *mat;
+};
+
+// the commitment after the row tree (job.cpp commit_job, fri_commit_impl in channel
+// mode): gamma from the trace root, g and its leaves, then per layer its node levels,
+// the challenge drawn from its root and the fold (with the next layer's leaves); last the
+// remainder, with the roots gathered into `res`
+void fri_stages(const zs_job &j, uint32_t beta, uint32_t remainder_log, uint8_t *region, const FriShared &sh,
+                uint32_t *res, std::vector<PStage> &out, size_t *lds) {
+    const uint32_t dlog = j.log_n + beta, L = dlog - remainder_log, rc = 1u << (remainder_log - beta);
+    const uint64_t N = uint64_t(1) << dlog;
+    const FriLayout f = fri_layout(j.n_cols, dlog, L);
+    auto at = [&](uint64_t off) { return reinterpret_cast<uint32_t *>(region + off); };
+    B3Hasher *chan = reinterpret_cast<B3Hasher *>(region + f.chan);
+    const uint32_t *row_root = reinterpret_cast<const uint32_t *>(j.d_nodes) + tree_words(dlog) - 8;
+    constexpr size_t draw_lds = (2 * ((sizeof(B3Hasher) / 4 + 1) & ~size_t(1)) + 24 + 16) * 4;
+    auto draw = [&](const B3Hasher *init, const uint32_t *digest, uint32_t *dst) {
+        PStage s{};
+        s.kind = PK_DRAW;
+        s.u.draw = DrawArgs{chan, init, digest, dst};
+        *lds = std::max(*lds, draw_lds);
+        out.push_back(s);
+    };
+    draw(sh.init, row_root, at(f.gamma));
+    {
+        PStage s{};
+        s.kind = PK_POWERS;
+        s.u.pow = PowArgs{at(f.gamma), at(f.powers), j.n_cols};
+        out.push_back(s);
+    }
+    {
+        PStage s{};
+        s.kind = PK_COMBINE;
+        s.u.comb = CombArgs{j.d_lde, at(f.powers), N, N, at(f.layer[0]), at(f.tree[0]), j.n_cols};
+        *lds = std::max(*lds, size_t(16) * j.n_cols);
+        out.push_back(s);
+    }
+    for (uint32_t i = 0; i < L; i++) {
+        const uint32_t li = dlog - i;
+        level_stages(at(f.tree[i]), li, out, lds);
+        draw(nullptr, at(f.tree[i]) + tree_words(li) - 8, at(f.alphas) + 4 * i);
+        PStage s{};
+        s.kind = PK_FOLD;
+        FoldArgs &fa = s.u.fold;
+        fa.cur = at(f.layer[i]);
+        fa.tables = sh.tables;
+        fa.alpha = at(f.alphas) + 4 * i;
+        fa.next = at(f.layer[i + 1]);
+        fa.next_leaves = i + 1 < L ? at(f.tree[i + 1]) : nullptr;
+        fa.half = (N >> i) / 2;
+        fa.inv_off = qm_inv(fri_offset_pow(i));
+        fa.layer = i;
+        out.push_back(s);
+    }
+    PStage s{};
+    s.kind = PK_REMAINDER;
+    s.u.rem = RemArgs{at(f.layer[L]), sh.mat, at(f.tree[0]), row_root, res, rc, L, dlog};
+    out.push_back(s);
+}
+
 }  // namespace
 
 // rows of at most 256 columns (one BLAKE3 chunk: the chunk-tree leaf's stack would give
@@ -430,24 +664,36 @@ bool persistent_supported(uint32_t n_cols, uint32_t log_n, uint32_t log_blowup) 
     return n_cols >= 1 && n_cols <= 256 && log_n >= 3 && log_blowup <= 8 && !log_sum_over(log_n, log_blowup, 26);
 }
 
-size_t persistent_pool_bytes(uint32_t n_cols, uint32_t log_n, uint32_t log_blowup) {
-    // the big k-space array of the LDE (or, with one forward pass, the coefficients)
-    return ((size_t(n_cols) << (log_n + log_blowup)) * 4 + 255) & ~size_t(255);
+// a trace's pool region: the LDE's big k-space array (or, with one forward pass, its
+// coefficients), later its FRI commitment's buffers
+size_t persistent_pool_bytes(uint32_t n_cols, uint32_t log_n, uint32_t log_blowup, uint32_t remainder_log) {
+    const uint32_t dlog = log_n + log_blowup;
+    const uint64_t lde = (uint64_t(n_cols) << dlog) * 4;
+    const uint64_t fri = fri_layout(n_cols, dlog, dlog - remainder_log).total;
+    return (std::max(lde, fri) + 255) & ~uint64_t(255);
 }
 
-int32_t persistent_lde_trees(Ctx *ctx, zs_job *jobs, const std::vector<size_t> &idx, uint32_t log_blowup) {
+int32_t persistent_commit(Ctx *ctx, zs_job *jobs, const std::vector<size_t> &idx, uint32_t log_blowup,
+                          uint32_t remainder_log) {
     if (idx.empty()) return ZS_OK;
-    // device memory: one pool region per trace, then the plan
+    const uint32_t rc = 1u << (remainder_log - log_blowup);
+    // device memory: one pool region per trace, then the plan and the results
     size_t pool = 0;
     for (size_t i : idx) {
         const zs_job &j = jobs[i];
-        if (!persistent_supported(j.n_cols, j.log_n, log_blowup) || !j.d_cols || !j.d_lde || !j.d_nodes)
+        if (!persistent_supported(j.n_cols, j.log_n, log_blowup) || !j.d_cols || !j.d_lde || !j.d_nodes ||
+            remainder_log < log_blowup || remainder_log >= j.log_n + log_blowup ||
+            remainder_log - log_blowup > kMaxRemainderLog)
             return ZS_ERR_INPUT;
-        pool += persistent_pool_bytes(j.n_cols, j.log_n, log_blowup);
+        pool += persistent_pool_bytes(j.n_cols, j.log_n, log_blowup, remainder_log);
     }
     ZS_TRY(ctx->ensure_scratch(10, pool));
-    // twiddle tables of every size in the batch (built and cached before the launch)
-    std::vector<const uint32_t *> inv_tw(32, nullptr), fwd_tw(32, nullptr);
+    // twiddle, FRI and remainder tables of every size in the batch, the channel's start
+    // (built and cached before the launch)
+    std::vector<const uint32_t *> inv_tw(32, nullptr), fwd_tw(32, nullptr), fri_tab(32, nullptr),
+        rem_mat(32, nullptr);
+    const B3Hasher *init;
+    ZS_TRY(device_commit_channel(ctx, &init));
     for (size_t i : idx) {
         const uint32_t L = jobs[i].log_n, LT = L + log_blowup;
         const TwiddleTable *t;
@@ -458,53 +704,100 @@ int32_t persistent_lde_trees(Ctx *ctx, zs_job *jobs, const std::vector<size_t> &
         if (!fwd_tw[LT]) {
             ZS_TRY(get_canonic_twiddles(ctx, LT, &t));
             fwd_tw[LT] = t->fwd.as<uint32_t>();
+            ZS_TRY(get_fri_tables(ctx, LT, &fri_tab[LT]));
+            ZS_TRY(get_remainder_matrix(ctx, LT - remainder_log, remainder_log, rc, &rem_mat[LT]));
         }
+    }
+    // plan buffer: control block, phase table, stages, results
+    std::vector<uint32_t> res_off(idx.size());
+    uint64_t res_words = 0;
+    for (size_t k = 0; k < idx.size(); k++) {
+        res_off[k] = (uint32_t)res_words;
+        res_words += result_words(jobs[idx[k]].log_n + log_blowup - remainder_log, rc);
     }
     // stages per trace
     std::vector<TraceStages> ts(idx.size());
+    std::vector<uint8_t *> regions(idx.size());
     size_t lds = 0, off = 0;
     uint32_t n_phases = 0;
     uint8_t *pool_base = ctx->scratch[10].as<uint8_t>();
+    std::vector<std::vector<PStage>> fri(idx.size());
     for (size_t k = 0; k < idx.size(); k++) {
         const zs_job &j = jobs[idx[k]];
-        uint32_t *region = reinterpret_cast<uint32_t *>(pool_base + off);
-        off += persistent_pool_bytes(j.n_cols, j.log_n, log_blowup);
+        uint8_t *region = regions[k] = pool_base + off;
+        off += persistent_pool_bytes(j.n_cols, j.log_n, log_blowup, remainder_log);
+        const uint32_t LT = j.log_n + log_blowup;
         const bool one_fwd = !pipelined_lde(j.log_n, log_blowup) && plan_passes(j.log_n, false, PASS_CMAX).size() == 1;
         // coefficients in the output's first c x n words unless one forward pass reads them
         // while writing the output
-        uint32_t *coef = one_fwd ? region : j.d_lde;
-        lde_stages(j, log_blowup, inv_tw[j.log_n], fwd_tw[j.log_n + log_blowup], coef, region, ts[k].st, &lds);
-        tree_stages(j, log_blowup, ts[k].st, &lds);
+        uint32_t *big = reinterpret_cast<uint32_t *>(region);
+        uint32_t *coef = one_fwd ? big : j.d_lde;
+        lde_stages(j, log_blowup, inv_tw[j.log_n], fwd_tw[LT], coef, big, ts[k].st, &lds);
+        {
+            PStage s{};
+            s.kind = PK_LEAF;
+            s.u.leaf = LeafArgs{j.d_lde, uint64_t(1) << LT, uint64_t(1) << LT, j.n_cols,
+                                reinterpret_cast<uint32_t *>(j.d_nodes)};
+            ts[k].st.push_back(s);
+        }
+        level_stages(reinterpret_cast<uint32_t *>(j.d_nodes), LT, ts[k].st, &lds);
         n_phases = std::max<uint32_t>(n_phases, (uint32_t)ts[k].st.size());
     }
-    if (n_phases > kMaxPhases) return ZS_ERR_INPUT;
-    // phase k: stage k of every trace, the stages with the most tasks first
+    // the results area's address is known once the plan's size is: FRI stages last
+    std::vector<uint32_t> first;
     std::vector<PStage> all;
-    std::vector<uint32_t> first(n_phases + 1, 0);
-    for (uint32_t ph = 0; ph < n_phases; ph++) {
-        first[ph] = (uint32_t)all.size();
-        std::vector<PStage> v;
-        for (auto &t : ts)
-            if (ph < t.st.size()) v.push_back(t.st[ph]);
-        std::stable_sort(v.begin(), v.end(),
-                         [](const PStage &a, const PStage &b) { return stage_tasks(a) > stage_tasks(b); });
-        uint64_t end = 0;
-        for (auto &s : v) {
-            end += stage_tasks(s);
-            s.end = end;
-            all.push_back(s);
+    size_t st_off = 0, res_off_bytes = 0, ph_off = 0;
+    for (int pass = 0; pass < 2; pass++) {
+        // pass 0 sizes the plan; pass 1 fills the FRI stages with the results' address
+        uint32_t *res = nullptr;
+        if (pass == 1) res = reinterpret_cast<uint32_t *>(ctx->scratch[11].as<uint8_t>() + res_off_bytes);
+        size_t n_st = 0;
+        uint32_t phases = n_phases;
+        for (size_t k = 0; k < idx.size(); k++) {
+            const zs_job &j = jobs[idx[k]];
+            fri[k].clear();
+            const uint32_t LT = j.log_n + log_blowup;
+            FriShared sh{init, fri_tab[LT], rem_mat[LT]};
+            fri_stages(j, log_blowup, remainder_log, regions[k], sh, res ? res + res_off[k] : nullptr, fri[k], &lds);
+            n_st += ts[k].st.size() + fri[k].size();
+            phases = std::max<uint32_t>(phases, (uint32_t)(ts[k].st.size() + fri[k].size()));
         }
+        if (phases > kMaxPhases) return ZS_ERR_INPUT;
+        const size_t ph_bytes = (phases + 1) * 4, st_bytes = n_st * sizeof(PStage);
+        ph_off = (sizeof(PCtl) + 255) & ~size_t(255);
+        st_off = (ph_off + ph_bytes + 255) & ~size_t(255);
+        res_off_bytes = (st_off + st_bytes + 255) & ~size_t(255);
+        if (pass == 0) {
+            ZS_TRY(ctx->ensure_scratch(11, res_off_bytes + res_words * 4));
+            continue;
+        }
+        // phase k: stage k of every trace, the stages with the most tasks first
+        first.assign(phases + 1, 0);
+        for (uint32_t ph = 0; ph < phases; ph++) {
+            first[ph] = (uint32_t)all.size();
+            std::vector<PStage> v;
+            for (size_t k = 0; k < idx.size(); k++) {
+                const size_t nl = ts[k].st.size();
+                if (ph < nl) v.push_back(ts[k].st[ph]);
+                else if (ph - nl < fri[k].size()) v.push_back(fri[k][ph - nl]);
+            }
+            std::stable_sort(v.begin(), v.end(),
+                             [](const PStage &a, const PStage &b) { return stage_tasks(a) > stage_tasks(b); });
+            uint64_t end = 0;
+            for (auto &s : v) {
+                end += stage_tasks(s);
+                s.end = end;
+                all.push_back(s);
+            }
+        }
+        first[phases] = (uint32_t)all.size();
+        n_phases = phases;
     }
-    first[n_phases] = (uint32_t)all.size();
-    // plan and control block on the device
-    const size_t st_bytes = all.size() * sizeof(PStage), ph_bytes = first.size() * 4;
-    const size_t ctl_off = 0, ph_off = (sizeof(PCtl) + 255) & ~size_t(255),
-                 st_off = (ph_off + ph_bytes + 255) & ~size_t(255);
-    ZS_TRY(ctx->ensure_scratch(11, st_off + st_bytes));
     uint8_t *plan = ctx->scratch[11].as<uint8_t>();
-    ZS_HIP(ctx, hipMemsetAsync(plan + ctl_off, 0, sizeof(PCtl), ctx->stream));
-    ZS_HIP(ctx, hipMemcpyAsync(plan + ph_off, first.data(), ph_bytes, hipMemcpyHostToDevice, ctx->stream));
-    ZS_HIP(ctx, hipMemcpyAsync(plan + st_off, all.data(), st_bytes, hipMemcpyHostToDevice, ctx->stream));
+    ZS_HIP(ctx, hipMemsetAsync(plan, 0, sizeof(PCtl), ctx->stream));
+    ZS_HIP(ctx, hipMemcpyAsync(plan + ph_off, first.data(), first.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    ZS_HIP(ctx, hipMemcpyAsync(plan + st_off, all.data(), all.size() * sizeof(PStage), hipMemcpyHostToDevice,
+                               ctx->stream));
     // as many workgroups as stay resident together
     if (lds > 160 * 1024) return ZS_ERR_INPUT;
     int per_cu = 0, cus = 0, clock_khz = 0;
@@ -518,7 +811,7 @@ int32_t persistent_lde_trees(Ctx *ctx, zs_job *jobs, const std::vector<size_t> &
     PPlan pl{};
     pl.stages = reinterpret_cast<const PStage *>(plan + st_off);
     pl.phase_first = reinterpret_cast<const uint32_t *>(plan + ph_off);
-    pl.ctl = reinterpret_cast<PCtl *>(plan + ctl_off);
+    pl.ctl = reinterpret_cast<PCtl *>(plan);
     pl.wait_ticks = uint64_t(clock_khz) * 1000 * 20;  // 20 s
     pl.n_phases = n_phases;
     pl.n_wg = (uint32_t)(per_cu * cus);
@@ -526,11 +819,22 @@ int32_t persistent_lde_trees(Ctx *ctx, zs_job *jobs, const std::vector<size_t> &
               ctx->stream, pl);
     ZS_HIP(ctx, hipGetLastError());
     uint32_t abort = 0;
+    std::vector<uint32_t> res(res_words);
     ZS_HIP(ctx, hipMemcpyAsync(&abort, &pl.ctl->abort, 4, hipMemcpyDeviceToHost, ctx->stream));
+    ZS_HIP(ctx, hipMemcpyAsync(res.data(), plan + res_off_bytes, res_words * 4, hipMemcpyDeviceToHost, ctx->stream));
     ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (abort) {
         ctx->set_error("persistent batch kernel: a grid barrier wait ran past its bound; results incomplete");
         return ZS_ERR_DEVICE;
+    }
+    for (size_t k = 0; k < idx.size(); k++) {
+        zs_job &j = jobs[idx[k]];
+        const uint32_t L = j.log_n + log_blowup - remainder_log;
+        const uint32_t *r = res.data() + res_off[k];
+        std::memcpy(j.trace_root, r, 32);
+        if (j.fri_roots) std::memcpy(j.fri_roots, r + 8, size_t(L) * 32);
+        if (j.remainder) std::memcpy(j.remainder, r + 8 + 8 * L, size_t(rc) * 16);
+        j.status = ZS_OK;
     }
     return ZS_OK;
 }

@@ -34,6 +34,7 @@ struct VB {
     __device__ static VB hw() { return {blockIdx.x, blockIdx.y, blockIdx.z, gridDim.x}; }
 };
 
+
 template <int C>
 struct TC {
     static constexpr int RB = C - 6;  // log2 elements per lane
@@ -228,7 +229,7 @@ __device__ __forceinline__ void load_low_twiddles(uint32_t (&tr)[LowTw<C>::N], c
     for (int b = 0; b < LowTw<C>::RL; b++) {
         const uint32_t *src = tw + (1u << (LT - 1 - b)) + (kb >> (b + 1)) + (lane << (RB - 1 - b));
 #pragma unroll
-        for (int k = 0; k < (1 << (RB - 1 - b)); k++) tr[LowTw<C>::off(b) + k] = src[k] << 1;
+        for (int k = 0; k < (1 << (RB - 1 - b)); k++) tr[LowTw<C>::off(b) + k] = ldg(src + k) << 1;
     }
 }
 
@@ -264,7 +265,7 @@ __device__ __forceinline__ void stage_twiddles(uint32_t *twl, const uint32_t *tw
         strided_count(t0, step, 1u << C),
         [&](uint32_t i) {
             const uint32_t idx = t0 + i * step, lvl = 31 - __builtin_clz(idx), s = s_lo + (C - 1 - lvl);
-            return tw[(1u << (LT - 1 - s)) + (Kbase >> (s + 1)) + (idx - (1u << lvl))];
+            return ldg(tw + (1u << (LT - 1 - s)) + (Kbase >> (s + 1)) + (idx - (1u << lvl)));
         },
         [&](uint32_t i, uint32_t v) {
             const uint32_t idx = t0 + i * step, lvl = 31 - __builtin_clz(idx);
@@ -644,7 +645,7 @@ __device__ __forceinline__ void lde_eval_last_body(const FftPass &p, const VB vb
     if (c0 < c1) {
         const uint32_t *src = p.in + (uint64_t)c0 * p.in_cs + in_off + lane;
 #pragma unroll
-        for (uint32_t j = 0; j < PER; j++) pf[j] = src[j * 64];
+        for (uint32_t j = 0; j < PER; j++) pf[j] = ldg(src + j * 64);
     }
     // tile set (q, g) covers k-space ((qblk + q) << L) | (bitrev(tau0 + g) << C); the lowest
     // group's stages are this wave's registers, the others its set in LDS
@@ -654,7 +655,7 @@ __device__ __forceinline__ void lde_eval_last_body(const FftPass &p, const VB vb
     uint32_t tr[LW::N];
     load_low_twiddles<C>(tr, p.tw, p.LT, set_kb(wave), lane);
     stage_twiddle_sets<C, LW::SLOG>(twl, Q * G, [&](uint32_t set, uint32_t ls, uint32_t i) {
-        return p.tw[(1u << (p.LT - 1 - ls)) + (set_kb(set) >> (ls + 1)) + i];
+        return ldg(p.tw + (1u << (p.LT - 1 - ls)) + (set_kb(set) >> (ls + 1)) + i);
     });
     __syncthreads();  // twiddle sets staged
     for (uint32_t col = c0; col < c1; col++) {
@@ -663,12 +664,12 @@ __device__ __forceinline__ void lde_eval_last_body(const FftPass &p, const VB vb
         if (col > c0) {
             uint32_t *dst = p.out + (uint64_t)(col - 1) * p.out_cs + out_off;
 #pragma unroll
-            for (uint32_t i = 0; i < PER; i++) dst[(uint64_t)bitrev32(i, C - 6) << sh] = res[i];
+            for (uint32_t i = 0; i < PER; i++) stg(dst + ((uint64_t)bitrev32(i, C - 6) << sh), res[i]);
         }
         if (col + 1 < c1) {
             const uint32_t *src = p.in + (uint64_t)(col + 1) * p.in_cs + in_off + lane;
 #pragma unroll
-            for (uint32_t j = 0; j < PER; j++) pf[j] = src[j * 64];
+            for (uint32_t j = 0; j < PER; j++) pf[j] = ldg(src + j * 64);
         }
         wave_lds_sync();
         tile_run_rt<false, C, 0>(tl, lane, twt, tr);
@@ -680,7 +681,7 @@ __device__ __forceinline__ void lde_eval_last_body(const FftPass &p, const VB vb
     if (c0 < c1) {
         uint32_t *dst = p.out + (uint64_t)(c1 - 1) * p.out_cs + out_off;
 #pragma unroll
-        for (uint32_t i = 0; i < PER; i++) dst[(uint64_t)bitrev32(i, C - 6) << sh] = res[i];
+        for (uint32_t i = 0; i < PER; i++) stg(dst + ((uint64_t)bitrev32(i, C - 6) << sh), res[i]);
     }
 }
 
@@ -713,7 +714,7 @@ __device__ __forceinline__ void lde_interp_first_body(const FftPass &p, const VB
     if (c0 < c1) {
         const uint32_t *src = p.in + (uint64_t)c0 * p.in_cs + in_off;
 #pragma unroll
-        for (uint32_t j = 0; j < PER; j++) pf[j] = src[(uint64_t)bitrev32(j, C - 6) << sh];
+        for (uint32_t j = 0; j < PER; j++) pf[j] = ldg(src + ((uint64_t)bitrev32(j, C - 6) << sh));
     }
     // tile set g covers k-space bitrev(tau0 + g) << C; the lowest group's stages are this
     // wave's registers, the others its set in LDS
@@ -721,7 +722,7 @@ __device__ __forceinline__ void lde_interp_first_body(const FftPass &p, const VB
     load_low_twiddles<C>(tr, p.tw, p.LT, bitrev32(tau0 + wave, p.L - C) << C, lane);
     stage_twiddle_sets<C, LW::SLOG>(twl, G, [&](uint32_t set, uint32_t ls, uint32_t i) {
         const uint32_t kb = bitrev32(tau0 + set, p.L - C) << C;
-        return p.tw[(1u << (p.LT - 1 - ls)) + (kb >> (ls + 1)) + i];
+        return ldg(p.tw + (1u << (p.LT - 1 - ls)) + (kb >> (ls + 1)) + i);
     });
     __syncthreads();  // twiddle sets staged
     for (uint32_t col = c0; col < c1; col++) {
@@ -730,12 +731,12 @@ __device__ __forceinline__ void lde_interp_first_body(const FftPass &p, const VB
         if (col > c0) {
             uint32_t *dst = p.out + (uint64_t)(col - 1) * p.out_cs + out_off;
 #pragma unroll
-            for (uint32_t j = 0; j < PER; j++) dst[j * 64] = res[j];
+            for (uint32_t j = 0; j < PER; j++) stg(dst + j * 64, res[j]);
         }
         if (col + 1 < c1) {
             const uint32_t *src = p.in + (uint64_t)(col + 1) * p.in_cs + in_off;
 #pragma unroll
-            for (uint32_t j = 0; j < PER; j++) pf[j] = src[(uint64_t)bitrev32(j, C - 6) << sh];
+            for (uint32_t j = 0; j < PER; j++) pf[j] = ldg(src + ((uint64_t)bitrev32(j, C - 6) << sh));
         }
         __syncthreads();  // every tile loaded
         tile_run_rt<true, C, 0>(tl, lane, twt, tr);
@@ -746,7 +747,7 @@ __device__ __forceinline__ void lde_interp_first_body(const FftPass &p, const VB
     if (c0 < c1) {
         uint32_t *dst = p.out + (uint64_t)(c1 - 1) * p.out_cs + out_off;
 #pragma unroll
-        for (uint32_t j = 0; j < PER; j++) dst[j * 64] = res[j];
+        for (uint32_t j = 0; j < PER; j++) stg(dst + j * 64, res[j]);
     }
 }
 // ---------------------------------------------------------------------------
@@ -802,15 +803,15 @@ __device__ __forceinline__ void lde_mid_body(const LdeFused &f, const VB vb) {
     if (c0 < c1) {
         const uint32_t *src = p.in + (uint64_t)c0 * p.in_cs + off;
 #pragma unroll
-        for (uint32_t j = 0; j < PER; j++) pf[j] = src[(uint64_t)j << jsh];
+        for (uint32_t j = 0; j < PER; j++) pf[j] = ldg(src + ((uint64_t)j << jsh));
     }
     // set 0: the interpolation's stages s_lo.. of the size-2^L table; set 1 + q: coset q's of the
     // size-2^(L + beta) table
     stage_twiddle_sets<C>(twl, 1 + NQ, [&](uint32_t set, uint32_t ls, uint32_t i) {
         const uint32_t s = p.s_lo + ls;
-        if (set == 0) return f.tw_inv[(1u << (p.L - 1 - s)) + (hi_bits >> (s + 1)) + i];
+        if (set == 0) return ldg(f.tw_inv + (1u << (p.L - 1 - s)) + (hi_bits >> (s + 1)) + i);
         const uint32_t kb = ((set - 1) << p.L) | hi_bits;
-        return f.tw_fwd[(1u << (p.L + p.beta - 1 - s)) + (kb >> (s + 1)) + i];
+        return ldg(f.tw_fwd + (1u << (p.L + p.beta - 1 - s)) + (kb >> (s + 1)) + i);
     });
     __syncthreads();  // twiddle sets staged
     for (uint32_t col = c0; col < c1; col++) {
@@ -819,12 +820,12 @@ __device__ __forceinline__ void lde_mid_body(const LdeFused &f, const VB vb) {
         if (col > c0) {  // the previous column's last coset, held back until this column's tiles landed
             uint32_t *dst = p.out + (uint64_t)(col - 1) * p.out_cs + ((uint64_t)(NQ - 1) << p.L) + off;
 #pragma unroll
-            for (uint32_t j = 0; j < PER; j++) dst[(uint64_t)j << jsh] = res[j];
+            for (uint32_t j = 0; j < PER; j++) stg(dst + ((uint64_t)j << jsh), res[j]);
         }
         if (col + 1 < c1) {
             const uint32_t *src = p.in + (uint64_t)(col + 1) * p.in_cs + off;
 #pragma unroll
-            for (uint32_t j = 0; j < PER; j++) pf[j] = src[(uint64_t)j << jsh];
+            for (uint32_t j = 0; j < PER; j++) pf[j] = ldg(src + ((uint64_t)j << jsh));
         }
         __syncthreads();
         uint32_t c[TC<C>::R];
@@ -845,7 +846,7 @@ __device__ __forceinline__ void lde_mid_body(const LdeFused &f, const VB vb) {
             if (q + 1 < NQ) {
                 uint32_t *dst = out_col + ((uint64_t)q << p.L);
 #pragma unroll
-                for (uint32_t j = 0; j < PER; j++) dst[(uint64_t)j << jsh] = res[j];
+                for (uint32_t j = 0; j < PER; j++) stg(dst + ((uint64_t)j << jsh), res[j]);
             }
         }
         __syncthreads();  // every read done: the tiles may take the next column
@@ -853,7 +854,7 @@ __device__ __forceinline__ void lde_mid_body(const LdeFused &f, const VB vb) {
     if (c0 < c1) {
         uint32_t *dst = p.out + (uint64_t)(c1 - 1) * p.out_cs + ((uint64_t)(NQ - 1) << p.L) + off;
 #pragma unroll
-        for (uint32_t j = 0; j < PER; j++) dst[(uint64_t)j << jsh] = res[j];
+        for (uint32_t j = 0; j < PER; j++) stg(dst + ((uint64_t)j << jsh), res[j]);
     }
 }
 
