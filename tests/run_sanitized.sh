@@ -14,4 +14,7 @@ export ZS_ORACLE_LIBRARY="$ROOT/oracle/_build_san/liboracle_san.so"
 # the interpreter itself is not instrumented: leak reports would be its own
 export ASAN_OPTIONS=detect_leaks=0:abort_on_error=1:strict_string_checks=1:detect_stack_use_after_return=1
 export UBSAN_OPTIONS=print_stacktrace=1:halt_on_error=1
-LD_PRELOAD="$RT" python3 -m pytest "$ROOT/tests" -q -m "not gpu" -p no:cacheprovider "$@"
+# the ASan runtime goes first; anything already preloaded stays (and is recorded, so that
+# tests starting their own sanitized programs give them the original list)
+export ZS_BASE_PRELOAD="${LD_PRELOAD:-}"
+LD_PRELOAD="$RT${LD_PRELOAD:+:$LD_PRELOAD}" python3 -m pytest "$ROOT/tests" -q -m "not gpu" -p no:cacheprovider "$@"

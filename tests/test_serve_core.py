@@ -23,9 +23,14 @@ def test_serve_core_under_sanitizer(tmp_path, kind):
              "-o", exe]
     r = subprocess.run(build, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr[-3000:]
+    # the harness links its own sanitizer runtime, so ASan is told not to insist on
+    # coming first among preloaded libraries; under tests/run_sanitized.sh (whose ASan
+    # runtime is preloaded into this process) it gets the preload list that script found
     env = dict(os.environ, TSAN_OPTIONS="halt_on_error=1:second_deadlock_stack=1",
-               ASAN_OPTIONS="detect_leaks=1:abort_on_error=1", UBSAN_OPTIONS="halt_on_error=1:print_stacktrace=1")
-    env.pop("LD_PRELOAD", None)  # the harness carries its own runtime
+               ASAN_OPTIONS="detect_leaks=1:abort_on_error=1:verify_asan_link_order=0",
+               UBSAN_OPTIONS="halt_on_error=1:print_stacktrace=1")
+    if "ZS_BASE_PRELOAD" in env:
+        env["LD_PRELOAD"] = env["ZS_BASE_PRELOAD"]
     r = subprocess.run([exe], capture_output=True, text=True, timeout=600, env=env)
     assert r.returncode == 0, r.stderr[-5000:]
     assert r.stdout.strip() == "ok"
