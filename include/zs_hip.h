@@ -199,7 +199,6 @@ int32_t zs_fri_commit(zs_hip_ctx *ctx, const uint32_t *codeword, uint32_t dlog, 
  *   paths_out: per query, per layer i, path0 then path1 of (dlog - i) digests. */
 int32_t zs_fri_query(zs_fri_result *res, zs_channel *chan, size_t num_queries, size_t *idx_out, uint32_t *vals_out,
                      uint8_t *paths_out);
-/* Copy FRI layer `layer` (2^(dlog - layer) QM31 values) to host memory. */
 /* Fri.proveCodeword (src/m31/fri.zig:157-273) on a device codeword of
  * 2^(log_size + log_blowup) QM31 values, with every challenge and query index
  * drawn from `chan`, returned as the reference's wire bytes
@@ -211,6 +210,9 @@ int32_t zs_fri_query(zs_fri_result *res, zs_channel *chan, size_t num_queries, s
 int32_t zs_fri_prove_dev(zs_hip_ctx *ctx, const uint32_t *d_codeword, uint32_t log_size, uint32_t log_blowup,
                          uint32_t remainder_log, size_t num_queries, zs_channel *chan, zs_host_allocator host,
                          uint8_t **proof_out, size_t *proof_len);
+/* Copy FRI layer `layer` (2^(dlog - layer) QM31 values) of a kept commitment to host
+ * memory (the layers proveCodeword holds between its commit and query phases,
+ * fri.zig:185-219). */
 int32_t zs_fri_read_layer(const zs_fri_result *res, uint32_t layer, uint32_t *out);
 void zs_fri_result_free(zs_fri_result *res);
 /* Fri.verify (src/m31/fri.zig:277-336) of a proof in the wire format above, on
