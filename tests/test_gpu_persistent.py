@@ -85,3 +85,25 @@ def test_persistent_batch_repeats_and_reuses_its_pool(zctx, torch_dev):
         for k in range(len(shapes)):
             assert got[k][0] == ref[k][0] and got[k][1] == ref[k][1]
             assert torch.equal(bufs[k][1][0][1], bufs[k][1][1][1])
+
+
+def test_persistent_barrier_bound_fails_cleanly(zctx, torch_dev, monkeypatch):
+    """A grid-barrier wait past its bound (forced here from the host: a bound of 0 ms)
+    makes every workgroup leave, and the call returns a device error instead of
+    waiting; the next call, with the default bound, gives the worker path's results."""
+    import torch
+    shapes = [(8, 16), (4, 12), (6, 17)]
+    inputs, bufs = make_batch(torch_dev, shapes, 2, seed=11)
+    print("persistent batch, barrier bound 0 ms", flush=True)
+    monkeypatch.setenv("ZS_PERSIST_WAIT_MS", "0")
+    with pytest.raises(Z.ZsError) as e:
+        zctx.commit_batch_dev(specs_of(inputs, bufs, 1), 2, 4, n_workers=2, flags=Z.ZS_FLAG_PERSISTENT)
+    assert e.value.code == Z.ZS_ERR_DEVICE
+    monkeypatch.delenv("ZS_PERSIST_WAIT_MS")
+    print("persistent batch, default bound", flush=True)
+    ref = zctx.commit_batch_dev(specs_of(inputs, bufs, 0), 2, 4, n_workers=2)
+    got = zctx.commit_batch_dev(specs_of(inputs, bufs, 1), 2, 4, n_workers=2, flags=Z.ZS_FLAG_PERSISTENT)
+    torch.cuda.synchronize()
+    for k in range(len(shapes)):
+        assert got[k][0] == ref[k][0] and got[k][1] == ref[k][1] and np.array_equal(got[k][2], ref[k][2])
+        assert torch.equal(bufs[k][1][0][1], bufs[k][1][1][1])

@@ -20,6 +20,7 @@
 // abort flag, every workgroup leaves at its next claim or barrier, and the host
 // reports a device error instead of waiting on the launch.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -812,7 +813,11 @@ int32_t persistent_commit(Ctx *ctx, zs_job *jobs, const std::vector<size_t> &idx
     pl.stages = reinterpret_cast<const PStage *>(plan + st_off);
     pl.phase_first = reinterpret_cast<const uint32_t *>(plan + ph_off);
     pl.ctl = reinterpret_cast<PCtl *>(plan);
-    pl.wait_ticks = uint64_t(clock_khz) * 1000 * 20;  // 20 s
+    // a barrier waits at most 20 s; ZS_PERSIST_WAIT_MS sets another bound (0: give up at
+    // once, so that a test can take the abort path with every workgroup still exiting)
+    const char *wait_env = getenv("ZS_PERSIST_WAIT_MS");
+    const uint64_t wait_ms = wait_env ? strtoull(wait_env, nullptr, 10) : 20000;
+    pl.wait_ticks = uint64_t(clock_khz) * wait_ms;
     pl.n_phases = n_phases;
     pl.n_wg = (uint32_t)(per_cu * cus);
     ZS_LAUNCH(ctx, "persistent_batch_kernel", persistent_batch_kernel, dim3(pl.n_wg), dim3(kPThreads), lds,
