@@ -752,15 +752,25 @@ int32_t zs::fri_commit_impl(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t 
         if (alphas) {
             std::vector<uint64_t> offs(L);
             std::vector<uint32_t> logs(L);
-            for (uint32_t i = 0; i < L; i++) {
-                uint32_t li = dlog - i;
-                if (i < i_tail) {
+            // folds two layers per launch while both are below the tail (layer i+1 then
+            // has leaves: i + 1 < i_tail <= L)
+            for (uint32_t i = 0; i < i_tail;) {
+                const uint32_t li = dlog - i;
+                if (i + 1 < i_tail) {
+                    uint32_t *leaves2 = (i + 2 < L) ? trees + r->tree_off[i + 2] : nullptr;
+                    ZS_TRY(launch_fri_fold2(ctx, layer(i), size_t(1) << li, dlog, i, alpha_dev + 4 * i, layer(i + 1),
+                                            trees + r->tree_off[i + 1], layer(i + 2), leaves2));
+                    i += 2;
+                } else {
                     uint32_t *next_leaves = (i + 1 < L) ? trees + r->tree_off[i + 1] : nullptr;
                     ZS_TRY(launch_fri_fold(ctx, layer(i), size_t(1) << li, dlog, i, alpha_dev + 4 * i, layer(i + 1),
                                            next_leaves));
+                    i += 1;
                 }
+            }
+            for (uint32_t i = 0; i < L; i++) {
                 offs[i] = r->tree_off[i];
-                logs[i] = li;
+                logs[i] = dlog - i;
             }
             if (i_tail < L) ZS_TRY(tail());
             return launch_trees_multi(ctx, trees, offs.data(), logs.data(), L);

@@ -91,6 +91,48 @@ __global__ __launch_bounds__(256) void fri_fold_kernel(const uint32_t *cur, uint
     if (next_leaves) store_qm31_leaf(next_leaves + j * 8, v);
 }
 
+// Two folds in one launch, for challenges known up front: thread k of layer i (n
+// values) reads cur[k], cur[k + n/2], cur[k + n/4], cur[k + 3n/4], folds them into
+// layer i+1's values at k and k + n/4 (fri.zig:211-219, the same fold_value), and
+// those into layer i+2's value at k, writing both layers and their leaves.  Layer
+// i+1 is never read back, and one launch replaces two.
+__global__ __launch_bounds__(256) void fri_fold2_kernel(const uint32_t *cur, uint64_t quarter, uint32_t layer,
+                                                        const uint32_t *tables, qm31 inv_off0, qm31 inv_off1,
+                                                        const uint32_t *alpha_p, uint32_t *next1, uint32_t *leaves1,
+                                                        uint32_t *next2, uint32_t *leaves2) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= quarter) return;
+    const qm31 a0 = qm_half(qm_mul(ldg_qm(alpha_p), inv_off0)), a1 = qm_half(qm_mul(ldg_qm(alpha_p + 4), inv_off1));
+    const uint64_t half = 2 * quarter;
+    const qm31 lo = fold_value(cur, half, k, layer, tables, a0);
+    const qm31 hi = fold_value(cur, half, k + quarter, layer, tables, a0);
+    stg_qm(next1 + 4 * k, lo);
+    stg_qm(next1 + 4 * (k + quarter), hi);
+    store_qm31_leaf(leaves1 + 8 * k, lo);
+    store_qm31_leaf(leaves1 + 8 * (k + quarter), hi);
+    // layer i+1's fold at k from the two values in registers
+    const uint64_t e = k << (layer + 1);
+    const qm31 t = qm_mul(ldg_qm(tables + 4 * (e & 1023)), ldg_qm(tables + 4 * (1024 + (e >> 10))));
+    const qm31 v = qm_add(qm_half(qm_add(lo, hi)), qm_mul(qm_mul(a1, t), qm_sub(lo, hi)));
+    stg_qm(next2 + 4 * k, v);
+    if (leaves2) store_qm31_leaf(leaves2 + 8 * k, v);
+}
+
+int32_t launch_fri_fold2(Ctx *ctx, const uint32_t *d_cur, uint64_t n, uint32_t dlog, uint32_t layer,
+                         const uint32_t *d_alpha, uint32_t *d_next1, uint32_t *d_leaves1, uint32_t *d_next2,
+                         uint32_t *d_leaves2) {
+    if (n < 4 || !d_leaves1) return ZS_ERR_INPUT;
+    const uint32_t *tables;
+    ZS_TRY(get_fri_tables(ctx, dlog, &tables));
+    const qm31 inv0 = qm_inv(fri_offset_pow(layer)), inv1 = qm_inv(fri_offset_pow(layer + 1));
+    const uint64_t quarter = n / 4;
+    ZS_LAUNCH(ctx, "fri_fold2_kernel", fri_fold2_kernel, dim3((uint32_t)((quarter + 255) / 256)), dim3(256), 0,
+              ctx->stream, d_cur, quarter, layer, tables, inv0, inv1, d_alpha, d_next1, d_leaves1, d_next2,
+              d_leaves2);
+    ZS_HIP(ctx, hipGetLastError());
+    return ZS_OK;
+}
+
 int32_t launch_fri_fold(Ctx *ctx, const uint32_t *d_cur, uint64_t n, uint32_t dlog, uint32_t layer,
                         const uint32_t *d_alpha, uint32_t *d_next, uint32_t *d_next_leaves) {
     const uint32_t *tables;

@@ -258,6 +258,10 @@ int32_t get_fri_tables(Ctx *ctx, uint32_t dlog, const uint32_t **out);
 // hashQM31 leaves of `next` into d_next_leaves when non-null.
 int32_t launch_fri_fold(Ctx *ctx, const uint32_t *d_cur, uint64_t n, uint32_t dlog, uint32_t layer,
                         const uint32_t *d_alpha, uint32_t *d_next, uint32_t *d_next_leaves);
+// folds of layers `layer` and layer + 1 in one launch (challenges at d_alpha, d_alpha + 4)
+int32_t launch_fri_fold2(Ctx *ctx, const uint32_t *d_cur, uint64_t n, uint32_t dlog, uint32_t layer,
+                         const uint32_t *d_alpha, uint32_t *d_next1, uint32_t *d_leaves1, uint32_t *d_next2,
+                         uint32_t *d_leaves2);
 
 // g[i] = sum_c powers[c] * lde_c[i]: QM31 combination of n_cols M31 columns of n values
 // device Fiat-Shamir: absorb the digest at d_digest, write a QM31 challenge to d_out
