@@ -172,3 +172,24 @@ def multiplicity_trace(n):
 LOOKUP_AIRS = {"range_check": (range_check_air, range_check_trace),
                "and_table": (and_table_air, and_table_trace),
                "multiplicity": (multiplicity_air, multiplicity_trace)}
+
+
+def combined_air(n, a0, a1, claimed):
+    """FibSeededAir's two columns and transitions with both lookup relations beside
+    them: [a, b, value, mult, sel, x, y, sel2], tables [table, ta, tb]; relation 0 is
+    RangeCheckAir's (value in table), relation 1 AndTableAir's ((x, y) in (ta, tb), the
+    selector as multiplicity).  Exercises several relations over shared column
+    challenges (alpha_cols), per-relation key and table offsets, and transition,
+    boundary, lookup and binarity terms in one composition."""
+    return Air(8, constraints=[[(1, [("next", 0)]), (P - 1, [("cur", 1)])],
+                               [(1, [("next", 1)]), (P - 1, [("cur", 0)]), (P - 1, [("cur", 1)])]],
+               boundary=[(0, 0, a0), (1, 0, a1), (0, n - 1, claimed)], num_preprocessed=3,
+               lookups=[(4, 3, [2], [0]), (7, 7, [5, 6], [1, 2])], num_lookup_columns=2, degree=(1, 0))
+
+
+def combined_trace(fib, n):
+    """fib: a (2, n, 4) Fibonacci trace; the lookup halves of range_check_trace and
+    and_table_trace beside it."""
+    rt, rtab = range_check_trace(n)
+    at, atab = and_table_trace(n)
+    return np.concatenate([fib, rt, at]), np.concatenate([rtab, atab])

@@ -106,3 +106,37 @@ def test_multi_tile_traces_verify(zctx, torch_dev, name, tl):
     bad[0, (1 << (tl - 1)) - 3, 0] = 9  # a selected row of the last tile of the lookup half
     got, _ = gpu_prove(zctx, torch_dev, air, bad, table, 2, 20, 3, LABEL)
     assert not Z.stark_verify(got, air.struct(), Z.Channel(LABEL))
+
+
+def combined(n, a1=12345):
+    fib = O.fib_trace(1, a1, n)
+    trace, table = airs.combined_trace(fib, n)
+    return airs.combined_air(n, [1, 0, 0, 0], [a1, 0, 0, 0], fib[0, -1]), trace, table
+
+
+@pytest.mark.parametrize("tl,lb,nq,rl", [(4, 3, 16, 3), (9, 2, 20, 3), (7, 1, 10, 2)])
+def test_two_relations_with_transitions_match_oracle(zctx, torch_dev, tl, lb, nq, rl):
+    """Fibonacci transitions and boundary assertions beside two LogUp relations (range
+    check, AND table) with shared column challenges: the device proof is the oracle's,
+    byte for byte, and verifies on the host and in the oracle."""
+    air, trace, table = combined(1 << tl)
+    label = b"zig-stark:stark:combined"
+    got, ch = gpu_prove(zctx, torch_dev, air, trace, table, lb, nq, rl, label)
+    och = O.Channel(label)
+    assert got == O.stark_prove(air, table, trace, lb, nq, rl, och)
+    assert ch.sample_bytes(32) == och.sample_bytes(32)
+    assert Z.stark_verify(got, air.struct(), Z.Channel(label))
+    assert O.stark_verify(air, got, O.Channel(label))
+
+
+def test_two_relations_multi_tile_verify(zctx, torch_dev):
+    """2^14 rows (several accumulator-scan and evaluation tiles, two scan rows): honest
+    proof verifies; one AND-table key missing from the table does not."""
+    air, trace, table = combined(1 << 14)
+    label = b"zig-stark:stark:combined"
+    got, _ = gpu_prove(zctx, torch_dev, air, trace, table, 2, 20, 3, label)
+    assert Z.stark_verify(got, air.struct(), Z.Channel(label))
+    bad = trace.copy()
+    bad[5, (1 << 13) - 5, 0] = 3  # x = 3 is in no (ta, tb) row
+    got, _ = gpu_prove(zctx, torch_dev, air, bad, table, 2, 20, 3, label)
+    assert not Z.stark_verify(got, air.struct(), Z.Channel(label))
