@@ -41,9 +41,8 @@ enum PKind : uint32_t {
     PK_LEAF,              // row leaves (<= 256 columns: one BLAKE3 chunk per row)
     PK_SUBTREE3,          // three node levels, one lane per 8-child subtree
     PK_BLOCK,             // up to eight node levels, one workgroup per subtree
-    PK_DRAW,              // absorb a root into the trace's channel, draw a challenge (one workgroup)
-    PK_POWERS,            // gamma^c for every column (one workgroup)
-    PK_COMBINE,           // g = sum_c gamma^c lde_c and g's leaves (FRI layer 0)
+    PK_DRAW,              // start the trace's channel, absorb its root, draw gamma (one workgroup)
+    PK_COMBINE,           // gamma^c, then g = sum_c gamma^c lde_c and g's leaves (FRI layer 0)
     PK_FOLD,              // one FRI fold and the next layer's leaves
     PK_REMAINDER,         // the remainder coefficients; the roots gathered for the host
 };
@@ -78,13 +77,8 @@ struct DrawArgs {
     const uint32_t *digest;
     uint32_t *out;
 };
-struct PowArgs {
-    const uint32_t *x;
-    uint32_t *out;
-    uint32_t n;
-};
 struct CombArgs {
-    const uint32_t *lde, *powers;
+    const uint32_t *lde, *gamma;
     uint64_t cs, n;
     uint32_t *g, *leaves;
     uint32_t n_cols;
@@ -112,7 +106,6 @@ struct PStage {
         SubArgs sub;
         BlockArgs blk;
         DrawArgs draw;
-        PowArgs pow;
         CombArgs comb;
         FoldArgs fold;
         RemArgs rem;
@@ -138,11 +131,16 @@ __device__ __forceinline__ uint32_t ld_relaxed(uint32_t *p) {
 }
 
 // Every workgroup's writes so far are visible to every workgroup after it; k counts the
-// barriers (1, 2, ...).  False when the wait gave up or another workgroup's did.
+// barriers (1, 2, ...).  False when the wait gave up or another workgroup's did.  Each
+// wave waits until its own stores have reached the L2; then one thread per workgroup
+// makes them visible device-wide (the release: the XCD's L2 written back), arrives,
+// waits, and takes the acquire (its CU's and XCD's caches invalidated) for the whole
+// workgroup -- a device-scope fence in every wave made each barrier cost ~0.15 ms.
 __device__ bool grid_barrier(const PPlan &pl, uint32_t k, uint32_t *s_ok) {
-    __threadfence();
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (threadIdx.x == 0) {
+        __threadfence();
         atomicAdd(&pl.ctl->arrive, 1u);
         const uint32_t want = pl.n_wg * k;
         const uint64_t t0 = wall_clock64();
@@ -159,10 +157,10 @@ __device__ bool grid_barrier(const PPlan &pl, uint32_t k, uint32_t *s_ok) {
             }
             __builtin_amdgcn_s_sleep(4);
         }
+        __threadfence();
         *s_ok = ok;
     }
     __syncthreads();
-    __threadfence();
     return *s_ok != 0;
 }
 
@@ -235,21 +233,23 @@ ZS_TASK void task_draw(const PStage *sp, uint64_t) {
     channel_draw_body(a.chan, a.init, a.digest, a.out, *h, *snap, lds + 2 * HW, lds + 2 * HW + 24, threadIdx.x,
                       kPThreads);
 }
-ZS_TASK void task_powers(const PStage *sp, uint64_t) {
-    const PowArgs a = uniform_copy(sp->u.pow);
-    const uint32_t c = threadIdx.x;
-    if (c >= a.n) return;
-    qm31 b = ldg_qm(a.x), r = {{1, 0}, {0, 0}};
-    for (uint32_t e = c; e; e >>= 1) {
-        if (e & 1u) r = qm_mul(r, b);
-        b = qm_mul(b, b);
-    }
-    stg_qm(a.out + 4 * c, r);
-}
+// the powers gamma^c (c < n_cols <= 256, square-and-multiply per thread, as
+// qm31_powers_kernel) into LDS, then g and its leaves for 512 rows
 ZS_TASK void task_combine(const PStage *sp, uint64_t u) {
     extern __shared__ uint32_t lds[];
     const CombArgs a = uniform_copy(sp->u.comb);
-    for (uint32_t k = threadIdx.x; k < 4 * a.n_cols; k += kPThreads) lds[k] = ldg(a.powers + k);
+    if (threadIdx.x < a.n_cols) {
+        qm31 b = ldg_qm(a.gamma), r = {{1, 0}, {0, 0}};
+        for (uint32_t e = threadIdx.x; e; e >>= 1) {
+            if (e & 1u) r = qm_mul(r, b);
+            b = qm_mul(b, b);
+        }
+        uint32_t *o = lds + 4 * threadIdx.x;
+        o[0] = r.a.c0;
+        o[1] = r.a.c1;
+        o[2] = r.b.c0;
+        o[3] = r.b.c1;
+    }
     __syncthreads();
     const uint64_t i = uniform_u64(u) * kPThreads + threadIdx.x;
     if (i >= a.n) return;
@@ -312,7 +312,6 @@ __device__ __forceinline__ void run_task(const PStage *sp, uint64_t u) {
         case PK_SUBTREE3: task_subtree3(sp, u); break;
         case PK_BLOCK: task_block(sp, u); break;
         case PK_DRAW: task_draw(sp, u); break;
-        case PK_POWERS: task_powers(sp, u); break;
         case PK_COMBINE: task_combine(sp, u); break;
         case PK_FOLD: task_fold(sp, u); break;
         default: task_remainder(sp, u); break;  // PK_REMAINDER
@@ -381,10 +380,6 @@ __global__ __launch_bounds__(kPThreads, 4) void persistent_batch_kernel(const PP
 
 // ---- host: per-trace stage lists ----
 
-struct TraceStages {
-    std::vector<PStage> st;
-};
-
 uint64_t stage_tasks(const PStage &s) {
     switch (s.kind) {
         case PK_INTERP_FIRST:
@@ -407,18 +402,15 @@ uint64_t stage_tasks(const PStage &s) {
             return (s.u.comb.n + kPThreads - 1) / kPThreads;
         case PK_FOLD:
             return (s.u.fold.half + kPThreads - 1) / kPThreads;
-        default:  // PK_DRAW, PK_POWERS, PK_REMAINDER: one workgroup
+        default:  // PK_DRAW, PK_REMAINDER: one workgroup
             return 1;
     }
 }
 
 // columns one pass task runs with one set of staged twiddles (the pass bodies overlap a
 // column's loads with the previous column's butterflies, so more columns per task also
-// amortise the claim and the call)
-#ifndef ZS_TASK_COLS
-#define ZS_TASK_COLS 8
-#endif
-constexpr uint32_t kTaskCols = ZS_TASK_COLS;
+// amortise the claim and the call; 4 and 16 measured no better)
+constexpr uint32_t kTaskCols = 8;
 
 uint32_t ncb_for(uint32_t n_cols, uint32_t cols_per_wg) { return std::max<uint32_t>(1, std::min(cols_per_wg, n_cols)); }
 
@@ -537,7 +529,10 @@ void lde_stages(const zs_job &j, uint32_t beta, const uint32_t *tw_inv, const ui
 // the node levels of a tree of 2^LT leaves at `nodes` (merkle.hip launch_trees_multi for
 // one tree: 3-level lane subtrees while 2^18 children or more remain, then 8-level
 // workgroup subtrees)
-void level_stages(uint32_t *nodes, uint32_t LT, std::vector<PStage> &out, size_t *lds) {
+// With draw_chan, the last stage (one workgroup: the root) also absorbs the root into
+// that channel and draws the next challenge into draw_alpha (tree_block_body's draw).
+void level_stages(uint32_t *nodes, uint32_t LT, std::vector<PStage> &out, size_t *lds,
+                  B3Hasher *draw_chan = nullptr, uint32_t *draw_alpha = nullptr) {
     uint32_t level = 0;
     while (LT - level >= 18) {
         PStage s{};
@@ -558,9 +553,10 @@ void level_stages(uint32_t *nodes, uint32_t LT, std::vector<PStage> &out, size_t
         b.bt.r[0] = rem;
         b.bt.K[0] = K;
         b.bt.wg_end[0] = 1u << (rem - K);
-        b.bt.chan = nullptr;
-        b.bt.alpha = nullptr;
-        *lds = std::max(*lds, block_lds_words(K, kPThreads) * 4);
+        const bool root = level + K == LT;
+        b.bt.chan = root ? draw_chan : nullptr;
+        b.bt.alpha = root ? draw_alpha : nullptr;
+        *lds = std::max(*lds, (block_lds_words(K, kPThreads) + (root && draw_chan ? kDrawLdsWords : 0)) * 4);
         out.push_back(s);
         level += K;
     }
@@ -568,11 +564,11 @@ void level_stages(uint32_t *nodes, uint32_t LT, std::vector<PStage> &out, size_t
 
 // Device memory of a trace's FRI commitment (job.cpp commit_job's scratch), laid out in
 // the trace's pool region once its LDE no longer needs it: the channel, gamma and its
-// powers, layers 0..L (layer 0 = g), the L layer trees, the L challenges.
+// layers 0..L (layer 0 = g), the L layer trees, the L challenges.
 struct FriLayout {
-    uint64_t chan, gamma, powers, layer[29], tree[28], alphas, total;
+    uint64_t chan, gamma, layer[29], tree[28], alphas, total;
 };
-FriLayout fri_layout(uint32_t n_cols, uint32_t dlog, uint32_t L) {
+FriLayout fri_layout(uint32_t dlog, uint32_t L) {
     FriLayout f{};
     uint64_t o = 0;
     auto take = [&](uint64_t bytes) {
@@ -582,7 +578,6 @@ FriLayout fri_layout(uint32_t n_cols, uint32_t dlog, uint32_t L) {
     };
     f.chan = take(sizeof(B3Hasher));
     f.gamma = take(16);
-    f.powers = take(16 * uint64_t(n_cols));
     for (uint32_t i = 0; i <= L; i++) f.layer[i] = take((uint64_t(16) << dlog) >> i);
     // the trees back to back (the remainder task finds the roots by their sizes)
     uint64_t tw = 0;
@@ -601,43 +596,35 @@ struct FriShared {
 };
 
 // the commitment after the row tree (job.cpp commit_job, fri_commit_impl in channel
-// mode): gamma from the trace root, g and its leaves, then per layer its node levels,
-// the challenge drawn from its root and the fold (with the next layer's leaves); last the
-// remainder, with the roots gathered into `res`
+// mode): gamma from the trace root, g and its leaves, then per layer its node levels
+// (the root's task also draws the layer's challenge) and the fold (with the next
+// layer's leaves); last the remainder, with the roots gathered into `res`
 void fri_stages(const zs_job &j, uint32_t beta, uint32_t remainder_log, uint8_t *region, const FriShared &sh,
                 uint32_t *res, std::vector<PStage> &out, size_t *lds) {
     const uint32_t dlog = j.log_n + beta, L = dlog - remainder_log, rc = 1u << (remainder_log - beta);
     const uint64_t N = uint64_t(1) << dlog;
-    const FriLayout f = fri_layout(j.n_cols, dlog, L);
+    const FriLayout f = fri_layout(dlog, L);
     auto at = [&](uint64_t off) { return reinterpret_cast<uint32_t *>(region + off); };
     B3Hasher *chan = reinterpret_cast<B3Hasher *>(region + f.chan);
     const uint32_t *row_root = reinterpret_cast<const uint32_t *>(j.d_nodes) + tree_words(dlog) - 8;
     constexpr size_t draw_lds = (2 * ((sizeof(B3Hasher) / 4 + 1) & ~size_t(1)) + 24 + 16) * 4;
-    auto draw = [&](const B3Hasher *init, const uint32_t *digest, uint32_t *dst) {
+    {  // gamma: the channel starts from Channel.init and absorbs the trace root
         PStage s{};
         s.kind = PK_DRAW;
-        s.u.draw = DrawArgs{chan, init, digest, dst};
+        s.u.draw = DrawArgs{chan, sh.init, row_root, at(f.gamma)};
         *lds = std::max(*lds, draw_lds);
-        out.push_back(s);
-    };
-    draw(sh.init, row_root, at(f.gamma));
-    {
-        PStage s{};
-        s.kind = PK_POWERS;
-        s.u.pow = PowArgs{at(f.gamma), at(f.powers), j.n_cols};
         out.push_back(s);
     }
     {
         PStage s{};
         s.kind = PK_COMBINE;
-        s.u.comb = CombArgs{j.d_lde, at(f.powers), N, N, at(f.layer[0]), at(f.tree[0]), j.n_cols};
+        s.u.comb = CombArgs{j.d_lde, at(f.gamma), N, N, at(f.layer[0]), at(f.tree[0]), j.n_cols};
         *lds = std::max(*lds, size_t(16) * j.n_cols);
         out.push_back(s);
     }
     for (uint32_t i = 0; i < L; i++) {
         const uint32_t li = dlog - i;
-        level_stages(at(f.tree[i]), li, out, lds);
-        draw(nullptr, at(f.tree[i]) + tree_words(li) - 8, at(f.alphas) + 4 * i);
+        level_stages(at(f.tree[i]), li, out, lds, chan, at(f.alphas) + 4 * i);  // the root's task draws
         PStage s{};
         s.kind = PK_FOLD;
         FoldArgs &fa = s.u.fold;
@@ -670,7 +657,7 @@ bool persistent_supported(uint32_t n_cols, uint32_t log_n, uint32_t log_blowup) 
 size_t persistent_pool_bytes(uint32_t n_cols, uint32_t log_n, uint32_t log_blowup, uint32_t remainder_log) {
     const uint32_t dlog = log_n + log_blowup;
     const uint64_t lde = (uint64_t(n_cols) << dlog) * 4;
-    const uint64_t fri = fri_layout(n_cols, dlog, dlog - remainder_log).total;
+    const uint64_t fri = fri_layout(dlog, dlog - remainder_log).total;
     return (std::max(lde, fri) + 255) & ~uint64_t(255);
 }
 
@@ -709,7 +696,7 @@ int32_t persistent_commit(Ctx *ctx, zs_job *jobs, const std::vector<size_t> &idx
             ZS_TRY(get_remainder_matrix(ctx, LT - remainder_log, remainder_log, rc, &rem_mat[LT]));
         }
     }
-    // plan buffer: control block, phase table, stages, results
+    // each trace's words in the results area (its roots and remainder, for one copy out)
     std::vector<uint32_t> res_off(idx.size());
     uint64_t res_words = 0;
     for (size_t k = 0; k < idx.size(); k++) {
@@ -717,7 +704,7 @@ int32_t persistent_commit(Ctx *ctx, zs_job *jobs, const std::vector<size_t> &idx
         res_words += result_words(jobs[idx[k]].log_n + log_blowup - remainder_log, rc);
     }
     // stages per trace
-    std::vector<TraceStages> ts(idx.size());
+    std::vector<std::vector<PStage>> ts(idx.size());  // LDE, leaves, row tree
     std::vector<uint8_t *> regions(idx.size());
     size_t lds = 0, off = 0;
     uint32_t n_phases = 0;
@@ -733,67 +720,57 @@ int32_t persistent_commit(Ctx *ctx, zs_job *jobs, const std::vector<size_t> &idx
         // while writing the output
         uint32_t *big = reinterpret_cast<uint32_t *>(region);
         uint32_t *coef = one_fwd ? big : j.d_lde;
-        lde_stages(j, log_blowup, inv_tw[j.log_n], fwd_tw[LT], coef, big, ts[k].st, &lds);
+        lde_stages(j, log_blowup, inv_tw[j.log_n], fwd_tw[LT], coef, big, ts[k], &lds);
         {
             PStage s{};
             s.kind = PK_LEAF;
             s.u.leaf = LeafArgs{j.d_lde, uint64_t(1) << LT, uint64_t(1) << LT, j.n_cols,
                                 reinterpret_cast<uint32_t *>(j.d_nodes)};
-            ts[k].st.push_back(s);
+            ts[k].push_back(s);
         }
-        level_stages(reinterpret_cast<uint32_t *>(j.d_nodes), LT, ts[k].st, &lds);
-        n_phases = std::max<uint32_t>(n_phases, (uint32_t)ts[k].st.size());
+        level_stages(reinterpret_cast<uint32_t *>(j.d_nodes), LT, ts[k], &lds);
+        n_phases = std::max<uint32_t>(n_phases, (uint32_t)ts[k].size());
     }
-    // the results area's address is known once the plan's size is: FRI stages last
-    std::vector<uint32_t> first;
+    // FRI stages; each trace's last one (the remainder's) gets its results address once
+    // the plan buffer exists
+    size_t n_st = 0;
+    for (size_t k = 0; k < idx.size(); k++) {
+        const uint32_t LT = jobs[idx[k]].log_n + log_blowup;
+        const FriShared sh{init, fri_tab[LT], rem_mat[LT]};
+        fri_stages(jobs[idx[k]], log_blowup, remainder_log, regions[k], sh, nullptr, fri[k], &lds);
+        n_st += ts[k].size() + fri[k].size();
+        n_phases = std::max<uint32_t>(n_phases, (uint32_t)(ts[k].size() + fri[k].size()));
+    }
+    if (n_phases > kMaxPhases) return ZS_ERR_INPUT;
+    // plan buffer: control block, phase table, stages, results
+    const size_t ph_off = (sizeof(PCtl) + 255) & ~size_t(255);
+    const size_t st_off = (ph_off + (n_phases + 1) * 4 + 255) & ~size_t(255);
+    const size_t res_off_bytes = (st_off + n_st * sizeof(PStage) + 255) & ~size_t(255);
+    ZS_TRY(ctx->ensure_scratch(11, res_off_bytes + res_words * 4));
+    uint32_t *res_dev = reinterpret_cast<uint32_t *>(ctx->scratch[11].as<uint8_t>() + res_off_bytes);
+    for (size_t k = 0; k < idx.size(); k++) fri[k].back().u.rem.out = res_dev + res_off[k];
+    // phase k: stage k of every trace, the stages with the most tasks first
+    std::vector<uint32_t> first(n_phases + 1, 0);
     std::vector<PStage> all;
-    size_t st_off = 0, res_off_bytes = 0, ph_off = 0;
-    for (int pass = 0; pass < 2; pass++) {
-        // pass 0 sizes the plan; pass 1 fills the FRI stages with the results' address
-        uint32_t *res = nullptr;
-        if (pass == 1) res = reinterpret_cast<uint32_t *>(ctx->scratch[11].as<uint8_t>() + res_off_bytes);
-        size_t n_st = 0;
-        uint32_t phases = n_phases;
+    all.reserve(n_st);
+    for (uint32_t ph = 0; ph < n_phases; ph++) {
+        first[ph] = (uint32_t)all.size();
+        std::vector<PStage> v;
         for (size_t k = 0; k < idx.size(); k++) {
-            const zs_job &j = jobs[idx[k]];
-            fri[k].clear();
-            const uint32_t LT = j.log_n + log_blowup;
-            FriShared sh{init, fri_tab[LT], rem_mat[LT]};
-            fri_stages(j, log_blowup, remainder_log, regions[k], sh, res ? res + res_off[k] : nullptr, fri[k], &lds);
-            n_st += ts[k].st.size() + fri[k].size();
-            phases = std::max<uint32_t>(phases, (uint32_t)(ts[k].st.size() + fri[k].size()));
+            const size_t nl = ts[k].size();
+            if (ph < nl) v.push_back(ts[k][ph]);
+            else if (ph - nl < fri[k].size()) v.push_back(fri[k][ph - nl]);
         }
-        if (phases > kMaxPhases) return ZS_ERR_INPUT;
-        const size_t ph_bytes = (phases + 1) * 4, st_bytes = n_st * sizeof(PStage);
-        ph_off = (sizeof(PCtl) + 255) & ~size_t(255);
-        st_off = (ph_off + ph_bytes + 255) & ~size_t(255);
-        res_off_bytes = (st_off + st_bytes + 255) & ~size_t(255);
-        if (pass == 0) {
-            ZS_TRY(ctx->ensure_scratch(11, res_off_bytes + res_words * 4));
-            continue;
+        std::stable_sort(v.begin(), v.end(),
+                         [](const PStage &x, const PStage &y) { return stage_tasks(x) > stage_tasks(y); });
+        uint64_t end = 0;
+        for (auto &st : v) {
+            end += stage_tasks(st);
+            st.end = end;
+            all.push_back(st);
         }
-        // phase k: stage k of every trace, the stages with the most tasks first
-        first.assign(phases + 1, 0);
-        for (uint32_t ph = 0; ph < phases; ph++) {
-            first[ph] = (uint32_t)all.size();
-            std::vector<PStage> v;
-            for (size_t k = 0; k < idx.size(); k++) {
-                const size_t nl = ts[k].st.size();
-                if (ph < nl) v.push_back(ts[k].st[ph]);
-                else if (ph - nl < fri[k].size()) v.push_back(fri[k][ph - nl]);
-            }
-            std::stable_sort(v.begin(), v.end(),
-                             [](const PStage &a, const PStage &b) { return stage_tasks(a) > stage_tasks(b); });
-            uint64_t end = 0;
-            for (auto &s : v) {
-                end += stage_tasks(s);
-                s.end = end;
-                all.push_back(s);
-            }
-        }
-        first[phases] = (uint32_t)all.size();
-        n_phases = phases;
     }
+    first[n_phases] = (uint32_t)all.size();
     uint8_t *plan = ctx->scratch[11].as<uint8_t>();
     ZS_HIP(ctx, hipMemsetAsync(plan, 0, sizeof(PCtl), ctx->stream));
     ZS_HIP(ctx, hipMemcpyAsync(plan + ph_off, first.data(), first.size() * 4, hipMemcpyHostToDevice, ctx->stream));
