@@ -29,16 +29,24 @@ template <bool INV, int C>
 __global__ __launch_bounds__(tile_block_max<C>()) void fft_tile_kernel(FftPass p) {
     fft_tile_body<INV, C>(p, VB::hw());
 }
+// The pipelined LDE passes run two 512-thread workgroups per CU (their LDS allows
+// no third), i.e. 4 waves per SIMD: stating that lets the compiler use up to 128
+// VGPRs for latency hiding (twiddle loads issued early, butterflies interleaved)
+// instead of squeezing registers for an occupancy the LDS never allows.
+#define ZS_LDE_PASS_BOUNDS __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(ZS_LDE_WAVES, ZS_LDE_WAVES)))
+#ifndef ZS_LDE_WAVES
+#define ZS_LDE_WAVES 4
+#endif
 template <int C>
-__global__ __launch_bounds__(512, 2) void lde_eval_last_kernel(FftPass p) {
+__global__ ZS_LDE_PASS_BOUNDS void lde_eval_last_kernel(FftPass p) {
     lde_eval_last_body<C>(p, VB::hw());
 }
 template <int C>
-__global__ __launch_bounds__(512, 2) void lde_interp_first_kernel(FftPass p) {
+__global__ ZS_LDE_PASS_BOUNDS void lde_interp_first_kernel(FftPass p) {
     lde_interp_first_body<C>(p, VB::hw());
 }
 template <int C>
-__global__ __launch_bounds__(512, 2) void lde_mid_kernel(LdeFused f) {
+__global__ ZS_LDE_PASS_BOUNDS void lde_mid_kernel(LdeFused f) {
     lde_mid_body<C>(f, VB::hw());
 }
 

@@ -129,10 +129,85 @@ __device__ __forceinline__ void g_stage(uint32_t (&v)[TC<C>::R], uint32_t lane, 
 }
 
 template <bool INV, int C, int LO, int r, int i>
-__device__ __forceinline__ void g_stages(uint32_t (&v)[TC<C>::R], uint32_t lane, const uint32_t *twl) {
+__device__ __forceinline__ void g_stages_lds(uint32_t (&v)[TC<C>::R], uint32_t lane, const uint32_t *twl) {
     constexpr int b = INV ? i : (r - 1 - i);
     g_stage<INV, C, LO, r, b>(v, lane, twl);
-    if constexpr (i + 1 < r) g_stages<INV, C, LO, r, i + 1>(v, lane, twl);
+    if constexpr (i + 1 < r) g_stages_lds<INV, C, LO, r, i + 1>(v, lane, twl);
+}
+
+// The same stages with every twiddle of the group read from LDS up front (2^r - 1
+// per lane for a full group), before the first butterfly: left to itself the
+// compiler issues each read just before its butterfly and waits for it there, one
+// LDS latency per butterfly pair.  The scheduling barrier keeps the reads ahead.
+template <int C, int LO, int r, int b>
+struct TwIdx {
+    static constexpr int RB = TC<C>::RB;
+    static constexpr int per_o = (1 << (r - 1 - b));                       // distinct twiddles of stage b per o
+    static constexpr int base = (1 << (RB - r)) * ((1 << r) - (1 << (r - b)));  // stages below b first
+};
+template <bool INV, int C, int LO, int r, int b>
+__device__ __forceinline__ void g_stage_w(uint32_t (&v)[TC<C>::R], const uint32_t *w) {
+    constexpr int RB = TC<C>::RB;
+    using TI = TwIdx<C, LO, r, b>;
+#pragma unroll
+    for (int o = 0; o < (1 << (RB - r)); o++) {
+#pragma unroll
+        for (int u = 0; u < (1 << (r - 1 - b)); u++) {
+            const uint32_t wt = w[TI::base + o * TI::per_o + u];
+            const int jb = (o << r) | (u << (b + 1));
+#pragma unroll
+            for (int l = 0; l < (1 << b); l++) {
+                const int j0 = jb | l, j1 = j0 | (1 << b);
+                const uint32_t a = v[j0];
+                if (INV) {
+                    v[j0] = m31_add(a, v[j1]);
+                    v[j1] = m31_mul_dbl(m31_sub(a, v[j1]), wt);
+                } else {
+                    const uint32_t m = m31_mul_dbl(v[j1], wt);
+                    v[j0] = m31_add(a, m);
+                    v[j1] = m31_sub(a, m);
+                }
+            }
+        }
+    }
+}
+template <int C, int LO, int r, int b>
+__device__ __forceinline__ void g_load_tw(uint32_t *w, uint32_t lane, const uint32_t *twl) {
+    constexpr int RB = TC<C>::RB, ls = LO + b;
+    using TI = TwIdx<C, LO, r, b>;
+    const uint32_t *tws = twl + (1u << (C - 1 - ls));
+#pragma unroll
+    for (int o = 0; o < (1 << (RB - r)); o++) {
+#pragma unroll
+        for (int u = 0; u < (1 << (r - 1 - b)); u++) {
+            const uint32_t jb = ((uint32_t)o << r) | ((uint32_t)u << (b + 1));
+            const uint32_t t0 = t_of<RB, LO, r>(lane, jb);
+            w[TI::base + o * TI::per_o + u] = tws[tw_pos<C>(ls, t0 >> (ls + 1))];
+        }
+    }
+    if constexpr (b > 0) g_load_tw<C, LO, r, b - 1>(w, lane, twl);
+}
+template <bool INV, int C, int LO, int r, int i>
+__device__ __forceinline__ void g_stages_w(uint32_t (&v)[TC<C>::R], const uint32_t *w) {
+    constexpr int b = INV ? i : (r - 1 - i);
+    g_stage_w<INV, C, LO, r, b>(v, w);
+    if constexpr (i + 1 < r) g_stages_w<INV, C, LO, r, i + 1>(v, w);
+}
+#ifndef ZS_TW_PRELOAD
+#define ZS_TW_PRELOAD 1
+#endif
+template <bool INV, int C, int LO, int r, int i>
+__device__ __forceinline__ void g_stages(uint32_t (&v)[TC<C>::R], uint32_t lane, const uint32_t *twl) {
+    static_assert(i == 0, "a group's stages run together");
+    if constexpr (ZS_TW_PRELOAD) {
+        constexpr int NW = (1 << (TC<C>::RB - r)) * ((1 << r) - 1);
+        uint32_t w[NW];
+        g_load_tw<C, LO, r, r - 1>(w, lane, twl);
+        __builtin_amdgcn_sched_barrier(0);
+        g_stages_w<INV, C, LO, r, 0>(v, w);
+    } else {
+        g_stages_lds<INV, C, LO, r, 0>(v, lane, twl);
+    }
 }
 
 // group GI in execution order (forward: top bits first; inverse: bottom first)
@@ -219,6 +294,37 @@ __device__ __forceinline__ void tile_run_rt(uint32_t *tl, uint32_t lane, const u
     wave_lds_sync();
     if constexpr (GI + 1 < TC<C>::NG) tile_run_rt<INV, C, GI + 1>(tl, lane, twl, tr);
 }
+// Groups GI.. of a tile whose group-GI values are already in v (a register copy
+// of the layout g_load would give), so the tile's first LDS round trip is
+// skipped.  KEEP: the last group's values stay in v (nothing stored); otherwise
+// they are stored to LDS, as tile_run leaves them.  RT: the lowest group takes
+// its twiddles from tr.
+template <bool INV, int C, int GI, bool KEEP, bool RT>
+__device__ __forceinline__ void tile_regs(uint32_t (&v)[TC<C>::R], uint32_t *tl, uint32_t lane, const uint32_t *twl,
+                                          const uint32_t (&tr)[LowTw<C>::N]) {
+    using Gp = Grp<INV, C, GI>;
+    if constexpr (RT && Gp::GF + 1 == TC<C>::NG) g_stages_rt<INV, C, 0>(v, tr);
+    else g_stages<INV, C, Gp::LO, Gp::r, 0>(v, lane, twl);
+    if constexpr (GI + 1 < TC<C>::NG) {
+        using Gn = Grp<INV, C, GI + 1>;
+        g_store<C, Gp::LO, Gp::r>(v, tl, lane);
+        wave_lds_sync();
+        g_load<C, Gn::LO, Gn::r>(v, tl, lane);
+        tile_regs<INV, C, GI + 1, KEEP, RT>(v, tl, lane, twl, tr);
+    } else if constexpr (!KEEP) {
+        g_store<C, Gp::LO, Gp::r>(v, tl, lane);
+        wave_lds_sync();
+    }
+}
+// the same for a tile held in LDS: group 0 is loaded first
+template <bool INV, int C, bool KEEP, bool RT>
+__device__ __forceinline__ void tile_lds(uint32_t (&v)[TC<C>::R], uint32_t *tl, uint32_t lane, const uint32_t *twl,
+                                         const uint32_t (&tr)[LowTw<C>::N]) {
+    using G0 = Grp<INV, C, 0>;
+    g_load<C, G0::LO, G0::r>(v, tl, lane);
+    tile_regs<INV, C, 0, KEEP, RT>(v, tl, lane, twl, tr);
+}
+
 // this lane's lowest-group twiddles (doubled) of the tile whose heap entries start at
 // heap(ls) + (kb >> (ls + 1)) in the table of log LT
 template <int C>
@@ -659,8 +765,10 @@ __device__ __forceinline__ void lde_eval_last_body(const FftPass &p, const VB vb
     });
     __syncthreads();  // twiddle sets staged
     for (uint32_t col = c0; col < c1; col++) {
+        // the top group's layout is the load's (element lane + 64 j): it starts from registers
+        uint32_t v[PER];
 #pragma unroll
-        for (uint32_t j = 0; j < PER; j++) tl[lidx(lane + j * 64)] = pf[j];
+        for (uint32_t j = 0; j < PER; j++) v[j] = pf[j];
         if (col > c0) {
             uint32_t *dst = p.out + (uint64_t)(col - 1) * p.out_cs + out_off;
 #pragma unroll
@@ -671,8 +779,7 @@ __device__ __forceinline__ void lde_eval_last_body(const FftPass &p, const VB vb
 #pragma unroll
             for (uint32_t j = 0; j < PER; j++) pf[j] = ldg(src + j * 64);
         }
-        wave_lds_sync();
-        tile_run_rt<false, C, 0>(tl, lane, twt, tr);
+        tile_regs<false, C, 0, false, true>(v, tl, lane, twt, tr);
         __syncthreads();  // every tile of the workgroup transformed
 #pragma unroll
         for (uint32_t i = 0; i < PER; i++) res[i] = rd[lidx(i << 6)];
@@ -739,9 +846,8 @@ __device__ __forceinline__ void lde_interp_first_body(const FftPass &p, const VB
             for (uint32_t j = 0; j < PER; j++) pf[j] = ldg(src + ((uint64_t)bitrev32(j, C - 6) << sh));
         }
         __syncthreads();  // every tile loaded
-        tile_run_rt<true, C, 0>(tl, lane, twt, tr);
-#pragma unroll
-        for (uint32_t j = 0; j < PER; j++) res[j] = tl[lidx(lane + j * 64)];
+        // the last (top) group's layout is the store's (element lane + 64 j): it stays in registers
+        tile_lds<true, C, true, true>(res, tl, lane, twt, tr);
         __syncthreads();  // every read done: the tiles may take the next column
     }
     if (c0 < c1) {
@@ -789,7 +895,8 @@ __device__ __forceinline__ void lde_mid_body(const LdeFused &f, const VB vb) {
     const uint32_t NQ = 1u << p.beta;
     uint32_t *twl = lds + G * TS;  // set 0: interpolation, set 1 + q: evaluation of q
     const uint32_t hi_bits = tile_fixed(p, tg, 0) & ~((1u << p.s_lo) - 1);
-    using Top = Grp<false, C, 0>;
+    static_assert(Grp<true, C, TC<C>::NG - 1>::LO == Grp<false, C, 0>::LO, "interpolation ends on the top group");
+    const uint32_t no_tr[LowTw<C>::N] = {};  // every group's twiddles are in LDS here
     uint32_t *tl = lds + wave * TS;
     // the strided copies: thread (g, t0) moves elements t = t0 + 64 j of tile g, which sit
     // at (hi, t, lo0 + g) of a column (runs of G words along g)
@@ -828,9 +935,9 @@ __device__ __forceinline__ void lde_mid_body(const LdeFused &f, const VB vb) {
             for (uint32_t j = 0; j < PER; j++) pf[j] = ldg(src + ((uint64_t)j << jsh));
         }
         __syncthreads();
+        // the interpolation's last group is the evaluation's first (top bits): it stays in registers
         uint32_t c[TC<C>::R];
-        tile_run<true, C, 0>(tl, lane, twl);
-        g_load<C, Top::LO, Top::r>(c, tl, lane);
+        tile_lds<true, C, true, false>(c, tl, lane, twl, no_tr);
 #pragma unroll
         for (int j = 0; j < TC<C>::R; j++) c[j] = m31_mul(c[j], f.inv_n);
         uint32_t *out_col = p.out + (uint64_t)col * p.out_cs + off;
