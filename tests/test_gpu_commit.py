@@ -188,3 +188,29 @@ def test_autotune_picks_reused_by_a_second_process(tmp_path):
     assert cache.read_text().splitlines() == lines  # nothing tuned, nothing appended
     assert second["picks"] == first["picks"]
     assert second["s"] < first["s"]
+
+
+@pytest.mark.parametrize("n_cols,lg", [(24, 21), (12, 22)])
+def test_lde_commit_11_stage_passes(zctx, torch_dev, n_cols, lg):
+    """2^21 and 2^22 rows at blowup 4 take 11-stage passes (first interpolation pass,
+    fused middle pass, last evaluation pass) with several columns per workgroup:
+    every LDE word and tree node equals the reference-shaped plain path's, and the
+    first and last columns equal the oracle's LDE."""
+    import torch
+    beta = 2
+    n, N = 1 << lg, 1 << (lg + beta)
+    d_cols = torch.empty((n_cols, n), dtype=torch.int32, device=torch_dev)
+    zctx.fill_m31_dev(lg, d_cols.data_ptr(), n_cols * n)
+    outs = []
+    for flags in (0, Z.ZS_FLAG_PLAIN):
+        d_lde = torch.zeros((n_cols, N), dtype=torch.int32, device=torch_dev)
+        d_nodes = torch.zeros((2 * N - 1) * 8, dtype=torch.int32, device=torch_dev)
+        zctx.lde_commit_dev(d_cols.data_ptr(), n_cols, lg, beta, d_lde.data_ptr(), d_nodes.data_ptr(), flags)
+        zctx.synchronize()
+        outs.append((d_lde, d_nodes))
+    assert torch.equal(outs[0][0], outs[1][0])
+    assert torch.equal(outs[0][1], outs[1][1])
+    lde_h = from_dev(outs[0][0])
+    cols_h = from_dev(d_cols)
+    for c in (0, n_cols - 1):
+        assert np.array_equal(lde_h[c], O.lde(cols_h[c], beta)), c

@@ -33,10 +33,12 @@ __global__ __launch_bounds__(tile_block_max<C>()) void fft_tile_kernel(FftPass p
 // no third), i.e. 4 waves per SIMD: stating that lets the compiler use up to 128
 // VGPRs for latency hiding (twiddle loads issued early, butterflies interleaved)
 // instead of squeezing registers for an occupancy the LDS never allows.
-#define ZS_LDE_PASS_BOUNDS __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(ZS_LDE_WAVES, ZS_LDE_WAVES)))
-#ifndef ZS_LDE_WAVES
-#define ZS_LDE_WAVES 4
-#endif
+// An 11-stage pass's LDS (2^11-word tiles and twiddle sets) leaves room for 8 waves
+// per CU, 2 per SIMD, so it may use up to 256 VGPRs.
+template <int C>
+constexpr int lde_pass_waves() { return C >= 11 ? 2 : 4; }
+#define ZS_LDE_PASS_BOUNDS \
+    __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(lde_pass_waves<C>(), lde_pass_waves<C>())))
 template <int C>
 __global__ ZS_LDE_PASS_BOUNDS void lde_eval_last_kernel(FftPass p) {
     lde_eval_last_body<C>(p, VB::hw());
@@ -121,8 +123,8 @@ static uint32_t pick_ncb(const Ctx *ctx, uint32_t n_cols, uint32_t wgs_per_col) 
 
 int32_t run_tile_pass(Ctx *ctx, FftPass p, bool inverse, uint32_t n_cols) {
     // the LDE's first interpolation and last evaluation passes have pipelined
-    // one-wave-per-tile kernels (8..10 stages, at most 8 tiles per workgroup)
-    const bool pipe_c = p.C >= 8 && p.C <= 10 && p.s_lo == 0 && p.natural_tiles && !p.scale;
+    // one-wave-per-tile kernels (8..11 stages, at most 8 tiles per workgroup)
+    const bool pipe_c = p.C >= 8 && p.C <= 11 && p.s_lo == 0 && p.natural_tiles && !p.scale;
     const bool pipe_interp = pipe_c && inverse && p.in_mode == IN_NATURAL && p.out_mode == OUT_KSPACE &&
                              p.Q_log == 0 && p.beta == 0;
     const bool pipe_eval = pipe_c && !inverse && p.in_mode == IN_KSPACE && p.out_mode == OUT_NATURAL;
@@ -160,6 +162,7 @@ int32_t run_tile_pass(Ctx *ctx, FftPass p, bool inverse, uint32_t n_cols) {
                 ZS_PIPE_CASE(8)
                 ZS_PIPE_CASE(9)
                 ZS_PIPE_CASE(10)
+                ZS_PIPE_CASE(11)
             }
 #undef ZS_PIPE_CASE
             ZS_HIP(ctx, hipGetLastError());

@@ -199,7 +199,8 @@ __device__ __forceinline__ void g_stages_w(uint32_t (&v)[TC<C>::R], const uint32
 template <bool INV, int C, int LO, int r, int i>
 __device__ __forceinline__ void g_stages(uint32_t (&v)[TC<C>::R], uint32_t lane, const uint32_t *twl) {
     static_assert(i == 0, "a group's stages run together");
-    if constexpr (ZS_TW_PRELOAD) {
+    // an 11-stage tile holds 32 values per lane: its 31 group twiddles stay in LDS
+    if constexpr (ZS_TW_PRELOAD && C <= 10) {
         constexpr int NW = (1 << (TC<C>::RB - r)) * ((1 << r) - 1);
         uint32_t w[NW];
         g_load_tw<C, LO, r, r - 1>(w, lane, twl);
@@ -886,6 +887,12 @@ struct LdeFused {
 template <int C>
 __device__ __forceinline__ void lde_mid_body(const LdeFused &f, const VB vb) {
     constexpr uint32_t PER = (1u << C) >> 6;  // elements per thread of one column's G tiles
+    // Up to 10 stages the next column is prefetched into registers and the last coset's
+    // stores are deferred; an 11-stage tile (32 values per lane, held twice: the
+    // interpolated tile and the coset being evaluated) has no registers left for either
+    // within 128 VGPRs, so it loads and stores each column in place.
+    constexpr bool PIPE = true;
+    constexpr uint32_t NPF = PIPE ? PER : 1;
     extern __shared__ uint32_t lds[];
     const FftPass &p = f.p;
     const uint32_t TS = p.TS;
@@ -906,11 +913,11 @@ __device__ __forceinline__ void lde_mid_body(const LdeFused &f, const VB vb) {
     uint32_t *cp = lds + g * TS + lidx(t0);
     const uint32_t c0 = vb.y * p.ncb;
     const uint32_t c1 = min(c0 + p.ncb, p.n_cols);
-    uint32_t pf[PER], res[PER];
-    if (c0 < c1) {
+    uint32_t pf[NPF], res[NPF];
+    if (PIPE && c0 < c1) {
         const uint32_t *src = p.in + (uint64_t)c0 * p.in_cs + off;
 #pragma unroll
-        for (uint32_t j = 0; j < PER; j++) pf[j] = ldg(src + ((uint64_t)j << jsh));
+        for (uint32_t j = 0; j < NPF; j++) pf[j] = ldg(src + ((uint64_t)j << jsh));
     }
     // set 0: the interpolation's stages s_lo.. of the size-2^L table; set 1 + q: coset q's of the
     // size-2^(L + beta) table
@@ -922,17 +929,24 @@ __device__ __forceinline__ void lde_mid_body(const LdeFused &f, const VB vb) {
     });
     __syncthreads();  // twiddle sets staged
     for (uint32_t col = c0; col < c1; col++) {
+        if constexpr (PIPE) {
 #pragma unroll
-        for (uint32_t j = 0; j < PER; j++) cp[lidx(j << 6)] = pf[j];
-        if (col > c0) {  // the previous column's last coset, held back until this column's tiles landed
-            uint32_t *dst = p.out + (uint64_t)(col - 1) * p.out_cs + ((uint64_t)(NQ - 1) << p.L) + off;
+            for (uint32_t j = 0; j < PER; j++) cp[lidx(j << 6)] = pf[j];
+            if (col > c0) {  // the previous column's last coset, held back until this column's tiles landed
+                uint32_t *dst = p.out + (uint64_t)(col - 1) * p.out_cs + ((uint64_t)(NQ - 1) << p.L) + off;
 #pragma unroll
-            for (uint32_t j = 0; j < PER; j++) stg(dst + ((uint64_t)j << jsh), res[j]);
-        }
-        if (col + 1 < c1) {
-            const uint32_t *src = p.in + (uint64_t)(col + 1) * p.in_cs + off;
+                for (uint32_t j = 0; j < PER; j++) stg(dst + ((uint64_t)j << jsh), res[j]);
+            }
+            if (col + 1 < c1) {
+                const uint32_t *src = p.in + (uint64_t)(col + 1) * p.in_cs + off;
 #pragma unroll
-            for (uint32_t j = 0; j < PER; j++) pf[j] = ldg(src + ((uint64_t)j << jsh));
+                for (uint32_t j = 0; j < PER; j++) pf[j] = ldg(src + ((uint64_t)j << jsh));
+            }
+        } else {
+            const uint32_t *src = p.in + (uint64_t)col * p.in_cs + off;
+            batched_copy<8>(
+                PER, [&](uint32_t j) { return ldg(src + ((uint64_t)j << jsh)); },
+                [&](uint32_t j, uint32_t v) { cp[lidx(j << 6)] = v; });
         }
         __syncthreads();
         // the interpolation's last group is the evaluation's first (top bits): it stays in registers
@@ -948,20 +962,26 @@ __device__ __forceinline__ void lde_mid_body(const LdeFused &f, const VB vb) {
             for (int j = 0; j < TC<C>::R; j++) v[j] = c[j];
             fwd_from_regs<C>(v, tl, lane, twl + ((1 + q) << C));
             __syncthreads();
+            uint32_t *dst = out_col + ((uint64_t)q << p.L);
+            if constexpr (PIPE) {
 #pragma unroll
-            for (uint32_t j = 0; j < PER; j++) res[j] = cp[lidx(j << 6)];
-            if (q + 1 < NQ) {
-                uint32_t *dst = out_col + ((uint64_t)q << p.L);
+                for (uint32_t j = 0; j < PER; j++) res[j] = cp[lidx(j << 6)];
+                if (q + 1 < NQ) {
 #pragma unroll
-                for (uint32_t j = 0; j < PER; j++) stg(dst + ((uint64_t)j << jsh), res[j]);
+                    for (uint32_t j = 0; j < PER; j++) stg(dst + ((uint64_t)j << jsh), res[j]);
+                }
+            } else {
+                batched_copy<8>(
+                    PER, [&](uint32_t j) { return cp[lidx(j << 6)]; },
+                    [&](uint32_t j, uint32_t v) { stg(dst + ((uint64_t)j << jsh), v); });
             }
         }
         __syncthreads();  // every read done: the tiles may take the next column
     }
-    if (c0 < c1) {
+    if (PIPE && c0 < c1) {
         uint32_t *dst = p.out + (uint64_t)(c1 - 1) * p.out_cs + ((uint64_t)(NQ - 1) << p.L) + off;
 #pragma unroll
-        for (uint32_t j = 0; j < PER; j++) stg(dst + ((uint64_t)j << jsh), res[j]);
+        for (uint32_t j = 0; j < NPF; j++) stg(dst + ((uint64_t)j << jsh), res[j]);
     }
 }
 
