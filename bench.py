@@ -342,12 +342,25 @@ def batch_leg(Z, torch, dev, stream, n_traces, autotune):
             "autotuned_shapes": tuned, "autotune_s": round(t_tune, 2)}
 
 
-def serve_leg(Z, n_requests, budget=200 * 10**9, worker_counts=(4, 8, 12), calib_requests=64):
+def serve_leg(Z, n_requests, budget=200 * 10**9, worker_counts=(8, 12, 16), calib_requests=64, autotune=True):
     """Continuous batching under Poisson arrivals: 200/s, then 100/s (below
     capacity, so the latencies are service times).  The worker count is picked by
     measurement: a shorter run at 200/s for each candidate, the one completing
-    the most requests per second kept."""
+    the most requests per second kept.  Launch shapes of every (log rows, columns
+    rounded up to a power of two) the workload draws are tuned first; the server's
+    workers read the picks from the per-device table they share with this process."""
     import numpy as np
+
+    tuned, t_tune = 0, 0.0
+    if autotune:
+        t0 = time.perf_counter()
+        tctx = Z.Context(0)
+        for lg in range(14, 23):
+            for cb in (16, 32, 64, 128, 256):
+                tctx.autotune_lde(cb, lg, 2)
+                tuned += 1
+        tctx.close()
+        t_tune = time.perf_counter() - t0
 
     def start(n_workers):
         srv = Z.Server(0, n_workers=n_workers, hbm_budget=budget)
@@ -398,6 +411,7 @@ def serve_leg(Z, n_requests, budget=200 * 10**9, worker_counts=(4, 8, 12), calib
                        f"columns U{{16..256}} (seed 5), blowup 4, LDE + row tree + FRI to 2^5, "
                        f"inputs generated on the device from the request seed",
            "workers": n_workers, "workers_completed_per_s": calib,
+           "autotuned_shapes": tuned, "autotune_s": round(t_tune, 2),
            "budget_bytes": budget, "peak_admitted_bytes": peak, "all_ok": ok}
     out.update(over)
     out["at_100_per_s"] = dict(under, workload=f"{max(1, (3 * n_requests) // 4)} requests, Poisson mean 100/s, "
@@ -617,7 +631,7 @@ def main():
     logup = logup_leg(Z, ctx, torch, dev, not args.no_cpu)
     batch = (batch_leg(Z, torch, dev, stream, args.batch_traces, not args.no_autotune)
              if args.batch_traces > 0 else None)
-    serve = serve_leg(Z, args.serve_requests) if args.serve_requests > 0 else None
+    serve = serve_leg(Z, args.serve_requests, autotune=not args.no_autotune) if args.serve_requests > 0 else None
 
     elems = n_cols * n
     value = elems / (ms * 1e-3) / 1e6

@@ -294,13 +294,7 @@ int32_t zs_m31_lde_dev(zs_hip_ctx *h, const uint32_t *d_cols, uint32_t n_cols, u
                        uint32_t *d_lde) {
     if (!h || !d_cols || !d_lde || n_cols == 0 || log_n < 1 || log_sum_over(log_n, log_blowup, 28)) return ZS_ERR_INPUT;
     Ctx *ctx = &h->c;
-    {
-        // the launch configuration chosen for this shape, if it was tuned
-        DeviceCaches &dc = *ctx->caches;
-        std::lock_guard<std::mutex> lk(dc.mu);
-        auto it = dc.tuned.find(lde_shape_key(n_cols, log_n, log_blowup));
-        ctx->lde_tiles = it != dc.tuned.end() ? it->second : default_lde_tiles();
-    }
+    ctx->lde_tiles = lde_tiles_for(ctx->device, n_cols, log_n, log_blowup);
     size_t n = size_t(1) << log_n, N = size_t(1) << (log_n + log_blowup);
     size_t coef_words;
     size_t big_words = lde_scratch_words(n_cols, log_n, log_blowup, &coef_words);
@@ -344,30 +338,75 @@ std::string device_key(int device) {
     return buf;
 }
 
-bool autotune_cache_lookup(int device, uint64_t key, LdeTiles *t, uint32_t *us) {
+bool autotune_refresh() {
+    const char *r = getenv("ZS_AUTOTUNE_REFRESH");
+    return r && atoi(r);
+}
+
+// every persisted pick of this device into `into` (the last line of a key wins)
+void autotune_cache_load(int device, std::map<uint64_t, TunedPick> &into) {
     const std::string path = autotune_cache_path(), dev = device_key(device);
-    if (path.empty() || dev.empty() || (getenv("ZS_AUTOTUNE_REFRESH") && atoi(getenv("ZS_AUTOTUNE_REFRESH"))))
-        return false;
+    if (path.empty() || dev.empty() || autotune_refresh()) return;
     FILE *f = fopen(path.c_str(), "r");
-    if (!f) return false;
+    if (!f) return;
     char line[512];
-    bool found = false;
     while (fgets(line, sizeof line, f)) {
         char d[200];
         unsigned long long k;
         unsigned v[7];
         if (sscanf(line, "%199s %llx %u %u %u %u %u %u %u", d, &k, &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6]) != 9)
             continue;
-        if (dev != d || k != key) continue;
+        if (dev != d) continue;
         bool ok = true;
         for (int r = 0; r < 3; r++) ok = ok && v[2 * r] >= 11 && v[2 * r] <= 14 && v[2 * r + 1] >= 1 && v[2 * r + 1] <= 8;
         if (!ok) continue;  // not a configuration the tuner can pick: ignored
-        for (int r = 0; r < 3; r++) t->role[r] = {v[2 * r], v[2 * r + 1]};
-        *us = v[6];
-        found = true;  // keep reading: the last line of a key wins
+        TunedPick tp;
+        for (int r = 0; r < 3; r++) tp.t.role[r] = {v[2 * r], v[2 * r + 1]};
+        tp.us = v[6];
+        into[k] = tp;
     }
     fclose(f);
-    return found;
+}
+
+// Tuned launch shapes are per device and shared by every context of the process
+// (a context, its batch workers, a server's workers), so a shape tuned by any of
+// them serves all.  The first lookup on a device reads that device's persisted picks.
+struct TunedTable {
+    std::mutex mu;
+    bool loaded = false;
+    std::map<uint64_t, TunedPick> picks;
+};
+TunedTable &tuned_table(int device) {
+    static std::mutex m;
+    static std::map<int, std::unique_ptr<TunedTable>> tables;
+    std::lock_guard<std::mutex> lk(m);
+    auto &t = tables[device];
+    if (!t) t = std::make_unique<TunedTable>();
+    return *t;
+}
+void tuned_load_locked(TunedTable &tb, int device) {
+    if (tb.loaded) return;
+    tb.loaded = true;
+    autotune_cache_load(device, tb.picks);
+}
+void tuned_set(int device, uint64_t key, const LdeTiles &t, uint32_t us) {
+    TunedTable &tb = tuned_table(device);
+    std::lock_guard<std::mutex> lk(tb.mu);
+    tb.picks[key] = {t, us};
+}
+void tuned_erase(int device, uint64_t key) {
+    TunedTable &tb = tuned_table(device);
+    std::lock_guard<std::mutex> lk(tb.mu);
+    tb.picks.erase(key);
+}
+bool tuned_find(int device, uint64_t key, TunedPick *out) {
+    TunedTable &tb = tuned_table(device);
+    std::lock_guard<std::mutex> lk(tb.mu);
+    tuned_load_locked(tb, device);
+    auto it = tb.picks.find(key);
+    if (it == tb.picks.end()) return false;
+    *out = it->second;
+    return true;
 }
 
 void autotune_cache_store(int device, uint64_t key, const LdeTiles &t, uint32_t us) {
@@ -386,6 +425,28 @@ void autotune_cache_store(int device, uint64_t key, const LdeTiles &t, uint32_t 
 }
 }  // namespace
 
+// The launch shapes of an LDE: the pick tuned for this shape (columns rounded up
+// to a power of two), else the pick of the nearest tuned column count with the
+// same log sizes, else the defaults.
+LdeTiles zs::lde_tiles_for(int device, uint32_t n_cols, uint32_t log_n, uint32_t log_blowup) {
+    const uint64_t key = lde_shape_key(n_cols, log_n, log_blowup);
+    TunedTable &tb = tuned_table(device);
+    std::lock_guard<std::mutex> lk(tb.mu);
+    tuned_load_locked(tb, device);
+    auto it = tb.picks.find(key);
+    if (it != tb.picks.end()) return it->second.t;
+    const TunedPick *near = nullptr;
+    int best = 1 << 30;
+    for (auto lo = tb.picks.lower_bound(key & ~uint64_t(0xff)); lo != tb.picks.end() && (lo->first >> 8) == (key >> 8); ++lo) {
+        const int d = std::abs((int)(lo->first & 0xff) - (int)(key & 0xff));
+        if (d <= best) {  // ties: the larger column count
+            best = d;
+            near = &lo->second;
+        }
+    }
+    return near ? near->t : default_lde_tiles();
+}
+
 extern "C" {
 
 // Times each candidate configuration with per-kernel events and keeps, for each
@@ -398,20 +459,15 @@ int32_t zs_autotune_lde(zs_hip_ctx *h, uint32_t n_cols, uint32_t log_n, uint32_t
     Ctx *ctx = &h->c;
     ZS_HIP(ctx, hipSetDevice(ctx->device));
     {
-        LdeTiles cached;
-        uint32_t us = 0;
-        const uint64_t key = lde_shape_key(n_cols, log_n, log_blowup);
-        if (autotune_cache_lookup(ctx->device, key, &cached, &us)) {
-            {
-                std::lock_guard<std::mutex> lk(ctx->caches->mu);
-                ctx->caches->tuned[key] = cached;
-            }
+        // tuned before, in this process or (persisted) by an earlier one
+        TunedPick cached;
+        if (tuned_find(ctx->device, lde_shape_key(n_cols, log_n, log_blowup), &cached)) {
             if (out) {
                 for (int k = 0; k < 3; k++) {
-                    out[2 * k] = cached.role[k].elems_log;
-                    out[2 * k + 1] = cached.role[k].cols_per_wg;
+                    out[2 * k] = cached.t.role[k].elems_log;
+                    out[2 * k + 1] = cached.t.role[k].cols_per_wg;
                 }
-                out[6] = us;
+                out[6] = cached.us;
             }
             return ZS_OK;
         }
@@ -434,10 +490,8 @@ int32_t zs_autotune_lde(zs_hip_ctx *h, uint32_t n_cols, uint32_t log_n, uint32_t
     uint8_t ref_root[32];
     bool have_ref = false;
     int32_t rc = ZS_OK;
-    auto set_tuned = [&](const LdeTiles &t) {
-        std::lock_guard<std::mutex> lk(ctx->caches->mu);
-        ctx->caches->tuned[key] = t;
-    };
+    // the candidate being timed is installed as this shape's pick
+    auto set_tuned = [&](const LdeTiles &t) { tuned_set(ctx->device, key, t, 0); };
     for (uint32_t el = 11; el <= 14 && rc == ZS_OK; el++) {
         for (uint32_t cpw = 1; cpw <= 8 && rc == ZS_OK; cpw *= 2) {
             TileConfig cand;
@@ -523,11 +577,8 @@ int32_t zs_autotune_lde(zs_hip_ctx *h, uint32_t n_cols, uint32_t log_n, uint32_t
     prof_clear(ctx);
     ctx->prof.swap(saved_prof);
     ctx->profiling = was_profiling;
-    {
-        std::lock_guard<std::mutex> lk(ctx->caches->mu);
-        if (rc == ZS_OK) ctx->caches->tuned[key] = best;
-        else ctx->caches->tuned.erase(key);
-    }
+    if (rc == ZS_OK) tuned_set(ctx->device, key, best, (uint32_t)(total_ms * 1000.0f));
+    else tuned_erase(ctx->device, key);
     if (rc == ZS_OK) autotune_cache_store(ctx->device, key, best, (uint32_t)(total_ms * 1000.0f));
     if (rc == ZS_OK && out) {
         for (int k = 0; k < 3; k++) {

@@ -39,12 +39,19 @@ template <int C>
 constexpr int lde_pass_waves() { return C >= 11 ? 2 : 4; }
 #define ZS_LDE_PASS_BOUNDS \
     __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(lde_pass_waves<C>(), lde_pass_waves<C>())))
+// The first and last passes of up to 10 stages may run 16 tiles per workgroup
+// (1024 threads, 86 KB of LDS at 10 stages: one workgroup per CU, still 4 waves per
+// SIMD), so the natural-order copies move 64-byte runs.
 template <int C>
-__global__ ZS_LDE_PASS_BOUNDS void lde_eval_last_kernel(FftPass p) {
+constexpr int lde_pipe_threads() { return C >= 11 ? 512 : 1024; }
+#define ZS_LDE_PIPE_BOUNDS \
+    __launch_bounds__(lde_pipe_threads<C>()) __attribute__((amdgpu_waves_per_eu(lde_pass_waves<C>(), lde_pass_waves<C>())))
+template <int C>
+__global__ ZS_LDE_PIPE_BOUNDS void lde_eval_last_kernel(FftPass p) {
     lde_eval_last_body<C>(p, VB::hw());
 }
 template <int C>
-__global__ ZS_LDE_PASS_BOUNDS void lde_interp_first_kernel(FftPass p) {
+__global__ ZS_LDE_PIPE_BOUNDS void lde_interp_first_kernel(FftPass p) {
     lde_interp_first_body<C>(p, VB::hw());
 }
 template <int C>
@@ -123,15 +130,17 @@ static uint32_t pick_ncb(const Ctx *ctx, uint32_t n_cols, uint32_t wgs_per_col) 
 
 int32_t run_tile_pass(Ctx *ctx, FftPass p, bool inverse, uint32_t n_cols) {
     // the LDE's first interpolation and last evaluation passes have pipelined
-    // one-wave-per-tile kernels (8..11 stages, at most 8 tiles per workgroup)
+    // one-wave-per-tile kernels (8..11 stages, at most 16 tiles per workgroup up to
+    // 10 stages, 8 at 11)
     const bool pipe_c = p.C >= 8 && p.C <= 11 && p.s_lo == 0 && p.natural_tiles && !p.scale;
     const bool pipe_interp = pipe_c && inverse && p.in_mode == IN_NATURAL && p.out_mode == OUT_KSPACE &&
                              p.Q_log == 0 && p.beta == 0;
     const bool pipe_eval = pipe_c && !inverse && p.in_mode == IN_KSPACE && p.out_mode == OUT_NATURAL;
-    if (pipe_eval) p.Q_log = std::min<uint32_t>(p.Q_log, 3);
+    const int32_t pipe_tiles_log = p.C >= 11 ? 3 : 4;
+    if (pipe_eval) p.Q_log = std::min<uint32_t>(p.Q_log, (uint32_t)pipe_tiles_log);
     uint32_t avail = p.s_lo > 0 ? p.s_lo : (p.L - p.C);
     int32_t want = (int32_t)ctx->tile.elems_log - (int32_t)p.C - (int32_t)p.Q_log;
-    if (pipe_interp || pipe_eval) want = std::min<int32_t>(want, 3 - (int32_t)p.Q_log);
+    if (pipe_interp || pipe_eval) want = std::min<int32_t>(want, pipe_tiles_log - (int32_t)p.Q_log);
     p.G_log = (uint32_t)std::max<int32_t>(0, std::min<int32_t>((int32_t)avail, want));
     uint32_t tiles = 1u << (p.L - p.C);
     uint32_t ntile = 1u << (p.G_log + p.Q_log);
@@ -143,7 +152,7 @@ int32_t run_tile_pass(Ctx *ctx, FftPass p, bool inverse, uint32_t n_cols) {
     dim3 grid(gx, (n_cols + p.ncb - 1) / p.ncb, gz);
     uint32_t maxw = p.C >= 11 ? 8 : 16;
     dim3 block(64 * std::min<uint32_t>(ntile, maxw));
-    if ((pipe_interp || pipe_eval) && ntile <= 8) {
+    if ((pipe_interp || pipe_eval) && ntile <= (1u << pipe_tiles_log)) {
         // a set keeps only the stages above the lowest register group
         const uint32_t set_log = p.C - low_group_stages((int)p.C);
         const size_t lds = (size_t(ntile) * p.TS + (size_t(ntile) << set_log)) * 4;

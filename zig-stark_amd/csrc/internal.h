@@ -112,6 +112,13 @@ struct LdeTiles {
     TileConfig role[3];
 };
 LdeTiles default_lde_tiles();
+// a tuned pick and the LDE time it was measured at (microseconds)
+struct TunedPick {
+    LdeTiles t;
+    uint32_t us = 0;
+};
+// the launch shapes for an LDE on `device`: tuned, nearest tuned column count, or default
+LdeTiles lde_tiles_for(int device, uint32_t n_cols, uint32_t log_n, uint32_t log_blowup);
 // a + b > max for log sizes, without the wrap of a 32-bit sum
 inline bool log_sum_over(uint32_t a, uint32_t b, uint32_t max) { return a > max || b > max || a + b > max; }
 // key of an LDE shape in the tuned table: columns rounded up to a power of two
@@ -134,8 +141,6 @@ struct DeviceCaches {
     std::map<uint64_t, std::unique_ptr<DevBuf>> qtables;
     // FRI remainder interpolation matrices, keyed by (L << 32) | (remainder_log << 16) | coefficients
     std::map<uint64_t, std::unique_ptr<DevBuf>> remainder_mats;
-    // launch configurations chosen by zs_autotune_lde, keyed by lde_shape_key
-    std::map<uint64_t, LdeTiles> tuned;
     // a commit job's starting channel state (Channel.init("zig-stark:hip:commit")) on the device
     std::unique_ptr<DevBuf> commit_channel;
 };

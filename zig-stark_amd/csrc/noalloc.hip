@@ -111,12 +111,7 @@ extern "C" int32_t zs_m31_lde_commit_dev_noalloc(zs_hip_ctx *h, const uint32_t *
         ctx->set_error(d.error);
         return d.status;
     }
-    {
-        // the launch configuration chosen for this shape, if it was tuned
-        std::lock_guard<std::mutex> lk(ctx->caches->mu);
-        auto it = ctx->caches->tuned.find(lde_shape_key(n_cols, log_n, log_blowup));
-        ctx->lde_tiles = it != ctx->caches->tuned.end() ? it->second : default_lde_tiles();
-    }
+    ctx->lde_tiles = lde_tiles_for(ctx->device, n_cols, log_n, log_blowup);
     uint32_t *buf = d.ptr + kNaTwWords;
     const LdeTables tabs{log_n >= 3 ? tw_inv(d.ptr, log_n) : nullptr, LT >= 3 ? tw_fwd(d.ptr, LT) : nullptr};
     const size_t n = size_t(1) << log_n, N = size_t(1) << LT;
