@@ -428,8 +428,17 @@ static int32_t launch_lde_impl(Ctx *ctx, const uint32_t *d_in, uint64_t in_strid
     {
         auto pi = plan_passes(log_n, true, PASS_CMAX);
         auto pf = plan_passes(log_n, false, PASS_CMAX);
+        if (log_n >= 12 && log_n <= 15) {
+            // two even passes would be 6..8 stages each, too few for the register-group
+            // kernels: the fused middle pass takes 8..10 stages and a generic LDS pass the
+            // 4..5 below it, on each side
+            const uint32_t cm = std::min<uint32_t>(10, log_n - 4);
+            pi = {{0, log_n - cm}, {log_n - cm, cm}};
+            pf = {{log_n - cm, cm}, {0, log_n - cm}};
+        }
+        // the fused middle pass needs a register-group shape; the other passes run on the
+        // register-group kernels where their stage count allows, else on the generic pass
         bool tiled = log_n >= 8 && lde_fused_ok(pf[0].C, log_blowup, pf.size() == 1);
-        for (auto &r : pf) tiled = tiled && tile_pass_ok(r.C);
         if (tiled) return launch_lde_tiled(ctx, d_in, in_stride, n_cols, log_n, log_blowup, d_out, out_stride,
                                            d_coef, d_big, hc_small, hc_big, pi, pf, given);
     }
