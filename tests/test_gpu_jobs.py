@@ -40,6 +40,25 @@ def test_commit_job_matches_oracle(zctx, torch_dev, n_cols, lg):
     assert (root2, roots2) == (root, roots) and np.array_equal(rem2, rem)
 
 
+@pytest.mark.parametrize("n_cols,lg,beta", [(96, 12, 2), (130, 13, 2), (97, 12, 3), (128, 14, 1), (256, 12, 2)])
+def test_commit_job_combination_before_blowup(zctx, torch_dev, n_cols, lg, beta):
+    """From 96 columns on the QM31 combination is formed over the input rows and blown up
+    as 4 M31 columns (the LDE is linear); the commitment equals the oracle's, which
+    combines the LDE columns, and the plain path's, which combines after the blowup too."""
+    rng = np.random.default_rng(n_cols * 1000 + lg * 10 + beta)
+    cols = rng.integers(0, P, (n_cols, 1 << lg), dtype=np.uint32)
+    d_cols, d_lde, d_nodes = dev_job(torch_dev, cols, beta)
+    rem_log = beta + 3
+    got = zctx.commit_job_dev(d_cols.data_ptr(), n_cols, lg, beta, rem_log, d_lde.data_ptr(), d_nodes.data_ptr())
+    plain = zctx.commit_job_dev(d_cols.data_ptr(), n_cols, lg, beta, rem_log, d_lde.data_ptr(), d_nodes.data_ptr(),
+                                Z.ZS_FLAG_PLAIN)
+    oroot, oroots, orem, _, _ = O.commit_job(cols, beta, rem_log)
+    for root, roots, rem in (got, plain):
+        assert root == oroot
+        assert roots == oroots
+        assert np.array_equal(rem, orem)
+
+
 def test_uneven_batch_equals_single_jobs(zctx, torch_dev):
     rng = np.random.default_rng(7)
     shapes = [(int(rng.integers(8, 40)), int(rng.integers(6, 13))) for _ in range(14)]

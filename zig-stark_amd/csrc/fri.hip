@@ -173,6 +173,43 @@ int32_t launch_combine(Ctx *ctx, const uint32_t *d_lde, uint64_t cs, uint32_t n_
     return ZS_OK;
 }
 
+__global__ __launch_bounds__(256) void combine_planes_kernel(const uint32_t *cols, uint64_t cs, uint32_t n_cols,
+                                                             uint64_t n, const uint32_t *powers, uint32_t *planes) {
+    extern __shared__ uint32_t pw[];
+    for (uint32_t k = threadIdx.x; k < 4 * n_cols; k += blockDim.x) pw[k] = powers[k];
+    __syncthreads();
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint4 v = combine_row(cols, cs, n_cols, i, pw);
+    stg(planes + i, v.x);
+    stg(planes + n + i, v.y);
+    stg(planes + 2 * n + i, v.z);
+    stg(planes + 3 * n + i, v.w);
+}
+
+int32_t launch_combine_planes(Ctx *ctx, const uint32_t *d_cols, uint64_t cs, uint32_t n_cols, uint64_t n,
+                              const uint32_t *d_powers, uint32_t *d_planes) {
+    size_t lds = size_t(n_cols) * 16;
+    if (lds > 64 * 1024) return ZS_ERR_INPUT;
+    ZS_LAUNCH(ctx, "combine_planes_kernel", combine_planes_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), lds,
+              ctx->stream, d_cols, cs, n_cols, n, d_powers, d_planes);
+    ZS_HIP(ctx, hipGetLastError());
+    return ZS_OK;
+}
+
+__global__ __launch_bounds__(256) void interleave4_kernel(const uint32_t *planes, uint64_t n, uint32_t *out) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    stg4(out + 4 * i, make_uint4(ldg(planes + i), ldg(planes + n + i), ldg(planes + 2 * n + i), ldg(planes + 3 * n + i)));
+}
+
+int32_t launch_interleave4(Ctx *ctx, const uint32_t *d_planes, uint64_t n, uint32_t *d_out) {
+    ZS_LAUNCH(ctx, "interleave4_kernel", interleave4_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0,
+              ctx->stream, d_planes, n, d_out);
+    ZS_HIP(ctx, hipGetLastError());
+    return ZS_OK;
+}
+
 }  // namespace zs
 
 namespace zs {

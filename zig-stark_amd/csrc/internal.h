@@ -303,6 +303,12 @@ int32_t fri_commit_impl(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t dlog
                         uint8_t *stage = nullptr);
 int32_t launch_combine(Ctx *ctx, const uint32_t *d_lde, uint64_t cs, uint32_t n_cols, uint64_t n,
                        const uint32_t *d_powers, uint32_t *d_out);
+// the same combination with its four M31 components written as planes: component k of row
+// i at d_planes[k * n + i]
+int32_t launch_combine_planes(Ctx *ctx, const uint32_t *d_cols, uint64_t cs, uint32_t n_cols, uint64_t n,
+                              const uint32_t *d_powers, uint32_t *d_planes);
+// four planes of n words -> n interleaved QM31 values (d_out[4 i + k] = d_planes[k * n + i])
+int32_t launch_interleave4(Ctx *ctx, const uint32_t *d_planes, uint64_t n, uint32_t *d_out);
 
 // FRI remainders have at most 2^kMaxRemainderLog coefficients (remainder_log - log_blowup <= 6)
 constexpr uint32_t kMaxRemainderLog = 6;
