@@ -364,7 +364,7 @@ static int32_t launch_lde_tiled(Ctx *ctx, const uint32_t *d_in, uint64_t in_stri
     }
     ctx->tile = ctx->lde_tiles.role[LDE_INTERP];
     ctx->lde_role = LDE_INTERP;
-    for (size_t i = 0; i + 1 < np; i++) {
+    for (size_t i = 0; i + 1 < pi.size(); i++) {
         FftPass p{};
         bool first = i == 0;
         p.in = first ? d_in : d_coef;
