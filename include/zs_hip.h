@@ -138,7 +138,11 @@ int32_t zs_m31_lde_commit_dev_noalloc(zs_hip_ctx *ctx, const uint32_t *d_cols, u
  * with events, and keeps for each kind of pass (interpolation passes, the fused
  * middle kernel, evaluation passes) the candidate it ran fastest under, for that
  * shape (log_n, log_blowup, n_cols rounded up to a power of two): later LDE
- * calls of the shape on this context and its batch workers use it.
+ * calls of the shape on the same device use it, from every context of the
+ * process (batch workers and server workers included); a shape not tuned takes
+ * the pick of the nearest tuned column count with the same log sizes.  Picks are
+ * appended to a per-device file ($ZS_AUTOTUNE_CACHE, else the user's cache
+ * directory; "off" disables it) that the next process reads instead of tuning.
  * out (may be NULL): tile elements log2 and columns per workgroup of the
  * interpolation, middle and evaluation passes, then the microseconds of one
  * LDE under the chosen combination. */
