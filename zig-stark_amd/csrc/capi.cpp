@@ -741,6 +741,9 @@ int32_t zs::fri_commit_impl(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t 
         r->tree_off.push_back(tw);
         tw += tree_words(dlog - i);
     }
+    // the caller wrote layer 0's leaves where its tree goes (only into the scratch trees)
+    const bool leaves0_given = (flags & kFriLeaves0Given) != 0;
+    if (leaves0_given && out_result) return ZS_ERR_INPUT;
     // A kept result owns its layers (layer 0 copied, for the query phase) and trees.
     // Otherwise layer 0 is read in place and layers 1..L and the trees live in the
     // context's scratch.
@@ -798,8 +801,11 @@ int32_t zs::fri_commit_impl(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t 
         }
         return launch_fri_tail(ctx, t);
     };
+    // with the challenges given and two folds in the first launch, that launch hashes
+    // layer 0's leaves too (it reads every layer-0 value)
+    const bool leaves0_in_fold2 = alphas && i_tail >= 2;
     auto enqueue = [&]() -> int32_t {
-        ZS_TRY(launch_qm31_leaves(ctx, layer(0), n0, trees));
+        if (!leaves0_given && !leaves0_in_fold2) ZS_TRY(launch_qm31_leaves(ctx, layer(0), n0, trees));
         if (alphas) {
             std::vector<uint64_t> offs(L);
             std::vector<uint32_t> logs(L);
@@ -809,8 +815,9 @@ int32_t zs::fri_commit_impl(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t 
                 const uint32_t li = dlog - i;
                 if (i + 1 < i_tail) {
                     uint32_t *leaves2 = (i + 2 < L) ? trees + r->tree_off[i + 2] : nullptr;
-                    ZS_TRY(launch_fri_fold2(ctx, layer(i), size_t(1) << li, dlog, i, alpha_dev + 4 * i, layer(i + 1),
-                                            trees + r->tree_off[i + 1], layer(i + 2), leaves2));
+                    uint32_t *leaves0 = (i == 0 && !leaves0_given) ? trees : nullptr;
+                    ZS_TRY(launch_fri_fold2(ctx, layer(i), size_t(1) << li, dlog, i, alpha_dev + 4 * i, leaves0,
+                                            layer(i + 1), trees + r->tree_off[i + 1], layer(i + 2), leaves2));
                     i += 2;
                 } else {
                     uint32_t *next_leaves = (i + 1 < L) ? trees + r->tree_off[i + 1] : nullptr;
@@ -852,7 +859,7 @@ int32_t zs::fri_commit_impl(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t 
         std::array<uint64_t, 6> key = {reinterpret_cast<uint64_t>(lay0), reinterpret_cast<uint64_t>(layers),
                                        reinterpret_cast<uint64_t>(trees), reinterpret_cast<uint64_t>(alpha_dev),
                                        reinterpret_cast<uint64_t>(d_chan),
-                                       (uint64_t(dlog) << 32) | (uint64_t(L) << 16) | rc};
+                                       (uint64_t(leaves0_given) << 48) | (uint64_t(dlog) << 32) | (uint64_t(L) << 16) | rc};
         auto it = ctx->fri_graphs.find(key);
         if (it == ctx->fri_graphs.end()) {
             if (ctx->fri_graphs.size() >= 32) {
@@ -890,9 +897,15 @@ int32_t zs::fri_commit_impl(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t 
                                    ctx->stream));
         return ZS_OK;
     }
-    ZS_HIP(ctx, hipMemcpyAsync(roots_out, roots_dev, size_t(L) * 32, hipMemcpyDeviceToHost, ctx->stream));
-    ZS_HIP(ctx, hipMemcpyAsync(remainder_out, rem_dev, size_t(rc) * 16, hipMemcpyDeviceToHost, ctx->stream));
+    // roots and remainder are contiguous on the device: one copy into the context's
+    // pinned buffer (two copies into pageable memory cost ~35 us of host staging)
+    const size_t out_bytes = size_t(L) * 32 + size_t(rc) * 16;
+    ZS_TRY(ctx->ensure_pinned(out_bytes));
+    const uint8_t *pin = static_cast<const uint8_t *>(ctx->pinned);
+    ZS_HIP(ctx, hipMemcpyAsync(ctx->pinned, roots_dev, out_bytes, hipMemcpyDeviceToHost, ctx->stream));
     ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::memcpy(roots_out, pin, size_t(L) * 32);
+    std::memcpy(remainder_out, pin + size_t(L) * 32, size_t(rc) * 16);
     if (out_result) *out_result = guard.release();
     return ZS_OK;
 }

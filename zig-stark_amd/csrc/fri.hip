@@ -95,17 +95,27 @@ __global__ __launch_bounds__(256) void fri_fold_kernel(const uint32_t *cur, uint
 // values) reads cur[k], cur[k + n/2], cur[k + n/4], cur[k + 3n/4], folds them into
 // layer i+1's values at k and k + n/4 (fri.zig:211-219, the same fold_value), and
 // those into layer i+2's value at k, writing both layers and their leaves.  Layer
-// i+1 is never read back, and one launch replaces two.
+// i+1 is never read back, and one launch replaces two.  With leaves0 the thread also
+// writes the hashQM31 leaves of the four layer-i values it read (layer 0's leaves,
+// with no launch of their own).
 __global__ __launch_bounds__(256) void fri_fold2_kernel(const uint32_t *cur, uint64_t quarter, uint32_t layer,
                                                         const uint32_t *tables, qm31 inv_off0, qm31 inv_off1,
-                                                        const uint32_t *alpha_p, uint32_t *next1, uint32_t *leaves1,
-                                                        uint32_t *next2, uint32_t *leaves2) {
+                                                        const uint32_t *alpha_p, uint32_t *leaves0, uint32_t *next1,
+                                                        uint32_t *leaves1, uint32_t *next2, uint32_t *leaves2) {
     const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= quarter) return;
     const qm31 a0 = qm_half(qm_mul(ldg_qm(alpha_p), inv_off0)), a1 = qm_half(qm_mul(ldg_qm(alpha_p + 4), inv_off1));
     const uint64_t half = 2 * quarter;
-    const qm31 lo = fold_value(cur, half, k, layer, tables, a0);
-    const qm31 hi = fold_value(cur, half, k + quarter, layer, tables, a0);
+    const qm31 y0 = ldg_qm(cur + 4 * k), y1 = ldg_qm(cur + 4 * (k + half));
+    const qm31 y2 = ldg_qm(cur + 4 * (k + quarter)), y3 = ldg_qm(cur + 4 * (k + quarter + half));
+    if (leaves0) {
+        store_qm31_leaf(leaves0 + 8 * k, y0);
+        store_qm31_leaf(leaves0 + 8 * (k + half), y1);
+        store_qm31_leaf(leaves0 + 8 * (k + quarter), y2);
+        store_qm31_leaf(leaves0 + 8 * (k + quarter + half), y3);
+    }
+    const qm31 lo = fold_values(y0, y1, k, layer, tables, a0);
+    const qm31 hi = fold_values(y2, y3, k + quarter, layer, tables, a0);
     stg_qm(next1 + 4 * k, lo);
     stg_qm(next1 + 4 * (k + quarter), hi);
     store_qm31_leaf(leaves1 + 8 * k, lo);
@@ -119,15 +129,15 @@ __global__ __launch_bounds__(256) void fri_fold2_kernel(const uint32_t *cur, uin
 }
 
 int32_t launch_fri_fold2(Ctx *ctx, const uint32_t *d_cur, uint64_t n, uint32_t dlog, uint32_t layer,
-                         const uint32_t *d_alpha, uint32_t *d_next1, uint32_t *d_leaves1, uint32_t *d_next2,
-                         uint32_t *d_leaves2) {
+                         const uint32_t *d_alpha, uint32_t *d_leaves0, uint32_t *d_next1, uint32_t *d_leaves1,
+                         uint32_t *d_next2, uint32_t *d_leaves2) {
     if (n < 4 || !d_leaves1) return ZS_ERR_INPUT;
     const uint32_t *tables;
     ZS_TRY(get_fri_tables(ctx, dlog, &tables));
     const qm31 inv0 = qm_inv(fri_offset_pow(layer)), inv1 = qm_inv(fri_offset_pow(layer + 1));
     const uint64_t quarter = n / 4;
     ZS_LAUNCH(ctx, "fri_fold2_kernel", fri_fold2_kernel, dim3((uint32_t)((quarter + 255) / 256)), dim3(256), 0,
-              ctx->stream, d_cur, quarter, layer, tables, inv0, inv1, d_alpha, d_next1, d_leaves1, d_next2,
+              ctx->stream, d_cur, quarter, layer, tables, inv0, inv1, d_alpha, d_leaves0, d_next1, d_leaves1, d_next2,
               d_leaves2);
     ZS_HIP(ctx, hipGetLastError());
     return ZS_OK;
@@ -153,22 +163,25 @@ namespace zs {
 // Products of two canonical values are < 2^62, so four of them are summed in 64 bits
 // (one multiply-add each) before a reduction.
 
+// with leaves: also the hashQM31 leaf of each value (the FRI's layer-0 leaves)
 __global__ __launch_bounds__(256) void combine_kernel(const uint32_t *lde, uint64_t cs, uint32_t n_cols, uint64_t n,
-                                                      const uint32_t *powers, uint32_t *out) {
+                                                      const uint32_t *powers, uint32_t *out, uint32_t *leaves) {
     extern __shared__ uint32_t pw[];
     for (uint32_t k = threadIdx.x; k < 4 * n_cols; k += blockDim.x) pw[k] = powers[k];
     __syncthreads();
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    stg4(out + 4 * i, combine_row(lde, cs, n_cols, i, pw));
+    const uint4 v = combine_row(lde, cs, n_cols, i, pw);
+    stg4(out + 4 * i, v);
+    if (leaves) store_qm31_leaf(leaves + 8 * i, {{v.x, v.y}, {v.z, v.w}});
 }
 
 int32_t launch_combine(Ctx *ctx, const uint32_t *d_lde, uint64_t cs, uint32_t n_cols, uint64_t n,
-                       const uint32_t *d_powers, uint32_t *d_out) {
+                       const uint32_t *d_powers, uint32_t *d_out, uint32_t *d_leaves) {
     size_t lds = size_t(n_cols) * 16;
     if (lds > 64 * 1024) return ZS_ERR_INPUT;
     ZS_LAUNCH(ctx, "combine_kernel", combine_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), lds, ctx->stream,
-              d_lde, cs, n_cols, n, d_powers, d_out);
+              d_lde, cs, n_cols, n, d_powers, d_out, d_leaves);
     ZS_HIP(ctx, hipGetLastError());
     return ZS_OK;
 }
@@ -197,15 +210,18 @@ int32_t launch_combine_planes(Ctx *ctx, const uint32_t *d_cols, uint64_t cs, uin
     return ZS_OK;
 }
 
-__global__ __launch_bounds__(256) void interleave4_kernel(const uint32_t *planes, uint64_t n, uint32_t *out) {
+__global__ __launch_bounds__(256) void interleave4_kernel(const uint32_t *planes, uint64_t n, uint32_t *out,
+                                                          uint32_t *leaves) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    stg4(out + 4 * i, make_uint4(ldg(planes + i), ldg(planes + n + i), ldg(planes + 2 * n + i), ldg(planes + 3 * n + i)));
+    const qm31 v = {{ldg(planes + i), ldg(planes + n + i)}, {ldg(planes + 2 * n + i), ldg(planes + 3 * n + i)}};
+    stg_qm(out + 4 * i, v);
+    if (leaves) store_qm31_leaf(leaves + 8 * i, v);
 }
 
-int32_t launch_interleave4(Ctx *ctx, const uint32_t *d_planes, uint64_t n, uint32_t *d_out) {
+int32_t launch_interleave4(Ctx *ctx, const uint32_t *d_planes, uint64_t n, uint32_t *d_out, uint32_t *d_leaves) {
     ZS_LAUNCH(ctx, "interleave4_kernel", interleave4_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0,
-              ctx->stream, d_planes, n, d_out);
+              ctx->stream, d_planes, n, d_out, d_leaves);
     ZS_HIP(ctx, hipGetLastError());
     return ZS_OK;
 }

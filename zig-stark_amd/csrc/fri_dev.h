@@ -43,13 +43,16 @@ __device__ __forceinline__ uint4 combine_row(const uint32_t *lde, uint64_t cs, u
 
 // next[j] of layer `layer` (fri.zig:211-219): (y0 + y1)/2 + alpha (y0 - y1) x^-1 / 2 with
 // x^-1 = inv_off * t_lo * t_hi from the two power tables; a_off = alpha * inv_off / 2
-__device__ __forceinline__ qm31 fold_value(const uint32_t *cur, uint64_t half, uint64_t j, uint32_t layer,
-                                           const uint32_t *tables, qm31 a_off) {
-    const qm31 y0 = ldg_qm(cur + 4 * j), y1 = ldg_qm(cur + 4 * (j + half));
+__device__ __forceinline__ qm31 fold_values(qm31 y0, qm31 y1, uint64_t j, uint32_t layer, const uint32_t *tables,
+                                            qm31 a_off) {
     const uint64_t e = j << layer;
     const qm31 t = qm_mul(ldg_qm(tables + 4 * (e & 1023)), ldg_qm(tables + 4 * (1024 + (e >> 10))));
     const qm31 hs = qm_half(qm_add(y0, y1));
     return qm_add(hs, qm_mul(qm_mul(a_off, t), qm_sub(y0, y1)));
+}
+__device__ __forceinline__ qm31 fold_value(const uint32_t *cur, uint64_t half, uint64_t j, uint32_t layer,
+                                           const uint32_t *tables, qm31 a_off) {
+    return fold_values(ldg_qm(cur + 4 * j), ldg_qm(cur + 4 * (j + half)), j, layer, tables, a_off);
 }
 
 // hashQM31 leaf of v at leaf

@@ -264,9 +264,10 @@ int32_t get_fri_tables(Ctx *ctx, uint32_t dlog, const uint32_t **out);
 int32_t launch_fri_fold(Ctx *ctx, const uint32_t *d_cur, uint64_t n, uint32_t dlog, uint32_t layer,
                         const uint32_t *d_alpha, uint32_t *d_next, uint32_t *d_next_leaves);
 // folds of layers `layer` and layer + 1 in one launch (challenges at d_alpha, d_alpha + 4)
+// (d_leaves0, when non-null, gets the hashQM31 leaves of layer `layer` itself)
 int32_t launch_fri_fold2(Ctx *ctx, const uint32_t *d_cur, uint64_t n, uint32_t dlog, uint32_t layer,
-                         const uint32_t *d_alpha, uint32_t *d_next1, uint32_t *d_leaves1, uint32_t *d_next2,
-                         uint32_t *d_leaves2);
+                         const uint32_t *d_alpha, uint32_t *d_leaves0, uint32_t *d_next1, uint32_t *d_leaves1,
+                         uint32_t *d_next2, uint32_t *d_leaves2);
 
 // g[i] = sum_c powers[c] * lde_c[i]: QM31 combination of n_cols M31 columns of n values
 // device Fiat-Shamir: absorb the digest at d_digest, write a QM31 challenge to d_out
@@ -302,13 +303,18 @@ int32_t fri_commit_impl(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t dlog
                         uint8_t *roots_out, uint32_t *remainder_out, zs_fri_result **out_result,
                         uint8_t *stage = nullptr);
 int32_t launch_combine(Ctx *ctx, const uint32_t *d_lde, uint64_t cs, uint32_t n_cols, uint64_t n,
-                       const uint32_t *d_powers, uint32_t *d_out);
+                       const uint32_t *d_powers, uint32_t *d_out, uint32_t *d_leaves = nullptr);
 // the same combination with its four M31 components written as planes: component k of row
 // i at d_planes[k * n + i]
 int32_t launch_combine_planes(Ctx *ctx, const uint32_t *d_cols, uint64_t cs, uint32_t n_cols, uint64_t n,
                               const uint32_t *d_powers, uint32_t *d_planes);
-// four planes of n words -> n interleaved QM31 values (d_out[4 i + k] = d_planes[k * n + i])
-int32_t launch_interleave4(Ctx *ctx, const uint32_t *d_planes, uint64_t n, uint32_t *d_out);
+// four planes of n words -> n interleaved QM31 values (d_out[4 i + k] = d_planes[k * n + i]);
+// d_leaves, when non-null, gets each value's hashQM31 leaf
+int32_t launch_interleave4(Ctx *ctx, const uint32_t *d_planes, uint64_t n, uint32_t *d_out,
+                           uint32_t *d_leaves = nullptr);
+// fri_commit_impl flag (internal): layer 0's hashQM31 leaves are already at the start of
+// scratch slot 5, where its tree goes; the commit phase does not hash them again
+constexpr uint32_t kFriLeaves0Given = 1u << 30;
 
 // FRI remainders have at most 2^kMaxRemainderLog coefficients (remainder_log - log_blowup <= 6)
 constexpr uint32_t kMaxRemainderLog = 6;
@@ -317,6 +323,13 @@ qm31 fri_offset_pow(uint32_t layer);  // FRI_OFFSET^(2^layer)
 qm31 qm31_root_of_unity(uint32_t log_size);
 
 inline size_t tree_words(uint32_t log_leaves) { return ((size_t(2) << log_leaves) - 1) * 8; }
+// FRI commit-phase trees of a codeword of 2^dlog values, L layers: their total words
+// (layer i's tree at the sum of the earlier layers' tree_words)
+inline size_t fri_tree_words(uint32_t dlog, uint32_t L) {
+    size_t tw = 0;
+    for (uint32_t i = 0; i < L; i++) tw += tree_words(dlog - i);
+    return tw;
+}
 inline size_t level_offset_words(uint32_t log_leaves, uint32_t level) {
     // digests before `level`: sum_{l<level} 2^(log_leaves - l)
     size_t n = size_t(1) << log_leaves;

@@ -107,6 +107,10 @@ int32_t commit_job(zs_hip_ctx *h, const uint32_t *d_cols, uint32_t n_cols, uint3
         ZS_TRY(launch_tree_levels(ctx, nodes, LT, 0, d_chan, d_gamma));
     }
     ZS_TRY(launch_qm31_powers(ctx, d_gamma, d_powers, n_cols));
+    // the combination also writes the FRI's layer-0 leaves, where the commit phase's
+    // first tree goes (the start of scratch slot 5)
+    ZS_TRY(ctx->ensure_scratch(5, fri_tree_words(LT, LT - remainder_log) * 4));
+    uint32_t *leaves0 = ctx->scratch[5].as<uint32_t>();
     const size_t n = size_t(1) << log_n;
     const bool before_lde = !(flags & ZS_FLAG_PLAIN) && n_cols >= kCombineBeforeLdeCols &&
                             ctx->scratch[1].bytes >= 8 * n * 4 && ctx->scratch[2].bytes >= 8 * N * 4;
@@ -116,9 +120,9 @@ int32_t commit_job(zs_hip_ctx *h, const uint32_t *d_cols, uint32_t n_cols, uint3
         ZS_TRY(launch_combine_planes(ctx, d_cols, n, n_cols, n, d_powers, h_planes));
         ctx->lde_tiles = lde_tiles_for(ctx->device, 4, log_n, log_blowup);
         ZS_TRY(launch_lde(ctx, h_planes, n, 4, log_n, log_blowup, g_planes, N, coef, big));
-        ZS_TRY(launch_interleave4(ctx, g_planes, N, ctx->scratch[8].as<uint32_t>()));
+        ZS_TRY(launch_interleave4(ctx, g_planes, N, ctx->scratch[8].as<uint32_t>(), leaves0));
     } else {
-        ZS_TRY(launch_combine(ctx, d_lde, N, n_cols, N, d_powers, ctx->scratch[8].as<uint32_t>()));
+        ZS_TRY(launch_combine(ctx, d_lde, N, n_cols, N, d_powers, ctx->scratch[8].as<uint32_t>(), leaves0));
     }
     const bool own_stage = stage == nullptr;
     if (own_stage) {
@@ -127,7 +131,7 @@ int32_t commit_job(zs_hip_ctx *h, const uint32_t *d_cols, uint32_t n_cols, uint3
     }
     ZS_HIP(ctx, hipMemcpyAsync(stage, d_root, 32, hipMemcpyDeviceToHost, ctx->stream));
     ZS_TRY(fri_commit_impl(h, ctx->scratch[8].as<uint32_t>(), LT, log_blowup, remainder_log, nullptr, d_chan,
-                           flags & ZS_FLAG_NO_GRAPH, nullptr, nullptr, nullptr, stage + 32));
+                           (flags & ZS_FLAG_NO_GRAPH) | kFriLeaves0Given, nullptr, nullptr, nullptr, stage + 32));
     if (!own_stage) return ZS_OK;
     ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     const uint32_t L = LT - remainder_log;
