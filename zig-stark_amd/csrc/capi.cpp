@@ -711,8 +711,9 @@ void zs::interpolate_host(const std::vector<qm31> &xs, const std::vector<qm31> &
 int32_t zs::fri_commit_impl(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t dlog, uint32_t log_blowup,
                             uint32_t remainder_log, const uint32_t *alphas, B3Hasher *d_chan, uint32_t flags,
                             uint8_t *roots_out, uint32_t *remainder_out, zs_fri_result **out_result,
-                            uint8_t *stage) {
-    if (!h || !d_codeword || (!stage && (!roots_out || !remainder_out)) || (stage && out_result))
+                            uint8_t *stage, const uint32_t *d_prefix) {
+    if (!h || !d_codeword || (!stage && (!roots_out || !remainder_out)) || (stage && out_result) ||
+        (d_prefix && !stage))
         return ZS_ERR_INPUT;
     if (dlog > 28 || remainder_log > dlog || remainder_log < log_blowup || dlog - remainder_log < 1)
         return ZS_ERR_INPUT;
@@ -766,12 +767,13 @@ int32_t zs::fri_commit_impl(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t 
     auto layer = [&](uint32_t i) -> uint32_t * {
         return i == 0 ? const_cast<uint32_t *>(lay0) : layers + r->layer_off[i];
     };
-    // slot 6: the challenges (16 B per layer), then the layer roots gathered for one copy (32 B
-    // per layer) and right after them the remainder coefficients (16 B each)
+    // slot 6: the challenges (16 B per layer), 32 B for a prefix digest, then the layer roots
+    // gathered for one copy (32 B per layer) and right after them the remainder
+    // coefficients (16 B each)
     const uint32_t rc = 1u << (remainder_log - log_blowup);
-    ZS_TRY(ctx->ensure_scratch(6, size_t(L) * 48 + size_t(rc) * 16));
+    ZS_TRY(ctx->ensure_scratch(6, size_t(L) * 48 + 32 + size_t(rc) * 16));
     alpha_dev = ctx->scratch[6].as<uint32_t>();
-    uint32_t *roots_dev = alpha_dev + 4 * size_t(L);
+    uint32_t *roots_dev = alpha_dev + 4 * size_t(L) + 8;
     uint32_t *rem_dev = roots_dev + 8 * size_t(L);
     const uint32_t *fri_tab, *rem_mat;
     ZS_TRY(get_fri_tables(ctx, dlog, &fri_tab));  // cached before any capture
@@ -849,17 +851,17 @@ int32_t zs::fri_commit_impl(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t 
         ZS_TRY(enqueue());
         std::vector<uint64_t> root_off(L);
         for (uint32_t i = 0; i < L; i++) root_off[i] = r->tree_off[i] + tree_words(dlog - i) - 8;
-        ZS_TRY(launch_gather_roots(ctx, trees, root_off.data(), L, roots_dev));
-        return launch_fri_remainder(ctx, layer(L), rem_mat, rc, rem_dev);
+        return launch_fri_finish(ctx, trees, root_off.data(), L, d_prefix, roots_dev, layer(L), rem_mat, rc, rem_dev);
     };
     const bool graph = !out_result && !ctx->profiling && !(flags & ZS_FLAG_NO_GRAPH);
     if (!graph) {
         ZS_TRY(enqueue_all());
     } else {
-        std::array<uint64_t, 6> key = {reinterpret_cast<uint64_t>(lay0), reinterpret_cast<uint64_t>(layers),
+        std::array<uint64_t, 7> key = {reinterpret_cast<uint64_t>(lay0), reinterpret_cast<uint64_t>(layers),
                                        reinterpret_cast<uint64_t>(trees), reinterpret_cast<uint64_t>(alpha_dev),
                                        reinterpret_cast<uint64_t>(d_chan),
-                                       (uint64_t(leaves0_given) << 48) | (uint64_t(dlog) << 32) | (uint64_t(L) << 16) | rc};
+                                       (uint64_t(leaves0_given) << 48) | (uint64_t(dlog) << 32) | (uint64_t(L) << 16) | rc,
+                                       reinterpret_cast<uint64_t>(d_prefix)};
         auto it = ctx->fri_graphs.find(key);
         if (it == ctx->fri_graphs.end()) {
             if (ctx->fri_graphs.size() >= 32) {
@@ -892,9 +894,10 @@ int32_t zs::fri_commit_impl(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t 
         }
         ZS_HIP(ctx, hipGraphLaunch(it->second, ctx->stream));
     }
-    if (stage) {  // the caller synchronises later and reads the roots, then the remainder, from stage
-        ZS_HIP(ctx, hipMemcpyAsync(stage, roots_dev, size_t(L) * 32 + size_t(rc) * 16, hipMemcpyDeviceToHost,
-                                   ctx->stream));
+    if (stage) {  // the caller synchronises later and reads (the prefix,) the roots, then the remainder
+        const size_t pre = d_prefix ? 8 : 0;
+        ZS_HIP(ctx, hipMemcpyAsync(stage, roots_dev - pre, pre * 4 + size_t(L) * 32 + size_t(rc) * 16,
+                                   hipMemcpyDeviceToHost, ctx->stream));
         return ZS_OK;
     }
     // roots and remainder are contiguous on the device: one copy into the context's

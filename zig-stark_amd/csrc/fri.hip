@@ -407,15 +407,6 @@ int32_t launch_fri_tail(Ctx *ctx, const FriTail &t) {
 // coeffs = M * ys with M the inverse Vandermonde matrix, built once per shape on
 // the host (column j = the interpolant of the j-th unit vector) and cached on
 // the device.  Thread k computes coefficient k.
-__global__ __launch_bounds__(64) void fri_remainder_kernel(const uint32_t *last, const uint32_t *mat, uint32_t rc,
-                                                           uint32_t *out) {
-    const uint32_t k = threadIdx.x;
-    if (k >= rc) return;
-    qm31 acc = {{0, 0}, {0, 0}};
-    for (uint32_t j = 0; j < rc; j++) acc = qm_add(acc, qm_mul(ld_qm(mat + 4 * (k * rc + j)), ld_qm(last + 4 * j)));
-    st_qm(out + 4 * k, acc);
-}
-
 int32_t get_remainder_matrix(Ctx *ctx, uint32_t L, uint32_t remainder_log, uint32_t rc, const uint32_t **out) {
     DeviceCaches &dc = *ctx->caches;
     const uint64_t key = (uint64_t(L) << 32) | (uint64_t(remainder_log) << 16) | rc;
@@ -456,10 +447,30 @@ int32_t get_remainder_matrix(Ctx *ctx, uint32_t L, uint32_t remainder_log, uint3
     return ZS_OK;
 }
 
-int32_t launch_fri_remainder(Ctx *ctx, const uint32_t *d_last, const uint32_t *d_mat, uint32_t rc, uint32_t *d_out) {
-    if (rc == 0 || rc > 64) return ZS_ERR_INPUT;
-    ZS_LAUNCH(ctx, "fri_remainder_kernel", fri_remainder_kernel, dim3(1), dim3(64), 0, ctx->stream, d_last, d_mat, rc,
-              d_out);
+// The end of a commit phase in one launch: every layer's root gathered into roots
+// (after the 8 words of prefix, when given: the commit job's trace root), and the
+// remainder coefficients right after the roots.
+__global__ __launch_bounds__(256) void fri_finish_kernel(const uint32_t *trees, RootOffsets ro, uint32_t n_roots,
+                                                         const uint32_t *prefix, uint32_t *roots, const uint32_t *last,
+                                                         const uint32_t *mat, uint32_t rc, uint32_t *rem) {
+    const uint32_t t = threadIdx.x;
+    if (t < 8 * n_roots) roots[t] = trees[ro.off[t >> 3] + (t & 7u)];
+    if (prefix && t < 8) (roots - 8)[t] = prefix[t];  // the 8 words just before the roots
+    if (t < rc) {
+        qm31 acc = {{0, 0}, {0, 0}};
+        for (uint32_t j = 0; j < rc; j++) acc = qm_add(acc, qm_mul(ld_qm(mat + 4 * (t * rc + j)), ld_qm(last + 4 * j)));
+        st_qm(rem + 4 * t, acc);
+    }
+}
+
+int32_t launch_fri_finish(Ctx *ctx, const uint32_t *d_trees, const uint64_t *root_off, uint32_t n_roots,
+                          const uint32_t *d_prefix, uint32_t *d_roots, const uint32_t *d_last, const uint32_t *d_mat,
+                          uint32_t rc, uint32_t *d_rem) {
+    if (n_roots > 28 || rc == 0 || rc > 64) return ZS_ERR_INPUT;
+    RootOffsets ro{};
+    for (uint32_t k = 0; k < n_roots; k++) ro.off[k] = root_off[k];
+    ZS_LAUNCH(ctx, "fri_finish_kernel", fri_finish_kernel, dim3(1), dim3(256), 0, ctx->stream, d_trees, ro, n_roots,
+              d_prefix, d_roots, d_last, d_mat, rc, d_rem);
     ZS_HIP(ctx, hipGetLastError());
     return ZS_OK;
 }

@@ -163,7 +163,7 @@ struct Ctx {
     // FRI commit phases with challenges given up front, captured once per buffer set and
     // replayed (key: codeword, layers, trees, challenges, channel, dlog / layer count / remainder length)
     hipStream_t capture_stream = nullptr;
-    std::map<std::array<uint64_t, 6>, hipGraphExec_t> fri_graphs;
+    std::map<std::array<uint64_t, 7>, hipGraphExec_t> fri_graphs;
 
     // grow-only pinned host buffer (staged results of batched jobs)
     void *pinned = nullptr;
@@ -235,13 +235,21 @@ int32_t launch_row_leaves(Ctx *ctx, const uint32_t *d_cols, uint64_t col_stride,
                           uint32_t *d_nodes);
 // node levels above an already written level 0 of 2^log_leaves digests; with
 // draw_chan, the launch that computes the root also absorbs it into that device
-// channel and draws a QM31 challenge into draw_alpha (log_leaves - from_level >= 1)
+// channel and draws a QM31 challenge into draw_alpha (log_leaves - from_level >= 1),
+// the channel starting from chan_init when given; with powers, that launch also
+// writes the challenge's powers 0 .. n_powers - 1 (QM31, 4 words each)
 int32_t launch_tree_levels(Ctx *ctx, uint32_t *d_nodes, uint32_t log_leaves, uint32_t from_level,
-                           B3Hasher *draw_chan = nullptr, uint32_t *draw_alpha = nullptr);
+                           B3Hasher *draw_chan = nullptr, uint32_t *draw_alpha = nullptr,
+                           const B3Hasher *chan_init = nullptr, uint32_t *powers = nullptr, uint32_t n_powers = 0);
 int32_t launch_qm31_leaves(Ctx *ctx, const uint32_t *d_vals, uint64_t n, uint32_t *d_nodes);
 // node levels of n_trees independent trees with written leaves, tree k at base + off_words[k]
 int32_t launch_trees_multi(Ctx *ctx, uint32_t *base, const uint64_t *off_words, const uint32_t *log_leaves,
-                           uint32_t n_trees, B3Hasher *draw_chan = nullptr, uint32_t *draw_alpha = nullptr);
+                           uint32_t n_trees, B3Hasher *draw_chan = nullptr, uint32_t *draw_alpha = nullptr,
+                           const B3Hasher *chan_init = nullptr, uint32_t *powers = nullptr, uint32_t n_powers = 0);
+// word offsets of up to 28 tree roots (a gather's argument)
+struct RootOffsets {
+    uint64_t off[28];
+};
 // Merkle path gather: for each request, log_leaves sibling digests written at out_off_words.
 struct PathReq {
     uint64_t tree_off_words;
@@ -250,9 +258,6 @@ struct PathReq {
     uint32_t index;
 };
 int32_t launch_gather_paths(Ctx *ctx, const uint32_t *d_trees, const std::vector<PathReq> &reqs, uint32_t *d_out);
-// the roots (last 8 words) at word offsets off_words[k] from d_base, k < n <= 28, into d_out (8 n words)
-int32_t launch_gather_roots(Ctx *ctx, const uint32_t *d_base, const uint64_t *off_words, uint32_t n,
-                            uint32_t *d_out);
 // copies `words` words from src + pairs[2r] to out + pairs[2r+1] for each r; synchronous
 int32_t launch_gather_words(Ctx *ctx, const uint32_t *d_src, const std::vector<uint64_t> &pairs, uint32_t words,
                             uint32_t *d_out);
@@ -291,17 +296,23 @@ int32_t launch_qm31_powers(Ctx *ctx, const uint32_t *d_x, uint32_t *d_out, uint3
 void interpolate_host(const std::vector<qm31> &xs, const std::vector<qm31> &ys, std::vector<qm31> &coeffs);
 // the FRI remainder as a device map: coefficients = M * (first rc values of layer L)
 int32_t get_remainder_matrix(Ctx *ctx, uint32_t L, uint32_t remainder_log, uint32_t rc, const uint32_t **out);
-int32_t launch_fri_remainder(Ctx *ctx, const uint32_t *d_last, const uint32_t *d_mat, uint32_t rc, uint32_t *d_out);
 // FRI commit phase on a device codeword; challenges from `alphas` (host) or drawn
 // on the device from the channel state at d_chan (left advanced past the last root).
 // With `stage` (host memory, pinned for a true asynchronous copy) the L roots and
 // then the remainder words are copied there on the stream and the call returns
 // without synchronising (roots_out and remainder_out are then unused); otherwise
 // they are in roots_out / remainder_out when it returns.
+// d_prefix (with stage only): 8 device words copied to the stage ahead of the roots
+// (the commit job's trace root), in the same copy.
 int32_t fri_commit_impl(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t dlog, uint32_t log_blowup,
                         uint32_t remainder_log, const uint32_t *alphas, B3Hasher *d_chan, uint32_t flags,
                         uint8_t *roots_out, uint32_t *remainder_out, zs_fri_result **out_result,
-                        uint8_t *stage = nullptr);
+                        uint8_t *stage = nullptr, const uint32_t *d_prefix = nullptr);
+// every layer root (tree word offsets root_off) into d_roots, d_prefix's 8 words (if
+// given) at d_roots - 8, and the remainder (rc coefficients) into d_rem, in one launch
+int32_t launch_fri_finish(Ctx *ctx, const uint32_t *d_trees, const uint64_t *root_off, uint32_t n_roots,
+                          const uint32_t *d_prefix, uint32_t *d_roots, const uint32_t *d_last, const uint32_t *d_mat,
+                          uint32_t rc, uint32_t *d_rem);
 int32_t launch_combine(Ctx *ctx, const uint32_t *d_lde, uint64_t cs, uint32_t n_cols, uint64_t n,
                        const uint32_t *d_powers, uint32_t *d_out, uint32_t *d_leaves = nullptr);
 // the same combination with its four M31 components written as planes: component k of row

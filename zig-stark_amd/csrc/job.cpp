@@ -100,13 +100,13 @@ int32_t commit_job(zs_hip_ctx *h, const uint32_t *d_cols, uint32_t n_cols, uint3
         ZS_HIP(ctx, hipMemcpyAsync(d_chan, d_init, sizeof(B3Hasher), hipMemcpyDeviceToDevice, ctx->stream));
         ZS_TRY(launch_channel_draw(ctx, d_chan, d_root, d_gamma));
     } else {
-        // LDE, row leaves, then the node levels, whose root launch also draws gamma
+        // LDE, row leaves, then the node levels, whose root launch also starts the channel
+        // from its initial state, draws gamma and writes gamma's powers
         ZS_TRY(zs_m31_lde_dev(h, d_cols, n_cols, log_n, log_blowup, d_lde));
-        ZS_HIP(ctx, hipMemcpyAsync(d_chan, d_init, sizeof(B3Hasher), hipMemcpyDeviceToDevice, ctx->stream));
         ZS_TRY(launch_row_leaves(ctx, d_lde, N, n_cols, N, nodes));
-        ZS_TRY(launch_tree_levels(ctx, nodes, LT, 0, d_chan, d_gamma));
+        ZS_TRY(launch_tree_levels(ctx, nodes, LT, 0, d_chan, d_gamma, d_init, d_powers, n_cols));
     }
-    ZS_TRY(launch_qm31_powers(ctx, d_gamma, d_powers, n_cols));
+    if (flags & ZS_FLAG_PLAIN) ZS_TRY(launch_qm31_powers(ctx, d_gamma, d_powers, n_cols));
     // the combination also writes the FRI's layer-0 leaves, where the commit phase's
     // first tree goes (the start of scratch slot 5)
     ZS_TRY(ctx->ensure_scratch(5, fri_tree_words(LT, LT - remainder_log) * 4));
@@ -129,9 +129,9 @@ int32_t commit_job(zs_hip_ctx *h, const uint32_t *d_cols, uint32_t n_cols, uint3
         ZS_TRY(ctx->ensure_pinned(kStageBytes));
         stage = static_cast<uint8_t *>(ctx->pinned);
     }
-    ZS_HIP(ctx, hipMemcpyAsync(stage, d_root, 32, hipMemcpyDeviceToHost, ctx->stream));
+    // the trace root reaches the stage ahead of the FRI roots, in the commit phase's one copy
     ZS_TRY(fri_commit_impl(h, ctx->scratch[8].as<uint32_t>(), LT, log_blowup, remainder_log, nullptr, d_chan,
-                           (flags & ZS_FLAG_NO_GRAPH) | kFriLeaves0Given, nullptr, nullptr, nullptr, stage + 32));
+                           (flags & ZS_FLAG_NO_GRAPH) | kFriLeaves0Given, nullptr, nullptr, nullptr, stage, d_root));
     if (!own_stage) return ZS_OK;
     ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     const uint32_t L = LT - remainder_log;
@@ -165,8 +165,9 @@ JobLayout job_layout(uint32_t n_cols, uint32_t log_n, uint32_t log_blowup, uint3
     for (uint32_t i = 0; i < L; i++) ft += tree_words(LT - i) * 4;
     j.len[JobLayout::FRI_LAYERS] = fl;
     j.len[JobLayout::FRI_TREES] = ft;
-    // challenges, then the gathered layer roots and the remainder coefficients (fri_commit_impl's slot 6)
-    j.len[JobLayout::ALPHAS] = uint64_t(L) * 48 + (uint64_t(16) << (remainder_log - log_blowup));
+    // challenges, the trace root, then the gathered layer roots and the remainder coefficients
+    // (fri_commit_impl's slot 6)
+    j.len[JobLayout::ALPHAS] = uint64_t(L) * 48 + 32 + (uint64_t(16) << (remainder_log - log_blowup));
     j.len[JobLayout::CHAN] = sizeof(B3Hasher);
     j.len[JobLayout::COMB] = 16 * N;
     j.len[JobLayout::POWERS] = 16 * (c + 1);

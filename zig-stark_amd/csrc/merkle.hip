@@ -88,7 +88,8 @@ constexpr uint32_t kSubtreeJoinLog = 11;
 // launch); then workgroup subtrees of 8 levels, then one workgroup per tree for
 // the rest.
 int32_t launch_trees_multi(Ctx *ctx, uint32_t *base, const uint64_t *off_words, const uint32_t *log_leaves,
-                           uint32_t n_trees, B3Hasher *draw_chan, uint32_t *draw_alpha) {
+                           uint32_t n_trees, B3Hasher *draw_chan, uint32_t *draw_alpha, const B3Hasher *chan_init,
+                           uint32_t *powers, uint32_t n_powers) {
     // a fused draw needs the root computed by one workgroup: one tree with at least one level
     if (draw_chan && (n_trees != 1 || log_leaves[0] == 0)) return ZS_ERR_INPUT;
     std::vector<uint32_t> level(n_trees, 0);
@@ -138,6 +139,9 @@ int32_t launch_trees_multi(Ctx *ctx, uint32_t *base, const uint64_t *off_words, 
             const bool last = draw_chan && bt.K[0] == bt.r[0];  // this launch computes the root
             bt.chan = last ? draw_chan : nullptr;
             bt.alpha = last ? draw_alpha : nullptr;
+            bt.chan_init = last ? chan_init : nullptr;
+            bt.powers = last ? powers : nullptr;
+            bt.n_powers = last ? n_powers : 0u;
             ZS_LAUNCH(ctx, "tree_block_kernel", tree_block_kernel, dim3(wgs), dim3(BLOCK_NT),
                       (block_lds_words(kmax) + (last ? kDrawLdsWords : 0)) * 4, ctx->stream, base, bt);
             ZS_HIP(ctx, hipGetLastError());
@@ -150,10 +154,11 @@ int32_t launch_trees_multi(Ctx *ctx, uint32_t *base, const uint64_t *off_words, 
 // Build levels from_level+1 .. top over level from_level (already written): the
 // levels from there up are laid out like a tree of 2^(log_leaves - from_level) leaves.
 int32_t launch_tree_levels(Ctx *ctx, uint32_t *d_nodes, uint32_t log_leaves, uint32_t from_level,
-                           B3Hasher *draw_chan, uint32_t *draw_alpha) {
+                           B3Hasher *draw_chan, uint32_t *draw_alpha, const B3Hasher *chan_init, uint32_t *powers,
+                           uint32_t n_powers) {
     const uint64_t off = level_offset_words(log_leaves, from_level);
     const uint32_t lg = log_leaves - from_level;
-    return launch_trees_multi(ctx, d_nodes, &off, &lg, 1, draw_chan, draw_alpha);
+    return launch_trees_multi(ctx, d_nodes, &off, &lg, 1, draw_chan, draw_alpha, chan_init, powers, n_powers);
 }
 
 int32_t launch_row_commit(Ctx *ctx, const uint32_t *d_cols, uint64_t col_stride, uint32_t n_cols, uint32_t log_rows,
@@ -236,26 +241,6 @@ int32_t launch_gather_paths(Ctx *ctx, const uint32_t *d_trees, const std::vector
 }
 
 // roots of up to 28 trees (8 words each at off[k]) gathered into out, one word per thread
-struct RootOffsets {
-    uint64_t off[28];
-};
-__global__ __launch_bounds__(256) void gather_roots_kernel(const uint32_t *base, RootOffsets ro, uint32_t n,
-                                                           uint32_t *out) {
-    const uint32_t t = threadIdx.x;
-    if (t < 8 * n) out[t] = base[ro.off[t >> 3] + (t & 7u)];
-}
-
-int32_t launch_gather_roots(Ctx *ctx, const uint32_t *d_base, const uint64_t *off_words, uint32_t n,
-                            uint32_t *d_out) {
-    if (n == 0) return ZS_OK;
-    if (n > 28) return ZS_ERR_INPUT;
-    RootOffsets ro{};
-    for (uint32_t k = 0; k < n; k++) ro.off[k] = off_words[k];
-    hipLaunchKernelGGL(gather_roots_kernel, dim3(1), dim3(256), 0, ctx->stream, d_base, ro, n, d_out);
-    ZS_HIP(ctx, hipGetLastError());
-    return ZS_OK;
-}
-
 int32_t launch_gather_words(Ctx *ctx, const uint32_t *d_src, const std::vector<uint64_t> &pairs, uint32_t words,
                             uint32_t *d_out) {
     if (pairs.empty()) return ZS_OK;

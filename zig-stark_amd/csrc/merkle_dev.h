@@ -138,8 +138,13 @@ struct BlockTrees {
     uint32_t r[MAX], K[MAX], wg_end[MAX];
     // set on the launch that computes the root of its one tree: the workgroup then
     // absorbs the root into this device channel and draws a challenge into alpha
+    // (starting the channel from chan_init when given); with powers, it also writes
+    // the challenge's powers 0 .. n_powers - 1 there
     B3Hasher *chan;
     uint32_t *alpha;
+    const B3Hasher *chan_init;
+    uint32_t *powers;
+    uint32_t n_powers;
 };
 
 constexpr uint32_t BLOCK_NT = 256;
@@ -156,6 +161,9 @@ struct OneTree {
     uint32_t r[1], K[1], wg_end[1];
     B3Hasher *chan;
     uint32_t *alpha;
+    const B3Hasher *chan_init;
+    uint32_t *powers;
+    uint32_t n_powers;
 };
 
 // workgroup s of tree t (levels from its child level, r child levels left, K of them here);
@@ -218,7 +226,7 @@ __device__ __forceinline__ void tree_block_body(uint32_t *base, const BT &bt, ui
         B3Hasher *hs = reinterpret_cast<B3Hasher *>(sm + block_lds_words(K, NT));
         B3Hasher *snap = hs + 1;
         uint32_t *qs = reinterpret_cast<uint32_t *>(snap + 1), *work = qs + 24;
-        const uint32_t *cw = reinterpret_cast<const uint32_t *>(bt.chan);
+        const uint32_t *cw = reinterpret_cast<const uint32_t *>(bt.chan_init ? bt.chan_init : bt.chan);
         uint32_t *hw = reinterpret_cast<uint32_t *>(hs);
         for (uint32_t k = tid; k < HW; k += NT) hw[k] = cw[k];
         __syncthreads();
@@ -226,12 +234,26 @@ __device__ __forceinline__ void tree_block_body(uint32_t *base, const BT &bt, ui
             uint32_t d[8], q[4];
             for (int w = 0; w < 8; w++) d[w] = src[w];
             quad_channel_draw(*hs, *snap, qs, work, d, q, j, mo);
-            if (tid == 0)
+            if (tid == 0) {
                 for (int k = 0; k < 4; k++) bt.alpha[k] = q[k];
+                for (int k = 0; k < 4; k++) work[k] = q[k];  // for the powers below
+            }
         }
         __syncthreads();
         uint32_t *ow = reinterpret_cast<uint32_t *>(bt.chan);
         for (uint32_t k = tid; k < HW; k += NT) ow[k] = hw[k];
+        if (bt.powers) {
+            // powers of the challenge, square-and-multiply per power (as qm31_powers_kernel)
+            const qm31 g = {{work[0], work[1]}, {work[2], work[3]}};
+            for (uint32_t c = tid; c < bt.n_powers; c += NT) {
+                qm31 b = g, r = {{1u, 0u}, {0u, 0u}};
+                for (uint32_t e = c; e; e >>= 1) {
+                    if (e & 1u) r = qm_mul(r, b);
+                    b = qm_mul(b, b);
+                }
+                stg4(bt.powers + 4 * c, make_uint4(r.a.c0, r.a.c1, r.b.c0, r.b.c1));
+            }
+        }
     }
 }
 
