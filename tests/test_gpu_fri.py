@@ -224,3 +224,19 @@ def test_fri_2p22_layer_trees_match_per_level_trees(zctx, torch_dev):
         top = ((2 << lg) - 1) * 8
         root = d_nodes[top - 8:top].cpu().numpy().view(np.uint32).tobytes()
         assert root == roots[i], i
+
+
+def test_fri_commit_dev_ignores_flags_it_does_not_take(zctx, torch_dev):
+    """Only ZS_FLAG_NO_GRAPH applies to zs_fri_commit_dev; other bits a caller passes
+    (here the bit the commit job uses internally to say layer 0's leaves are already
+    hashed) leave the commitment unchanged."""
+    from conftest import to_dev
+    dlog, beta, rem_log = 12, 2, 5
+    rng = np.random.default_rng(77)
+    cw = rng.integers(0, P, (1 << dlog, 4), dtype=np.uint32)
+    alphas = rng.integers(0, P, (dlog - rem_log, 4), dtype=np.uint32)
+    d_cw = to_dev(cw.reshape(-1), torch_dev)
+    oroots, _, orem, _ = O.fri_commit(cw, beta, rem_log, alphas=alphas, want_layers=False)
+    for flags in (0, 1 << 30, (1 << 30) | Z.ZS_FLAG_NO_GRAPH, 0xFFFFFFF0):
+        roots, rem, _ = zctx.fri_commit_dev(d_cw.data_ptr(), dlog, beta, rem_log, alphas=alphas, flags=flags)
+        assert roots == oroots and np.array_equal(rem, orem), hex(flags)

@@ -919,6 +919,7 @@ int32_t zs_fri_commit_dev(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t dl
                           uint32_t remainder_log, const uint32_t *alphas, zs_channel *chan, uint32_t flags,
                           uint8_t *roots_out, uint32_t *remainder_out, zs_fri_result **out_result) {
     if (!h || (!alphas && !chan)) return ZS_ERR_INPUT;
+    flags &= ZS_FLAG_NO_GRAPH;  // the only flag of this entry point (the others are internal)
     if (alphas)
         return fri_commit_impl(h, d_codeword, dlog, log_blowup, remainder_log, alphas, nullptr, flags, roots_out,
                                remainder_out, out_result);
