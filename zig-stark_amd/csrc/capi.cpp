@@ -835,10 +835,29 @@ int32_t zs::fri_commit_impl(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t 
             if (i_tail < L) ZS_TRY(tail());
             return launch_trees_multi(ctx, trees, offs.data(), logs.data(), L);
         }
+        // A layer's fold into the next one goes into the next tree's first launch when
+        // that is a workgroup-subtree launch (fewer than 2^kSubtreeMinLog leaves) and the
+        // next tree is built here (not in the tail): one launch fewer per layer, and the
+        // leaves are hashed where the tree reads them.
+        bool folded = false;  // layer i's values and leaves come from its tree's first launch
         for (uint32_t i = 0; i < i_tail; i++) {
             uint32_t li = dlog - i;
+            FoldLeaves fl{};
+            if (folded) {
+                fl.cur = layer(i - 1);
+                fl.half = size_t(1) << li;
+                fl.layer = i - 1;
+                fl.tables = fri_tab;
+                fl.alpha = alpha_dev + 4 * (i - 1);
+                fl.inv_off = qm_inv(fri_offset_pow(i - 1));
+                fl.next = layer(i);
+            }
             // the launch that computes this layer's root also draws its challenge on the device
-            ZS_TRY(launch_tree_levels(ctx, trees + r->tree_off[i], li, 0, d_chan, alpha_dev + 4 * i));
+            uint64_t off = r->tree_off[i];
+            ZS_TRY(launch_trees_multi(ctx, trees, &off, &li, 1, d_chan, alpha_dev + 4 * i, nullptr, nullptr, 0,
+                                      folded ? &fl : nullptr));
+            folded = i + 1 < i_tail && li - 1 < kSubtreeMinLog;
+            if (folded) continue;
             uint32_t *next_leaves = (i + 1 < L) ? trees + r->tree_off[i + 1] : nullptr;
             ZS_TRY(launch_fri_fold(ctx, layer(i), size_t(1) << li, dlog, i, alpha_dev + 4 * i, layer(i + 1),
                                    next_leaves));

@@ -12,6 +12,7 @@
 // reference's value for value.  The kernel also writes the hashQM31 leaf of
 // each new value, fusing the next layer's leaf pass into the fold.
 #include "fri_dev.h"
+#include "merkle_dev.h"
 
 namespace zs {
 
@@ -139,6 +140,35 @@ int32_t launch_fri_fold2(Ctx *ctx, const uint32_t *d_cur, uint64_t n, uint32_t d
     ZS_LAUNCH(ctx, "fri_fold2_kernel", fri_fold2_kernel, dim3((uint32_t)((quarter + 255) / 256)), dim3(256), 0,
               ctx->stream, d_cur, quarter, layer, tables, inv0, inv1, d_alpha, d_leaves0, d_next1, d_leaves1, d_next2,
               d_leaves2);
+    ZS_HIP(ctx, hipGetLastError());
+    return ZS_OK;
+}
+
+// Channel mode, a layer of fewer than 2^kSubtreeMinLog values: its fold and leaves go
+// into the first launch of its tree, each workgroup folding the 2^K values under its
+// subtree (no fold launch, and the leaves are hashed from LDS, not read back).
+struct FoldLeafSource {
+    static constexpr bool on = true;
+    FoldLeaves f;
+    __device__ void operator()(uint64_t j, uint32_t (&d)[8]) const {
+        const qm31 a_off = qm_half(qm_mul(ldg_qm(f.alpha), f.inv_off));
+        const qm31 v = fold_value(f.cur, f.half, j, f.layer, f.tables, a_off);
+        stg_qm(f.next + 4 * j, v);
+        const uint32_t w[4] = {v.a.c0, v.a.c1, v.b.c0, v.b.c1};
+        b3_hash_qm31(w, d);
+    }
+};
+
+__global__ __launch_bounds__(BLOCK_NT) void tree_block_fold_kernel(uint32_t *base, const BlockTrees bt,
+                                                                   const FoldLeafSource src) {
+    tree_block_body<BLOCK_NT>(base, bt, blockIdx.x, threadIdx.x, src);
+}
+
+int32_t launch_tree_block_fold(Ctx *ctx, uint32_t *base, const BlockTrees &bt, uint32_t wgs, size_t lds_bytes,
+                               const FoldLeaves &fold) {
+    if (bt.n != 1 || (fold.half >> bt.r[0]) != 1) return ZS_ERR_INPUT;
+    ZS_LAUNCH(ctx, "tree_block_fold_kernel", tree_block_fold_kernel, dim3(wgs), dim3(BLOCK_NT), lds_bytes,
+              ctx->stream, base, bt, FoldLeafSource{fold});
     ZS_HIP(ctx, hipGetLastError());
     return ZS_OK;
 }

@@ -242,10 +242,29 @@ int32_t launch_tree_levels(Ctx *ctx, uint32_t *d_nodes, uint32_t log_leaves, uin
                            B3Hasher *draw_chan = nullptr, uint32_t *draw_alpha = nullptr,
                            const B3Hasher *chan_init = nullptr, uint32_t *powers = nullptr, uint32_t n_powers = 0);
 int32_t launch_qm31_leaves(Ctx *ctx, const uint32_t *d_vals, uint64_t n, uint32_t *d_nodes);
-// node levels of n_trees independent trees with written leaves, tree k at base + off_words[k]
+// Tree levels with at least 2^kSubtreeMinLog children go 3 at a time, one lane per
+// 8-child subtree; narrower ones in workgroup subtrees of up to 8 levels.
+constexpr uint32_t kSubtreeMinLog = 18;
+// An FRI fold whose outputs are the leaves of a tree (fri.zig:211-219): leaf j is
+// hashQM31(next[j]), next[j] = fold of cur[j] and cur[j + half] with the challenge at
+// alpha; the tree's first workgroup-subtree launch computes them (and writes next and
+// the leaves) instead of reading written leaves.
+struct FoldLeaves {
+    const uint32_t *cur;
+    uint64_t half;
+    uint32_t layer;
+    const uint32_t *tables;  // get_fri_tables(dlog)
+    const uint32_t *alpha;
+    qm31 inv_off;            // (FRI_OFFSET^(2^layer))^-1
+    uint32_t *next;
+};
+// node levels of n_trees independent trees with written leaves, tree k at base + off_words[k];
+// with fold (one tree of fewer than 2^kSubtreeMinLog leaves), the leaves are not written
+// yet: the first launch computes them from the fold
 int32_t launch_trees_multi(Ctx *ctx, uint32_t *base, const uint64_t *off_words, const uint32_t *log_leaves,
                            uint32_t n_trees, B3Hasher *draw_chan = nullptr, uint32_t *draw_alpha = nullptr,
-                           const B3Hasher *chan_init = nullptr, uint32_t *powers = nullptr, uint32_t n_powers = 0);
+                           const B3Hasher *chan_init = nullptr, uint32_t *powers = nullptr, uint32_t n_powers = 0,
+                           const FoldLeaves *fold = nullptr);
 // word offsets of up to 28 tree roots (a gather's argument)
 struct RootOffsets {
     uint64_t off[28];
@@ -268,6 +287,10 @@ int32_t get_fri_tables(Ctx *ctx, uint32_t dlog, const uint32_t **out);
 // hashQM31 leaves of `next` into d_next_leaves when non-null.
 int32_t launch_fri_fold(Ctx *ctx, const uint32_t *d_cur, uint64_t n, uint32_t dlog, uint32_t layer,
                         const uint32_t *d_alpha, uint32_t *d_next, uint32_t *d_next_leaves);
+// one workgroup-subtree launch of a single tree (bt) whose leaves come from fold (fri.hip)
+struct BlockTrees;
+int32_t launch_tree_block_fold(Ctx *ctx, uint32_t *base, const BlockTrees &bt, uint32_t wgs, size_t lds_bytes,
+                               const FoldLeaves &fold);
 // folds of layers `layer` and layer + 1 in one launch (challenges at d_alpha, d_alpha + 4)
 // (d_leaves0, when non-null, gets the hashQM31 leaves of layer `layer` itself)
 int32_t launch_fri_fold2(Ctx *ctx, const uint32_t *d_cur, uint64_t n, uint32_t dlog, uint32_t layer,

@@ -71,10 +71,9 @@ __global__ __launch_bounds__(BLOCK_NT) void tree_block_kernel(uint32_t *base, co
     tree_block_body<BLOCK_NT>(base, bt, blockIdx.x, threadIdx.x);
 }
 
-// levels with at least 2^18 children are hashed 3 at a time by one lane per
-// 8-child subtree (throughput); below that, by workgroup subtrees whose quads
+// levels with at least 2^kSubtreeMinLog children are hashed 3 at a time by one lane
+// per 8-child subtree (throughput); below that, by workgroup subtrees whose quads
 // share each hash (a lane's chain of 14 compressions would set the latency)
-constexpr uint32_t kSubtreeMinLog = 18;
 // once a tree that wide makes the 3-level pass worth a launch, narrower trees of the
 // same launch join it down to 2^11 children: their lanes add little to a pass whose
 // time one lane's 7 compressions set, and it takes 3 levels (7/8 of the workgroups)
@@ -89,9 +88,11 @@ constexpr uint32_t kSubtreeJoinLog = 11;
 // the rest.
 int32_t launch_trees_multi(Ctx *ctx, uint32_t *base, const uint64_t *off_words, const uint32_t *log_leaves,
                            uint32_t n_trees, B3Hasher *draw_chan, uint32_t *draw_alpha, const B3Hasher *chan_init,
-                           uint32_t *powers, uint32_t n_powers) {
+                           uint32_t *powers, uint32_t n_powers, const FoldLeaves *fold) {
     // a fused draw needs the root computed by one workgroup: one tree with at least one level
     if (draw_chan && (n_trees != 1 || log_leaves[0] == 0)) return ZS_ERR_INPUT;
+    // computed leaves go into the first launch: one tree whose first launch is a workgroup-subtree one
+    if (fold && (n_trees != 1 || log_leaves[0] == 0 || log_leaves[0] >= kSubtreeMinLog)) return ZS_ERR_INPUT;
     std::vector<uint32_t> level(n_trees, 0);
     for (uint32_t start = 0; start < n_trees; start += MultiTrees::MAX) {
         const uint32_t cnt = std::min<uint32_t>(MultiTrees::MAX, n_trees - start);
@@ -142,9 +143,14 @@ int32_t launch_trees_multi(Ctx *ctx, uint32_t *base, const uint64_t *off_words, 
             bt.chan_init = last ? chan_init : nullptr;
             bt.powers = last ? powers : nullptr;
             bt.n_powers = last ? n_powers : 0u;
-            ZS_LAUNCH(ctx, "tree_block_kernel", tree_block_kernel, dim3(wgs), dim3(BLOCK_NT),
-                      (block_lds_words(kmax) + (last ? kDrawLdsWords : 0)) * 4, ctx->stream, base, bt);
-            ZS_HIP(ctx, hipGetLastError());
+            if (fold && level[in[0]] == 0) {
+                ZS_TRY(launch_tree_block_fold(ctx, base, bt, wgs, (block_leaf_lds_words(kmax) + (8u << kmax)) * 4,
+                                              *fold));
+            } else {
+                ZS_LAUNCH(ctx, "tree_block_kernel", tree_block_kernel, dim3(wgs), dim3(BLOCK_NT),
+                          (block_lds_words(kmax) + (last ? kDrawLdsWords : 0)) * 4, ctx->stream, base, bt);
+                ZS_HIP(ctx, hipGetLastError());
+            }
             for (uint32_t i = 0; i < bt.n; i++) level[in[i]] += bt.K[i];
         }
     }

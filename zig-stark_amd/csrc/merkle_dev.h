@@ -166,10 +166,22 @@ struct OneTree {
     uint32_t n_powers;
 };
 
+// The children of a workgroup subtree are read from their level in HBM, or, with a
+// leaf source, computed: LEAVES::on, and leaves(j, d) sets d to child j's digest.
+struct ReadLeaves {
+    static constexpr bool on = false;
+    __device__ void operator()(uint64_t, uint32_t (&)[8]) const {}
+};
+// LDS words after the tree's and the draw's where computed children are kept
+ZS_HD size_t block_leaf_lds_words(uint32_t K, uint32_t NT = BLOCK_NT) {
+    return block_lds_words(K, NT) + kDrawLdsWords;
+}
+
 // workgroup s of tree t (levels from its child level, r child levels left, K of them here);
 // BT is BlockTrees or OneTree
-template <uint32_t NT, class BT>
-__device__ __forceinline__ void tree_block_body(uint32_t *base, const BT &bt, uint32_t b, uint32_t tid) {
+template <uint32_t NT, class BT, class LEAVES = ReadLeaves>
+__device__ __forceinline__ void tree_block_body(uint32_t *base, const BT &bt, uint32_t b, uint32_t tid,
+                                                const LEAVES &leaves = LEAVES{}) {
     extern __shared__ uint32_t sm[];
     uint32_t t = 0;
     while (b >= bt.wg_end[t]) t++;
@@ -189,6 +201,18 @@ __device__ __forceinline__ void tree_block_body(uint32_t *base, const BT &bt, ui
     uint64_t lvl = bt.child[t];  // word offset of the current child level
     uint64_t width = uint64_t(1) << r;
     const uint32_t *src = base + lvl + (uint64_t)s * (8u << K);
+    if constexpr (LEAVES::on) {
+        // this subtree's 2^K children: computed, written to their level, kept in LDS
+        uint32_t *kids = sm + block_leaf_lds_words(K, NT);
+        uint32_t *kids_g = base + lvl + (uint64_t)s * (8u << K);
+        for (uint32_t p = tid; p < (1u << K); p += NT) {
+            uint32_t d[8];
+            leaves(((uint64_t)s << K) + p, d);
+            st_digest_g(kids_g + 8 * p, d);
+            st_digest(kids + 8 * p, d);
+        }
+        src = kids;
+    }
     __syncthreads();
     for (uint32_t l = 0; l < K; l++) {
         const uint32_t P = 1u << (K - 1 - l);
