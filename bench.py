@@ -101,17 +101,17 @@ def fri_leg(Z, ctx, torch, dev):
     def call(flags=0):
         return ctx.fri_commit_dev(d_cw.data_ptr(), dlog, beta, rem_log, alphas=alphas, flags=flags)
 
-    r_graph = call()
-    r_eager = call(Z.ZS_FLAG_NO_GRAPH)
+    r_graph = call(Z.ZS_FLAG_GRAPH)
+    r_eager = call()
     same = r_graph[0] == r_eager[0] and bool(np.array_equal(r_graph[1], r_eager[1]))
     reps = 10
     t0 = time.perf_counter()
     for _ in range(reps):
-        call()
+        call(Z.ZS_FLAG_GRAPH)
     graph_ms = (time.perf_counter() - t0) * 1e3 / reps
     t0 = time.perf_counter()
     for _ in range(reps):
-        call(Z.ZS_FLAG_NO_GRAPH)
+        call()
     eager_ms = (time.perf_counter() - t0) * 1e3 / reps
     ctx.profile(True)
     call()

@@ -43,8 +43,9 @@ extern "C" {
 
 /* flags for zs_m31_lde_commit_dev / zs_fri_commit_dev */
 #define ZS_FLAG_PLAIN 1u /* reference-shaped per-column, per-layer launches (comparison path) */
-#define ZS_FLAG_NO_GRAPH 2u /* FRI with given challenges: launch each kernel instead of replaying a graph */
+#define ZS_FLAG_NO_GRAPH 2u /* FRI commit: launch each kernel (the default; kept for callers that pass it) */
 #define ZS_FLAG_PERSISTENT 4u /* batch: LDE, row leaves and node levels of all traces in one persistent launch */
+#define ZS_FLAG_GRAPH 8u /* FRI commit (and a commit job's FRI): capture the launches once per buffer set as a HIP graph and replay it */
 
 typedef struct zs_hip_ctx zs_hip_ctx;
 typedef struct zs_tree zs_tree;
@@ -186,10 +187,11 @@ size_t zs_channel_sample_index(zs_channel *ch, size_t n);                     /*
  *           NULL the remainder is absorbed into `chan` afterwards.
  *   roots_out: L * 32 bytes; remainder_out: 4 * 2^(remainder_log - log_blowup) words.
  *   out_result: optional; keeps every layer and tree on the device for queries.
- * d_codeword is a device pointer; it is not modified.  With alphas given and no
- * out_result, the leaf, node-level and fold kernels of every layer are captured
- * once per buffer set into a HIP graph and replayed on later calls
- * (ZS_FLAG_NO_GRAPH launches them one by one instead). */
+ * d_codeword is a device pointer; it is not modified.  The leaf, node-level and
+ * fold kernels of every layer are launched one by one; with ZS_FLAG_GRAPH (and no
+ * out_result) they are captured once per buffer set into a HIP graph and replayed
+ * on later calls (on ROCm 7 the replay is 0.5-3 % slower than the launches it
+ * replaces, so it is opt-in).  ZS_FLAG_NO_GRAPH overrides ZS_FLAG_GRAPH. */
 int32_t zs_fri_commit_dev(zs_hip_ctx *ctx, const uint32_t *d_codeword, uint32_t dlog, uint32_t log_blowup,
                           uint32_t remainder_log, const uint32_t *alphas, zs_channel *chan, uint32_t flags,
                           uint8_t *roots_out, uint32_t *remainder_out, zs_fri_result **out_result);

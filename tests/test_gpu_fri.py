@@ -134,9 +134,9 @@ def test_fri_graph_replays_match_eager_and_oracle(zctx, torch_dev, dlog, beta, r
         cw = rng.integers(0, P, (1 << dlog, 4), dtype=np.uint32)
         d_cw.copy_(to_dev(cw.reshape(-1), torch_dev))
         alphas = rng.integers(0, P, (L, 4), dtype=np.uint32)
-        g_roots, g_rem, _ = zctx.fri_commit_dev(d_cw.data_ptr(), dlog, beta, rem_log, alphas=alphas)
-        e_roots, e_rem, _ = zctx.fri_commit_dev(d_cw.data_ptr(), dlog, beta, rem_log, alphas=alphas,
-                                                flags=Z.ZS_FLAG_NO_GRAPH)
+        g_roots, g_rem, _ = zctx.fri_commit_dev(d_cw.data_ptr(), dlog, beta, rem_log, alphas=alphas,
+                                                flags=Z.ZS_FLAG_GRAPH)
+        e_roots, e_rem, _ = zctx.fri_commit_dev(d_cw.data_ptr(), dlog, beta, rem_log, alphas=alphas)
         assert g_roots == e_roots and np.array_equal(g_rem, e_rem), rep
         if dlog <= 15:
             oroots, _, orem, _ = O.fri_commit(cw, beta, rem_log, alphas=alphas, want_layers=False)
@@ -156,7 +156,7 @@ def test_fri_device_channel_graph_replays(zctx, torch_dev, dlog, beta, rem_log):
     for rep in range(3):
         cw = rng.integers(0, P, (1 << dlog, 4), dtype=np.uint32)
         d_cw.copy_(to_dev(cw.reshape(-1), torch_dev))
-        for flags in (0, Z.ZS_FLAG_NO_GRAPH):
+        for flags in (Z.ZS_FLAG_GRAPH, 0):
             ch = Z.Channel(b"zig-stark:fri:dev")
             roots, rem, _ = zctx.fri_commit_dev(d_cw.data_ptr(), dlog, beta, rem_log, channel=ch, flags=flags)
             och = O.Channel(b"zig-stark:fri:dev")
@@ -175,7 +175,7 @@ def test_fri_device_channel_2p20_matches_oracle(zctx, torch_dev):
     d_cw = to_dev(cw.reshape(-1), torch_dev)
     och = O.Channel(b"zig-stark:fri:2p20")
     oroots, _, orem, _ = O.fri_commit(cw, beta, rem_log, channel=och, want_layers=False)
-    for flags in (0, Z.ZS_FLAG_NO_GRAPH):
+    for flags in (Z.ZS_FLAG_GRAPH, 0):
         ch = Z.Channel(b"zig-stark:fri:2p20")
         roots, rem, _ = zctx.fri_commit_dev(d_cw.data_ptr(), dlog, beta, rem_log, channel=ch, flags=flags)
         assert roots == oroots and np.array_equal(rem, orem), flags
@@ -227,7 +227,7 @@ def test_fri_2p22_layer_trees_match_per_level_trees(zctx, torch_dev):
 
 
 def test_fri_commit_dev_ignores_flags_it_does_not_take(zctx, torch_dev):
-    """Only ZS_FLAG_NO_GRAPH applies to zs_fri_commit_dev; other bits a caller passes
+    """Only ZS_FLAG_GRAPH and ZS_FLAG_NO_GRAPH apply to zs_fri_commit_dev; other bits a caller passes
     (here the bit the commit job uses internally to say layer 0's leaves are already
     hashed) leave the commitment unchanged."""
     from conftest import to_dev
@@ -237,6 +237,6 @@ def test_fri_commit_dev_ignores_flags_it_does_not_take(zctx, torch_dev):
     alphas = rng.integers(0, P, (dlog - rem_log, 4), dtype=np.uint32)
     d_cw = to_dev(cw.reshape(-1), torch_dev)
     oroots, _, orem, _ = O.fri_commit(cw, beta, rem_log, alphas=alphas, want_layers=False)
-    for flags in (0, 1 << 30, (1 << 30) | Z.ZS_FLAG_NO_GRAPH, 0xFFFFFFF0):
+    for flags in (0, 1 << 30, (1 << 30) | Z.ZS_FLAG_GRAPH, (1 << 30) | Z.ZS_FLAG_NO_GRAPH, 0xFFFFFFF0):
         roots, rem, _ = zctx.fri_commit_dev(d_cw.data_ptr(), dlog, beta, rem_log, alphas=alphas, flags=flags)
         assert roots == oroots and np.array_equal(rem, orem), hex(flags)

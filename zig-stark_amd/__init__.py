@@ -28,6 +28,7 @@ ZS_OK, ZS_ERR_GENERIC, ZS_ERR_INPUT, ZS_ERR_OOM, ZS_ERR_PROTOCOL, ZS_ERR_DEVICE 
 ZS_FLAG_PLAIN = 1
 ZS_FLAG_NO_GRAPH = 2
 ZS_FLAG_PERSISTENT = 4
+ZS_FLAG_GRAPH = 8
 
 _lib = None
 

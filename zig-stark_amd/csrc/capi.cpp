@@ -493,7 +493,8 @@ int32_t zs_autotune_lde(zs_hip_ctx *h, uint32_t n_cols, uint32_t log_n, uint32_t
     // the candidate being timed is installed as this shape's pick
     auto set_tuned = [&](const LdeTiles &t) { tuned_set(ctx->device, key, t, 0); };
     for (uint32_t el = 11; el <= 14 && rc == ZS_OK; el++) {
-        for (uint32_t cpw = 1; cpw <= 8 && rc == ZS_OK; cpw *= 2) {
+        for (uint32_t cpw = 1; cpw <= 32 && rc == ZS_OK; cpw *= 2) {
+            if (cpw > 1 && cpw / 2 >= n_cols) continue;  // the same launch as cpw / 2
             TileConfig cand;
             cand.elems_log = el;
             cand.cols_per_wg = cpw;
@@ -872,7 +873,7 @@ int32_t zs::fri_commit_impl(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t 
         for (uint32_t i = 0; i < L; i++) root_off[i] = r->tree_off[i] + tree_words(dlog - i) - 8;
         return launch_fri_finish(ctx, trees, root_off.data(), L, d_prefix, roots_dev, layer(L), rem_mat, rc, rem_dev);
     };
-    const bool graph = !out_result && !ctx->profiling && !(flags & ZS_FLAG_NO_GRAPH);
+    const bool graph = !out_result && !ctx->profiling && (flags & ZS_FLAG_GRAPH) && !(flags & ZS_FLAG_NO_GRAPH);
     if (!graph) {
         ZS_TRY(enqueue_all());
     } else {
@@ -938,7 +939,7 @@ int32_t zs_fri_commit_dev(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t dl
                           uint32_t remainder_log, const uint32_t *alphas, zs_channel *chan, uint32_t flags,
                           uint8_t *roots_out, uint32_t *remainder_out, zs_fri_result **out_result) {
     if (!h || (!alphas && !chan)) return ZS_ERR_INPUT;
-    flags &= ZS_FLAG_NO_GRAPH;  // the only flag of this entry point (the others are internal)
+    flags &= ZS_FLAG_NO_GRAPH | ZS_FLAG_GRAPH;  // the only flags of this entry point (the others are internal)
     if (alphas)
         return fri_commit_impl(h, d_codeword, dlog, log_blowup, remainder_log, alphas, nullptr, flags, roots_out,
                                remainder_out, out_result);

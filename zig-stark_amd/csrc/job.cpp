@@ -131,7 +131,7 @@ int32_t commit_job(zs_hip_ctx *h, const uint32_t *d_cols, uint32_t n_cols, uint3
     }
     // the trace root reaches the stage ahead of the FRI roots, in the commit phase's one copy
     ZS_TRY(fri_commit_impl(h, ctx->scratch[8].as<uint32_t>(), LT, log_blowup, remainder_log, nullptr, d_chan,
-                           (flags & ZS_FLAG_NO_GRAPH) | kFriLeaves0Given, nullptr, nullptr, nullptr, stage, d_root));
+                           (flags & (ZS_FLAG_NO_GRAPH | ZS_FLAG_GRAPH)) | kFriLeaves0Given, nullptr, nullptr, nullptr, stage, d_root));
     if (!own_stage) return ZS_OK;
     ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     const uint32_t L = LT - remainder_log;
