@@ -709,12 +709,36 @@ void zs::interpolate_host(const std::vector<qm31> &xs, const std::vector<qm31> &
     coeffs = acc;
 }
 
+zs::DeferLayout zs::defer_layout(uint32_t dlog, uint32_t remainder_log) {
+    DeferLayout d{};
+    const uint32_t L = dlog - remainder_log;
+    const uint32_t i_tail = std::min(L, dlog > FRI_TAIL_LOG ? dlog - FRI_TAIL_LOG : 0u);
+    auto up = [](size_t w) { return (w + 63) & ~size_t(63); };  // 256-byte aligned parts
+    d.alphas = up((sizeof(B3Hasher) + 3) / 4);
+    d.layers = d.alphas + up(4 * size_t(L));
+    size_t o = 0;
+    for (uint32_t i = i_tail; i <= L && i - i_tail < 12; i++) {
+        d.lay_off[i - i_tail] = o;
+        o += size_t(4) << (dlog - i);
+    }
+    d.trees = d.layers + up(o);
+    o = 0;
+    for (uint32_t i = i_tail; i < L && i - i_tail < 11; i++) {
+        d.tree_off[i - i_tail] = o;
+        o += tree_words(dlog - i);
+    }
+    d.total = d.trees + up(o);
+    return d;
+}
+
 int32_t zs::fri_commit_impl(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t dlog, uint32_t log_blowup,
                             uint32_t remainder_log, const uint32_t *alphas, B3Hasher *d_chan, uint32_t flags,
                             uint8_t *roots_out, uint32_t *remainder_out, zs_fri_result **out_result,
-                            uint8_t *stage, const uint32_t *d_prefix) {
-    if (!h || !d_codeword || (!stage && (!roots_out || !remainder_out)) || (stage && out_result) ||
-        (d_prefix && !stage))
+                            uint8_t *stage, const uint32_t *d_prefix, FriDefer *defer) {
+    if (defer && (!d_chan || alphas || stage || out_result || !defer->region || !defer->results))
+        return ZS_ERR_INPUT;
+    if (!h || !d_codeword || (!stage && !defer && (!roots_out || !remainder_out)) || (stage && out_result) ||
+        (d_prefix && !stage && !defer))
         return ZS_ERR_INPUT;
     if (dlog > 28 || remainder_log > dlog || remainder_log < log_blowup || dlog - remainder_log < 1)
         return ZS_ERR_INPUT;
@@ -765,17 +789,37 @@ int32_t zs::fri_commit_impl(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t 
         trees = ctx->scratch[5].as<uint32_t>();
         lay0 = d_codeword;
     }
+    // layers of at most 2^FRI_TAIL_LOG values go to one single-workgroup launch
+    const uint32_t i_tail = std::min(L, dlog > FRI_TAIL_LOG ? dlog - FRI_TAIL_LOG : 0u);
+    // A deferred tail reads and writes only the caller's region: layer i_tail onwards,
+    // the trees from layer i_tail's on, the challenges and the channel.  (With no chain
+    // layer before the tail, or no tail, nothing is deferred: the tail, if any, runs here.)
+    const bool deferring = defer && i_tail > 0 && i_tail < L;
+    DeferLayout dl{};
+    if (defer) dl = defer_layout(dlog, remainder_log);
     auto layer = [&](uint32_t i) -> uint32_t * {
+        if (deferring && i >= i_tail) return defer->region + dl.layers + dl.lay_off[i - i_tail];
         return i == 0 ? const_cast<uint32_t *>(lay0) : layers + r->layer_off[i];
     };
-    // slot 6: the challenges (16 B per layer), 32 B for a prefix digest, then the layer roots
-    // gathered for one copy (32 B per layer) and right after them the remainder
-    // coefficients (16 B each)
+    // leaves of layer i, where its tree starts
+    auto leaves_of = [&](uint32_t i) -> uint32_t * {
+        if (deferring && i >= i_tail) return defer->region + dl.trees + dl.tree_off[i - i_tail];
+        return trees + r->tree_off[i];
+    };
+    // slot 6 (or the deferred region and results): the challenges (16 B per layer), 32 B
+    // for a prefix digest, then the layer roots gathered for one copy (32 B per layer) and
+    // right after them the remainder coefficients (16 B each)
     const uint32_t rc = 1u << (remainder_log - log_blowup);
-    ZS_TRY(ctx->ensure_scratch(6, size_t(L) * 48 + 32 + size_t(rc) * 16));
-    alpha_dev = ctx->scratch[6].as<uint32_t>();
-    uint32_t *roots_dev = alpha_dev + 4 * size_t(L) + 8;
-    uint32_t *rem_dev = roots_dev + 8 * size_t(L);
+    uint32_t *roots_dev, *rem_dev;
+    if (defer) {
+        alpha_dev = defer->region + dl.alphas;
+        roots_dev = defer->results + 8;
+    } else {
+        ZS_TRY(ctx->ensure_scratch(6, size_t(L) * 48 + 32 + size_t(rc) * 16));
+        alpha_dev = ctx->scratch[6].as<uint32_t>();
+        roots_dev = alpha_dev + 4 * size_t(L) + 8;
+    }
+    rem_dev = roots_dev + 8 * size_t(L);
     const uint32_t *fri_tab, *rem_mat;
     ZS_TRY(get_fri_tables(ctx, dlog, &fri_tab));  // cached before any capture
     ZS_TRY(get_remainder_matrix(ctx, L, remainder_log, rc, &rem_mat));
@@ -784,9 +828,7 @@ int32_t zs::fri_commit_impl(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t 
     // Leaves of layer 0, then per layer: node levels, (challenge), fold + next
     // layer's leaves.  With the challenges given, no fold waits for a root: the
     // folds run first and the trees of all layers are built together after them.
-    // layers of at most 2^FRI_TAIL_LOG values go to one single-workgroup launch
-    const uint32_t i_tail = std::min(L, dlog > FRI_TAIL_LOG ? dlog - FRI_TAIL_LOG : 0u);
-    auto tail = [&]() -> int32_t {
+    auto tail_desc = [&]() -> FriTail {
         FriTail t{};
         t.in = layer(i_tail);
         t.layers = layers;
@@ -802,7 +844,27 @@ int32_t zs::fri_commit_impl(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t 
             t.tree_off[i] = r->tree_off[i];
             t.inv_off[i] = qm_inv(fri_offset_pow(i));
         }
-        return launch_fri_tail(ctx, t);
+        if (deferring) {
+            // everything the tail touches is in the region; it also writes its roots and
+            // the remainder (the finish's work for those layers)
+            t.layers = defer->region + dl.layers;
+            t.trees = defer->region + dl.trees;
+            for (uint32_t i = i_tail + 1; i <= L; i++) t.lay_off[i] = dl.lay_off[i - i_tail];
+            for (uint32_t i = i_tail; i < L; i++) t.tree_off[i] = dl.tree_off[i - i_tail];
+            t.roots = roots_dev;
+            t.rem_mat = rem_mat;
+            t.rc = rc;
+            t.rem = rem_dev;
+        }
+        return t;
+    };
+    auto tail = [&]() -> int32_t {
+        if (deferring) {
+            defer->tail = tail_desc();
+            defer->pending = true;
+            return ZS_OK;
+        }
+        return launch_fri_tail(ctx, tail_desc());
     };
     // with the challenges given and two folds in the first launch, that launch hashes
     // layer 0's leaves too (it reads every layer-0 value)
@@ -859,7 +921,7 @@ int32_t zs::fri_commit_impl(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t 
                                       folded ? &fl : nullptr));
             folded = i + 1 < i_tail && li - 1 < kSubtreeMinLog;
             if (folded) continue;
-            uint32_t *next_leaves = (i + 1 < L) ? trees + r->tree_off[i + 1] : nullptr;
+            uint32_t *next_leaves = (i + 1 < L) ? leaves_of(i + 1) : nullptr;
             ZS_TRY(launch_fri_fold(ctx, layer(i), size_t(1) << li, dlog, i, alpha_dev + 4 * i, layer(i + 1),
                                    next_leaves));
         }
@@ -869,10 +931,17 @@ int32_t zs::fri_commit_impl(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t 
     // the interpolant of the first rc points of layer L), so the host reads both with one copy
     auto enqueue_all = [&]() -> int32_t {
         ZS_TRY(enqueue());
-        std::vector<uint64_t> root_off(L);
-        for (uint32_t i = 0; i < L; i++) root_off[i] = r->tree_off[i] + tree_words(dlog - i) - 8;
-        return launch_fri_finish(ctx, trees, root_off.data(), L, d_prefix, roots_dev, layer(L), rem_mat, rc, rem_dev);
+        // a deferred tail gathers its own roots and computes the remainder
+        const uint32_t n_roots = deferring ? i_tail : L;
+        std::vector<uint64_t> root_off(n_roots);
+        for (uint32_t i = 0; i < n_roots; i++) root_off[i] = r->tree_off[i] + tree_words(dlog - i) - 8;
+        return launch_fri_finish(ctx, trees, root_off.data(), n_roots, d_prefix, roots_dev, layer(L), rem_mat,
+                                 deferring ? 0 : rc, rem_dev);
     };
+    if (defer) {  // queued only: the caller runs the tail (if pending) and reads the results
+        defer->pending = false;
+        return enqueue_all();
+    }
     const bool graph = !out_result && !ctx->profiling && (flags & ZS_FLAG_GRAPH) && !(flags & ZS_FLAG_NO_GRAPH);
     if (!graph) {
         ZS_TRY(enqueue_all());

@@ -293,7 +293,7 @@ int32_t launch_channel_draw(Ctx *ctx, B3Hasher *d_state, const uint32_t *d_diges
 // ---------------------------------------------------------------------------
 constexpr uint32_t FRI_TAIL_NT = 512;
 
-__global__ __launch_bounds__(FRI_TAIL_NT) void fri_tail_kernel(FriTail t) {
+__device__ __forceinline__ void fri_tail_body(const FriTail &t) {
     extern __shared__ uint32_t sm[];
     constexpr uint32_t NV = 1u << FRI_TAIL_LOG;
     uint32_t *const val0 = sm, *const val1 = sm + 4 * NV;       // layer values (QM31 words), ping-pong
@@ -374,6 +374,7 @@ __global__ __launch_bounds__(FRI_TAIL_NT) void fri_tail_kernel(FriTail t) {
             if (tid < 64) {  // wave 0
                 uint32_t d[8], q[4];
                 for (int w = 0; w < 8; w++) d[w] = lv(src)[w];
+                if (t.roots && tid < 8) t.roots[8 * i + tid] = lv(src)[tid];
                 quad_channel_draw(*h, *snap, qs, work, d, q, j, mo);
                 if (tid == 0)
                     for (int k = 0; k < 4; k++) {
@@ -420,12 +421,35 @@ __global__ __launch_bounds__(FRI_TAIL_NT) void fri_tail_kernel(FriTail t) {
         const uint32_t *hw = reinterpret_cast<const uint32_t *>(h);
         for (uint32_t k = tid; k < HW; k += FRI_TAIL_NT) cw[k] = hw[k];
     }
+    // the remainder from layer L in LDS (fri_finish_kernel's map, fri.zig:228-233)
+    if (t.rem && tid < t.rc) {
+        const uint32_t *last = val(cur);
+        qm31 acc = {{0, 0}, {0, 0}};
+        for (uint32_t k = 0; k < t.rc; k++)
+            acc = qm_add(acc, qm_mul(ld_qm(t.rem_mat + 4 * (tid * t.rc + k)), ld_qm(last + 4 * k)));
+        st_qm(t.rem + 4 * tid, acc);
+    }
 }
 
+__global__ __launch_bounds__(FRI_TAIL_NT) void fri_tail_kernel(FriTail t) { fri_tail_body(t); }
+
+// workgroup b runs the tail of descriptor b
+__global__ __launch_bounds__(FRI_TAIL_NT) void fri_tail_multi_kernel(const FriTail *ts) { fri_tail_body(ts[blockIdx.x]); }
+
+constexpr size_t kFriTailLds = (size_t(20) << FRI_TAIL_LOG) * 4 + size_t(32) * (FRI_TAIL_NT / 4) * 4 +
+                               2 * sizeof(B3Hasher) + (24 + 16 + 4) * 4;
+
 int32_t launch_fri_tail(Ctx *ctx, const FriTail &t) {
-    const size_t lds = (size_t(20) << FRI_TAIL_LOG) * 4 + size_t(32) * (FRI_TAIL_NT / 4) * 4 + 2 * sizeof(B3Hasher) +
-                       (24 + 16 + 4) * 4;
-    ZS_LAUNCH(ctx, "fri_tail_kernel", fri_tail_kernel, dim3(1), dim3(FRI_TAIL_NT), lds, ctx->stream, t);
+    if (t.rem && (t.rc == 0 || t.rc > 64 || t.rc > (1u << (t.dlog - t.L)))) return ZS_ERR_INPUT;
+    ZS_LAUNCH(ctx, "fri_tail_kernel", fri_tail_kernel, dim3(1), dim3(FRI_TAIL_NT), kFriTailLds, ctx->stream, t);
+    ZS_HIP(ctx, hipGetLastError());
+    return ZS_OK;
+}
+
+int32_t launch_fri_tail_multi(Ctx *ctx, const FriTail *d_tails, uint32_t n) {
+    if (n == 0) return ZS_OK;
+    ZS_LAUNCH(ctx, "fri_tail_multi_kernel", fri_tail_multi_kernel, dim3(n), dim3(FRI_TAIL_NT), kFriTailLds,
+              ctx->stream, d_tails);
     ZS_HIP(ctx, hipGetLastError());
     return ZS_OK;
 }
@@ -496,7 +520,7 @@ __global__ __launch_bounds__(256) void fri_finish_kernel(const uint32_t *trees, 
 int32_t launch_fri_finish(Ctx *ctx, const uint32_t *d_trees, const uint64_t *root_off, uint32_t n_roots,
                           const uint32_t *d_prefix, uint32_t *d_roots, const uint32_t *d_last, const uint32_t *d_mat,
                           uint32_t rc, uint32_t *d_rem) {
-    if (n_roots > 28 || rc == 0 || rc > 64) return ZS_ERR_INPUT;
+    if (n_roots > 28 || rc > 64) return ZS_ERR_INPUT;  // rc == 0: the roots only
     RootOffsets ro{};
     for (uint32_t k = 0; k < n_roots; k++) ro.off[k] = root_off[k];
     ZS_LAUNCH(ctx, "fri_finish_kernel", fri_finish_kernel, dim3(1), dim3(256), 0, ctx->stream, d_trees, ro, n_roots,

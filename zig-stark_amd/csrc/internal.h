@@ -159,7 +159,8 @@ struct Ctx {
     LdeTiles lde_tiles = default_lde_tiles();  // used by the next LDE, per kind of pass
     int lde_role = -1;                         // the kind of LDE pass being launched (profiling)
     // grow-only scratch arenas
-    DevBuf scratch[12];  // 0 transforms, 1-2 LDE, 3 plain path, 4-6 FRI, 7-9 commit jobs, 10-11 persistent batch
+    DevBuf scratch[14];  // 0 transforms, 1-2 LDE, 3 plain path, 4-6 FRI, 7-9 commit jobs, 10-11 persistent batch,
+                         // 12-13 a batch's deferred FRI tails
     // FRI commit phases with challenges given up front, captured once per buffer set and
     // replayed (key: codeword, layers, trees, challenges, channel, dlog / layer count / remainder length)
     hipStream_t capture_stream = nullptr;
@@ -312,8 +313,39 @@ struct FriTail {
     uint32_t dlog, i0, L;
     uint64_t lay_off[29], tree_off[28];
     qm31 inv_off[28];  // (FRI_OFFSET^(2^i))^-1
+    // optional, channel mode: the root of layer i (i0 <= i < L) written to roots + 8 i
+    uint32_t *roots;
+    // optional: the remainder coefficients, rem_mat * (first rc values of layer L), to rem
+    const uint32_t *rem_mat;
+    uint32_t rc;
+    uint32_t *rem;
 };
 int32_t launch_fri_tail(Ctx *ctx, const FriTail &t);
+// the tails of several commit phases in one launch, one workgroup each (d_tails: n
+// descriptors in device memory)
+int32_t launch_fri_tail_multi(Ctx *ctx, const FriTail *d_tails, uint32_t n);
+// A channel-mode commit phase whose tail (the layers of at most 2^FRI_TAIL_LOG values,
+// the remainder) is left to a later launch that runs the tails of many commit phases
+// at once.  Everything the tail reads or writes lives in `region` (fri_defer_words
+// words: the channel, the challenges, the tail's layers and trees) and `results`
+// (the stage layout: 8 words of prefix, 8 per root, 4 per remainder coefficient), not
+// in the context's scratch, so the context may run other commit phases meanwhile.
+struct FriDefer {
+    uint32_t *region;   // in: device, fri_defer_words(dlog, remainder_log) words
+    uint32_t *results;  // in: device, 8 + 8 L + 4 rc words
+    bool pending;       // out: `tail` still has to run (false: results complete)
+    FriTail tail;       // out
+};
+// A deferred region (word offsets): the channel state at 0, the challenges (4 words
+// per layer), then layers i_tail..L and the trees of layers i_tail..L-1 (entry k is
+// layer i_tail + k; i_tail = min(L, dlog - FRI_TAIL_LOG), where fri_commit_impl splits
+// the tail off).
+struct DeferLayout {
+    size_t alphas, layers, trees, total;
+    size_t lay_off[12], tree_off[11];
+};
+DeferLayout defer_layout(uint32_t dlog, uint32_t remainder_log);
+inline size_t fri_defer_words(uint32_t dlog, uint32_t remainder_log) { return defer_layout(dlog, remainder_log).total; }
 int32_t launch_qm31_powers(Ctx *ctx, const uint32_t *d_x, uint32_t *d_out, uint32_t n);
 // UnivariateQM31.interpolate (univariate.zig:79-139) on the host, O(n^2) Lagrange
 void interpolate_host(const std::vector<qm31> &xs, const std::vector<qm31> &ys, std::vector<qm31> &coeffs);
@@ -325,12 +357,16 @@ int32_t get_remainder_matrix(Ctx *ctx, uint32_t L, uint32_t remainder_log, uint3
 // then the remainder words are copied there on the stream and the call returns
 // without synchronising (roots_out and remainder_out are then unused); otherwise
 // they are in roots_out / remainder_out when it returns.
-// d_prefix (with stage only): 8 device words copied to the stage ahead of the roots
-// (the commit job's trace root), in the same copy.
+// d_prefix (with stage or defer only): 8 device words copied to the stage ahead of the
+// roots (the commit job's trace root), in the same copy.
+// With `defer` (channel mode; d_chan must be the region's channel) the call queues
+// everything but the tail, writes the prefix and the roots it computed into
+// defer->results and returns without synchronising or copying; defer->tail is then
+// the tail to launch (launch_fri_tail_multi) before the results are read.
 int32_t fri_commit_impl(zs_hip_ctx *h, const uint32_t *d_codeword, uint32_t dlog, uint32_t log_blowup,
                         uint32_t remainder_log, const uint32_t *alphas, B3Hasher *d_chan, uint32_t flags,
                         uint8_t *roots_out, uint32_t *remainder_out, zs_fri_result **out_result,
-                        uint8_t *stage = nullptr, const uint32_t *d_prefix = nullptr);
+                        uint8_t *stage = nullptr, const uint32_t *d_prefix = nullptr, FriDefer *defer = nullptr);
 // every layer root (tree word offsets root_off) into d_roots, d_prefix's 8 words (if
 // given) at d_roots - 8, and the remainder (rc coefficients) into d_rem, in one launch
 int32_t launch_fri_finish(Ctx *ctx, const uint32_t *d_trees, const uint64_t *root_off, uint32_t n_roots,

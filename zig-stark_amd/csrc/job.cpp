@@ -77,11 +77,18 @@ constexpr uint32_t kCombineBeforeLdeCols = 96;
 // are unused: the caller synchronises the stream before reading the stage.
 // Without it the context's own pinned buffer is the stage, and the call returns
 // with the results in trace_root / fri_roots / remainder.
+//
+// With `defer` (not with the plain path) the channel lives in defer->region, the
+// results (trace root, FRI roots, remainder) are written to defer->results on the
+// device, and the FRI tail is left to the caller (defer->pending, defer->tail): the
+// call returns without synchronising or copying.
 int32_t commit_job(zs_hip_ctx *h, const uint32_t *d_cols, uint32_t n_cols, uint32_t log_n, uint32_t log_blowup,
                    uint32_t remainder_log, uint32_t *d_lde, uint8_t *d_nodes, uint8_t trace_root[32],
-                   uint8_t *fri_roots, uint32_t *remainder, uint32_t flags, uint8_t *stage = nullptr) {
-    if (!h || !d_cols || !d_lde || !d_nodes || (!stage && (!trace_root || !fri_roots || !remainder)))
+                   uint8_t *fri_roots, uint32_t *remainder, uint32_t flags, uint8_t *stage = nullptr,
+                   FriDefer *defer = nullptr) {
+    if (!h || !d_cols || !d_lde || !d_nodes || (!stage && !defer && (!trace_root || !fri_roots || !remainder)))
         return ZS_ERR_INPUT;
+    if (defer && (stage || (flags & ZS_FLAG_PLAIN))) return ZS_ERR_INPUT;
     const uint32_t LT = log_n + log_blowup;
     if (!job_shape_ok(n_cols, log_n, log_blowup, remainder_log)) return ZS_ERR_INPUT;
     Ctx *ctx = &h->c;
@@ -91,7 +98,7 @@ int32_t commit_job(zs_hip_ctx *h, const uint32_t *d_cols, uint32_t n_cols, uint3
     ZS_TRY(ctx->ensure_scratch(7, sizeof(B3Hasher)));
     ZS_TRY(ctx->ensure_scratch(8, N * 16));
     ZS_TRY(ctx->ensure_scratch(9, (size_t(n_cols) + 1) * 16));
-    B3Hasher *d_chan = ctx->scratch[7].as<B3Hasher>();
+    B3Hasher *d_chan = defer ? reinterpret_cast<B3Hasher *>(defer->region) : ctx->scratch[7].as<B3Hasher>();
     const B3Hasher *d_init;
     ZS_TRY(device_commit_channel(ctx, &d_init));
     uint32_t *d_gamma = ctx->scratch[9].as<uint32_t>(), *d_powers = d_gamma + 4;
@@ -124,6 +131,9 @@ int32_t commit_job(zs_hip_ctx *h, const uint32_t *d_cols, uint32_t n_cols, uint3
     } else {
         ZS_TRY(launch_combine(ctx, d_lde, N, n_cols, N, d_powers, ctx->scratch[8].as<uint32_t>(), leaves0));
     }
+    if (defer)
+        return fri_commit_impl(h, ctx->scratch[8].as<uint32_t>(), LT, log_blowup, remainder_log, nullptr, d_chan,
+                               kFriLeaves0Given, nullptr, nullptr, nullptr, nullptr, d_root, defer);
     const bool own_stage = stage == nullptr;
     if (own_stage) {
         ZS_TRY(ctx->ensure_pinned(kStageBytes));
@@ -283,11 +293,12 @@ int32_t zs_commit_batch_dev(zs_hip_ctx *h, zs_job *jobs, size_t n_jobs, uint32_t
             if (rc != ZS_OK) return rc;
         }
     }
-    auto run_one = [&](zs_hip_ctx *w, size_t i, uint8_t *stage) {
+    auto run_one = [&](zs_hip_ctx *w, size_t i, uint8_t *stage, FriDefer *defer = nullptr) {
         zs_job &j = jobs[i];
+        const bool host_out = !stage && !defer;
         return commit_job(w, j.d_cols, j.n_cols, j.log_n, log_blowup, remainder_log, j.d_lde, j.d_nodes,
-                          stage ? nullptr : j.trace_root, stage ? nullptr : j.fri_roots,
-                          stage ? nullptr : j.remainder, flags, stage);
+                          host_out ? j.trace_root : nullptr, host_out ? j.fri_roots : nullptr,
+                          host_out ? j.remainder : nullptr, flags, stage, defer);
     };
     std::vector<size_t> rest;
     for (size_t i = 0; i < n_jobs; i++)
@@ -318,6 +329,27 @@ int32_t zs_commit_batch_dev(zs_hip_ctx *h, zs_job *jobs, size_t n_jobs, uint32_t
         constexpr size_t kInFlight = 2;
         ZS_TRY(ctx->ensure_pinned(n_jobs * kStageBytes));
         uint8_t *stage = static_cast<uint8_t *>(ctx->pinned);
+        // Each job's FRI tail (its layers of at most 2^10 values and the remainder, one
+        // workgroup's latency-bound chain) is deferred: the jobs' tails run together in one
+        // launch after the workers, from per-job regions (slot 12), and every job's results
+        // (slot 13, the stage layout) reach the host in one copy.
+        std::vector<FriDefer> defers(n_jobs);
+        {
+            std::vector<size_t> region_off(n_jobs, 0);
+            size_t words = 0;
+            for (size_t i : rest) {
+                region_off[i] = words;
+                if (job_shape_ok(jobs[i].n_cols, jobs[i].log_n, log_blowup, remainder_log))
+                    words += (fri_defer_words(jobs[i].log_n + log_blowup, remainder_log) + 63) & ~size_t(63);
+            }
+            const size_t res_bytes = (n_jobs * kStageBytes + 255) & ~size_t(255);
+            ZS_TRY(ctx->ensure_scratch(12, std::max<size_t>(words, 64) * 4));
+            ZS_TRY(ctx->ensure_scratch(13, res_bytes + n_jobs * sizeof(FriTail)));
+            for (size_t i : rest) {
+                defers[i].region = ctx->scratch[12].as<uint32_t>() + region_off[i];
+                defers[i].results = ctx->scratch[13].as<uint32_t>() + i * (kStageBytes / 4);
+            }
+        }
         std::atomic<size_t> next{0};
         std::vector<uint32_t> ran_on(n_jobs, 0);
         std::vector<uint8_t> stream_failed(n_workers, 0);
@@ -336,7 +368,7 @@ int32_t zs_commit_batch_dev(zs_hip_ctx *h, zs_job *jobs, size_t n_jobs, uint32_t
                     if ((k = next.fetch_add(1)) >= order.size()) break;
                     const size_t i = order[k];
                     ran_on[i] = t;
-                    jobs[i].status = run_one(w, i, stage + i * kStageBytes);
+                    jobs[i].status = run_one(w, i, nullptr, &defers[i]);
                     if (!ev_ok || hipEventRecord(ev[queued % kInFlight], w->c.stream) != hipSuccess) {
                         ev_ok = false;  // no completion events: one job at a time
                         (void)hipStreamSynchronize(w->c.stream);
@@ -349,9 +381,24 @@ int32_t zs_commit_batch_dev(zs_hip_ctx *h, zs_job *jobs, size_t n_jobs, uint32_t
             });
         }
         for (auto &x : th) x.join();
+        // every worker stream has finished: the deferred tails, then one copy of the results
+        std::vector<FriTail> tails;
+        for (size_t i : rest) {
+            if (stream_failed[ran_on[i]]) jobs[i].status = ZS_ERR_DEVICE;  // no result of that stream is complete
+            if (jobs[i].status == ZS_OK && defers[i].pending) tails.push_back(defers[i].tail);
+        }
+        FriTail *d_tails = reinterpret_cast<FriTail *>(ctx->scratch[13].as<uint8_t>() +
+                                                       ((n_jobs * kStageBytes + 255) & ~size_t(255)));
+        if (!tails.empty()) {
+            ZS_HIP(ctx, hipMemcpyAsync(d_tails, tails.data(), tails.size() * sizeof(FriTail), hipMemcpyHostToDevice,
+                                       ctx->stream));
+            ZS_TRY(launch_fri_tail_multi(ctx, d_tails, (uint32_t)tails.size()));
+        }
+        ZS_HIP(ctx, hipMemcpyAsync(stage, ctx->scratch[13].ptr, n_jobs * kStageBytes, hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        ZS_HIP(ctx, hipStreamSynchronize(ctx->stream));
         for (size_t i : rest) {
             zs_job &j = jobs[i];
-            if (stream_failed[ran_on[i]]) j.status = ZS_ERR_DEVICE;  // no result of that stream is complete
             if (j.status != ZS_OK) continue;
             const uint32_t L = j.log_n + log_blowup - remainder_log;
             const size_t rc = size_t(1) << (remainder_log - log_blowup);
