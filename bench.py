@@ -314,7 +314,7 @@ def batch_leg(Z, torch, dev, stream, n_traces, autotune):
         tuned, t_tune = len(shapes), time.perf_counter() - t0
     run(4)  # warm: twiddles, FRI tables, worker scratch
     tune = {}
-    for w in (2, 4, 8):
+    for w in (4, 8, 16):
         run(w)
         tune[w] = round(run(w)[0], 2)
     best_w = min(tune, key=tune.get)
@@ -542,7 +542,13 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=200, help="serving leg size (0 = skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device copy (for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--hw-queues", type=int, default=16,
+                    help="GPU_MAX_HW_QUEUES for this process (0 = leave the environment's): the batch and "
+                         "serving legs run up to 16 worker streams, which share in-order hardware queues "
+                         "when there are fewer queues than streams")
     args = ap.parse_args()
+    if args.hw_queues > 0:  # read by the HIP runtime when it starts (before torch touches the GPU)
+        os.environ["GPU_MAX_HW_QUEUES"] = str(min(args.hw_queues, 32))
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: the benchmark measures one GPU "
               f"(the commitment path is not sharded across devices); run with --gpus 1", file=sys.stderr)
@@ -682,6 +688,7 @@ def main():
         "batch": batch,
         "serve": serve,
         "root": root_fused.hex(),
+        "runtime": {"GPU_MAX_HW_QUEUES": os.environ.get("GPU_MAX_HW_QUEUES")},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n_cols, lg, beta)
