@@ -77,6 +77,44 @@ def test_uneven_batch_equals_single_jobs(zctx, torch_dev):
             assert alone[0] == oroot and alone[1] == oroots and np.array_equal(alone[2], orem)
 
 
+def test_batch_with_a_rejected_job_keeps_the_others(zctx, torch_dev):
+    """One job of a worker-stream batch has a shape commit jobs reject (no columns):
+    the call reports ZS_ERR_INPUT and that job's status says so, while every other job
+    (their FRI tails deferred to the launch after the workers) still gets its full
+    result, equal to the same job alone."""
+    import ctypes
+    rng = np.random.default_rng(11)
+    shapes = [(12, 11), (9, 8), (20, 12), (7, 13), (16, 10)]
+    inputs = [rng.integers(0, P, (c, 1 << lg), dtype=np.uint32) for c, lg in shapes]
+    bufs = [dev_job(torch_dev, x, 2) for x in inputs]
+    bad = 2
+    jobs = (Z.Job * len(inputs))()
+    keep = []
+    for k, (x, b) in enumerate(zip(inputs, bufs)):
+        lg = x.shape[1].bit_length() - 1
+        L = lg + 2 - 5
+        fr = ctypes.create_string_buffer(32 * L)
+        rem = np.zeros(4 << (5 - 2), np.uint32)
+        keep.append((fr, rem, L))
+        jobs[k].d_cols, jobs[k].n_cols, jobs[k].log_n = b[0].data_ptr(), 0 if k == bad else x.shape[0], lg
+        jobs[k].d_lde, jobs[k].d_nodes = b[1].data_ptr(), b[2].data_ptr()
+        jobs[k].fri_roots = ctypes.cast(fr, ctypes.c_void_p)
+        jobs[k].remainder = rem.ctypes.data
+    rc = Z.lib().zs_commit_batch_dev(zctx.h, jobs, len(inputs), 2, 5, 4, 0)
+    assert rc == Z.ZS_ERR_INPUT
+    assert jobs[bad].status == Z.ZS_ERR_INPUT
+    for k, (x, b) in enumerate(zip(inputs, bufs)):
+        if k == bad:
+            continue
+        assert jobs[k].status == 0, k
+        alone = zctx.commit_job_dev(b[0].data_ptr(), x.shape[0], x.shape[1].bit_length() - 1, 2, 5,
+                                    b[1].data_ptr(), b[2].data_ptr())
+        fr, rem, L = keep[k]
+        assert bytes(jobs[k].trace_root) == alone[0], k
+        assert [fr.raw[32 * i:32 * (i + 1)] for i in range(L)] == alone[1], k
+        assert np.array_equal(rem.reshape(-1, 4), alone[2]), k
+
+
 def test_server_results_same_alone_and_mixed(zctx, torch_dev):
     """Requests submitted together (overlapping on the device under a memory budget)
     give the results each one gives alone."""
